@@ -1,0 +1,154 @@
+/*
+ * ophip.h -- C-ABI of libophip, the MI355X (gfx950) operator-method pricing
+ * library.  Drop-in for OPLib's opcuda (the .cu files of oplib/opcuda), reached from the
+ * C# opmodel layer through [DllImport]; every opcuda_* entry point keeps the
+ * reference's name, argument order and meaning.  Each declaration cites the
+ * reference definition it replaces.
+ *
+ * Device buffers are 32-bit handles: byte offsets into one per-device arena
+ * (0 = null).  The reference's 32-bit "device pointers" (CArray.cs:46,
+ * util.cu:209-214) and the caller's arithmetic on them (e.g.
+ * CModel Device.cs:193, ptr + j*d*d*4) carry over unchanged.  Device-side
+ * pointer tables and descriptors (opcuda_ssqmm, opcuda_sgemm4,
+ * opcuda_sgemv4) hold handles too.  Host pointers are 64-bit.
+ *
+ * Status: every call sets a thread-local status read by opcuda_get_status();
+ * entry points that return int return that status (0 = success).  Codes are
+ * hipError_t values or OPHIP_ERR_* below.
+ *
+ * Streams: all work goes to one non-blocking stream per device;
+ * opcuda_thread_synchronize() waits for it; copies are synchronous.
+ *
+ * Matrices are column-major; element (y0,y1) of a d x d block is at
+ * y0 + d*y1 and row y0 holds the rates/probabilities out of state y0.
+ */
+#ifndef OPHIP_H
+#define OPHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OPHIP_ERR_BAD_HANDLE  0x10001u
+#define OPHIP_ERR_OUT_OF_ARENA 0x10002u
+#define OPHIP_ERR_BAD_ARG     0x10003u
+#define OPHIP_ERR_NO_DEVICE   0x10004u
+
+/* ---- context, device, status (util.cu) ---- */
+int      opcuda_cublas_init(void);                     /* util.cu:123 (creates arena + stream) */
+int      opcuda_shutdown(void);                        /* util.cu:287 */
+void     opcuda_thread_synchronize(void);              /* util.cu:131 */
+void     opcuda_set_device(int dev);                   /* util.cu:139 */
+int      opcuda_get_device(void);                      /* util.cu:145 */
+int      opcuda_device_get_count(void);                /* util.cu:156 */
+int      opcuda_device_count(void);                    /* util.cu:240 */
+int      opcuda_multi_processor_count(int dev);        /* util.cu:248 */
+int      opcuda_max_threads_per_block(int dev);        /* util.cu:255 */
+unsigned opcuda_get_status(void);                      /* util.cu:204 */
+
+/* ---- memory (util.cu:182-236) ---- */
+unsigned      opcuda_mem_alloc(unsigned sz);                          /* util.cu:209 */
+void          opcuda_mem_free_device(unsigned p);                     /* util.cu:217 */
+unsigned long opcuda_mem_alloc_host(unsigned sz);                     /* util.cu:223 */
+void          opcuda_mem_free_host(unsigned long h);                  /* util.cu:232 */
+int opcuda_memcpy_h2d(unsigned dptr, unsigned long hptr, unsigned sz);/* util.cu:182 */
+int opcuda_memcpy_d2h(unsigned dptr, unsigned long hptr, unsigned sz);/* util.cu:190 */
+int opcuda_memcpy_d2d(unsigned dst, unsigned src, unsigned sz);       /* util.cu:198 */
+
+/* ---- timers (util.cu:62-113) ---- */
+void   opcuda_create_timer(void);
+void   opcuda_reset_timer(void);
+void   opcuda_stop_timer(void);
+double opcuda_get_timer_value(void);
+
+/* ---- copies and fills ---- */
+void opcuda_scopy(int n, unsigned x, int incx, unsigned y, int incy); /* util.cu:278 */
+int  opcuda_scopy1(unsigned dst, unsigned src, unsigned n);           /* scopy2.cu:137 (returns status; C# declares int) */
+void opcuda_ssetall(unsigned x, int n, float c, int incx);            /* sset.cu:54 */
+void opcuda_ssetone(unsigned x, int n, float c, int i);               /* sset.cu:69 */
+
+/* ---- BLAS (sgemm.cu, sgemv.cu) ---- */
+void opcuda_sgemm(int m, int n, int k, float alpha, unsigned A, int lda, unsigned B, int ldb,
+                  float beta, unsigned C, int ldc);                    /* sgemm.cu:41 */
+void opcuda_sgemv(int m, int n, float alpha, unsigned A, int lda, unsigned x, int incx,
+                  float beta, unsigned y, int incy);                   /* sgemv.cu:44 */
+void opcuda_sgemvT(int m, int n, float alpha, unsigned A, int lda, unsigned x, int incx,
+                   float beta, unsigned y, int incy);                  /* sgemv.cu:55 */
+
+/* ---- batched products (ssqmm.cu, sgemm4.cu) ---- */
+void opcuda_ssqmm(int d, int ni, unsigned A_i, unsigned B_i, unsigned C_i); /* ssqmm.cu:174 */
+void opcuda_sgemm4(int nblocks, unsigned argptr_bid);                 /* sgemm4.cu:148; 64x16 tiles */
+
+/* ---- gather GEMV (sgemv3.cu, sgemv4.cu) ---- */
+void opcuda_sgemv3(int d, int nz, int ncol, unsigned A, unsigned Z, unsigned col0,
+                   unsigned col1);                                     /* sgemv3.cu:163 */
+void opcuda_sgemv4(int nblocks, unsigned argptr_bid);                 /* sgemv4.cu:174 */
+
+/* ---- first step, tables, generator (sek.cu, sck.cu, sglvg.cu) ---- */
+void opcuda_sek(int d, unsigned gen, float dt, unsigned tpk);          /* sek.cu:106 */
+void opcuda_sekd(int d, unsigned gen, float dt, unsigned tpk, unsigned r);          /* sek.cu:117 */
+void opcuda_sekd_lambda(int d, unsigned gen, float dt, unsigned tpk, unsigned r,
+                        unsigned lambda);                               /* sek.cu:129 */
+void opcuda_sck(int d, int nm, unsigned tpk, unsigned ctpk);           /* sck.cu:88 */
+void opcuda_sglvg(unsigned gen, int ni, int d, unsigned invm, unsigned pars); /* sglvg.cu:104 */
+
+/* ---- Monte Carlo (smc.cu) ---- */
+int  opcuda_mc_load_mt_gpu(char *stream, long sz);                     /* smc.cu:87 (MT19937: no-op) */
+int  opcuda_mc_nrng(void);                                             /* smc.cu:98: number of streams */
+int  opcuda_mc_status_sz(void);                                        /* smc.cu:103: 625 words/stream */
+void opcuda_mc_setseed(unsigned long host_seedptr, unsigned mtptr);    /* smc.cu:137 */
+int  opcuda_mc1f(unsigned mt, int y0, unsigned y_sk, int nscen, int nk, int d, unsigned ctpk,
+                 unsigned m_k, unsigned long yhost);                   /* smc.cu:610 (yhost 0: no copy) */
+int  opcuda_mt_benchmark(unsigned mt, unsigned unif, int nscen);       /* smc.cu:731 */
+
+/* ==== MI355X additions (no reference counterpart unless cited) ==== */
+
+/* raw device address of a handle; arena size; library stream */
+unsigned long ophip_device_address(unsigned h);
+unsigned long ophip_arena_bytes(void);
+unsigned long ophip_stream(void);
+
+/* FP64 regime-switching local-vol lattice on y = x + nx*r (S2DGrid.cs:89):
+ * price rates from the 2x2 stencil inverse of S1DGrid.cs:78-147 applied to
+ * (drift, vol^2) as CLVModel.cs:99-114, regime rate q_off at equal x,
+ * diagonal = -sum.  xval: nx doubles; drift, vol: nx*nr doubles (price
+ * units).  nr = 1 is the 1-D LV generator of CLVModel.host_d_gen. */
+int ophip_dgen_rs(int nx, int nr, unsigned xval, unsigned drift, unsigned vol, double q_off,
+                  unsigned gen);
+/* the same generator fused with the first step, evaluated in FP64:
+ * out_kind 0: FP32 I+dt*L, 1: FP64 I+dt*L, 2: FP32 dt*L, 3: FP64 dt*L
+ * (host_d_mc_ek, CModel Host Double.cs:89-121) */
+int ophip_sek_rs(int nx, int nr, unsigned xval, unsigned drift, unsigned vol, double q_off,
+                 double dt, int out_kind, unsigned out);
+/* cumulative tables stored row-contiguous: crow[m*d*d + row*d + y]; the
+ * same values as opcuda_sck (sck.cu:43-84) */
+int ophip_sck_rows(int d, int nm, unsigned tpk, unsigned crow);
+
+/* strided batched FP32 GEMM on the matrix cores */
+int ophip_sgemm_strided_batched(int m, int n, int k, float alpha, unsigned A, int lda,
+                                long long strideA, unsigned B, int ldb, long long strideB,
+                                float beta, unsigned C, int ldc, long long strideC, int batch);
+/* Y = alpha*A*X, A m x K column-major, X (K x ncp) and Y (m x ncp)
+ * row-major, ncp in {16,32,48,64}, m % 16 == 0: one backward-induction
+ * step over many payoff columns (the role of sgemv3/sgemv4) */
+int ophip_sgemm_skinny(int m, int K, int ncp, float alpha, unsigned A, int lda, unsigned X,
+                       int ldx, unsigned Y, int ldy);
+
+/* Monte Carlo */
+int ophip_mc_set_nrng(int nstreams);
+int ophip_mc1f_rows(unsigned mt, int y0, unsigned y_sk, int nscen, int nk, int d,
+                    unsigned crow, unsigned m_k, unsigned long yhost);
+
+/* graphs, events, workspace */
+int           ophip_reserve_workspace(unsigned long bytes);
+int           ophip_capture_begin(void);
+unsigned long ophip_capture_end(void);
+int           ophip_graph_launch(unsigned long graph_exec);
+void          ophip_graph_destroy(unsigned long graph_exec);
+int           ophip_event_record(int slot);
+double        ophip_event_elapsed_ms(int a, int b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

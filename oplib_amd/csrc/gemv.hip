@@ -1,0 +1,319 @@
+// gemv.hip -- matrix-vector products of the pricing path.
+//
+// Drop-in entry points:
+//   opcuda_sgemv / opcuda_sgemvT   BLAS sgemv 'N' / 'T' (sgemv.cu:44-62)
+//   opcuda_sgemv3                  gather/scatter GEMV over 16-column groups,
+//                                  Z[col1[c]+i] = sum_k A[i,k] Z[col0[c]+k]
+//                                  (sgemv3.cu:92-176)
+//   opcuda_sgemv4                  the same over per-tile descriptors
+//                                  {i,bx,by,d,nz,ncol,A,Z,col0,col1}
+//                                  (sgemv4.cu:90-180)
+// MI355X addition:
+//   ophip_sgemm_skinny   Y = alpha*A*X for n <= 64 right-hand columns,
+//                        X and Y row-major (X[k*ldx + c]); the backward-
+//                        induction step V_t = df*U*V_{t+1}.  One pass over
+//                        A from HBM on the matrix cores, split-K partials
+//                        combined deterministically.
+//
+// All reductions have a fixed order: results are bitwise reproducible run
+// to run.
+#include "ophip_internal.h"
+
+using namespace ophip;
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+__device__ inline f4 mfma4(float a, float b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------
+// sgemv 'N': partial sums over K chunks, then a fixed-order combine
+// ---------------------------------------------------------------------
+// grid (ceil(m/256), S); 64 threads, 4 rows each
+__global__ __launch_bounds__(64) void k_gemvN_part(int m, int n, const float *__restrict__ A,
+                                                   int lda, const float *__restrict__ x,
+                                                   int incx, int kc, float *__restrict__ part,
+                                                   int vec) {
+    const int row = blockIdx.x * 256 + threadIdx.x * 4;
+    const int k0 = blockIdx.y * kc, k1 = min(n, k0 + kc);
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+    if (vec && row + 3 < m) {
+        for (int k = k0; k < k1; k++) {
+            float xv = x[(long)k * incx];
+            f4 a = *(const f4 *)(A + row + (size_t)lda * k);
+            acc += a * xv;
+        }
+    } else {
+        for (int k = k0; k < k1; k++) {
+            float xv = x[(long)k * incx];
+            for (int e = 0; e < 4; e++)
+                if (row + e < m) acc[e] += A[row + e + (size_t)lda * k] * xv;
+        }
+    }
+    for (int e = 0; e < 4; e++)
+        if (row + e < m) part[(size_t)blockIdx.y * m + row + e] = acc[e];
+}
+
+__global__ void k_gemv_combine(int m, int S, const float *__restrict__ part, float alpha,
+                               float beta, float *__restrict__ y, int incy) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    float s = 0.0f;
+    for (int q = 0; q < S; q++) s += part[(size_t)q * m + i];
+    float v = alpha * s;
+    if (beta != 0.0f) v += beta * y[(long)i * incy];
+    y[(long)i * incy] = v;
+}
+
+// sgemv 'T': one wave per output column
+__global__ __launch_bounds__(256) void k_gemvT(int m, int n, const float *__restrict__ A, int lda,
+                                               const float *__restrict__ x, int incx, float alpha,
+                                               float beta, float *__restrict__ y, int incy) {
+    const int col = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (col >= n) return;
+    const float *a = A + (size_t)lda * col;
+    float s = 0.0f;
+    for (int i = lane; i < m; i += 64) s += a[i] * x[(long)i * incx];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) {
+        float v = alpha * s;
+        if (beta != 0.0f) v += beta * y[(long)col * incy];
+        y[(long)col * incy] = v;
+    }
+}
+
+// ---------------------------------------------------------------------
+// sgemv3 / sgemv4: 64-row x 16-column-group tile per workgroup (one wave)
+// ---------------------------------------------------------------------
+__device__ void gather_tile(int d, int nz, int ncol, const float *__restrict__ A,
+                            float *__restrict__ Z, const int *__restrict__ col0,
+                            const int *__restrict__ col1, int bx, int by) {
+    const int lane = threadIdx.x, r = lane & 15, grp = lane >> 4;
+    const int sentinel = 2 * nz * d;
+    // first sentinel of this group: stores stop there (sgemv3.cu:141-157)
+    int c = by * 16 + r;
+    bool valid = c < ncol && col1[c] < sentinel;
+    unsigned long long bal = __ballot(!valid);
+    int first_bad = (int)__ffsll((long long)(bal & 0xFFFFull)) - 1;  // lanes 0..15 carry the group
+    if (first_bad < 0) first_bad = 16;
+    const bool live = r < first_bad;
+    const int c0 = live ? col0[c] : 0;
+    const int row0 = bx * 64;
+    f4 acc[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) acc[i] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < d; k0 += 4) {
+        int kk = k0 + grp;
+        float b = (live && kk < d) ? Z[c0 + kk] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            int row = row0 + 16 * i + r;
+            float a = (kk < d && row < d) ? A[row + (size_t)d * kk] : 0.0f;
+            acc[i] = mfma4(a, b, acc[i]);
+        }
+    }
+    if (!live) return;
+    const int c1 = col1[c];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            int row = row0 + 16 * i + 4 * grp + e;
+            if (row < d) Z[c1 + row] = acc[i][e];
+        }
+}
+
+__global__ __launch_bounds__(64) void k_sgemv3(int d, int nz, int ncol, const float *A, float *Z,
+                                               const int *col0, const int *col1) {
+    gather_tile(d, nz, ncol, A, Z, col0, col1, blockIdx.x, blockIdx.y);
+}
+
+__global__ __launch_bounds__(64) void k_sgemv4(const char *__restrict__ base,
+                                               const uint32_t *__restrict__ argptr_bid) {
+    const uint32_t *arg = (const uint32_t *)(base + argptr_bid[blockIdx.x]);
+    int bx = (int)arg[1], by = (int)arg[2], d = (int)arg[3], nz = (int)arg[4];
+    int ncol = (int)arg[5];
+    const float *A = (const float *)(base + arg[6]);
+    float *Z = (float *)(base + arg[7]);
+    const int *col0 = (const int *)(base + arg[8]);
+    const int *col1 = (const int *)(base + arg[9]);
+    // the reference leaves ncol unused in the descriptor (sgemv4.cu:100-107);
+    // 0 there means "the group is full, stop only at sentinels"
+    if (ncol <= 0) ncol = (by + 1) * 16;
+    gather_tile(d, nz, ncol, A, Z, col0, col1, bx, by);
+}
+
+// ---------------------------------------------------------------------
+// skinny GEMM: Y[m x NCP] = alpha * A[m x K] * X[K x NCP], X/Y row-major
+// grid (m/16, KS); 4 waves split the workgroup's K range in 16-deep blocks
+// ---------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(256) void k_skinny(int m, int K, int kc, const float *__restrict__ A,
+                                                int lda, const float *__restrict__ X, int ldx,
+                                                float *__restrict__ out, int ldo, float alpha) {
+    __shared__ f4 red[4][NT][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, grp = lane >> 4;
+    const int row0 = blockIdx.x * 16;
+    const int kbeg = blockIdx.y * kc, kend = min(K, kbeg + kc);
+    f4 acc[NT];
+#pragma unroll
+    for (int j = 0; j < NT; j++) acc[j] = f4{0.f, 0.f, 0.f, 0.f};
+    const float *Ar = A + row0 + r;
+    for (int kb = kbeg + 16 * wave; kb < kend; kb += 64) {
+        // lane group g of MFMA step s takes k = kb + 4g + s
+        float a[4];
+        f4 xb[NT];
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            int k = kb + 4 * grp + s;
+            a[s] = (k < kend) ? Ar[(size_t)lda * k] : 0.0f;
+#pragma unroll
+            for (int j = 0; j < NT; j++) xb[j][s] = (k < kend) ? X[(size_t)k * ldx + 16 * j + r] : 0.0f;
+        }
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+#pragma unroll
+            for (int j = 0; j < NT; j++) acc[j] = mfma4(a[s], xb[j][s], acc[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < NT; j++) red[wave][j][lane] = acc[j];
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+            f4 v = red[0][j][lane];
+            v += red[1][j][lane];
+            v += red[2][j][lane];
+            v += red[3][j][lane];
+            v *= alpha;
+            // lane holds rows 4*grp+e of column 16j+r
+            float *o = out + (size_t)blockIdx.y * m * ldo;
+#pragma unroll
+            for (int e = 0; e < 4; e++) o[(size_t)(row0 + 4 * grp + e) * ldo + 16 * j + r] = v[e];
+        }
+    }
+}
+
+__global__ void k_skinny_combine(int m, int ncp, int KS, const float *__restrict__ part,
+                                 float *__restrict__ Y, int ldy) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= m * ncp) return;
+    int i = idx / ncp, c = idx - i * ncp;
+    float s = part[(size_t)i * ncp + c];
+    for (int q = 1; q < KS; q++) s += part[(size_t)q * m * ncp + (size_t)i * ncp + c];
+    Y[(size_t)i * ldy + c] = s;
+}
+
+}  // namespace
+
+OPHIP_EXPORT void opcuda_sgemv(int m, int n, float alpha, unsigned A, int lda, unsigned x,
+                               int incx, float beta, unsigned y, int incy) {
+    Device &D = dev();
+    if (m < 0 || n < 0 || lda < (m > 1 ? m : 1) || incx == 0 || incy == 0) {
+        set_status(kErrBadArg);
+        return;
+    }
+    if (m == 0) { set_status(kOk); return; }
+    const float *Ap = P<const float>(A), *xp = P<const float>(x);
+    float *yp = P<float>(y);
+    if (!yp || (n > 0 && (!Ap || !xp))) { set_status(kErrBadHandle); return; }
+    if (incx < 0) xp += (long)(n - 1) * (-incx);
+    if (incy < 0) yp += (long)(m - 1) * (-incy);
+    int rb = cdiv(m, 256);
+    int S = n > 0 ? cdiv(1024, rb) : 1;
+    if (S > n && n > 0) S = n;
+    int kc = n > 0 ? cdiv(n, S) : 0;
+    if (n > 0) S = cdiv(n, kc);
+    float *part = (float *)workspace((size_t)S * m * sizeof(float));
+    if (!part) return;
+    int vec = ((lda % 4) == 0 && (((uintptr_t)Ap) & 15) == 0);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_gemvN_part, dim3(rb, S), dim3(64), 0, D.stream, m, n, Ap, lda, xp, incx,
+                           kc, part, vec);
+    } else {
+        S = 0;
+    }
+    hipLaunchKernelGGL(k_gemv_combine, dim3(cdiv(m, 256)), dim3(256), 0, D.stream, m, S, part,
+                       alpha, beta, yp, incy);
+    check_launch();
+}
+
+OPHIP_EXPORT void opcuda_sgemvT(int m, int n, float alpha, unsigned A, int lda, unsigned x,
+                                int incx, float beta, unsigned y, int incy) {
+    Device &D = dev();
+    if (m < 0 || n < 0 || lda < (m > 1 ? m : 1) || incx == 0 || incy == 0) {
+        set_status(kErrBadArg);
+        return;
+    }
+    if (n == 0) { set_status(kOk); return; }
+    const float *Ap = P<const float>(A), *xp = P<const float>(x);
+    float *yp = P<float>(y);
+    if (!yp || (m > 0 && (!Ap || !xp))) { set_status(kErrBadHandle); return; }
+    if (incx < 0) xp += (long)(m - 1) * (-incx);
+    if (incy < 0) yp += (long)(n - 1) * (-incy);
+    hipLaunchKernelGGL(k_gemvT, dim3(cdiv(n, 4)), dim3(256), 0, D.stream, m, n, Ap, lda, xp, incx,
+                       alpha, beta, yp, incy);
+    check_launch();
+}
+
+OPHIP_EXPORT void opcuda_sgemv3(int d, int nz, int ncol, unsigned A, unsigned Z, unsigned col0,
+                                unsigned col1) {
+    Device &D = dev();
+    if (d <= 0 || ncol <= 0) { set_status(kOk); return; }
+    if (!ptr(A) || !ptr(Z) || !ptr(col0) || !ptr(col1)) { set_status(kErrBadHandle); return; }
+    hipLaunchKernelGGL(k_sgemv3, dim3(cdiv(d, 64), cdiv(ncol, 16)), dim3(64), 0, D.stream, d, nz,
+                       ncol, P<const float>(A), P<float>(Z), P<const int>(col0),
+                       P<const int>(col1));
+    check_launch();
+}
+
+OPHIP_EXPORT void opcuda_sgemv4(int nblocks, unsigned argptr_bid) {
+    Device &D = dev();
+    if (nblocks <= 0) { set_status(kOk); return; }
+    const uint32_t *tab = P<const uint32_t>(argptr_bid);
+    if (!tab) { set_status(kErrBadHandle); return; }
+    hipLaunchKernelGGL(k_sgemv4, dim3(nblocks), dim3(64), 0, D.stream, (const char *)D.base, tab);
+    check_launch();
+}
+
+// Y = alpha * A * X.  A: m x K column-major (lda); X: K x ncp row-major
+// (ldx >= ncp); Y: m x ncp row-major (ldy >= ncp).  ncp must be 16, 32, 48
+// or 64 (pad the right-hand side with zero columns); m % 16 == 0.
+OPHIP_EXPORT int ophip_sgemm_skinny(int m, int K, int ncp, float alpha, unsigned A, int lda,
+                                    unsigned X, int ldx, unsigned Y, int ldy) {
+    Device &D = dev();
+    if (m <= 0 || K <= 0 || (m % 16) || ncp <= 0 || ncp > 64 || (ncp % 16) || ldx < ncp ||
+        ldy < ncp || lda < m)
+        return (int)set_status(kErrBadArg);
+    const float *Ap = P<const float>(A), *Xp = P<const float>(X);
+    float *Yp = P<float>(Y);
+    if (!Ap || !Xp || !Yp) return (int)set_status(kErrBadHandle);
+    const int strips = m / 16;
+    int KS = cdiv(2 * (D.n_cu > 0 ? D.n_cu : 256), strips);
+    if (KS > 16) KS = 16;
+    int kc = cdiv(cdiv(K, KS), 64) * 64;
+    KS = cdiv(K, kc);
+    float *out = Yp;
+    int ldo = ldy;
+    if (KS > 1) {
+        out = (float *)workspace((size_t)KS * m * ncp * sizeof(float));
+        if (!out) return (int)kErrOutOfArena;
+        ldo = ncp;
+    }
+    dim3 grid(strips, KS), block(256);
+    switch (ncp / 16) {
+    case 1: hipLaunchKernelGGL(k_skinny<1>, grid, block, 0, D.stream, m, K, kc, Ap, lda, Xp, ldx, out, ldo, alpha); break;
+    case 2: hipLaunchKernelGGL(k_skinny<2>, grid, block, 0, D.stream, m, K, kc, Ap, lda, Xp, ldx, out, ldo, alpha); break;
+    case 3: hipLaunchKernelGGL(k_skinny<3>, grid, block, 0, D.stream, m, K, kc, Ap, lda, Xp, ldx, out, ldo, alpha); break;
+    default: hipLaunchKernelGGL(k_skinny<4>, grid, block, 0, D.stream, m, K, kc, Ap, lda, Xp, ldx, out, ldo, alpha); break;
+    }
+    if (KS > 1)
+        hipLaunchKernelGGL(k_skinny_combine, dim3(cdiv(m * ncp, 256)), dim3(256), 0, D.stream, m,
+                           ncp, KS, out, Yp, ldy);
+    return (int)check_launch();
+}

@@ -1,0 +1,315 @@
+// lattice.hip -- generator assembly, first exponentiation step and
+// cumulative transition tables.
+//
+// Drop-in entry points (FP32, the reference's semantics):
+//   opcuda_sek / opcuda_sekd / opcuda_sekd_lambda   (sek.cu:40-138)
+//   opcuda_sck                                      (sck.cu:43-93)
+//   opcuda_sglvg                                    (sglvg.cu:42-125)
+// MI355X additions (FP64 on the GPU where precision needs it):
+//   ophip_dgen_rs     FP64 generator of a regime-switching local-vol lattice
+//   ophip_sek_rs      the same generator fused with U0 = I + dt*L, evaluated
+//                     in FP64 and rounded once to FP32 (or kept as FP64);
+//                     the generator is never written to memory
+//   ophip_sck_rows    cumulative tables stored row-contiguous for the
+//                     sampler (one 64-row tile per workgroup, LDS transpose)
+#include "ophip_internal.h"
+
+using namespace ophip;
+
+// ---------------------------------------------------------------------
+// FP32 first step, drop-in (sek.cu:40-103)
+// ---------------------------------------------------------------------
+
+// mode 0: id + dt*gen; 1: (id + dt*gen)/(1 + dt*r[y1]); 2: ... /(1+dt*r*lambda)
+template <int MODE>
+__global__ void k_sek(int d, const float *__restrict__ gen, float dt, float *__restrict__ tpk,
+                      const float *__restrict__ r, const float *__restrict__ lambda) {
+    size_t n = (size_t)d * d;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        int y1 = (int)(e % d), y2 = (int)(e / d);
+        float v = fmaf(dt, gen[e], (y1 == y2) ? 1.0f : 0.0f);
+        if (MODE == 1) v = v / fmaf(dt, r[y1], 1.0f);
+        if (MODE == 2) v = v / fmaf(dt * r[y1], lambda[0], 1.0f);
+        tpk[e] = v;
+    }
+}
+
+static int grid1d(size_t n, int bs) {
+    size_t g = (n + bs - 1) / bs;
+    if (g > 16384) g = 16384;
+    return g ? (int)g : 1;
+}
+
+OPHIP_EXPORT void opcuda_sek(int d, unsigned gen, float dt, unsigned tpk) {
+    Device &D = dev();
+    if (d <= 0 || !ptr(gen) || !ptr(tpk)) { set_status(kErrBadArg); return; }
+    hipLaunchKernelGGL(k_sek<0>, dim3(grid1d((size_t)d * d, 256)), dim3(256), 0, D.stream, d,
+                       P<const float>(gen), dt, P<float>(tpk), nullptr, nullptr);
+    check_launch();
+}
+
+OPHIP_EXPORT void opcuda_sekd(int d, unsigned gen, float dt, unsigned tpk, unsigned r) {
+    Device &D = dev();
+    if (d <= 0 || !ptr(gen) || !ptr(tpk) || !ptr(r)) { set_status(kErrBadArg); return; }
+    hipLaunchKernelGGL(k_sek<1>, dim3(grid1d((size_t)d * d, 256)), dim3(256), 0, D.stream, d,
+                       P<const float>(gen), dt, P<float>(tpk), P<const float>(r), nullptr);
+    check_launch();
+}
+
+OPHIP_EXPORT void opcuda_sekd_lambda(int d, unsigned gen, float dt, unsigned tpk, unsigned r,
+                                     unsigned lambda) {
+    Device &D = dev();
+    if (d <= 0 || !ptr(gen) || !ptr(tpk) || !ptr(r) || !ptr(lambda)) {
+        set_status(kErrBadArg);
+        return;
+    }
+    hipLaunchKernelGGL(k_sek<2>, dim3(grid1d((size_t)d * d, 256)), dim3(256), 0, D.stream, d,
+                       P<const float>(gen), dt, P<float>(tpk), P<const float>(r),
+                       P<const float>(lambda));
+    check_launch();
+}
+
+// ---------------------------------------------------------------------
+// FP32 LV generator, drop-in (sglvg.cu:42-125).  One thread per (i, y0);
+// the row is zeroed and its two rates written.  pars = [drift(ni*d),
+// vol(ni*d)], invm = 4 floats per node (column-major 2x2).
+// ---------------------------------------------------------------------
+
+__global__ void k_sglvg(float *__restrict__ gen, int ni, int d, const float *__restrict__ invm,
+                        const float *__restrict__ drift, const float *__restrict__ vol) {
+    size_t total = (size_t)ni * d * d;
+    // zero pass, coalesced
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x)
+        gen[e] = 0.0f;
+}
+
+__global__ void k_sglvg_rates(float *__restrict__ gen, int ni, int d,
+                              const float *__restrict__ invm, const float *__restrict__ drift,
+                              const float *__restrict__ vol) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ni * d) return;
+    int i = t / d, y0 = t - i * d;
+    if (y0 == 0 || y0 == d - 1) return;  // absorbing boundary rows stay zero
+    float r0 = drift[y0 + d * i];
+    float v = vol[y0 + d * i];
+    float r1 = v * v;
+    float *g = gen + (size_t)i * d * d;
+    float lo = fmaf(invm[4 * y0 + 2], r1, invm[4 * y0 + 0] * r0);
+    float hi = fmaf(invm[4 * y0 + 3], r1, invm[4 * y0 + 1] * r0);
+    g[y0 + (size_t)d * (y0 - 1)] = lo;
+    g[y0 + (size_t)d * (y0 + 1)] = hi;
+    g[y0 + (size_t)d * y0] = -(lo + hi);
+}
+
+OPHIP_EXPORT void opcuda_sglvg(unsigned gen, int ni, int d, unsigned invm, unsigned pars) {
+    Device &D = dev();
+    if (ni <= 0 || d < 2 || !ptr(gen) || !ptr(invm) || !ptr(pars)) {
+        set_status(kErrBadArg);
+        return;
+    }
+    const float *drift = P<const float>(pars);
+    const float *vol = drift + (size_t)ni * d;
+    hipLaunchKernelGGL(k_sglvg, dim3(grid1d((size_t)ni * d * d, 256)), dim3(256), 0, D.stream,
+                       P<float>(gen), ni, d, P<const float>(invm), drift, vol);
+    hipLaunchKernelGGL(k_sglvg_rates, dim3(cdiv(ni * d, 256)), dim3(256), 0, D.stream,
+                       P<float>(gen), ni, d, P<const float>(invm), drift, vol);
+    check_launch();
+}
+
+// ---------------------------------------------------------------------
+// FP64 regime-switching local-vol lattice (LV is nr = 1).
+// States y = x + nx*r (S2DGrid.cs:89).  Row y: rates to x-1 and x+1 in the
+// same regime from the closed-form inverse of the 2x2 stencil of
+// S1DGrid.cs:78-147 applied to (drift, vol^2) as in CLVModel.cs:99-114;
+// rate q_off to every other regime at the same x; diagonal = -sum.  Rows at
+// x = 0 and x = nx-1 have no price moves (absorbing in x).
+// ---------------------------------------------------------------------
+
+struct RSLattice {
+    int nx, nr;
+    const double *xval;   // nx nodes
+    const double *drift;  // per state (nx*nr), in price units
+    const double *vol;    // per state (nx*nr), absolute vol in price units
+    double q_off;
+};
+
+__device__ inline void rs_rates(const RSLattice &L, int y, double &am, double &ap) {
+    int x = y % L.nx;
+    am = 0.0;
+    ap = 0.0;
+    if (x <= 0 || x >= L.nx - 1) return;
+    double m00 = L.xval[x - 1] - L.xval[x];
+    double m01 = L.xval[x + 1] - L.xval[x];
+    double m10 = m00 * m00, m11 = m01 * m01;
+    double det = m00 * m11 - m01 * m10;
+    double i00 = m11 / det, i10 = -m10 / det, i01 = -m01 / det, i11 = m00 / det;
+    double mu = L.drift[y], v = L.vol[y];
+    double s2 = v * v;
+    am = i00 * mu + i01 * s2;
+    ap = i10 * mu + i11 * s2;
+}
+
+// MODE 0: generator (FP64 out); MODE 1: I + dt*L; MODE 2: dt*L (U0 - I)
+template <int MODE, typename OUT>
+__global__ void k_rs_assemble(RSLattice L, double dt, OUT *__restrict__ out) {
+    const int nx = L.nx, d = L.nx * L.nr;
+    size_t n = (size_t)d * d;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        int i = (int)(e % d), j = (int)(e / d);  // row i (from), column j (to)
+        int xi = i % nx, ri = i / nx, xj = j % nx, rj = j / nx;
+        double g = 0.0;
+        if (ri == rj) {
+            if (xj == xi - 1 || xj == xi + 1 || xj == xi) {
+                double am, ap;
+                rs_rates(L, i, am, ap);
+                if (xj == xi - 1) g = am;
+                else if (xj == xi + 1) g = ap;
+                else g = -(am + ap + (double)(L.nr - 1) * L.q_off);
+            }
+        } else if (xi == xj) {
+            g = L.q_off;
+        }
+        double v;
+        if (MODE == 0) v = g;
+        else if (MODE == 1) v = (i == j ? 1.0 : 0.0) + dt * g;
+        else v = dt * g;
+        out[e] = (OUT)v;
+    }
+}
+
+static bool rs_args(int nx, int nr, unsigned xval, unsigned drift, unsigned vol, RSLattice &L) {
+    if (nx < 3 || nr < 1 || (long)nx * nr > 65535) return false;
+    L.nx = nx;
+    L.nr = nr;
+    L.xval = P<const double>(xval);
+    L.drift = P<const double>(drift);
+    L.vol = P<const double>(vol);
+    return L.xval && L.drift && L.vol;
+}
+
+// FP64 generator (d = nx*nr, d*d doubles at `gen`)
+OPHIP_EXPORT int ophip_dgen_rs(int nx, int nr, unsigned xval, unsigned drift, unsigned vol,
+                               double q_off, unsigned gen) {
+    Device &D = dev();
+    RSLattice L;
+    if (!rs_args(nx, nr, xval, drift, vol, L) || !ptr(gen)) return (int)set_status(kErrBadArg);
+    L.q_off = q_off;
+    size_t d = (size_t)nx * nr;
+    hipLaunchKernelGGL((k_rs_assemble<0, double>), dim3(grid1d(d * d, 256)), dim3(256), 0,
+                       D.stream, L, 0.0, P<double>(gen));
+    return (int)check_launch();
+}
+
+// Fused assembly + first step.  out_kind 0: FP32 I+dt*L; 1: FP64 I+dt*L;
+// 2: FP32 dt*L (the U-I form); 3: FP64 dt*L.
+OPHIP_EXPORT int ophip_sek_rs(int nx, int nr, unsigned xval, unsigned drift, unsigned vol,
+                              double q_off, double dt, int out_kind, unsigned out) {
+    Device &D = dev();
+    RSLattice L;
+    if (!rs_args(nx, nr, xval, drift, vol, L) || !ptr(out)) return (int)set_status(kErrBadArg);
+    L.q_off = q_off;
+    size_t d = (size_t)nx * nr;
+    dim3 g(grid1d(d * d, 256)), b(256);
+    switch (out_kind) {
+    case 0: hipLaunchKernelGGL((k_rs_assemble<1, float>), g, b, 0, D.stream, L, dt, P<float>(out)); break;
+    case 1: hipLaunchKernelGGL((k_rs_assemble<1, double>), g, b, 0, D.stream, L, dt, P<double>(out)); break;
+    case 2: hipLaunchKernelGGL((k_rs_assemble<2, float>), g, b, 0, D.stream, L, dt, P<float>(out)); break;
+    case 3: hipLaunchKernelGGL((k_rs_assemble<2, double>), g, b, 0, D.stream, L, dt, P<double>(out)); break;
+    default: return (int)set_status(kErrBadArg);
+    }
+    return (int)check_launch();
+}
+
+// ---------------------------------------------------------------------
+// cumulative tables
+// ---------------------------------------------------------------------
+
+// Drop-in layout (sck.cu:43-84): ctpk[m*d*d + row + d*y].  One thread per
+// (m,row): sequential FP32 running sum along the row, then divide by the
+// total and cap at 1.  Negative entries count as zero (a rounding-level
+// negative would break monotonicity; FP32 plain kernels have none).
+__global__ void k_sck(int d, int nm, const float *__restrict__ tpk, float *__restrict__ ctpk) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nm * d) return;
+    int m = t / d, row = t - m * d;
+    size_t base = (size_t)m * d * d + row;
+    const float *tp = tpk + base;
+    float *cp = ctpk + base;
+    float s = 0.0f;
+    for (int y = 0; y < d; y++) {
+        float v = tp[(size_t)d * y];
+        s += (v > 0.0f) ? v : 0.0f;
+        cp[(size_t)d * y] = s;
+    }
+    for (int y = 0; y < d; y++) {
+        float v = cp[(size_t)d * y] / s;
+        cp[(size_t)d * y] = (v < 1.0f) ? v : 1.0f;
+    }
+}
+
+OPHIP_EXPORT void opcuda_sck(int d, int nm, unsigned tpk, unsigned ctpk) {
+    Device &D = dev();
+    if (d <= 0 || nm <= 0 || !ptr(tpk) || !ptr(ctpk)) { set_status(kErrBadArg); return; }
+    hipLaunchKernelGGL(k_sck, dim3(cdiv(nm * d, 64)), dim3(64), 0, D.stream, d, nm,
+                       P<const float>(tpk), P<float>(ctpk));
+    check_launch();
+}
+
+// Row-contiguous table crow[m*d*d + row*d + y], same values bit for bit
+// (same sequential order per row).  Workgroup = 64 rows x one m; the
+// column-major input is read 64 states at a time through an LDS tile so
+// both the reads (64 consecutive rows) and the writes (64 consecutive y)
+// are coalesced.  Pass 1 forms the totals, pass 2 re-forms the running sums
+// in the same order and writes the normalised values.
+__global__ __launch_bounds__(64) void k_sck_rows(int d, const float *__restrict__ tpk,
+                                                  float *__restrict__ crow) {
+    __shared__ float tile[64][65];
+    const int lane = threadIdx.x;
+    const int row0 = blockIdx.x * 64;
+    const int m = blockIdx.y;
+    const size_t mb = (size_t)m * d * d;
+    const int row = row0 + lane;
+    float total = 0.0f;
+    for (int pass = 0; pass < 2; pass++) {
+        float s = 0.0f;
+        for (int y0 = 0; y0 < d; y0 += 64) {
+            int ny = min(64, d - y0);
+            // load tile: element (r, c) = tpk[row0 + r, y0 + c]
+            for (int c = 0; c < ny; c++) {
+                float v = 0.0f;
+                if (row0 + lane < d) v = tpk[mb + (size_t)(row0 + lane) + (size_t)d * (y0 + c)];
+                tile[lane][c] = v;
+            }
+            __syncthreads();
+            // each lane scans its own row through the tile
+            for (int c = 0; c < ny; c++) {
+                float v = tile[lane][c];
+                s += (v > 0.0f) ? v : 0.0f;
+                if (pass == 1) {
+                    float q = s / total;
+                    tile[lane][c] = (q < 1.0f) ? q : 1.0f;
+                }
+            }
+            __syncthreads();
+            if (pass == 1) {
+                // write rows: for each row r of the tile, lanes cover y
+                for (int r = 0; r < 64 && row0 + r < d; r++)
+                    if (lane < ny) crow[mb + (size_t)(row0 + r) * d + y0 + lane] = tile[r][lane];
+            }
+            __syncthreads();
+        }
+        total = s;
+    }
+    (void)row;
+}
+
+OPHIP_EXPORT int ophip_sck_rows(int d, int nm, unsigned tpk, unsigned crow) {
+    Device &D = dev();
+    if (d <= 0 || nm <= 0 || !ptr(tpk) || !ptr(crow)) return (int)set_status(kErrBadArg);
+    hipLaunchKernelGGL(k_sck_rows, dim3(cdiv(d, 64), nm), dim3(64), 0, D.stream, d,
+                       P<const float>(tpk), P<float>(crow));
+    return (int)check_launch();
+}

@@ -1,0 +1,273 @@
+// mc.hip -- MT19937 streams and the Markov-chain Monte Carlo sampler.
+//
+// Drop-in entry points (smc.cu:87-156, 610-656, 731-736):
+//   opcuda_mc_load_mt_gpu, opcuda_mc_nrng, opcuda_mc_status_sz,
+//   opcuda_mc_setseed, opcuda_mc1f, opcuda_mt_benchmark
+// MI355X additions:
+//   ophip_mc_set_nrng   number of streams in a state buffer
+//   ophip_mc1f_rows     sampler over row-contiguous cumulative tables
+//
+// Generator.  The reference runs 4096 per-lane DCMT generators of period
+// 2^607-1 (MersenneTwister.h:8-29) whose parameter file is absent and whose
+// loader is a no-op (smc.cu:87-96).  This library runs MT19937 (Matsumoto &
+// Nishimura 1998), one stream per wavefront: the 624-word state lives in
+// LDS and the 64 lanes twist it together in three dependency-free chunks
+// (i < 227 reads only old words; 227 <= i < 454 and 454 <= i < 624 read
+// words produced by the previous chunk).  State buffer: stream s occupies
+// words [625*s, 625*s + 625): 624 state words and the output index.
+//
+// Path mapping (fixed; the CPU oracle reproduces it): stream s serves G
+// groups of 64 paths, scenario = (s*G + g)*64 + lane.  Within a group the
+// draws are step-major, lane-minor: at step k lane l consumes the stream's
+// next output number k*64 + l.  Uniform u = (x >> 8) * 2^-24 in [0,1).
+// Chain: y_{-1} = y0, y_k = min{ j : u_k < C[m_k[k]][y_{k-1}, j] } clamped
+// to d-1 (the inverse-CDF of rangen.cpp:128-149).  Output is time-major
+// int16 y_sk[k*nscen + scen] (smc.cu:222).
+#include "ophip_internal.h"
+
+#include <vector>
+
+using namespace ophip;
+
+namespace {
+
+constexpr int kMTN = 624;
+constexpr int kMTW = 625;  // words per stream in the state buffer
+int g_nrng = 4096;
+
+__device__ inline uint32_t mt_f(uint32_t a, uint32_t b, uint32_t m) {
+    uint32_t y = (a & 0x80000000u) | (b & 0x7fffffffu);
+    return m ^ (y >> 1) ^ ((b & 1u) ? 0x9908b0dfu : 0u);
+}
+
+// dst = twist(src); 64 lanes; three chunks separated by barriers
+__device__ inline void mt_twist(const uint32_t *src, uint32_t *dst, int lane) {
+    for (int i = lane; i < 227; i += 64) dst[i] = mt_f(src[i], src[i + 1], src[i + 397]);
+    __syncthreads();
+    for (int i = 227 + lane; i < 454; i += 64) dst[i] = mt_f(src[i], src[i + 1], dst[i - 227]);
+    __syncthreads();
+    for (int i = 454 + lane; i < kMTN; i += 64)
+        dst[i] = mt_f(src[i], (i == kMTN - 1) ? dst[0] : src[i + 1], dst[i - 227]);
+    __syncthreads();
+}
+
+__device__ inline uint32_t mt_temper(uint32_t y) {
+    y ^= (y >> 11);
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= (y >> 18);
+    return y;
+}
+
+// One wavefront's view of its stream: two LDS buffers and the index.
+struct WaveMT {
+    uint32_t *buf0, *buf1;
+    int idx, cur;
+    __device__ void load(const uint32_t *st, int lane) {
+        for (int i = lane; i < kMTN; i += 64) buf0[i] = st[i];
+        idx = (int)st[kMTN];
+        cur = 0;
+        __syncthreads();
+    }
+    __device__ void store(uint32_t *st, int lane) {
+        const uint32_t *b = cur ? buf1 : buf0;
+        __syncthreads();
+        for (int i = lane; i < kMTN; i += 64) st[i] = b[i];
+        if (lane == 0) st[kMTN] = (uint32_t)idx;
+    }
+    // 64 consecutive outputs, lane l gets output idx + l
+    __device__ uint32_t next64(int lane) {
+        uint32_t *c = cur ? buf1 : buf0;
+        uint32_t *n = cur ? buf0 : buf1;
+        if (idx + 64 > kMTN) mt_twist(c, n, lane);
+        int pos = idx + lane;
+        uint32_t x = (pos < kMTN) ? c[pos] : n[pos - kMTN];
+        idx += 64;
+        if (idx >= kMTN) {
+            idx -= kMTN;
+            cur ^= 1;
+        }
+        return mt_temper(x);
+    }
+};
+
+__device__ inline float u01(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
+
+// init_genrand for every stream: one thread per stream
+__global__ void k_mt_seed(int nrng, const uint32_t *__restrict__ seeds, uint32_t *__restrict__ st) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nrng) return;
+    uint32_t *p = st + (size_t)s * kMTW;
+    uint32_t v = seeds[s];
+    p[0] = v;
+    for (uint32_t i = 1; i < kMTN; i++) {
+        v = 1812433253u * (v ^ (v >> 30)) + i;
+        p[i] = v;
+    }
+    p[kMTN] = kMTN;
+}
+
+template <bool ROWS>
+__global__ __launch_bounds__(64) void k_mc1f(uint32_t *__restrict__ mt, int G, int nk, int d,
+                                             int y0, const float *__restrict__ ctpk,
+                                             const int *__restrict__ m_k,
+                                             int16_t *__restrict__ y_sk, size_t nscen, int niter) {
+    __shared__ uint32_t b0[kMTN], b1[kMTN];
+    const int lane = threadIdx.x;
+    const int s = blockIdx.x;
+    WaveMT w{b0, b1, 0, 0};
+    uint32_t *st = mt + (size_t)s * kMTW;
+    w.load(st, lane);
+    const size_t d2 = (size_t)d * d;
+    for (int g = 0; g < G; g++) {
+        int y = y0;
+        const size_t scen = ((size_t)s * G + g) * 64 + lane;
+        for (int k = 0; k < nk; k++) {
+            const float u = u01(w.next64(lane));
+            const float *tab = ctpk + (size_t)m_k[k] * d2;
+            const float *row = ROWS ? tab + (size_t)y * d : tab + y;
+            const size_t stride = ROWS ? 1 : (size_t)d;
+            int lo = -1, hi = d - 1;
+            for (int it = 0; it < niter; it++) {
+                if (hi - lo > 1) {
+                    int mid = (lo + hi) >> 1;
+                    if (u < row[stride * mid]) hi = mid;
+                    else lo = mid;
+                }
+            }
+            y = hi;
+            y_sk[(size_t)k * nscen + scen] = (int16_t)y;
+        }
+    }
+    w.store(st, lane);
+}
+
+__global__ __launch_bounds__(64) void k_mt_bench(uint32_t *__restrict__ mt, int per_stream,
+                                                 float *__restrict__ unif) {
+    __shared__ uint32_t b0[kMTN], b1[kMTN];
+    const int lane = threadIdx.x;
+    const int s = blockIdx.x;
+    WaveMT w{b0, b1, 0, 0};
+    uint32_t *st = mt + (size_t)s * kMTW;
+    w.load(st, lane);
+    float *o = unif + (size_t)s * per_stream;
+    for (int j = 0; j < per_stream; j += 64) o[j + lane] = u01(w.next64(lane));
+    w.store(st, lane);
+}
+
+int ceil_log2(int d) {
+    int n = 0;
+    while ((1 << n) < d) n++;
+    return n;
+}
+
+int mc_run(bool rows, unsigned mtptr, int y0, unsigned y_skp, int nscen, int nk, int d,
+           unsigned ctpk, unsigned m_kp, unsigned long yhost) {
+    Device &D = dev();
+    const int nrng = g_nrng;
+    if (nscen <= 0 || nk <= 0 || d <= 0 || d > 32767 || y0 < 0 || y0 >= d ||
+        nscen % (64 * nrng) != 0)
+        return (int)set_status(kErrBadArg);
+    uint32_t *mt = P<uint32_t>(mtptr);
+    int16_t *ysk = P<int16_t>(y_skp);
+    const float *tab = P<const float>(ctpk);
+    const int *mk = P<const int>(m_kp);
+    if (!mt || !ysk || !tab || !mk) return (int)set_status(kErrBadHandle);
+    if (!handle_range_ok(mtptr, (size_t)nrng * kMTW * 4) ||
+        !handle_range_ok(y_skp, (size_t)nscen * nk * 2) || !handle_range_ok(m_kp, (size_t)nk * 4))
+        return (int)set_status(kErrBadHandle);
+    const int G = nscen / (64 * nrng);
+    const int niter = ceil_log2(d);
+    if (rows)
+        hipLaunchKernelGGL(k_mc1f<true>, dim3(nrng), dim3(64), 0, D.stream, mt, G, nk, d, y0, tab,
+                           mk, ysk, (size_t)nscen, niter);
+    else
+        hipLaunchKernelGGL(k_mc1f<false>, dim3(nrng), dim3(64), 0, D.stream, mt, G, nk, d, y0,
+                           tab, mk, ysk, (size_t)nscen, niter);
+    unsigned st = check_launch();
+    if (st) return (int)st;
+    if (yhost) {
+        // the reference copies every batch to the host (smc.cu:653)
+        hipError_t e = hipMemcpyAsync((void *)(uintptr_t)yhost, ysk, (size_t)nscen * nk * 2,
+                                      hipMemcpyDeviceToHost, D.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(D.stream);
+        return (int)set_status(hip_status(e));
+    }
+    return 0;
+}
+
+}  // namespace
+
+// MT19937 has fixed parameters: a DCMT parameter stream has nothing to
+// configure.  Accepted and ignored (the reference's loader copies the
+// buffer onto itself and changes nothing, smc.cu:89-94).
+OPHIP_EXPORT int opcuda_mc_load_mt_gpu(char *stream, long sz) {
+    (void)stream;
+    (void)sz;
+    return (int)set_status(kOk);
+}
+
+OPHIP_EXPORT int opcuda_mc_nrng(void) { return g_nrng; }
+
+OPHIP_EXPORT int opcuda_mc_status_sz(void) { return g_nrng * kMTW; }
+
+OPHIP_EXPORT int ophip_mc_set_nrng(int n) {
+    if (n <= 0 || n > (1 << 20)) return (int)set_status(kErrBadArg);
+    g_nrng = n;
+    return (int)set_status(kOk);
+}
+
+// seeds: host array of opcuda_mc_nrng() uint32 seeds (smc.cu:137-156);
+// stream s is init_genrand(seed[s]).
+OPHIP_EXPORT void opcuda_mc_setseed(unsigned long host_seedptr, unsigned mtptr) {
+    Device &D = dev();
+    const int nrng = g_nrng;
+    uint32_t *mt = P<uint32_t>(mtptr);
+    if (!host_seedptr || !mt || !handle_range_ok(mtptr, (size_t)nrng * kMTW * 4)) {
+        set_status(kErrBadHandle);
+        return;
+    }
+    uint32_t *seeds = (uint32_t *)workspace((size_t)nrng * 4);
+    if (!seeds) return;
+    hipError_t e = hipMemcpyAsync(seeds, (const void *)(uintptr_t)host_seedptr, (size_t)nrng * 4,
+                                  hipMemcpyHostToDevice, D.stream);
+    if (e != hipSuccess) { set_status(hip_status(e)); return; }
+    hipLaunchKernelGGL(k_mt_seed, dim3(cdiv(nrng, 64)), dim3(64), 0, D.stream, nrng, seeds, mt);
+    if (check_launch()) return;
+    e = hipStreamSynchronize(D.stream);
+    set_status(hip_status(e));
+}
+
+// Sample nk steps of nscen chains on the reference's column-major table
+// layout ctpk[m*d*d + yprev + d*y].  nscen must be a multiple of
+// 64*opcuda_mc_nrng() (the reference requires a multiple of its 4096
+// lanes, CModel Device.cs:345).  d up to 32767 (int16 states; the
+// reference stops at d < 1024, smc.cu:622).  yhost = 0 leaves the states
+// on the device (no host round trip).
+OPHIP_EXPORT int opcuda_mc1f(unsigned mtptr, int y0, unsigned y_sk, int nscen, int nk, int d,
+                             unsigned ctpk, unsigned m_k, unsigned long yhost) {
+    return mc_run(false, mtptr, y0, y_sk, nscen, nk, d, ctpk, m_k, yhost);
+}
+
+// Same chains on a row-contiguous table crow[m*d*d + yprev*d + y]
+// (ophip_sck_rows): identical states given identical tables.
+OPHIP_EXPORT int ophip_mc1f_rows(unsigned mtptr, int y0, unsigned y_sk, int nscen, int nk, int d,
+                                 unsigned crow, unsigned m_k, unsigned long yhost) {
+    return mc_run(true, mtptr, y0, y_sk, nscen, nk, d, crow, m_k, yhost);
+}
+
+// nscen uniforms: stream s writes its next nscen/nrng outputs to
+// unif[s*(nscen/nrng) + j] (smc.cu:664-736 writes lane-interleaved;
+// here each stream's outputs are contiguous).  nscen % (64*nrng) == 0.
+OPHIP_EXPORT int opcuda_mt_benchmark(unsigned mtptr, unsigned unif, int nscen) {
+    Device &D = dev();
+    const int nrng = g_nrng;
+    if (nscen <= 0 || nscen % (64 * nrng)) return (int)set_status(kErrBadArg);
+    uint32_t *mt = P<uint32_t>(mtptr);
+    float *u = P<float>(unif);
+    if (!mt || !u || !handle_range_ok(unif, (size_t)nscen * 4) ||
+        !handle_range_ok(mtptr, (size_t)nrng * kMTW * 4))
+        return (int)set_status(kErrBadHandle);
+    hipLaunchKernelGGL(k_mt_bench, dim3(nrng), dim3(64), 0, D.stream, mt, nscen / nrng, u);
+    return (int)check_launch();
+}

@@ -1,0 +1,246 @@
+"""ctypes binding of libopmodel.so (include/opmodel.h) and the named
+workload configurations.
+
+``Solver`` wraps one ``opm_solver``: build the lattice, assemble the FP64
+generator and first step on the GPU, square, price.  The configuration
+builders below fix every choice the workload texts leave open; each one is
+stated where it is made.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+from . import ophip  # loads libophip first (libopmodel links it)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libopmodel.so")
+
+OPM_VOL_REGIME, OPM_VOL_C1, OPM_VOL_CEV = 0, 1, 2
+OPM_PRICE_BACKWARD, OPM_PRICE_PROPAGATE = 0, 1
+OPM_PREC_FP32 = 0
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("nr", C.c_int),
+        ("xlo", C.c_double), ("xhi", C.c_double), ("spot", C.c_double),
+        ("rate", C.c_double), ("q_off", C.c_double),
+        ("vol_kind", C.c_int),
+        ("sigma_r", C.c_double * 16),
+        ("sigma0", C.c_double), ("beta", C.c_double), ("skew", C.c_double), ("curv", C.c_double),
+        ("DeltaT", C.c_double),
+        ("nsq", C.c_int),
+        ("price_kind", C.c_int),
+        ("nsteps", C.c_int),
+        ("nstrikes", C.c_int),
+        ("strikes", C.c_double * 64),
+        ("precision", C.c_int),
+    ]
+
+    @property
+    def d(self):
+        return self.nx * self.nr
+
+    def strike_list(self):
+        return [self.strikes[i] for i in range(self.nstrikes)]
+
+    def sigma_list(self):
+        return [self.sigma_r[i] for i in range(self.nr)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libopmodel.so not found at {LIB_PATH}: build it first")
+    lib = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    lib.opm_grid.argtypes = [P(Config), P(C.c_double), P(C.c_int)]
+    lib.opm_grid.restype = C.c_int
+    lib.opm_params.argtypes = [P(Config), P(C.c_double), P(C.c_double)]
+    lib.opm_params.restype = C.c_int
+    lib.opm_create.argtypes = [P(Config)]
+    lib.opm_create.restype = C.c_void_p
+    lib.opm_destroy.argtypes = [C.c_void_p]
+    lib.opm_destroy.restype = None
+    for n in ("opm_solve", "opm_capture", "opm_launch"):
+        getattr(lib, n).argtypes = [C.c_void_p]
+        getattr(lib, n).restype = C.c_int
+    lib.opm_prices.argtypes = [C.c_void_p, P(C.c_double)]
+    lib.opm_prices.restype = C.c_int
+    lib.opm_kernel.argtypes = [C.c_void_p, P(C.c_float)]
+    lib.opm_kernel.restype = C.c_int
+    lib.opm_dims.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int)]
+    lib.opm_dims.restype = C.c_int
+    return lib
+
+
+lib = _load()
+
+
+# ---------------------------------------------------------------------------
+# workload configurations
+# ---------------------------------------------------------------------------
+
+def _base(nx, nr, rate):
+    c = Config()
+    c.nx, c.nr = nx, nr
+    # uniform on [0,400], rescaled so the node nearest 100 lands on 100
+    # (SGrid.cs:540-557): for nx = 256 that is node 64 and h = 1.5625
+    c.xlo, c.xhi, c.spot = 0.0, 400.0, 100.0
+    c.rate = rate
+    c.precision = OPM_PREC_FP32
+    return c
+
+
+def config_lv1d():
+    """Plain method, 1-D local vol, d = 256.
+
+    sigma(x) = clip(0.25 (x/100)^-0.5, 0.05, 1), r = 0.02; dt0 = 2^-20 yr,
+    16 squarings to U(1/16 yr), then the propagation chain U(k/16) for
+    k = 1..16 (15 products; U(1/16) itself is the first) and European calls
+    K = 80..120 step 2 at every k (T = 1 yr is k = 16)."""
+    c = _base(256, 1, 0.02)
+    c.vol_kind = OPM_VOL_C1
+    c.DeltaT = 1.0 / 16.0
+    c.nsq = 16
+    c.price_kind = OPM_PRICE_PROPAGATE
+    c.nsteps = 16
+    ks = [80.0 + 2.0 * i for i in range(21)]
+    c.nstrikes = len(ks)
+    for i, k in enumerate(ks):
+        c.strikes[i] = k
+    return c
+
+
+def config_rs2048(nsteps=252):
+    """Plain method at GPU scale: 256 price nodes x 8 regimes, d = 2048.
+
+    sigma_k = 0.1..0.8, regime rate 0.5/yr off-diagonal, r = 0.02 (not
+    stated by the workload; the 1-D workload's rate).  16 squarings of the
+    FP32 kernel reach the daily step: dt0 = (1/252) 2^-16 yr (<= 2^-20),
+    U(1/252).  Backward induction over 252 daily steps (T = 1 yr) for 41
+    strikes K = 80..120 step 1; prices at the spot node in every regime."""
+    c = _base(256, 8, 0.02)
+    c.vol_kind = OPM_VOL_REGIME
+    for k in range(8):
+        c.sigma_r[k] = 0.1 * (k + 1)
+    c.q_off = 0.5
+    c.DeltaT = 1.0 / 252.0
+    c.nsq = 16
+    c.price_kind = OPM_PRICE_BACKWARD
+    c.nsteps = nsteps
+    ks = [80.0 + i for i in range(41)]
+    c.nstrikes = len(ks)
+    for i, k in enumerate(ks):
+        c.strikes[i] = k
+    return c
+
+
+def config_rs_monthly():
+    """Precision study generator (the 2048-state lattice above): 16
+    squarings to U(1/12), dt0 = (1/12) 2^-16, then the propagation chain to
+    the 12 monthly maturities, 41 strikes."""
+    c = config_rs2048()
+    c.DeltaT = 1.0 / 12.0
+    c.price_kind = OPM_PRICE_PROPAGATE
+    c.nsteps = 12
+    return c
+
+
+def config_small(nx=64, nr=2, price_kind=OPM_PRICE_BACKWARD, nsteps=8, nsq=10):
+    """Small lattices for parity tests (same family as the 2048-state one)."""
+    c = _base(nx, nr, 0.03)
+    c.vol_kind = OPM_VOL_REGIME
+    for k in range(nr):
+        c.sigma_r[k] = 0.15 + 0.1 * k
+    c.q_off = 0.5 if nr > 1 else 0.0
+    c.DeltaT = 1.0 / 52.0
+    c.nsq = nsq
+    c.price_kind = price_kind
+    c.nsteps = nsteps
+    ks = [90.0 + 2.5 * i for i in range(9)]
+    c.nstrikes = len(ks)
+    for i, k in enumerate(ks):
+        c.strikes[i] = k
+    return c
+
+
+# ---------------------------------------------------------------------------
+# solver
+# ---------------------------------------------------------------------------
+
+class Solver:
+    def __init__(self, cfg: Config):
+        ophip.init()
+        self.cfg = cfg
+        self.p = lib.opm_create(C.byref(cfg))
+        if not self.p:
+            raise ophip.OphipError(f"opm_create failed: 0x{ophip.lib.opcuda_get_status():x}")
+        d, nm = C.c_int(), C.c_int()
+        lib.opm_dims(self.p, C.byref(d), C.byref(nm))
+        self.d, self.n_mat = d.value, nm.value
+
+    def solve(self):
+        ophip.check(lib.opm_solve(self.p), "opm_solve")
+
+    def capture(self):
+        ophip.check(lib.opm_capture(self.p), "opm_capture")
+
+    def launch(self):
+        ophip.check(lib.opm_launch(self.p), "opm_launch")
+
+    def prices(self) -> np.ndarray:
+        c = self.cfg
+        out = np.zeros(c.nr * self.n_mat * c.nstrikes)
+        ophip.check(lib.opm_prices(self.p, out.ctypes.data_as(C.POINTER(C.c_double))), "prices")
+        return out.reshape(c.nr, self.n_mat, c.nstrikes)
+
+    def kernel(self) -> np.ndarray:
+        U = np.zeros(self.d * self.d, dtype=np.float32)
+        ophip.check(lib.opm_kernel(self.p, U.ctypes.data_as(C.POINTER(C.c_float))), "kernel")
+        return U.reshape(self.d, self.d, order="F")
+
+    def close(self):
+        if self.p:
+            lib.opm_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def grid(cfg: Config):
+    x = np.zeros(cfg.nx)
+    x0 = C.c_int()
+    lib.opm_grid(C.byref(cfg), x.ctypes.data_as(C.POINTER(C.c_double)), C.byref(x0))
+    return x, x0.value
+
+
+def params(cfg: Config):
+    n = cfg.nx * cfg.nr
+    dr, vo = np.zeros(n), np.zeros(n)
+    lib.opm_params(C.byref(cfg), dr.ctypes.data_as(C.POINTER(C.c_double)),
+                   vo.ctypes.data_as(C.POINTER(C.c_double)))
+    return dr, vo
+
+
+def flops(cfg: Config) -> float:
+    """Algorithmic FLOPs of one solve: 2d^3 per squaring/propagation GEMM,
+    2*d^2*nstrikes per backward step (unpadded strikes)."""
+    d = cfg.d
+    f = cfg.nsq * 2.0 * d ** 3
+    if cfg.price_kind == OPM_PRICE_BACKWARD:
+        f += cfg.nsteps * 2.0 * d * d * cfg.nstrikes
+    else:
+        f += (cfg.nsteps - 1) * 2.0 * d ** 3 + cfg.nsteps * cfg.nr * 2.0 * d * cfg.nstrikes
+    return f
+
+
+def df(cfg: Config, t: float) -> float:
+    return math.exp(-cfg.rate * t)

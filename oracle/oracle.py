@@ -1,0 +1,264 @@
+"""ctypes binding of the CPU oracle (liboracle.so) plus the FP64 scenario
+solve composed from it.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker or the timed CPU path,
+never by the product (oplib_amd/).
+
+The scenario solve restates the reference's CPU (FP64) path: grid and
+stencil (SGrid.cs:492-593, S1DGrid.cs:78-147), generator
+(CLVModel.cs:86-136), first step (CModel Host Double.cs:89-121), repeated
+squaring with DGEMM (CModel Host Double.cs:132-172, linalg.c:93-101), the
+propagation chain (CModel Host Double.cs:200-305) and pricing by payoff
+GEMV with discounting (OPPricer.cs:143-146) or backward induction.
+Configuration objects are read by attribute (nx, nr, xlo, ...), the same
+fields as include/opmodel.h's opm_config.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_sp = np.ctypeslib.ndpointer(dtype=np.int16, flags="C_CONTIGUOUS")
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} missing: make -C oracle")
+    L = C.CDLL(LIB_PATH)
+    I, D, F, U = C.c_int, C.c_double, C.c_float, C.c_uint32
+    sig = {
+        "or_xgrid_uniform": (None, [I, D, D, D, _dp, C.POINTER(I)]),
+        "or_stencil_invm": (None, [I, _dp, _dp]),
+        "or_lv_generator": (None, [I, _dp, _dp, _dp, _dp]),
+        "or_rs_generator": (None, [I, I, _dp, _dp, _dp, D, D, _dp]),
+        "or_lv_params_c1": (None, [I, _dp, D, _dp, _dp]),
+        "or_lv_params_cev": (None, [I, _dp, D, D, D, D, D, _dp, _dp]),
+        "or_plan_niter": (I, [D, D, D, C.POINTER(D)]),
+        "or_max_abs_diag": (D, [I, _dp]),
+        "or_first_step": (None, [I, _dp, D, _dp]),
+        "or_sek_f32": (None, [I, _fp, F, _fp]),
+        "or_dgemm": (None, [I, I, I, _dp, I, _dp, I, _dp, I]),
+        "or_sgemm_f64acc": (None, [I, I, I, F, _fp, I, _fp, I, F, _fp, I]),
+        "or_square_chain": (None, [I, I, _dp, _dp]),
+        "or_row_renorm": (None, [I, _dp]),
+        "or_set_threads": (I, [I]),
+        "or_sgemv_gather": (None, [I, _fp, _fp, I, _ip, _ip, I]),
+        "or_backward_induction": (None, [I, _dp, I, D, I, _dp]),
+        "or_sck_f32": (None, [I, I, _fp, _fp]),
+        "or_sck_rows_f32": (None, [I, I, _fp, _fp]),
+        "or_mt19937_init": (None, [U, _up]),
+        "or_mt19937_next": (U, [_up]),
+        "or_u01_from_bits": (F, [U]),
+        "or_mt_uniforms": (None, [U, I, _fp]),
+        "or_sample_linear": (I, [_fp, I, F]),
+        "or_mc_chain": (None, [I, U, I, I, I, I, _fp, _ip, I, _sp]),
+        "or_bs_call": (D, [D, D, D, D, D]),
+        "or_ncdf": (D, [D]),
+        "or_gelu": (D, [D]),
+        "or_gelu_grad": (D, [D]),
+        "or_adam_step": (None, [I, _dp, _dp, _dp, _dp, D, D, D, D, I]),
+    }
+    for n, (r, a) in sig.items():
+        f = getattr(L, n)
+        f.restype = r
+        f.argtypes = a
+    return L
+
+
+lib = _load()
+
+
+def set_threads(n: int) -> int:
+    return lib.or_set_threads(int(n))
+
+
+# ---------------------------------------------------------------------------
+# building blocks
+# ---------------------------------------------------------------------------
+
+def xgrid(nx, xlo, xhi, spot):
+    x = np.zeros(nx)
+    x0 = C.c_int()
+    lib.or_xgrid_uniform(nx, xlo, xhi, spot, x, C.byref(x0))
+    return x, x0.value
+
+
+def invm(x):
+    out = np.zeros(4 * len(x))
+    lib.or_stencil_invm(len(x), np.ascontiguousarray(x), out)
+    return out
+
+
+def lv_generator(x, drift, vol):
+    d = len(x)
+    g = np.zeros(d * d)
+    lib.or_lv_generator(d, invm(x), np.ascontiguousarray(drift, dtype=np.float64),
+                        np.ascontiguousarray(vol, dtype=np.float64), g)
+    return g.reshape(d, d, order="F")
+
+
+def rs_generator(x, sigma_r, rate, q_off):
+    nx, nr = len(x), len(sigma_r)
+    d = nx * nr
+    g = np.zeros(d * d)
+    lib.or_rs_generator(nx, nr, invm(x), np.ascontiguousarray(x),
+                        np.ascontiguousarray(sigma_r, dtype=np.float64), rate, q_off, g)
+    return g.reshape(d, d, order="F")
+
+
+def lv_params_c1(x, rate):
+    n = len(x)
+    dr, vo = np.zeros(n), np.zeros(n)
+    lib.or_lv_params_c1(n, np.ascontiguousarray(x), rate, dr, vo)
+    return dr, vo
+
+
+def lv_params_cev(x, rate, sigma0, beta, skew=0.0, curv=0.0):
+    n = len(x)
+    dr, vo = np.zeros(n), np.zeros(n)
+    lib.or_lv_params_cev(n, np.ascontiguousarray(x), rate, sigma0, beta, skew, curv, dr, vo)
+    return dr, vo
+
+
+def first_step(gen, dt):
+    d = gen.shape[0]
+    U = np.zeros(d * d)
+    lib.or_first_step(d, np.ascontiguousarray(gen.reshape(-1, order="F")), dt, U)
+    return U.reshape(d, d, order="F")
+
+
+def dgemm(A, B):
+    m, k = A.shape
+    k2, n = B.shape
+    assert k == k2
+    Cm = np.zeros(m * n)
+    lib.or_dgemm(m, n, k, np.ascontiguousarray(A.reshape(-1, order="F")), m,
+                 np.ascontiguousarray(B.reshape(-1, order="F")), k, Cm, m)
+    return Cm.reshape(m, n, order="F")
+
+
+def square_chain(U, nsq):
+    d = U.shape[0]
+    u = np.ascontiguousarray(U.reshape(-1, order="F")).copy()
+    w = np.zeros_like(u)
+    lib.or_square_chain(d, nsq, u, w)
+    return u.reshape(d, d, order="F")
+
+
+def sck_f32(tpk):
+    """tpk: (nm, d, d) float32 with [m][y0, y] semantics, column-major per
+    matrix; returns the same layout."""
+    nm, d, _ = tpk.shape
+    src = np.ascontiguousarray(np.stack([t.reshape(-1, order="F") for t in tpk]).reshape(-1))
+    out = np.zeros_like(src)
+    lib.or_sck_f32(d, nm, src, out)
+    return out
+
+
+def mt_uniforms(seed, n):
+    out = np.zeros(n, dtype=np.float32)
+    lib.or_mt_uniforms(seed, n, out)
+    return out
+
+
+def mt_raw(seed, n):
+    st = np.zeros(625, dtype=np.uint32)
+    lib.or_mt19937_init(seed, st)
+    return np.array([lib.or_mt19937_next(st) for _ in range(n)], dtype=np.uint32)
+
+
+def mc_chain(nstreams, seed0, groups, nk, d, y0, ctpk_flat, m_k, layout_rows=0):
+    nscen = nstreams * groups * 64
+    out = np.zeros(nscen * nk, dtype=np.int16)
+    lib.or_mc_chain(nstreams, seed0, groups, nk, d, y0, np.ascontiguousarray(ctpk_flat,
+                    dtype=np.float32), np.ascontiguousarray(m_k, dtype=np.int32),
+                    layout_rows, out)
+    return out.reshape(nk, nscen)
+
+
+def bs_call(S, K, r, sigma, T):
+    return lib.or_bs_call(S, K, r, sigma, T)
+
+
+# ---------------------------------------------------------------------------
+# scenario solve in FP64
+# ---------------------------------------------------------------------------
+
+def model_generator(cfg):
+    """Grid, FP64 generator and the spot node of a configuration."""
+    nx, nr = cfg.nx, cfg.nr
+    x, x0 = xgrid(nx, cfg.xlo, cfg.xhi, cfg.spot)
+    if cfg.vol_kind == 0:
+        sig = [cfg.sigma_r[i] for i in range(nr)]
+        g = rs_generator(x, sig, cfg.rate, cfg.q_off)
+    elif cfg.vol_kind == 1:
+        assert nr == 1
+        dr, vo = lv_params_c1(x, cfg.rate)
+        g = lv_generator(x, dr, vo)
+    else:
+        assert nr == 1
+        dr, vo = lv_params_cev(x, cfg.rate, cfg.sigma0, cfg.beta, cfg.skew, cfg.curv)
+        g = lv_generator(x, dr, vo)
+    return x, x0, g
+
+
+def kernel_fp64(cfg):
+    """U(DeltaT) = (I + L dt0)^(2^nsq), dt0 = DeltaT 2^-nsq, in FP64."""
+    x, x0, g = model_generator(cfg)
+    dt0 = math.ldexp(cfg.DeltaT, -cfg.nsq)
+    U = first_step(g, dt0)
+    return x, x0, square_chain(U, cfg.nsq)
+
+
+def payoff(cfg, x):
+    ks = np.array([cfg.strikes[i] for i in range(cfg.nstrikes)])
+    xs = np.tile(x, cfg.nr)  # state y -> price x[y % nx]
+    return np.maximum(xs[:, None] - ks[None, :], 0.0)  # d x nstrikes
+
+
+def solve_fp64(cfg, U=None):
+    """Prices [nr][n_mat][nstrikes] of the FP64 method (same discretisation
+    as the GPU method: same grid, dt0, squaring count, steps)."""
+    if U is None:
+        x, x0, U = kernel_fp64(cfg)
+    else:
+        x, x0 = xgrid(cfg.nx, cfg.xlo, cfg.xhi, cfg.spot)
+    d, nx, nr = U.shape[0], cfg.nx, cfg.nr
+    P = payoff(cfg, x)
+    rows = [x0 + nx * r for r in range(nr)]
+    if cfg.price_kind == 0:  # backward induction
+        V = np.ascontiguousarray(P.reshape(-1, order="F")).copy()
+        dfs = math.exp(-cfg.rate * cfg.DeltaT)
+        lib.or_backward_induction(d, np.ascontiguousarray(U.reshape(-1, order="F")),
+                                  cfg.nsteps, dfs, cfg.nstrikes, V)
+        V = V.reshape(d, cfg.nstrikes, order="F")
+        return np.stack([V[y][None, :] for y in rows])
+    out = np.zeros((nr, cfg.nsteps, cfg.nstrikes))
+    Uk = U
+    for k in range(1, cfg.nsteps + 1):
+        if k > 1:
+            Uk = dgemm(Uk, U)
+        dfk = math.exp(-cfg.rate * cfg.DeltaT * k)
+        for r, y in enumerate(rows):
+            out[r, k - 1] = dfk * (Uk[y, :] @ P)
+    return out
+
+
+def rel_err(a, b, floor=None):
+    """|a-b| / max(|b|, floor); floor defaults to 1e-3 * max|b|."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if floor is None:
+        floor = 1e-3 * float(np.max(np.abs(b)))
+    return np.abs(a - b) / np.maximum(np.abs(b), floor)

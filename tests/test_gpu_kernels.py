@@ -1,0 +1,429 @@
+"""GPU parity tests of every drop-in entry point, called through the C-ABI
+(libophip.so via ctypes) and checked against the CPU oracle.
+
+Bars: bit-exact for integer work (MT19937 streams, sampled states) and for
+kernels whose FP32 operation order is the reference's (sek, sck); for FP32
+contractions the reference's own self-check tolerance |a-b|/(1+|a|)
+(2e-3 GEMM, gsgemm4/Program.cs:283-291; 1e-3 GEMV, gsgemv4/Program.cs:
+384-391) and a tighter scale-relative bound stated per test.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def H(hip):
+    return hip
+
+
+def dev(H, a, order="F"):
+    return H.DeviceArray.from_numpy(a, order=order)
+
+
+def ref_tol(a, b):
+    """the reference's self-check metric |a-b|/(1+|a|)"""
+    return float(np.max(np.abs(a - b) / (1.0 + np.abs(a))))
+
+
+# ---- memory ----------------------------------------------------------------
+
+def test_alloc_copy_roundtrip_and_handles(H):
+    banner("alloc/copy")
+    a = np.arange(1000, dtype=np.float32)
+    d = dev(H, a)
+    assert d.h != 0 and d.h % 256 == 0
+    assert np.array_equal(d.download(), a)
+    # handle arithmetic, as the C# layer does (ptr + offset bytes)
+    out = np.zeros(10, dtype=np.float32)
+    H.check(H.lib.opcuda_memcpy_d2h(d.h + 4 * 100, out.ctypes.data, 40))
+    assert np.array_equal(out, a[100:110])
+    e = H.DeviceArray.empty(1000)
+    H.check(H.lib.opcuda_memcpy_d2d(e.h, d.h, 4000))
+    assert np.array_equal(e.download(), a)
+    # copying past the allocation is refused, not performed
+    st = H.lib.opcuda_memcpy_d2h(d.h, out.ctypes.data, 8000)
+    assert st != 0
+    d.free()
+    e.free()
+
+
+def test_fill_and_copies_do_not_overrun(H):
+    banner("sset/scopy/scopy1")
+    n = 1000 + 37
+    guard = 256
+    buf = dev(H, np.full(n + guard, -7.0, dtype=np.float32))
+    H.lib.opcuda_ssetall(buf.h, n, np.float32(2.5), 1)
+    H.check()
+    H.lib.opcuda_ssetone(buf.h, n, np.float32(9.0), 5)
+    H.check()
+    got = buf.download()
+    want = np.full(n + guard, -7.0, dtype=np.float32)
+    want[:n] = 2.5
+    want[5] = 9.0
+    assert np.array_equal(got, want)
+    src = dev(H, np.arange(n, dtype=np.float32))
+    dst = dev(H, np.full(n + guard, -1.0, dtype=np.float32))
+    H.check(H.lib.opcuda_scopy1(dst.h, src.h, n))
+    got = dst.download()
+    assert np.array_equal(got[:n], np.arange(n, dtype=np.float32))
+    assert np.all(got[n:] == -1.0)  # global_scopy1 overran here (scopy2.cu:91)
+    H.lib.opcuda_scopy(100, src.h, 3, dst.h, 2)
+    H.check()
+    got = dst.download()
+    assert np.array_equal(got[0:200:2], np.arange(0, 300, 3, dtype=np.float32))
+
+
+# ---- first step / generator -------------------------------------------------
+
+def test_sek_bit_exact(H):
+    banner("sek")
+    rng = np.random.default_rng(10)
+    d = 100
+    g = rng.standard_normal((d, d)).astype(np.float32)
+    gd = dev(H, g)
+    out = H.DeviceArray.empty((d, d))
+    H.lib.opcuda_sek(d, gd.h, np.float32(1e-3), out.h)
+    H.check()
+    ref = np.zeros(d * d, dtype=np.float32)
+    O.lib.or_sek_f32(d, np.ascontiguousarray(g.reshape(-1, order="F")), np.float32(1e-3), ref)
+    assert np.array_equal(out.download(), ref.reshape(d, d, order="F"))
+    r = rng.random(d).astype(np.float32) * 0.05
+    rd = dev(H, r)
+    H.lib.opcuda_sekd(d, gd.h, np.float32(1e-3), out.h, rd.h)
+    H.check()
+    want = (np.eye(d) + 1e-3 * g.astype(np.float64)) / (1 + 1e-3 * r[:, None].astype(np.float64))
+    assert np.max(np.abs(out.download() - want)) < 1e-6
+    lam = dev(H, np.array([0.5], dtype=np.float32))
+    H.lib.opcuda_sekd_lambda(d, gd.h, np.float32(1e-3), out.h, rd.h, lam.h)
+    H.check()
+    want = (np.eye(d) + 1e-3 * g.astype(np.float64)) / (1 + 0.5e-3 * r[:, None].astype(np.float64))
+    assert np.max(np.abs(out.download() - want)) < 1e-6
+
+
+def test_sglvg_vs_fp64(H):
+    banner("sglvg")
+    d, ni = 128, 2
+    x, _ = O.xgrid(d, 0.0, 400.0, 100.0)
+    inv = O.invm(x)
+    dr, vo = O.lv_params_c1(x, 0.02)
+    pars = np.concatenate([dr, 1.3 * dr, vo, 0.9 * vo]).astype(np.float32)
+    gen = H.DeviceArray.empty(ni * d * d)
+    H.lib.opcuda_sglvg(gen.h, ni, d, dev(H, inv.astype(np.float32)).h, dev(H, pars).h)
+    H.check()
+    got = gen.download().reshape(ni, d * d)
+    for i, (a, b) in enumerate(((dr, vo), (1.3 * dr, 0.9 * vo))):
+        ref = O.lv_generator(x, a, b)
+        scale = np.abs(ref).max()
+        assert np.abs(got[i].reshape(d, d, order="F") - ref).max() / scale < 1e-6
+
+
+def test_fp64_assembly_and_first_step(H):
+    banner("dgen_rs / sek_rs")
+    nx, nr = 48, 3
+    x, _ = O.xgrid(nx, 0.0, 400.0, 100.0)
+    sig = [0.2, 0.35, 0.5]
+    ref = O.rs_generator(x, sig, 0.02, 0.5)
+    d = nx * nr
+    drift = np.tile(0.02 * x, nr)
+    vol = np.concatenate([s * x for s in sig])
+    xd, dd, vd = dev(H, x), dev(H, drift), dev(H, vol)
+    g = H.DeviceArray.empty((d, d), dtype=np.float64)
+    H.check(H.lib.ophip_dgen_rs(nx, nr, xd.h, dd.h, vd.h, 0.5, g.h))
+    G = g.download()
+    assert np.allclose(G, ref, rtol=1e-13, atol=1e-10)
+    assert np.abs(G.sum(axis=1)).max() < 1e-9
+    dt = 2.0**-20
+    U = H.DeviceArray.empty((d, d), dtype=np.float64)
+    H.check(H.lib.ophip_sek_rs(nx, nr, xd.h, dd.h, vd.h, 0.5, dt, 1, U.h))
+    assert np.allclose(U.download(), O.first_step(ref, dt), rtol=0, atol=1e-15)
+    U32 = H.DeviceArray.empty((d, d), dtype=np.float32)
+    H.check(H.lib.ophip_sek_rs(nx, nr, xd.h, dd.h, vd.h, 0.5, dt, 0, U32.h))
+    # one rounding of the FP64 value
+    want = O.first_step(ref, dt).astype(np.float32)
+    got = U32.download()
+    assert np.max(np.abs(got.astype(np.float64) - want.astype(np.float64))) <= 6e-8
+
+
+# ---- GEMM ---------------------------------------------------------------------
+
+@pytest.mark.parametrize("m,n,k,lda,ldb,ldc,alpha,beta", [
+    (256, 256, 256, 256, 256, 256, 1.0, 0.0),      # fast path, 64-tiles
+    (2048, 2048, 2048, 2048, 2048, 2048, 1.0, 0.0),  # fast path, 128-tiles
+    (577, 301, 129, 600, 130, 580, 0.5, -1.25),    # generic path, odd shapes
+    (64, 1, 64, 64, 64, 64, 1.0, 0.0),
+    (1, 33, 7, 1, 7, 1, 2.0, 1.0),
+])
+def test_sgemm(H, m, n, k, lda, ldb, ldc, alpha, beta):
+    banner(f"sgemm {m}x{n}x{k}")
+    rng = np.random.default_rng(m + n + k)
+    A = rng.standard_normal((lda, k)).astype(np.float32)
+    B = rng.standard_normal((ldb, n)).astype(np.float32)
+    Cm = rng.standard_normal((ldc, n)).astype(np.float32)
+    Ad, Bd, Cd = dev(H, A), dev(H, B), dev(H, Cm)
+    H.lib.opcuda_sgemm(m, n, k, np.float32(alpha), Ad.h, lda, Bd.h, ldb, np.float32(beta), Cd.h, ldc)
+    H.check()
+    got = Cd.download()
+    want = alpha * (A[:m, :k].astype(np.float64) @ B[:k, :n].astype(np.float64))
+    if beta != 0.0:
+        want += beta * Cm[:m, :n].astype(np.float64)
+    assert ref_tol(want, got[:m, :n]) < 2e-3
+    # FP32 accumulation error bound ~ sqrt(k) ulp of the row norms
+    scale = np.sqrt(k) * np.abs(A[:m]).max() * np.abs(B[:k]).max() * abs(alpha) + 1
+    assert np.max(np.abs(want - got[:m, :n])) < 1e-5 * scale
+    # padding rows of C are untouched
+    if ldc > m:
+        assert np.array_equal(got[m:, :n], Cm[m:, :n])
+
+
+def test_ssqmm_pointer_tables(H):
+    banner("ssqmm")
+    rng = np.random.default_rng(5)
+    d, ni = 256, 3
+    mats = rng.random((ni, d, d)).astype(np.float32) / d
+    buf = H.DeviceArray.empty(3 * ni * d * d)
+    size = 4 * d * d
+    Ah = [buf.h + size * i for i in range(ni)]
+    Bh = [buf.h + size * (ni + (ni - 1 - i)) for i in range(ni)]
+    Ch = [buf.h + size * (2 * ni + i) for i in range(ni)]
+    for i in range(ni):
+        buf.upload(np.ascontiguousarray(mats[i].reshape(-1, order="F")), offset=size * i)
+        buf.upload(np.ascontiguousarray(mats[i].T.reshape(-1, order="F")), offset=size * (ni + ni - 1 - i))
+    tabs = [dev(H, np.array(t, dtype=np.uint32)) for t in (Ah, Bh, Ch)]
+    H.lib.opcuda_ssqmm(d, ni, tabs[0].h, tabs[1].h, tabs[2].h)
+    H.check()
+    out = buf.download(count=3 * ni * d * d).reshape(3 * ni, d * d)
+    for i in range(ni):
+        got = out[2 * ni + i].reshape(d, d, order="F")
+        want = mats[i].astype(np.float64) @ mats[i].T.astype(np.float64)
+        assert ref_tol(want, got) < 2e-3
+        assert np.abs(want - got).max() < 1e-6
+
+
+def test_sgemm4_descriptors(H):
+    banner("sgemm4")
+    rng = np.random.default_rng(6)
+    shapes = [(128, 48, 64), (70, 20, 33), (192, 64, 128)]
+    mats, descs = [], []
+    for (m, n, k) in shapes:
+        A = rng.standard_normal((m, k)).astype(np.float32)
+        B = rng.standard_normal((k, n)).astype(np.float32)
+        mats.append((dev(H, A), dev(H, B), H.DeviceArray.empty((m, n)), A, B))
+    # descriptor block {i,bx,by,m,n,k,A,B,C} per 64x16 tile (gsgemm4/Program.cs:186-205)
+    words = []
+    for i, ((m, n, k), (Ad, Bd, Cd, _, _)) in enumerate(zip(shapes, mats)):
+        for bx in range((m + 63) // 64):
+            for by in range((n + 15) // 16):
+                words.append([i, bx, by, m, n, k, Ad.h, Bd.h, Cd.h])
+    nb = len(words)
+    arg = dev(H, np.array(words, dtype=np.uint32).reshape(-1))
+    ptrs = dev(H, np.array([arg.h + 36 * b for b in range(nb)], dtype=np.uint32))
+    H.lib.opcuda_sgemm4(nb, ptrs.h)
+    H.check()
+    for (Ad, Bd, Cd, A, B) in mats:
+        want = A.astype(np.float64) @ B.astype(np.float64)
+        got = Cd.download()
+        assert ref_tol(want, got) < 2e-3
+        assert np.abs(want - got).max() < 1e-4
+
+
+def test_strided_batched(H):
+    banner("strided batched")
+    rng = np.random.default_rng(7)
+    b, m, n, k = 4, 128, 256, 64
+    A = rng.standard_normal((b, k, m)).astype(np.float32)  # each A_i stored F-order (m x k)
+    B = rng.standard_normal((b, n, k)).astype(np.float32)
+    Ad = dev(H, A.reshape(-1), order="C")
+    Bd = dev(H, B.reshape(-1), order="C")
+    Cd = H.DeviceArray.empty(b * m * n)
+    H.check(H.lib.ophip_sgemm_strided_batched(m, n, k, np.float32(1), Ad.h, m, m * k, Bd.h, k,
+                                               k * n, np.float32(0), Cd.h, m, m * n, b))
+    got = Cd.download().reshape(b, n, m)
+    for i in range(b):
+        want = A[i].T.astype(np.float64) @ B[i].T.astype(np.float64)
+        assert np.abs(got[i].T - want).max() < 1e-4
+
+
+# ---- GEMV -----------------------------------------------------------------------
+
+@pytest.mark.parametrize("m,n,lda,incx,incy,alpha,beta", [
+    (2048, 2048, 2048, 1, 1, 1.0, 0.0),
+    (300, 77, 301, 2, 3, 0.5, 2.0),
+    (21, 256, 21, 256, 1, 0.9, 0.0),
+])
+def test_sgemv_n_and_t(H, m, n, lda, incx, incy, alpha, beta):
+    banner(f"sgemv {m}x{n}")
+    rng = np.random.default_rng(m * 7 + n)
+    A = rng.standard_normal((lda, n)).astype(np.float32)
+    for trans in (False, True):
+        xl = (n if not trans else m)
+        yl = (m if not trans else n)
+        x = rng.standard_normal(xl * abs(incx)).astype(np.float32)
+        y = rng.standard_normal(yl * abs(incy)).astype(np.float32)
+        Ad, xd, yd = dev(H, A), dev(H, x), dev(H, y)
+        fn = H.lib.opcuda_sgemvT if trans else H.lib.opcuda_sgemv
+        fn(m, n, np.float32(alpha), Ad.h, lda, xd.h, incx, np.float32(beta), yd.h, incy)
+        H.check()
+        Am = A[:m].astype(np.float64)
+        xv = x[::incx][:xl].astype(np.float64)
+        want = alpha * ((Am.T if trans else Am) @ xv) + beta * y[::incy][:yl]
+        got = yd.download()[::incy][:yl]
+        assert ref_tol(want, got) < 1e-3
+        assert np.abs(want - got).max() < 2e-5 * (np.sqrt(max(m, n)) * 4 + 1)
+
+
+def test_sgemv3_gather_and_sentinel(H):
+    banner("sgemv3")
+    rng = np.random.default_rng(8)
+    d, nz = 192, 8
+    A = (rng.random((d, d)) / d).astype(np.float32)
+    Z = rng.random((d, 2 * nz)).astype(np.float32)  # columns 0..nz-1 "from", nz.. "to"
+    sentinel = 2 * nz * d
+    ncol = 32
+    col0 = np.zeros(ncol, dtype=np.int32)
+    col1 = np.full(ncol, sentinel, dtype=np.int32)
+    # group 0: 5 live columns then a sentinel; group 1: 16 live columns
+    live0 = [(3, 8), (0, 9), (7, 10), (3, 11), (1, 12)]
+    for c, (a, b) in enumerate(live0):
+        col0[c], col1[c] = a * d, b * d
+    col1[5] = sentinel
+    col0[6], col1[6] = 2 * d, 15 * d  # after a sentinel: must not be written
+    col0[16], col1[16] = 4 * d, 13 * d
+    col0[17], col1[17] = 5 * d, 14 * d
+    col1[18:] = sentinel + 5
+    Zd = dev(H, Z)
+    H.lib.opcuda_sgemv3(d, nz, ncol, dev(H, A).h, Zd.h, dev(H, col0).h, dev(H, col1).h)
+    H.check()
+    got = Zd.download()
+    want = Z.copy().reshape(-1, order="F")
+    O.lib.or_sgemv_gather(d, np.ascontiguousarray(A.reshape(-1, order="F")), want, 16,
+                          col0[:16].copy(), col1[:16].copy(), sentinel)
+    O.lib.or_sgemv_gather(d, np.ascontiguousarray(A.reshape(-1, order="F")), want, 16,
+                          col0[16:].copy(), col1[16:].copy(), sentinel)
+    want = want.reshape(d, 2 * nz, order="F")
+    assert np.abs(got - want).max() < 1e-5
+    assert np.array_equal(got[:, 15], Z[:, 15])  # the column after the sentinel
+
+
+def test_skinny_backward_step(H):
+    banner("skinny")
+    rng = np.random.default_rng(9)
+    for m, ncp in ((2048, 48), (256, 32), (384, 64)):
+        A = (rng.random((m, m)) / m).astype(np.float32)
+        X = rng.random((m, ncp)).astype(np.float32)
+        Y = H.DeviceArray.empty(m * ncp)
+        H.check(H.lib.ophip_sgemm_skinny(m, m, ncp, np.float32(0.99), dev(H, A).h, m,
+                                          dev(H, X, order="C").h, ncp, Y.h, ncp))
+        got = Y.download().reshape(m, ncp)
+        want = 0.99 * A.astype(np.float64) @ X.astype(np.float64)
+        assert np.abs(got - want).max() < 2e-6, (m, ncp)
+
+
+# ---- tables and Monte Carlo ---------------------------------------------------------
+
+def stochastic(rng, nm, d, power=6):
+    P = rng.random((nm, d, d)) ** power
+    P /= P.sum(axis=2, keepdims=True)
+    return P.astype(np.float32)
+
+
+def test_sck_bit_exact_both_layouts(H):
+    banner("sck")
+    rng = np.random.default_rng(11)
+    nm, d = 2, 200
+    P = stochastic(rng, nm, d)
+    P[0, 3, 50] = -1e-9  # a rounding-level negative counts as zero
+    flat = np.ascontiguousarray(np.stack([p.reshape(-1, order="F") for p in P]).reshape(-1))
+    t = dev(H, flat)
+    c = H.DeviceArray.empty(nm * d * d)
+    H.lib.opcuda_sck(d, nm, t.h, c.h)
+    H.check()
+    want = np.zeros_like(flat)
+    O.lib.or_sck_f32(d, nm, flat, want)
+    assert np.array_equal(c.download(), want)
+    cr = H.DeviceArray.empty(nm * d * d)
+    H.check(H.lib.ophip_sck_rows(d, nm, t.h, cr.h))
+    want_r = np.zeros_like(flat)
+    O.lib.or_sck_rows_f32(d, nm, flat, want_r)
+    assert np.array_equal(cr.download(), want_r)
+
+
+def test_mt19937_streams_bit_exact(H):
+    banner("mt19937")
+    nrng = 16
+    H.check(H.lib.ophip_mc_set_nrng(nrng))
+    try:
+        assert H.lib.opcuda_mc_nrng() == nrng
+        sz = H.lib.opcuda_mc_status_sz()
+        assert sz == 625 * nrng
+        mt = H.DeviceArray.empty(sz, dtype=np.uint32)
+        seeds = (5489 + np.arange(nrng)).astype(np.uint32)
+        H.lib.opcuda_mc_setseed(seeds.ctypes.data, mt.h)
+        H.check()
+        per = 64 * 30  # crosses several twists (624 words each)
+        u = H.DeviceArray.empty(nrng * per)
+        for rep in range(2):  # the state carries over between launches
+            H.check(H.lib.opcuda_mt_benchmark(mt.h, u.h, nrng * per))
+            got = u.download().reshape(nrng, per)
+            for s in (0, 1, 7, 15):
+                want = O.mt_uniforms(5489 + s, per * (rep + 1))[per * rep:]
+                assert np.array_equal(got[s], want), (s, rep)
+    finally:
+        H.lib.ophip_mc_set_nrng(4096)
+
+
+@pytest.mark.parametrize("d,nrng,groups,nk", [(50, 4, 2, 7), (256, 8, 1, 16), (700, 2, 3, 5)])
+def test_mc_chain_bit_exact(H, d, nrng, groups, nk):
+    banner(f"mc1f d={d}")
+    rng = np.random.default_rng(d)
+    nm = 3
+    P = stochastic(rng, nm, d, power=10)
+    flat = np.ascontiguousarray(np.stack([p.reshape(-1, order="F") for p in P]).reshape(-1))
+    t = dev(H, flat)
+    ct = H.DeviceArray.empty(nm * d * d)
+    H.lib.opcuda_sck(d, nm, t.h, ct.h)
+    H.check()
+    ctab = ct.download()
+    cr = H.DeviceArray.empty(nm * d * d)
+    H.check(H.lib.ophip_sck_rows(d, nm, t.h, cr.h))
+    m_k = (np.arange(nk) % nm).astype(np.int32)
+    mkd = dev(H, m_k)
+    nscen = 64 * nrng * groups
+    H.check(H.lib.ophip_mc_set_nrng(nrng))
+    try:
+        y0 = d // 3
+        want = O.mc_chain(nrng, 777, groups, nk, d, y0, ctab, m_k, 0)
+        for rows in (False, True):
+            mt = H.DeviceArray.empty(625 * nrng, dtype=np.uint32)
+            seeds = (777 + np.arange(nrng)).astype(np.uint32)
+            H.lib.opcuda_mc_setseed(seeds.ctypes.data, mt.h)
+            H.check()
+            ysk = H.DeviceArray.empty(nscen * nk, dtype=np.int16)
+            host = np.zeros(nscen * nk, dtype=np.int16)
+            fn = H.lib.ophip_mc1f_rows if rows else H.lib.opcuda_mc1f
+            H.check(fn(mt.h, y0, ysk.h, nscen, nk, d, (cr if rows else ct).h, mkd.h,
+                       host.ctypes.data))
+            assert np.array_equal(host.reshape(nk, nscen), want), rows
+    finally:
+        H.lib.ophip_mc_set_nrng(4096)
+
+
+def test_mc1f_rejects_bad_shapes(H):
+    banner("mc1f args")
+    H.check(H.lib.ophip_mc_set_nrng(4))
+    try:
+        st = H.lib.opcuda_mc1f(256, 0, 256, 100, 4, 8, 256, 256, 0)  # nscen % (64*nrng) != 0
+        assert st != 0
+    finally:
+        H.lib.ophip_mc_set_nrng(4096)

@@ -1,0 +1,288 @@
+"""CPU tests that pin the oracle to things other than itself: known
+answers, closed forms, library routines and brute force (no GPU)."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oplib_amd import opmodel as M
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ---- MT19937 -------------------------------------------------------------
+
+def test_mt19937_known_answers():
+    # C++ standard [rand.predef]: default-seeded mt19937's 10000th output
+    r = O.mt_raw(5489, 10000)
+    assert r[0] == 3499211612
+    assert r[9999] == 4123659995
+
+
+def test_mt19937_fixture_and_numpy():
+    g = np.load(os.path.join(GOLD, "mt19937.npz"))
+    assert np.array_equal(O.mt_raw(5489, 2000), g["s5489"])
+    assert np.array_equal(O.mt_raw(5490, 2000), g["s5490"])
+    # numpy's legacy RandomState is MT19937 with init_genrand for int seeds
+    rs = np.random.RandomState(1)
+    assert np.array_equal(rs.randint(0, 2**32, size=2000, dtype=np.uint64).astype(np.uint32),
+                          g["s1"])
+
+
+def test_uniform_mapping_exact():
+    raw = O.mt_raw(7, 5000)
+    u = O.mt_uniforms(7, 5000)
+    assert np.array_equal(u, ((raw >> 8).astype(np.float64) * 2.0**-24).astype(np.float32))
+    assert u.min() >= 0.0 and u.max() < 1.0
+
+
+# ---- lattice / generator -------------------------------------------------
+
+def test_grid_spot_on_node():
+    x, x0 = O.xgrid(256, 0.0, 400.0, 100.0)
+    assert x0 == 64
+    assert x[x0] == 100.0
+    assert np.allclose(np.diff(x), 1.5625, rtol=0, atol=1e-12)
+
+
+def test_stencil_inverse_closed_form():
+    x = np.cumsum(np.r_[0.0, np.random.default_rng(0).uniform(0.5, 2.0, 40)])
+    inv = O.invm(x).reshape(-1, 2, 2, order="C")  # [node][c][r] since col-major per node
+    for i in range(1, len(x) - 1):
+        dm, dp = x[i - 1] - x[i], x[i + 1] - x[i]
+        mat = np.array([[dm, dp], [dm * dm, dp * dp]])
+        mi = inv[i].T  # back to (r, c)
+        assert np.allclose(mat @ mi, np.eye(2), atol=1e-12)
+
+
+def test_lv_rates_uniform_closed_form():
+    x, _ = O.xgrid(64, 0.0, 400.0, 100.0)
+    h = x[1] - x[0]
+    dr, vo = O.lv_params_c1(x, 0.02)
+    g = O.lv_generator(x, dr, vo)
+    for i in range(1, 63):
+        s2, mu = vo[i] ** 2, dr[i]
+        a = (s2 / h**2 - mu / h) / 2
+        b = (s2 / h**2 + mu / h) / 2
+        assert g[i, i - 1] == pytest.approx(a, rel=1e-12)
+        assert g[i, i + 1] == pytest.approx(b, rel=1e-12)
+    assert np.all(g[0] == 0) and np.all(g[-1] == 0)  # absorbing
+    assert np.abs(g.sum(axis=1)).max() < 1e-9
+    # exactly tridiagonal
+    assert np.count_nonzero(np.triu(g, 2)) == 0 and np.count_nonzero(np.tril(g, -2)) == 0
+
+
+def test_regime_generator_structure_and_marginal():
+    c = M.config_small(nx=12, nr=3, nsq=14)
+    x, x0, g = O.model_generator(c)
+    nx, nr = 12, 3
+    assert np.abs(g.sum(axis=1)).max() < 1e-9
+    # off-diagonal regime blocks are q_off * I
+    for r1 in range(nr):
+        for r2 in range(nr):
+            blk = g[r1 * nx:(r1 + 1) * nx, r2 * nx:(r2 + 1) * nx]
+            if r1 != r2:
+                assert np.array_equal(blk, c.q_off * np.eye(nx))
+    # the regime marginal of U is exp(Q t): sum U over the x of each regime
+    U = O.kernel_fp64(c)[2]
+    Q = c.q_off * (np.ones((nr, nr)) - nr * np.eye(nr))
+    import scipy.linalg as sl
+    E = sl.expm(Q * c.DeltaT)
+    for r1 in range(nr):
+        y = 5 + nx * r1
+        marg = [U[y, r2 * nx:(r2 + 1) * nx].sum() for r2 in range(nr)]
+        # first-order discretisation error only (dt0 = DeltaT 2^-14)
+        assert np.allclose(marg, E[r1], atol=1e-6)
+
+
+# ---- planner -------------------------------------------------------------
+
+def test_planner_formula():
+    import ctypes as C
+    out = C.c_double()
+    n = O.lib.or_plan_niter(1.0 / 16, 1000.0, 2.0**-20, C.byref(out))
+    assert n == 16 and out.value == 2.0**-20
+    n = O.lib.or_plan_niter(1.0 / 252, 1000.0, 2.0**-20, C.byref(out))
+    assert n == math.ceil(math.log2((1 / 252) / 2.0**-20)) == 13
+    assert out.value == pytest.approx((1 / 252) * 2.0**-13, rel=1e-15)
+    # the 0.5/maxdiag cap binds when the generator is stiff
+    n = O.lib.or_plan_niter(1.0, 1e7, 1.0, C.byref(out))
+    assert out.value <= 0.5 / 1e7
+
+
+# ---- first step, squaring ------------------------------------------------
+
+def test_first_step_and_square_chain_vs_numpy():
+    c = M.config_small(nx=20, nr=2, nsq=6)
+    x, x0, g = O.model_generator(c)
+    dt0 = math.ldexp(c.DeltaT, -c.nsq)
+    U0 = O.first_step(g, dt0)
+    assert np.array_equal(U0, np.eye(g.shape[0]) + dt0 * g)
+    assert np.abs(U0.sum(axis=1) - 1).max() < 1e-14
+    U = O.square_chain(U0, c.nsq)
+    ref = np.linalg.matrix_power(U0, 2**c.nsq)
+    assert np.abs(U - ref).max() < 1e-13
+    # a Q-matrix (non-negative off-diagonal rates) gives a stochastic kernel
+    off = g - np.diag(np.diag(g))
+    if off.min() >= 0:
+        assert U.min() >= 0
+
+
+def test_fine_grid_kernel_is_stochastic():
+    # zero drift: the central-difference rates are all non-negative (with
+    # drift, nodes below r*h/sigma^2 get a negative down-rate, as in the
+    # reference's stencil)
+    c = M.config_small(nx=64, nr=2, nsq=8)
+    c.rate = 0.0
+    x, x0, g = O.model_generator(c)
+    assert (g - np.diag(np.diag(g))).min() >= 0
+    U = O.kernel_fp64(c)[2]
+    assert U.min() >= 0
+    assert np.abs(U.sum(axis=1) - 1).max() < 1e-12
+
+
+def test_square_chain_converges_to_expm_first_order():
+    f = np.load(os.path.join(GOLD, "expm_small.npz"))
+    L, T, E = f["L"], float(f["T"]), f["expm"]
+    errs = []
+    for nsq in (8, 10, 12):
+        U = O.square_chain(O.first_step(L, math.ldexp(T, -nsq)), nsq)
+        errs.append(np.abs(U - E).max())
+    # first-order method: error shrinks ~4x per two extra squarings
+    assert errs[1] < errs[0] / 3 and errs[2] < errs[1] / 3
+    assert errs[2] < 1e-4
+
+
+def test_sek_f32_formula():
+    rng = np.random.default_rng(1)
+    d = 33
+    g = rng.standard_normal((d, d)).astype(np.float32)
+    out = np.zeros(d * d, dtype=np.float32)
+    O.lib.or_sek_f32(d, np.ascontiguousarray(g.reshape(-1, order="F")), np.float32(0.01), out)
+    ref = (np.eye(d) + np.float64(np.float32(0.01)) * g.astype(np.float64)).astype(np.float32)
+    # single rounding of a fused multiply-add equals rounding the exact value
+    assert np.array_equal(out.reshape(d, d, order="F"), ref)
+
+
+# ---- cumulative tables and sampler ---------------------------------------
+
+def test_sck_properties_and_float64():
+    rng = np.random.default_rng(2)
+    nm, d = 3, 40
+    t = rng.random((nm, d, d)).astype(np.float32)
+    t[1, 5, :] = 0
+    t[1, 5, 7] = 1.0
+    c = O.sck_f32(t).reshape(nm, d, d)  # [m][col][row] (column-major per matrix)
+    for m in range(nm):
+        C = c[m].T  # [row][y]
+        assert np.all(C[:, -1] == 1.0)
+        assert np.all(np.diff(C, axis=1) >= 0)
+        ref = np.cumsum(t[m].astype(np.float64), axis=1) / t[m].astype(np.float64).sum(1)[:, None]
+        assert np.abs(C - ref).max() < 1e-5
+
+
+def test_sampler_linear_is_inverse_cdf():
+    rng = np.random.default_rng(3)
+    d = 17
+    p = rng.random(d)
+    p[[0, 4, 5]] = 0
+    cdf = np.minimum(np.cumsum(p) / p.sum(), 1).astype(np.float32)
+    for u in list(rng.random(200).astype(np.float32)) + [np.float32(0.0), cdf[3], cdf[6]]:
+        j = O.lib.or_sample_linear(cdf, d, float(u))
+        brute = next((i for i in range(d) if u < cdf[i]), d - 1)
+        assert j == brute
+        assert p[j] > 0 or j == d - 1
+
+
+def test_mc_chain_bisect_equals_linear():
+    # d <= 64 scans linearly, larger d bisects: both must agree on the same
+    # monotone tables (pad a 48-state chain to 96 states with zero mass)
+    rng = np.random.default_rng(4)
+    d1, d2 = 48, 96
+    P = rng.random((d1, d1)) ** 8
+    P /= P.sum(1)[:, None]
+    big = np.zeros((d2, d2))
+    big[:d1, :d1] = P
+    big[d1:, d1:] = np.eye(d2 - d1)
+    c1 = O.sck_f32(P.astype(np.float32)[None])
+    c2 = O.sck_f32(big.astype(np.float32)[None])
+    m_k = np.zeros(5, dtype=np.int32)
+    y1 = O.mc_chain(3, 5489, 2, 5, d1, 7, c1, m_k)
+    y2 = O.mc_chain(3, 5489, 2, 5, d2, 7, c2, m_k)
+    assert np.array_equal(y1, y2)
+
+
+def test_mc_chain_stream_mapping():
+    # identity transition: a state-dependent table makes each draw visible
+    d = 64
+    P = np.zeros((d, d))
+    for i in range(d):
+        P[i] = 1.0 / d  # uniform rows: y = floor(u * 64) exactly
+    c = O.sck_f32(P.astype(np.float32)[None])
+    y = O.mc_chain(2, 100, 2, 3, d, 0, c, np.zeros(3, dtype=np.int32))
+    for s in range(2):
+        u = O.mt_uniforms(100 + s, 2 * 3 * 64)
+        for g in range(2):
+            for k in range(3):
+                for lane in range(64):
+                    want = int(np.floor(u[g * 192 + k * 64 + lane] * 64))
+                    assert y[k, (s * 2 + g) * 64 + lane] == want
+
+
+# ---- pricing -------------------------------------------------------------
+
+def test_black_scholes_bstest_value():
+    # tests/bstest Program.cs: S=K=100, r=0.05, sigma=0.25, T=1
+    assert O.bs_call(100, 100, 0.05, 0.25, 1.0) == pytest.approx(12.336, abs=5e-4)
+    # put-call parity through the call formula
+    c = O.bs_call(100, 110, 0.03, 0.2, 0.5)
+    p = c - 100 + 110 * math.exp(-0.015)
+    assert p > 0
+
+
+def test_operator_method_matches_black_scholes():
+    # constant vol, one regime: the lattice price approaches Black-Scholes
+    c = M.config_small(nx=256, nr=1, nsq=14, nsteps=52)
+    c.sigma_r[0] = 0.25
+    c.rate = 0.05
+    c.q_off = 0.0
+    c.xhi = 400.0
+    p = O.solve_fp64(c)[0, 0]
+    for i, K in enumerate(c.strike_list()):
+        assert p[i] == pytest.approx(O.bs_call(100, K, 0.05, 0.25, 1.0), abs=0.03)
+
+
+def test_backward_equals_propagation():
+    c1 = M.config_small(nx=40, nr=2, nsteps=5)
+    c2 = M.config_small(nx=40, nr=2, nsteps=5, price_kind=M.OPM_PRICE_PROPAGATE)
+    p1 = O.solve_fp64(c1)[:, 0]
+    p2 = O.solve_fp64(c2)[:, -1]
+    assert np.allclose(p1, p2, rtol=1e-12, atol=1e-12)
+
+
+def test_golden_prices_reproduce():
+    f = np.load(os.path.join(GOLD, "prices_small_bi.npz"))
+    c = M.config_small(nx=64, nr=2)
+    assert np.allclose(O.solve_fp64(c), f["prices"], rtol=1e-12, atol=1e-12)
+
+
+# ---- surrogate pieces ----------------------------------------------------
+
+def test_gelu_grad_finite_difference():
+    for x in np.linspace(-4, 4, 17):
+        fd = (O.lib.or_gelu(x + 1e-6) - O.lib.or_gelu(x - 1e-6)) / 2e-6
+        assert O.lib.or_gelu_grad(x) == pytest.approx(fd, abs=1e-7)
+
+
+def test_adam_step_closed_form():
+    n = 5
+    p = np.linspace(-1, 1, n)
+    g = np.linspace(0.5, -0.5, n)
+    m = np.zeros(n)
+    v = np.zeros(n)
+    p0 = p.copy()
+    O.lib.or_adam_step(n, p, g, m, v, 1e-3, 0.9, 0.999, 1e-8, 1)
+    # first step: m_hat = g, v_hat = g^2 -> p -= lr * g/(|g| + eps)
+    assert np.allclose(p, p0 - 1e-3 * g / (np.abs(g) + 1e-8), atol=1e-15)
