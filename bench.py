@@ -58,6 +58,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mc", action="store_true", help="skip the Monte Carlo leg")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the calibration-sweep leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
@@ -127,6 +128,19 @@ def main():
     bi_gbs = bi_bytes / (bi_us * 1e-6) / 1e9
     log(f"[bench] backward step {bi_us:.2f} us, {bi_gbs:.0f} GB/s algorithmic")
 
+    # ---- the same scenario with the U = I + W mixed-precision squarings ----
+    methods = {"plain_fp32": {"scenarios_per_s": value, "ms": ms_per_step,
+                              "max_rel_err": float(rel.max()),
+                              "meets_1e-4": bool(rel.max() <= 1e-4)}}
+    for prec, name in ((1, "wform_fp16x1"), (2, "wform_fp16x3"), (3, "wform_bf16x3")):
+        methods[name] = bench_method(H, M, prec, args.steps, args.warmup, gold)
+        log(f"[bench] {name}: {methods[name]}")
+
+    # ---- calibration sweep: 64 lattices, plain FP32 one by one vs batched W form ----
+    sweep = None
+    if not args.no_sweep:
+        sweep = bench_sweep(H, M, max(3, args.steps // 10))
+
     # ---- Monte Carlo on the daily table ----
     mc = None
     if not args.no_mc:
@@ -166,10 +180,71 @@ def main():
         "gemv": {"kernel": "k_skinny<3> + combine (backward step)", "avg_us": bi_us,
                  "algorithmic_GBps": bi_gbs, "peak_GBps": HBM_PEAK_GBS,
                  "frac": bi_gbs / HBM_PEAK_GBS},
+        "methods": methods,
+        "sweep": sweep,
         "mc": mc,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def rel_err(p, gold):
+    den = np.maximum(np.abs(gold), 1e-3 * np.abs(gold).max())
+    return np.abs(p - gold) / den
+
+
+def bench_method(H, M, prec, steps, warmup, gold):
+    cfg = M.config_rs2048()
+    cfg.precision = prec
+    s = M.Solver(cfg)
+    s.solve()
+    s.capture()
+    for _ in range(warmup):
+        s.launch()
+    H.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.launch()
+    H.sync()
+    t = (time.perf_counter() - t0) / steps
+    e = rel_err(s.prices(), gold)
+    s.close()
+    return {"scenarios_per_s": 1.0 / t, "ms": t * 1e3, "max_rel_err": float(e.max()),
+            "rms_rel_err": float(np.sqrt(np.mean(e ** 2))), "meets_1e-4": bool(e.max() <= 1e-4)}
+
+
+def bench_sweep(H, M, reps):
+    gold_path = os.path.join(ROOT, "tests", "golden", "prices_sweep.npz")
+    gold = np.load(gold_path)["prices"] if os.path.exists(gold_path) else None
+    out = {"workload": "64 CEV lattices, d in {384..1536}, 17-18 squarings to U(0.1), "
+                       "21 strikes x 10 maturities", "scenarios": 64}
+    for prec, name in ((0, "plain_fp32_unbatched"), (2, "wform_fp16x3_batched"),
+                       (1, "wform_fp16x1_batched"), (3, "wform_bf16x3_batched")):
+        cfgs = M.config_sweep(precision=prec)
+        b = M.Batch(cfgs)
+        b.solve()
+        b.capture()
+        b.launch()
+        H.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            b.launch()
+        H.sync()
+        t = (time.perf_counter() - t0) / reps
+        p = b.prices()
+        r = {"sweeps_per_s": 1.0 / t, "scenarios_per_s": 64.0 / t, "ms_per_sweep": t * 1e3,
+             "tflops_algorithmic": M.sweep_flops(cfgs) / t / 1e12}
+        if gold is not None:
+            e = rel_err(p, gold)
+            r.update({"max_rel_err": float(e.max()), "meets_1e-4": bool(e.max() <= 1e-4)})
+        out[name] = r
+        log(f"[bench] sweep {name}: {r}")
+        b.close()
+    base = out["plain_fp32_unbatched"]["scenarios_per_s"]
+    for k in list(out):
+        if isinstance(out[k], dict) and k != "plain_fp32_unbatched":
+            out[k]["speedup_vs_plain"] = out[k]["scenarios_per_s"] / base
+    return out
 
 
 def bench_mc(H, U, d, cfg):

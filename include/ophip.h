@@ -123,6 +123,9 @@ int ophip_sek_rs(int nx, int nr, unsigned xval, unsigned drift, unsigned vol, do
 /* cumulative tables stored row-contiguous: crow[m*d*d + row*d + y]; the
  * same values as opcuda_sck (sck.cu:43-84) */
 int ophip_sck_rows(int d, int nm, unsigned tpk, unsigned crow);
+/* the same for U = I + W given W (plus_identity = 1 adds 1 to each
+ * diagonal entry before the running sum) */
+int ophip_sck_rows_ex(int d, int nm, unsigned tpk, unsigned crow, int plus_identity);
 
 /* strided batched FP32 GEMM on the matrix cores */
 int ophip_sgemm_strided_batched(int m, int n, int k, float alpha, unsigned A, int lda,
@@ -133,6 +136,48 @@ int ophip_sgemm_strided_batched(int m, int n, int k, float alpha, unsigned A, in
  * step over many payoff columns (the role of sgemv3/sgemv4) */
 int ophip_sgemm_skinny(int m, int K, int ncp, float alpha, unsigned A, int lda, unsigned X,
                        int ldx, unsigned Y, int ldy);
+
+/* Y = alpha*(A*X + X): one backward step with U = I + W held as W */
+int ophip_sgemm_skinny_ex(int m, int K, int ncp, float alpha, unsigned A, int lda, unsigned X,
+                          int ldx, unsigned Y, int ldy, int plus_x);
+/* C = a*A + b*B elementwise over n floats */
+int ophip_saxpby(int n, float a, unsigned A, float b, unsigned B, unsigned C);
+
+/* Repeated squaring in the U = I + W form on FP16/BF16 matrix cores with
+ * FP32 accumulation and per-step row-sum renormalisation (the reference
+ * squares U in FP32, ssqmm.cu:119-170, driven by CModel Device.cs:216-228).
+ * mode 0: FP16, one product H*H (H = fp16(W*2^15));
+ * mode 1: FP16, three products H*H + 2^-11 (H*L + L*H);
+ * mode 2: BF16, three products.
+ * A state (ophip_wsq_bytes(d, mode) bytes, d % 128 == 0) holds two FP32 W
+ * buffers, their 16-bit copies and partial row sums.  Write W0 = dt*L into
+ * buffer 0 (ophip_sek_rs out_kind 2); ophip_wsq_exec leaves W(2^nsq dt) in
+ * buffer nsq & 1.  A plan batches many states of different d into the same
+ * launches; executing it only launches kernels (graph-capturable).  nsqs
+ * (optional) gives each lattice its own squaring count, used when exec is
+ * called with nsq < 0: round r squares the lattices with nsqs[i] > r (the
+ * reference's per-round active lists, CModel Common.cs:508-545). */
+unsigned long ophip_wsq_bytes(int d, int mode);
+unsigned      ophip_wsq_w(unsigned state, int d, int mode, int buf);
+/* any array of a state: which = 0 W, 1 Hr, 2 Hc, 3 Lr, 4 Lc, 5 partial
+ * row sums (0 when the mode has no such array) */
+unsigned      ophip_wsq_array(unsigned state, int d, int mode, int buf, int which);
+unsigned long ophip_wsq_plan(int n, const unsigned *states, const int *ds, const int *nsqs,
+                             int mode, int tile, int grid);
+int           ophip_wsq_exec(unsigned long plan, int nsq);
+void          ophip_wsq_plan_free(unsigned long plan);
+
+/* Forward pricing of many lattices in one launch per step (calibration
+ * sweep): p_k = p_{k-1} U, or p_{k-1}(I + W) when plus_identity[i]; prices
+ * df_k * sum_j p_k[j] max(x_j - K_c, 0) for k = 1..nmat (maturity k*dT).
+ * P[i]: (nmat+1) x d row-major, row 0 = initial distribution; out[i]:
+ * nmat x nstrikes.  exec only launches kernels (graph-capturable). */
+unsigned long ophip_fwd_plan(int n, const int *d, const unsigned *K, const int *plus_identity,
+                             const unsigned *P, const unsigned *x, const unsigned *out,
+                             const float *rate, float dT, int nmat, const float *strikes,
+                             int nstrikes);
+int  ophip_fwd_exec(unsigned long plan);
+void ophip_fwd_plan_free(unsigned long plan);
 
 /* Monte Carlo */
 int ophip_mc_set_nrng(int nstreams);

@@ -32,7 +32,10 @@ extern "C" {
 #define OPM_PRICE_PROPAGATE 1 /* U_k = U_{k-1}*U, prices at every k from row y0 of U_k */
 
 /* squaring arithmetic */
-#define OPM_PREC_FP32 0       /* plain FP32 (the reference's device method) */
+#define OPM_PREC_FP32   0     /* plain FP32 on U (the reference's device method) */
+#define OPM_PREC_F16    1     /* U = I + W, FP16 one product, row-sum renormalised */
+#define OPM_PREC_F16X3  2     /* U = I + W, FP16 hi/lo three products, renormalised */
+#define OPM_PREC_BF16X3 3     /* U = I + W, BF16 hi/lo three products, renormalised */
 
 typedef struct {
     int nx, nr;
@@ -53,6 +56,11 @@ typedef struct {
 /* host lattice and parameters (FP64), for inspection and tests */
 int opm_grid(const opm_config *cfg, double *xval /* nx */, int *x0);
 int opm_params(const opm_config *cfg, double *drift /* nx*nr */, double *vol /* nx*nr */);
+/* the largest exit rate max_y |L_yy| of the configuration's generator */
+double opm_max_rate(const opm_config *cfg);
+/* the reference planner (CModel Common.cs:289-307) in years:
+ * dt = min(0.5/max|L_yy|, dt0_cap), niter = ceil(log2(DeltaT/dt)) */
+int opm_plan_nsq(const opm_config *cfg, double dt0_cap);
 
 /* a solver owns device buffers for one configuration */
 typedef struct opm_solver opm_solver;
@@ -66,9 +74,30 @@ int opm_launch(opm_solver *s);
 /* wait and copy results: prices[nr][n_mat][nstrikes] (n_mat = 1 for
  * backward induction, nsteps for propagation) */
 int opm_prices(opm_solver *s, double *prices);
-/* copy U(DeltaT) (FP32, d*d column-major) */
+/* copy U(DeltaT) (FP32, d*d column-major; I + W formed on the host in the
+ * W form) */
 int opm_kernel(opm_solver *s, float *U);
+/* the squared kernel as held on the device: U, or W when *is_w = 1 */
+int opm_kernel_raw(opm_solver *s, float *K, int *is_w);
+/* its device handle (valid after a solve) */
+unsigned opm_kernel_handle(opm_solver *s);
 int opm_dims(opm_solver *s, int *d, int *n_mat);
+
+/* ---- calibration sweep: many one-regime lattices of different size ----
+ * Every configuration shares precision, nsq, DeltaT, nsteps (maturities
+ * k*DeltaT, k = 1..nsteps) and the strikes; nx, the vol parameters and
+ * the rate differ.  Pricing is the forward step of the spot row,
+ * p_k = p_{k-1} U(DeltaT).  In the W form all lattices square together in
+ * one persistent launch per round (ophip_wsq_plan); the plain FP32 method
+ * squares them one launch each.  tile: 0 = autotuned/default, 64 or 128. */
+typedef struct opm_batch opm_batch;
+opm_batch *opm_batch_create(const opm_config *cfgs, int n, int tile);
+void opm_batch_destroy(opm_batch *b);
+int opm_batch_solve(opm_batch *b);
+int opm_batch_capture(opm_batch *b);
+int opm_batch_launch(opm_batch *b);
+/* prices[n][nsteps][nstrikes] */
+int opm_batch_prices(opm_batch *b, double *prices);
 
 #ifdef __cplusplus
 }

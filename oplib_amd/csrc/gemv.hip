@@ -148,65 +148,117 @@ __global__ __launch_bounds__(64) void k_sgemv4(const char *__restrict__ base,
 }
 
 // ---------------------------------------------------------------------
-// skinny GEMM: Y[m x NCP] = alpha * A[m x K] * X[K x NCP], X/Y row-major
-// grid (m/16, KS); 4 waves split the workgroup's K range in 16-deep blocks
+// skinny GEMM: Y[m x NCP] = alpha * (A[m x K] * X[K x NCP] [+ X]), X and Y
+// row-major.  Grid (m/16 strips, KS K-splits); 8 waves per workgroup split
+// the workgroup's K range; each wave keeps 4 k16-blocks of loads in flight.
+// With KS > 1 every workgroup writes its partial tile, and the workgroup
+// that finishes a strip last (per-strip arrival counter) adds the KS
+// partials in index order, so the result does not depend on which
+// workgroup came last.  The counter is reset by that workgroup.
 // ---------------------------------------------------------------------
+constexpr int kSkWaves = 8;
+
 template <int NT>
-__global__ __launch_bounds__(256) void k_skinny(int m, int K, int kc, const float *__restrict__ A,
+__global__ __launch_bounds__(512) void k_skinny(int m, int K, int kc, const float *__restrict__ A,
                                                 int lda, const float *__restrict__ X, int ldx,
-                                                float *__restrict__ out, int ldo, float alpha) {
-    __shared__ f4 red[4][NT][64];
+                                                float *__restrict__ Y, int ldy, float alpha,
+                                                int plus_x, int KS, float *__restrict__ part,
+                                                int *__restrict__ counters) {
+    __shared__ f4 red[kSkWaves][NT][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, grp = lane >> 4;
-    const int row0 = blockIdx.x * 16;
+    const int strip = blockIdx.x, row0 = strip * 16;
     const int kbeg = blockIdx.y * kc, kend = min(K, kbeg + kc);
     f4 acc[NT];
 #pragma unroll
     for (int j = 0; j < NT; j++) acc[j] = f4{0.f, 0.f, 0.f, 0.f};
     const float *Ar = A + row0 + r;
-    for (int kb = kbeg + 16 * wave; kb < kend; kb += 64) {
-        // lane group g of MFMA step s takes k = kb + 4g + s
-        float a[4];
-        f4 xb[NT];
+    constexpr int U = 4;  // k16-blocks per iteration
+    for (int kb0 = kbeg + 16 * wave; kb0 < kend; kb0 += 16 * kSkWaves * U) {
+        float a[U][4];
+        float xb[U][NT][4];
 #pragma unroll
-        for (int s = 0; s < 4; s++) {
-            int k = kb + 4 * grp + s;
-            a[s] = (k < kend) ? Ar[(size_t)lda * k] : 0.0f;
+        for (int u = 0; u < U; u++) {
+            const int kb = kb0 + 16 * kSkWaves * u;
 #pragma unroll
-            for (int j = 0; j < NT; j++) xb[j][s] = (k < kend) ? X[(size_t)k * ldx + 16 * j + r] : 0.0f;
+            for (int s = 0; s < 4; s++) {
+                // lane group g of MFMA step s takes k = kb + 4g + s
+                const int k = kb + 4 * grp + s;
+                const bool ok = k < kend;
+                a[u][s] = ok ? Ar[(size_t)lda * k] : 0.0f;
+#pragma unroll
+                for (int j = 0; j < NT; j++) xb[u][j][s] = ok ? X[(size_t)k * ldx + 16 * j + r] : 0.0f;
+            }
         }
 #pragma unroll
-        for (int s = 0; s < 4; s++)
+        for (int u = 0; u < U; u++)
 #pragma unroll
-            for (int j = 0; j < NT; j++) acc[j] = mfma4(a[s], xb[j][s], acc[j]);
+            for (int s = 0; s < 4; s++)
+#pragma unroll
+                for (int j = 0; j < NT; j++) acc[j] = mfma4(a[u][s], xb[u][j][s], acc[j]);
     }
 #pragma unroll
     for (int j = 0; j < NT; j++) red[wave][j][lane] = acc[j];
     __syncthreads();
-    if (wave == 0) {
+    if (wave != 0) return;
+    f4 v[NT];
 #pragma unroll
-        for (int j = 0; j < NT; j++) {
-            f4 v = red[0][j][lane];
-            v += red[1][j][lane];
-            v += red[2][j][lane];
-            v += red[3][j][lane];
-            v *= alpha;
-            // lane holds rows 4*grp+e of column 16j+r
-            float *o = out + (size_t)blockIdx.y * m * ldo;
+    for (int j = 0; j < NT; j++) {
+        v[j] = red[0][j][lane];
 #pragma unroll
-            for (int e = 0; e < 4; e++) o[(size_t)(row0 + 4 * grp + e) * ldo + 16 * j + r] = v[e];
-        }
+        for (int w = 1; w < kSkWaves; w++) v[j] += red[w][j][lane];
     }
+    // lane holds rows 4*grp+e of column 16j+r
+    if (KS > 1) {
+        float *pp = part + (size_t)blockIdx.y * m * 16 * NT;
+#pragma unroll
+        for (int j = 0; j < NT; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                pp[(size_t)(row0 + 4 * grp + e) * 16 * NT + 16 * j + r] = v[j][e];
+        __threadfence();  // release the partial before counting in
+        int old = 0;
+        if (lane == 0) old = atomicAdd(&counters[strip], 1);
+        old = __shfl(old, 0, 64);
+        if (old != KS - 1) return;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // see the other partials
+        // this workgroup completes the strip: sum the partials in order
+#pragma unroll
+        for (int j = 0; j < NT; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const size_t off = (size_t)(row0 + 4 * grp + e) * 16 * NT + 16 * j + r;
+                float sum = part[off];
+                for (int q = 1; q < KS; q++) sum += part[(size_t)q * m * 16 * NT + off];
+                v[j][e] = sum;
+            }
+        if (lane == 0) atomicExch(&counters[strip], 0);
+    }
+#pragma unroll
+    for (int j = 0; j < NT; j++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int row = row0 + 4 * grp + e;
+            float y = alpha * v[j][e];
+            if (plus_x) y += alpha * X[(size_t)row * ldx + 16 * j + r];
+            Y[(size_t)row * ldy + 16 * j + r] = y;
+        }
 }
 
-__global__ void k_skinny_combine(int m, int ncp, int KS, const float *__restrict__ part,
-                                 float *__restrict__ Y, int ldy) {
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= m * ncp) return;
-    int i = idx / ncp, c = idx - i * ncp;
-    float s = part[(size_t)i * ncp + c];
-    for (int q = 1; q < KS; q++) s += part[(size_t)q * m * ncp + (size_t)i * ncp + c];
-    Y[(size_t)i * ldy + c] = s;
+constexpr int kMaxStrips = 1 << 16;
+
+// per-strip arrival counters of k_skinny: zeroed once, and every use leaves
+// them zero again (the last workgroup of a strip resets its counter)
+int *strip_counters() {
+    static int *c[16] = {};
+    Device &D = dev();
+    int id = D.id < 0 ? 0 : D.id;
+    if (!c[id]) {
+        if (hipMalloc(&c[id], sizeof(int) * kMaxStrips) != hipSuccess) return nullptr;
+        if (hipMemset(c[id], 0, sizeof(int) * kMaxStrips) != hipSuccess) return nullptr;
+        if (hipDeviceSynchronize() != hipSuccess) return nullptr;
+    }
+    return c[id];
 }
 
 }  // namespace
@@ -281,39 +333,49 @@ OPHIP_EXPORT void opcuda_sgemv4(int nblocks, unsigned argptr_bid) {
     check_launch();
 }
 
-// Y = alpha * A * X.  A: m x K column-major (lda); X: K x ncp row-major
-// (ldx >= ncp); Y: m x ncp row-major (ldy >= ncp).  ncp must be 16, 32, 48
-// or 64 (pad the right-hand side with zero columns); m % 16 == 0.
-OPHIP_EXPORT int ophip_sgemm_skinny(int m, int K, int ncp, float alpha, unsigned A, int lda,
-                                    unsigned X, int ldx, unsigned Y, int ldy) {
+// Y = alpha * (A * X [+ X]).  A: m x K column-major (lda); X: K x ncp
+// row-major (ldx >= ncp); Y: m x ncp row-major (ldy >= ncp).  ncp must be
+// 16, 32, 48 or 64 (pad the right-hand side with zero columns); m % 16 == 0.
+// plus_x = 1 adds X itself (requires m == K): one backward step with
+// U = I + W stored as W.
+static int skinny(int m, int K, int ncp, float alpha, unsigned A, int lda, unsigned X, int ldx,
+                  unsigned Y, int ldy, int plus_x) {
     Device &D = dev();
     if (m <= 0 || K <= 0 || (m % 16) || ncp <= 0 || ncp > 64 || (ncp % 16) || ldx < ncp ||
-        ldy < ncp || lda < m)
+        ldy < ncp || lda < m || (plus_x && m != K) || m / 16 > kMaxStrips)
         return (int)set_status(kErrBadArg);
     const float *Ap = P<const float>(A), *Xp = P<const float>(X);
     float *Yp = P<float>(Y);
     if (!Ap || !Xp || !Yp) return (int)set_status(kErrBadHandle);
     const int strips = m / 16;
-    int KS = cdiv(2 * (D.n_cu > 0 ? D.n_cu : 256), strips);
+    const int ncu = D.n_cu > 0 ? D.n_cu : 256;
+    int KS = cdiv(ncu, strips);
     if (KS > 16) KS = 16;
-    int kc = cdiv(cdiv(K, KS), 64) * 64;
+    int kc = cdiv(cdiv(K, KS), 16) * 16;
     KS = cdiv(K, kc);
-    float *out = Yp;
-    int ldo = ldy;
+    float *part = nullptr;
     if (KS > 1) {
-        out = (float *)workspace((size_t)KS * m * ncp * sizeof(float));
-        if (!out) return (int)kErrOutOfArena;
-        ldo = ncp;
+        part = (float *)workspace((size_t)KS * m * ncp * sizeof(float));
+        if (!part) return (int)kErrOutOfArena;
     }
-    dim3 grid(strips, KS), block(256);
+    int *counters = strip_counters();
+    if (!counters) return (int)kErrOutOfArena;
+    dim3 grid(strips, KS), block(64 * kSkWaves);
     switch (ncp / 16) {
-    case 1: hipLaunchKernelGGL(k_skinny<1>, grid, block, 0, D.stream, m, K, kc, Ap, lda, Xp, ldx, out, ldo, alpha); break;
-    case 2: hipLaunchKernelGGL(k_skinny<2>, grid, block, 0, D.stream, m, K, kc, Ap, lda, Xp, ldx, out, ldo, alpha); break;
-    case 3: hipLaunchKernelGGL(k_skinny<3>, grid, block, 0, D.stream, m, K, kc, Ap, lda, Xp, ldx, out, ldo, alpha); break;
-    default: hipLaunchKernelGGL(k_skinny<4>, grid, block, 0, D.stream, m, K, kc, Ap, lda, Xp, ldx, out, ldo, alpha); break;
+    case 1: hipLaunchKernelGGL(k_skinny<1>, grid, block, 0, D.stream, m, K, kc, Ap, lda, Xp, ldx, Yp, ldy, alpha, plus_x, KS, part, counters); break;
+    case 2: hipLaunchKernelGGL(k_skinny<2>, grid, block, 0, D.stream, m, K, kc, Ap, lda, Xp, ldx, Yp, ldy, alpha, plus_x, KS, part, counters); break;
+    case 3: hipLaunchKernelGGL(k_skinny<3>, grid, block, 0, D.stream, m, K, kc, Ap, lda, Xp, ldx, Yp, ldy, alpha, plus_x, KS, part, counters); break;
+    default: hipLaunchKernelGGL(k_skinny<4>, grid, block, 0, D.stream, m, K, kc, Ap, lda, Xp, ldx, Yp, ldy, alpha, plus_x, KS, part, counters); break;
     }
-    if (KS > 1)
-        hipLaunchKernelGGL(k_skinny_combine, dim3(cdiv(m * ncp, 256)), dim3(256), 0, D.stream, m,
-                           ncp, KS, out, Yp, ldy);
     return (int)check_launch();
+}
+
+OPHIP_EXPORT int ophip_sgemm_skinny(int m, int K, int ncp, float alpha, unsigned A, int lda,
+                                    unsigned X, int ldx, unsigned Y, int ldy) {
+    return skinny(m, K, ncp, alpha, A, lda, X, ldx, Y, ldy, 0);
+}
+
+OPHIP_EXPORT int ophip_sgemm_skinny_ex(int m, int K, int ncp, float alpha, unsigned A, int lda,
+                                       unsigned X, int ldx, unsigned Y, int ldy, int plus_x) {
+    return skinny(m, K, ncp, alpha, A, lda, X, ldx, Y, ldy, plus_x);
 }

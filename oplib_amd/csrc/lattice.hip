@@ -265,7 +265,7 @@ OPHIP_EXPORT void opcuda_sck(int d, int nm, unsigned tpk, unsigned ctpk) {
 // are coalesced.  Pass 1 forms the totals, pass 2 re-forms the running sums
 // in the same order and writes the normalised values.
 __global__ __launch_bounds__(64) void k_sck_rows(int d, const float *__restrict__ tpk,
-                                                  float *__restrict__ crow) {
+                                                  float *__restrict__ crow, int plus_identity) {
     __shared__ float tile[64][65];
     const int lane = threadIdx.x;
     const int row0 = blockIdx.x * 64;
@@ -280,7 +280,10 @@ __global__ __launch_bounds__(64) void k_sck_rows(int d, const float *__restrict_
             // load tile: element (r, c) = tpk[row0 + r, y0 + c]
             for (int c = 0; c < ny; c++) {
                 float v = 0.0f;
-                if (row0 + lane < d) v = tpk[mb + (size_t)(row0 + lane) + (size_t)d * (y0 + c)];
+                if (row0 + lane < d) {
+                    v = tpk[mb + (size_t)(row0 + lane) + (size_t)d * (y0 + c)];
+                    if (plus_identity && row0 + lane == y0 + c) v += 1.0f;
+                }
                 tile[lane][c] = v;
             }
             __syncthreads();
@@ -306,10 +309,20 @@ __global__ __launch_bounds__(64) void k_sck_rows(int d, const float *__restrict_
     (void)row;
 }
 
-OPHIP_EXPORT int ophip_sck_rows(int d, int nm, unsigned tpk, unsigned crow) {
+static int sck_rows(int d, int nm, unsigned tpk, unsigned crow, int plus_identity) {
     Device &D = dev();
     if (d <= 0 || nm <= 0 || !ptr(tpk) || !ptr(crow)) return (int)set_status(kErrBadArg);
     hipLaunchKernelGGL(k_sck_rows, dim3(cdiv(d, 64), nm), dim3(64), 0, D.stream, d,
-                       P<const float>(tpk), P<float>(crow));
+                       P<const float>(tpk), P<float>(crow), plus_identity);
     return (int)check_launch();
+}
+
+OPHIP_EXPORT int ophip_sck_rows(int d, int nm, unsigned tpk, unsigned crow) {
+    return sck_rows(d, nm, tpk, crow, 0);
+}
+
+// the same table for U = I + W given W (the W-form kernels): 1 is added to
+// each diagonal entry in FP32 before the running sum
+OPHIP_EXPORT int ophip_sck_rows_ex(int d, int nm, unsigned tpk, unsigned crow, int plus_identity) {
+    return sck_rows(d, nm, tpk, crow, plus_identity);
 }

@@ -75,11 +75,14 @@ struct WaveMT {
         for (int i = lane; i < kMTN; i += 64) st[i] = b[i];
         if (lane == 0) st[kMTN] = (uint32_t)idx;
     }
-    // 64 consecutive outputs, lane l gets output idx + l
+    // 64 consecutive outputs, lane l gets output idx + l.  Whenever this
+    // call reaches the end of the current state (idx + 64 >= 624, including
+    // landing exactly on 624) the next state is twisted first, because the
+    // buffers swap at the end of the call.
     __device__ uint32_t next64(int lane) {
         uint32_t *c = cur ? buf1 : buf0;
         uint32_t *n = cur ? buf0 : buf1;
-        if (idx + 64 > kMTN) mt_twist(c, n, lane);
+        if (idx + 64 >= kMTN) mt_twist(c, n, lane);
         int pos = idx + lane;
         uint32_t x = (pos < kMTN) ? c[pos] : n[pos - kMTN];
         idx += 64;

@@ -15,6 +15,7 @@
 #include "ophip.h"
 #include "opmodel.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -105,6 +106,11 @@ struct opm_solver {
     Buf Uk[2], payT, out;  // propagation: U_k, payoff (nstrikes x d), prices
     int u_final = 0, v_final = 0;
     unsigned long graph = 0;
+    // U = I + W form (precision > 0): squaring state and plan
+    int wmode = -1;
+    Buf wstate;
+    unsigned long wplan = 0;
+    unsigned kernel_h = 0;  // handle of U (plain) or W (W form) after the squarings
 };
 
 extern "C" {
@@ -130,6 +136,35 @@ int opm_params(const opm_config *cfg, double *drift, double *vol) {
     return 0;
 }
 
+double opm_max_rate(const opm_config *c) {
+    if (!c || c->nx < 3) return 0.0;
+    std::vector<double> x, dr, vo;
+    int b;
+    make_grid(c, x, b);
+    make_params(c, x, dr, vo);
+    const int nx = c->nx;
+    double m = 0.0;
+    for (int r = 0; r < c->nr; r++)
+        for (int i = 1; i < nx - 1; i++) {
+            // the two stencil rates of S1DGrid.cs:78-147 / CLVModel.cs:99-114
+            const double m00 = x[i - 1] - x[i], m01 = x[i + 1] - x[i];
+            const double m10 = m00 * m00, m11 = m01 * m01, det = m00 * m11 - m01 * m10;
+            const double mu = dr[i + (size_t)nx * r], v = vo[i + (size_t)nx * r];
+            const double am = (m11 * mu - m01 * v * v) / det;
+            const double ap = (-m10 * mu + m00 * v * v) / det;
+            const double out = am + ap + (c->nr - 1) * c->q_off;
+            m = out > m ? out : m;
+        }
+    return m;
+}
+
+int opm_plan_nsq(const opm_config *c, double dt0_cap) {
+    const double md = opm_max_rate(c);
+    double dt = md > 0.0 ? 0.5 / md : dt0_cap;
+    if (dt > dt0_cap) dt = dt0_cap;
+    return (int)std::ceil(std::log(c->DeltaT / dt) / std::log(2.0));
+}
+
 opm_solver *opm_create(const opm_config *cfg) {
     if (!valid(cfg)) return nullptr;
     if (opcuda_cublas_init() != 0) return nullptr;
@@ -145,7 +180,19 @@ opm_solver *opm_create(const opm_config *cfg) {
     make_params(c, x, dr, vo);
     const size_t dd = (size_t)d * d;
     bool ok = alloc(s->x, sizeof(double) * nx) && alloc(s->drift, sizeof(double) * d) &&
-              alloc(s->vol, sizeof(double) * d) && alloc(s->U[0], 4 * dd) && alloc(s->U[1], 4 * dd);
+              alloc(s->vol, sizeof(double) * d);
+    if (c->precision == OPM_PREC_FP32) {
+        ok = ok && alloc(s->U[0], 4 * dd) && alloc(s->U[1], 4 * dd);
+    } else {
+        s->wmode = c->precision - 1;  // OPM_PREC_F16.. -> wsq modes 0..2
+        unsigned long wb = ophip_wsq_bytes(d, s->wmode);
+        ok = ok && wb && (d % 128) == 0 && alloc(s->wstate, wb);
+        if (ok) {
+            unsigned st = s->wstate.h;
+            s->wplan = ophip_wsq_plan(1, &st, &d, nullptr, s->wmode, 0, 0);
+            ok = s->wplan != 0;
+        }
+    }
     ok = ok && !upload(s->x, x.data(), s->x.bytes) && !upload(s->drift, dr.data(), s->drift.bytes) &&
          !upload(s->vol, vo.data(), s->vol.bytes);
     if (ok && c->price_kind == OPM_PRICE_BACKWARD) {
@@ -187,8 +234,9 @@ opm_solver *opm_create(const opm_config *cfg) {
 void opm_destroy(opm_solver *s) {
     if (!s) return;
     if (s->graph) ophip_graph_destroy(s->graph);
+    if (s->wplan) ophip_wsq_plan_free(s->wplan);
     Buf *all[] = {&s->x, &s->drift, &s->vol, &s->U[0], &s->U[1], &s->V[0], &s->V[1],
-                  &s->pay0, &s->Uk[0], &s->Uk[1], &s->payT, &s->out};
+                  &s->pay0, &s->Uk[0], &s->Uk[1], &s->payT, &s->out, &s->wstate};
     for (Buf *b : all) release(*b);
     delete s;
 }
@@ -197,45 +245,76 @@ void opm_destroy(opm_solver *s) {
 static int solve_body(opm_solver *s) {
     const opm_config *c = &s->cfg;
     const int d = s->d, nx = c->nx, nr = c->nr;
-    int st = ophip_sek_rs(nx, nr, s->x.h, s->drift.h, s->vol.h, c->q_off, s->dt0, 0, s->U[0].h);
-    if (st) return st;
-    // repeated squaring U(2dt) = U(dt)^2, ping-pong (CModel Device.cs:216-228)
-    for (int r = 0; r < c->nsq; r++) {
-        const Buf &a = s->U[r & 1];
-        const Buf &o = s->U[(r + 1) & 1];
-        st = ophip_sgemm_strided_batched(d, d, d, 1.0f, a.h, d, 0, a.h, d, 0, 0.0f, o.h, d, 0, 1);
+    const bool wform = s->wmode >= 0;
+    int st;
+    if (!wform) {
+        st = ophip_sek_rs(nx, nr, s->x.h, s->drift.h, s->vol.h, c->q_off, s->dt0, 0, s->U[0].h);
         if (st) return st;
+        // repeated squaring U(2dt) = U(dt)^2, ping-pong (CModel Device.cs:216-228)
+        for (int r = 0; r < c->nsq; r++) {
+            const Buf &a = s->U[r & 1];
+            const Buf &o = s->U[(r + 1) & 1];
+            st = ophip_sgemm_strided_batched(d, d, d, 1.0f, a.h, d, 0, a.h, d, 0, 0.0f, o.h, d, 0, 1);
+            if (st) return st;
+        }
+        s->u_final = c->nsq & 1;
+        s->kernel_h = s->U[s->u_final].h;
+    } else {
+        // W0 = dt*L in FP32 straight from the FP64 assembly, then the
+        // W-form squarings with renormalisation
+        const unsigned w0 = ophip_wsq_w(s->wstate.h, d, s->wmode, 0);
+        st = ophip_sek_rs(nx, nr, s->x.h, s->drift.h, s->vol.h, c->q_off, s->dt0, 2, w0);
+        if (st) return st;
+        st = ophip_wsq_exec(s->wplan, c->nsq);
+        if (st) return st;
+        s->u_final = c->nsq & 1;
+        s->kernel_h = ophip_wsq_w(s->wstate.h, d, s->wmode, s->u_final);
     }
-    s->u_final = c->nsq & 1;
-    const unsigned U = s->U[s->u_final].h;
+    const unsigned U = s->kernel_h;
     if (c->price_kind == OPM_PRICE_BACKWARD) {
         const float df = (float)std::exp(-c->rate * c->DeltaT);
         st = opcuda_scopy1(s->V[0].h, s->pay0.h, (unsigned)((size_t)d * s->ncp));
         if (st) return st;
         for (int t = 0; t < c->nsteps; t++) {
-            st = ophip_sgemm_skinny(d, d, s->ncp, df, U, d, s->V[t & 1].h, s->ncp,
-                                    s->V[(t + 1) & 1].h, s->ncp);
+            // V <- df*U*V, or df*(V + W*V) in the W form
+            st = ophip_sgemm_skinny_ex(d, d, s->ncp, df, U, d, s->V[t & 1].h, s->ncp,
+                                       s->V[(t + 1) & 1].h, s->ncp, wform ? 1 : 0);
             if (st) return st;
         }
         s->v_final = c->nsteps & 1;
     } else {
-        // U_1 = U; U_k = U_{k-1} * U (CModel Device.cs:269-279)
+        // U_1 = U; U_k = U_{k-1} * U (CModel Device.cs:269-279); in the W
+        // form W_k = W_{k-1} + W + W_{k-1} * W
         unsigned prev = U;
         for (int k = 1; k <= c->nsteps; k++) {
             unsigned cur = U;
             if (k > 1) {
                 cur = s->Uk[k & 1].h;
-                st = ophip_sgemm_strided_batched(d, d, d, 1.0f, prev, d, 0, U, d, 0, 0.0f, cur, d,
-                                                 0, 1);
+                float beta = 0.0f;
+                if (wform) {
+                    st = ophip_saxpby(d * d, 1.0f, prev, 1.0f, U, cur);
+                    if (st) return st;
+                    beta = 1.0f;
+                }
+                opcuda_sgemm(d, d, d, 1.0f, prev, d, U, d, beta, cur, d);
+                st = (int)opcuda_get_status();
                 if (st) return st;
             }
             const float df = (float)std::exp(-c->rate * c->DeltaT * k);
             for (int r = 0; r < nr; r++) {
                 // row y0 of U_k is the vector at element offset y0, stride d
-                const unsigned row = cur + 4u * (unsigned)(s->x0 + nx * r);
+                const int y0 = s->x0 + nx * r;
+                const unsigned row = cur + 4u * (unsigned)y0;
                 const unsigned dst =
                     s->out.h + 4u * (unsigned)(((k - 1) * nr + r) * c->nstrikes);
-                opcuda_sgemv(c->nstrikes, d, df, s->payT.h, c->nstrikes, row, d, 0.0f, dst, 1);
+                float beta = 0.0f;
+                if (wform) {
+                    // the identity's row: the payoffs at y0 themselves
+                    opcuda_scopy(c->nstrikes, s->payT.h + 4u * (unsigned)(c->nstrikes * y0), 1,
+                                 dst, 1);
+                    beta = df;
+                }
+                opcuda_sgemv(c->nstrikes, d, df, s->payT.h, c->nstrikes, row, d, beta, dst, 1);
                 st = (int)opcuda_get_status();
                 if (st) return st;
             }
@@ -304,17 +383,205 @@ int opm_prices(opm_solver *s, double *prices) {
     return 0;
 }
 
+// U(DeltaT) in FP32; in the W form U = I + W is formed on the host
 int opm_kernel(opm_solver *s, float *U) {
-    if (!s) return 1;
+    if (!s || !s->kernel_h) return 1;
     opcuda_thread_synchronize();
-    return opcuda_memcpy_d2h(s->U[s->u_final].h, (unsigned long)(uintptr_t)U,
+    const int d = s->d;
+    int st = opcuda_memcpy_d2h(s->kernel_h, (unsigned long)(uintptr_t)U,
+                               (unsigned)(4 * (size_t)d * d));
+    if (st || s->wmode < 0) return st;
+    for (int i = 0; i < d; i++) U[(size_t)i * d + i] += 1.0f;
+    return 0;
+}
+
+// the raw squared kernel as held on the device (U, or W in the W form)
+int opm_kernel_raw(opm_solver *s, float *K, int *is_w) {
+    if (!s || !s->kernel_h) return 1;
+    opcuda_thread_synchronize();
+    *is_w = s->wmode >= 0;
+    return opcuda_memcpy_d2h(s->kernel_h, (unsigned long)(uintptr_t)K,
                              (unsigned)(4 * (size_t)s->d * s->d));
 }
+
+unsigned opm_kernel_handle(opm_solver *s) { return s ? s->kernel_h : 0; }
 
 int opm_dims(opm_solver *s, int *d, int *n_mat) {
     if (!s) return 1;
     *d = s->d;
     *n_mat = s->n_mat;
+    return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------
+// calibration sweep (see opmodel.h)
+// ---------------------------------------------------------------------
+
+struct opm_batch {
+    int n = 0, nmat = 0, nstrikes = 0, wmode = -1, rounds = 0;
+    std::vector<opm_config> cfg;
+    std::vector<int> nsq;
+    std::vector<int> d, x0;
+    std::vector<double> dt0;
+    std::vector<Buf> x, drift, vol, x32, P, out, state, U1;  // U1: plain path ping-pong partner
+    unsigned long wplan = 0, fplan = 0, graph = 0;
+};
+
+extern "C" {
+
+void opm_batch_destroy(opm_batch *b) {
+    if (!b) return;
+    if (b->graph) ophip_graph_destroy(b->graph);
+    if (b->wplan) ophip_wsq_plan_free(b->wplan);
+    if (b->fplan) ophip_fwd_plan_free(b->fplan);
+    for (auto *v : {&b->x, &b->drift, &b->vol, &b->x32, &b->P, &b->out, &b->state, &b->U1})
+        for (Buf &q : *v) release(q);
+    delete b;
+}
+
+opm_batch *opm_batch_create(const opm_config *cfgs, int n, int tile) {
+    if (!cfgs || n <= 0) return nullptr;
+    if (opcuda_cublas_init() != 0) return nullptr;
+    const opm_config &c0 = cfgs[0];
+    for (int i = 0; i < n; i++) {
+        const opm_config &c = cfgs[i];
+        if (!valid(&c) || c.nr != 1 || c.precision != c0.precision ||
+            c.nsteps != c0.nsteps || c.DeltaT != c0.DeltaT || c.nstrikes != c0.nstrikes ||
+            std::memcmp(c.strikes, c0.strikes, sizeof(double) * c.nstrikes) != 0)
+            return nullptr;
+        if (c.precision != OPM_PREC_FP32 && c.nx % 128) return nullptr;
+    }
+    opm_batch *b = new opm_batch();
+    b->n = n;
+    b->nsq.resize(n);
+    for (int i = 0; i < n; i++) {
+        b->nsq[i] = cfgs[i].nsq;
+        b->rounds = std::max(b->rounds, cfgs[i].nsq);
+    }
+    b->nmat = c0.nsteps;
+    b->nstrikes = c0.nstrikes;
+    b->wmode = c0.precision == OPM_PREC_FP32 ? -1 : c0.precision - 1;
+    b->cfg.assign(cfgs, cfgs + n);
+    for (auto *v : {&b->x, &b->drift, &b->vol, &b->x32, &b->P, &b->out, &b->state, &b->U1})
+        v->resize(n);
+    b->d.resize(n);
+    b->x0.resize(n);
+    b->dt0.resize(n);
+    bool ok = true;
+    std::vector<unsigned> Kh(n), Ph(n), xh(n), oh(n);
+    std::vector<int> plus(n, b->wmode >= 0 ? 1 : 0);
+    std::vector<float> rate(n);
+    for (int i = 0; i < n && ok; i++) {
+        const opm_config *c = &b->cfg[i];
+        const int d = c->nx;
+        b->d[i] = d;
+        b->dt0[i] = std::ldexp(c->DeltaT, -c->nsq);
+        std::vector<double> x, dr, vo;
+        make_grid(c, x, b->x0[i]);
+        make_params(c, x, dr, vo);
+        const size_t dd = (size_t)d * d;
+        ok = alloc(b->x[i], 8 * (size_t)d) && alloc(b->drift[i], 8 * (size_t)d) &&
+             alloc(b->vol[i], 8 * (size_t)d) && alloc(b->x32[i], 4 * (size_t)d) &&
+             alloc(b->P[i], 4 * (size_t)(b->nmat + 1) * d) &&
+             alloc(b->out[i], 4 * (size_t)b->nmat * b->nstrikes);
+        if (ok && b->wmode >= 0) ok = alloc(b->state[i], ophip_wsq_bytes(d, b->wmode));
+        if (ok && b->wmode < 0) ok = alloc(b->state[i], 4 * dd) && alloc(b->U1[i], 4 * dd);
+        if (!ok) break;
+        std::vector<float> xf(x.begin(), x.end()), p0((size_t)d, 0.0f);
+        p0[b->x0[i]] = 1.0f;  // the chain starts at the spot node
+        ok = !upload(b->x[i], x.data(), 8 * (size_t)d) && !upload(b->drift[i], dr.data(), 8 * (size_t)d) &&
+             !upload(b->vol[i], vo.data(), 8 * (size_t)d) && !upload(b->x32[i], xf.data(), 4 * (size_t)d) &&
+             !upload(b->P[i], p0.data(), 4 * (size_t)d);
+        Ph[i] = b->P[i].h;
+        xh[i] = b->x32[i].h;
+        oh[i] = b->out[i].h;
+        rate[i] = (float)c->rate;
+        if (b->wmode >= 0) Kh[i] = ophip_wsq_w(b->state[i].h, d, b->wmode, b->nsq[i] & 1);
+        else Kh[i] = (b->nsq[i] & 1) ? b->U1[i].h : b->state[i].h;
+    }
+    if (ok && b->wmode >= 0) {
+        std::vector<unsigned> st(n);
+        for (int i = 0; i < n; i++) st[i] = b->state[i].h;
+        b->wplan = ophip_wsq_plan(n, st.data(), b->d.data(), b->nsq.data(), b->wmode, tile, 0);
+        ok = b->wplan != 0;
+    }
+    if (ok) {
+        std::vector<float> ks(c0.strikes, c0.strikes + c0.nstrikes);
+        b->fplan = ophip_fwd_plan(n, b->d.data(), Kh.data(), plus.data(), Ph.data(), xh.data(),
+                                  oh.data(), rate.data(), (float)c0.DeltaT, b->nmat, ks.data(),
+                                  b->nstrikes);
+        ok = b->fplan != 0;
+    }
+    if (!ok) {
+        opm_batch_destroy(b);
+        return nullptr;
+    }
+    return b;
+}
+
+static int batch_body(opm_batch *b) {
+    int st;
+    for (int i = 0; i < b->n; i++) {
+        const opm_config *c = &b->cfg[i];
+        const unsigned dst = b->wmode >= 0 ? ophip_wsq_w(b->state[i].h, b->d[i], b->wmode, 0)
+                                           : b->state[i].h;
+        st = ophip_sek_rs(c->nx, 1, b->x[i].h, b->drift[i].h, b->vol[i].h, 0.0, b->dt0[i],
+                          b->wmode >= 0 ? 2 : 0, dst);
+        if (st) return st;
+    }
+    if (b->wmode >= 0) {
+        st = ophip_wsq_exec(b->wplan, -1);
+        if (st) return st;
+    } else {
+        for (int r = 0; r < b->rounds; r++)
+            for (int i = 0; i < b->n; i++) {
+                if (r >= b->nsq[i]) continue;
+                const int d = b->d[i];
+                const unsigned a = (r & 1) ? b->U1[i].h : b->state[i].h;
+                const unsigned o = (r & 1) ? b->state[i].h : b->U1[i].h;
+                st = ophip_sgemm_strided_batched(d, d, d, 1.0f, a, d, 0, a, d, 0, 0.0f, o, d, 0, 1);
+                if (st) return st;
+            }
+    }
+    return ophip_fwd_exec(b->fplan);
+}
+
+int opm_batch_solve(opm_batch *b) { return b ? batch_body(b) : 1; }
+
+int opm_batch_capture(opm_batch *b) {
+    if (!b) return 1;
+    if (b->graph) {
+        ophip_graph_destroy(b->graph);
+        b->graph = 0;
+    }
+    ophip_reserve_workspace((unsigned long)64 << 20);
+    int st = ophip_capture_begin();
+    if (st) return st;
+    int body = batch_body(b);
+    unsigned long g = ophip_capture_end();
+    if (body) {
+        if (g) ophip_graph_destroy(g);
+        return body;
+    }
+    if (!g) return (int)opcuda_get_status();
+    b->graph = g;
+    return 0;
+}
+
+int opm_batch_launch(opm_batch *b) { return (b && b->graph) ? ophip_graph_launch(b->graph) : 1; }
+
+int opm_batch_prices(opm_batch *b, double *prices) {
+    if (!b) return 1;
+    opcuda_thread_synchronize();
+    std::vector<float> o((size_t)b->nmat * b->nstrikes);
+    for (int i = 0; i < b->n; i++) {
+        int st = opcuda_memcpy_d2h(b->out[i].h, (unsigned long)(uintptr_t)o.data(),
+                                   (unsigned)(4 * o.size()));
+        if (st) return st;
+        for (size_t e = 0; e < o.size(); e++) prices[(size_t)i * o.size() + e] = o[e];
+    }
     return 0;
 }
 

@@ -392,6 +392,25 @@ OPHIP_EXPORT void opcuda_ssetone(unsigned x, int n, float c, int i) {
     check_launch();
 }
 
+__global__ void k_axpby(size_t n, float a, const float *__restrict__ A, float b,
+                        const float *__restrict__ B, float *__restrict__ C) {
+    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < n;
+         j += (size_t)gridDim.x * blockDim.x)
+        C[j] = a * A[j] + b * B[j];
+}
+
+OPHIP_EXPORT int ophip_saxpby(int n, float a, unsigned A, float b, unsigned B, unsigned C) {
+    Device &d = dev();
+    if (n < 0) return (int)set_status(kErrBadArg);
+    if (n == 0) return (int)set_status(kOk);
+    if (!handle_range_ok(A, (size_t)n * 4) || !handle_range_ok(B, (size_t)n * 4) ||
+        !handle_range_ok(C, (size_t)n * 4))
+        return (int)set_status(kErrBadHandle);
+    hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256)), dim3(256), 0, d.stream, (size_t)n, a,
+                       P<const float>(A), b, P<const float>(B), P<float>(C));
+    return (int)check_launch();
+}
+
 // ---------------------------------------------------------------------
 // extensions: raw device address of a handle (for zero-copy interop with
 // torch tensors allocated elsewhere) and arena statistics

@@ -1,0 +1,587 @@
+// wsq.hip -- repeated squaring in the U = I + W form with per-step
+// probability-mass renormalisation, on FP16/BF16 matrix cores with FP32
+// accumulation (no reference counterpart: the reference squares U itself
+// in FP32, ssqmm.cu:119-170).
+//
+// Why the W form.  U(dt) = I + W with |W| ~ |L| dt small.  In FP32 the
+// diagonal 1 - eps of U keeps only ~2^-24/eps relative precision of the
+// mass that leaves a state, and every squaring doubles the row-sum error;
+// after 16 squarings and 252 backward steps the reference's FP32 method is
+// off by ~10% at the 2048-state lattice.  Squaring W instead,
+//     (I + W)^2 = I + (2W + W*W),
+// keeps full relative precision of the small entries; the product W*W is
+// the only GEMM and is rounded relative to |W|^2.  After each squaring the
+// diagonal is corrected so every row of W sums to exactly the FP64 sum of
+// its off-diagonal part with opposite sign (rows of U sum to 1).
+//
+// Operands.  W is kept in FP32 (column-major) and as 16-bit copies scaled by
+// 2^S (S = 15 for FP16 so entries <= 1 stay normal down to 2^-24/2^15;
+// S = 0 for BF16): H = rnd(W 2^S) and, for the three-product variant,
+// L = rnd((W 2^S - H) 2^G) (G = 11 FP16, 8 BF16).  The product is
+//     W*W ~ 2^-2S (H*H + 2^-G (H*L + L*H)).
+// The A operand is read from a row-major copy and the B operand from a
+// column-major copy so that both feed the MFMA with 8 consecutive k per lane.
+//
+// Kernels (all tiles 128x128 or 64x64, every d a multiple of the tile):
+//   k_wsplit   FP32 W -> 16-bit copies (both layouts) + partial row sums
+//   k_wsq      W' = 2W + W*W on v_mfma_f32_16x16x32_{f16,bf16}; epilogue
+//              writes W' (FP32), its 16-bit copies and partial row sums
+//   k_wfix     diagonal correction from the partial row sums (fixed order)
+// A scenario table lets one launch serve many lattices of different d.
+#include "ophip_internal.h"
+
+#include <algorithm>
+#include <type_traits>
+#include <vector>
+
+using namespace ophip;
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+typedef short s8 __attribute__((ext_vector_type(8)));
+
+namespace {
+
+// one scenario (lattice) of a batched W-form squaring round
+struct WScen {
+    int d;
+    int pad;
+    const float *W;    // FP32 W, column-major (input)
+    float *Wn;         // FP32 W' (output)
+    const uint16_t *Hr, *Hc, *Lr, *Lc;  // 16-bit copies of W (row-/column-major)
+    uint16_t *Hrn, *Hcn, *Lrn, *Lcn;    // copies of W' (outputs)
+    double *part;      // partial row sums of W': [d/64 column blocks][d]
+};
+
+// work item = (scenario, tile)
+struct WItem {
+    int s, tm, tn, pad;
+};
+
+template <bool BF> struct Num;
+template <> struct Num<false> {
+    static constexpr float S = 32768.0f, IS = 1.0f / 32768.0f;
+    static constexpr float G = 2048.0f, IG = 1.0f / 2048.0f;
+    __device__ static uint16_t enc(float x) {
+        _Float16 h = (_Float16)x;
+        return __builtin_bit_cast(uint16_t, h);
+    }
+    __device__ static float dec(uint16_t u) { return (float)__builtin_bit_cast(_Float16, u); }
+    __device__ static f4 mfma(s8 a, s8 b, f4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, a),
+                                                       __builtin_bit_cast(h8, b), c, 0, 0, 0);
+    }
+};
+template <> struct Num<true> {
+    static constexpr float S = 1.0f, IS = 1.0f;
+    static constexpr float G = 256.0f, IG = 1.0f / 256.0f;
+    __device__ static uint16_t enc(float x) {
+        __bf16 h = (__bf16)x;
+        return __builtin_bit_cast(uint16_t, h);
+    }
+    __device__ static float dec(uint16_t u) { return (float)__builtin_bit_cast(__bf16, u); }
+    __device__ static f4 mfma(s8 a, s8 b, f4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8, a),
+                                                        __builtin_bit_cast(b8, b), c, 0, 0, 0);
+    }
+};
+
+// ---------------------------------------------------------------------
+// k_wsplit: 64x64 tile per workgroup (64 threads), LDS transpose
+// ---------------------------------------------------------------------
+template <bool BF, bool LO>
+__global__ __launch_bounds__(64) void k_wsplit(const WScen *__restrict__ scen,
+                                               const WItem *__restrict__ items) {
+    using N = Num<BF>;
+    __shared__ uint16_t th[64][66], tl[64][66];
+    const WItem it = items[blockIdx.x];
+    const WScen sc = scen[it.s];
+    const int d = sc.d, lane = threadIdx.x;
+    const int i0 = it.tm * 64, j0 = it.tn * 64;
+    const int i = i0 + lane;
+    double rs = 0.0;
+    for (int c = 0; c < 64; c++) {
+        const size_t e = (size_t)i + (size_t)d * (j0 + c);
+        float w = sc.W[e];
+        rs += (double)w;
+        float ws = w * N::S;
+        uint16_t h = N::enc(ws);
+        sc.Hcn[e] = h;
+        th[lane][c] = h;
+        if (LO) {
+            uint16_t l = N::enc((ws - N::dec(h)) * N::G);
+            sc.Lcn[e] = l;
+            tl[lane][c] = l;
+        }
+    }
+    sc.part[(size_t)it.tn * d + i] = rs;  // slot = 64-column block
+    __syncthreads();
+    // row-major copies: row r of the tile, lanes over columns
+    for (int r = 0; r < 64; r++) {
+        const size_t e = (size_t)(i0 + r) * d + j0 + lane;
+        sc.Hrn[e] = th[r][lane];
+        if (LO) sc.Lrn[e] = tl[r][lane];
+    }
+}
+
+// ---------------------------------------------------------------------
+// k_wsq: W' = 2W + W*W.  Workgroup tile BT x BT (BT = 128: 4 waves of
+// 64x64; BT = 64: 4 waves of 32x32), K-step 32.
+// ---------------------------------------------------------------------
+template <bool BF, bool LO, int BT>
+__global__ __launch_bounds__(256) void k_wsq(const WScen *__restrict__ scen,
+                                             const WItem *__restrict__ items, int nitems) {
+    using N = Num<BF>;
+    constexpr int BK = 32, PITCH = BK + 8;  // 16-bit elements per LDS row
+    constexpr int WT = BT / 2, T = WT / 16;  // wave tile, MFMA tiles per side
+    constexpr int NOP = LO ? 2 : 1;
+    constexpr int LD16 = BT * BK / 8 / 256;  // 16-byte loads per thread per operand array
+    static_assert(LD16 >= 1, "tile");
+    __shared__ __attribute__((aligned(16))) uint16_t As[2][NOP][BT * PITCH];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][NOP][BT * PITCH];
+    __shared__ double rsum[2][BT];
+
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int wm0 = (wave & 1) * WT, wn0 = (wave >> 1) * WT;
+
+    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+        const WItem it = items[item];
+        const WScen sc = scen[it.s];
+        const int d = sc.d;
+        const int i0 = it.tm * BT, j0 = it.tn * BT;
+
+        f4 acc[T][T], cor[T][T];
+#pragma unroll
+        for (int a = 0; a < T; a++)
+#pragma unroll
+            for (int b = 0; b < T; b++) {
+                acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+                cor[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+            }
+
+        uint4 ra[NOP][LD16], rb[NOP][LD16];
+        auto gload = [&](int k0) {
+#pragma unroll
+            for (int q = 0; q < LD16; q++) {
+                const int idx = t + 256 * q;
+                const int row = idx >> 2, part = idx & 3;  // 4 x 16 B per 32-element row
+                const size_t ea = (size_t)(i0 + row) * d + k0 + 8 * part;  // row-major A
+                const size_t eb = (size_t)(j0 + row) * d + k0 + 8 * part;  // column-major B
+                ra[0][q] = *(const uint4 *)(sc.Hr + ea);
+                rb[0][q] = *(const uint4 *)(sc.Hc + eb);
+                if (LO) {
+                    ra[NOP - 1][q] = *(const uint4 *)(sc.Lr + ea);
+                    rb[NOP - 1][q] = *(const uint4 *)(sc.Lc + eb);
+                }
+            }
+        };
+        auto sstore = [&](int buf) {
+#pragma unroll
+            for (int q = 0; q < LD16; q++) {
+                const int idx = t + 256 * q;
+                const int row = idx >> 2, part = idx & 3;
+#pragma unroll
+                for (int o = 0; o < NOP; o++) {
+                    *(uint4 *)&As[buf][o][row * PITCH + 8 * part] = ra[o][q];
+                    *(uint4 *)&Bs[buf][o][row * PITCH + 8 * part] = rb[o][q];
+                }
+            }
+        };
+
+        const int nk = d / BK;
+        gload(0);
+        __syncthreads();  // previous item's readers of LDS are done
+        sstore(0);
+        __syncthreads();
+        for (int kt = 0; kt < nk; kt++) {
+            const int buf = kt & 1;
+            if (kt + 1 < nk) gload((kt + 1) * BK);
+            s8 fa[NOP][T], fb[NOP][T];
+#pragma unroll
+            for (int o = 0; o < NOP; o++)
+#pragma unroll
+                for (int a = 0; a < T; a++) {
+                    fa[o][a] = *(const s8 *)&As[buf][o][(wm0 + 16 * a + r) * PITCH + 8 * g];
+                    fb[o][a] = *(const s8 *)&Bs[buf][o][(wn0 + 16 * a + r) * PITCH + 8 * g];
+                }
+#pragma unroll
+            for (int b = 0; b < T; b++)
+#pragma unroll
+                for (int a = 0; a < T; a++) {
+                    acc[a][b] = N::mfma(fa[0][a], fb[0][b], acc[a][b]);
+                    if (LO) {
+                        cor[a][b] = N::mfma(fa[0][a], fb[1][b], cor[a][b]);
+                        cor[a][b] = N::mfma(fa[1][a], fb[0][b], cor[a][b]);
+                    }
+                }
+            if (kt + 1 < nk) sstore(buf ^ 1);
+            __syncthreads();
+        }
+
+        // epilogue: lane holds rows 4g+v of column r of every 16x16 tile
+        const float inv = N::IS * N::IS;
+        double rows[T][4];
+#pragma unroll
+        for (int a = 0; a < T; a++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) rows[a][v] = 0.0;
+#pragma unroll
+        for (int a = 0; a < T; a++) {
+            const int row = i0 + wm0 + 16 * a + 4 * g;
+#pragma unroll
+            for (int b = 0; b < T; b++) {
+                const int col = j0 + wn0 + 16 * b + r;
+                const size_t e = (size_t)row + (size_t)d * col;
+                f4 w = *(const f4 *)(sc.W + e);
+                f4 p = acc[a][b];
+                if (LO) p += cor[a][b] * N::IG;
+                f4 o = 2.0f * w + p * inv;
+                *(f4 *)(sc.Wn + e) = o;
+                uint16_t hc[4], lc[4];
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    rows[a][v] += (double)o[v];
+                    float os = o[v] * N::S;
+                    hc[v] = N::enc(os);
+                    if (LO) lc[v] = N::enc((os - N::dec(hc[v])) * N::G);
+                    sc.Hrn[(size_t)(row + v) * d + col] = hc[v];
+                    if (LO) sc.Lrn[(size_t)(row + v) * d + col] = lc[v];
+                }
+                uint2 ph, pl;
+                ph.x = (uint32_t)hc[0] | ((uint32_t)hc[1] << 16);
+                ph.y = (uint32_t)hc[2] | ((uint32_t)hc[3] << 16);
+                *(uint2 *)(sc.Hcn + e) = ph;
+                if (LO) {
+                    pl.x = (uint32_t)lc[0] | ((uint32_t)lc[1] << 16);
+                    pl.y = (uint32_t)lc[2] | ((uint32_t)lc[3] << 16);
+                    *(uint2 *)(sc.Lcn + e) = pl;
+                }
+            }
+        }
+        // row sums over this wave's WT columns: reduce across the 16 lanes r
+#pragma unroll
+        for (int a = 0; a < T; a++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                double x = rows[a][v];
+                x += __shfl_xor(x, 1, 64);
+                x += __shfl_xor(x, 2, 64);
+                x += __shfl_xor(x, 4, 64);
+                x += __shfl_xor(x, 8, 64);
+                rows[a][v] = x;
+            }
+        if (r == 0) {
+#pragma unroll
+            for (int a = 0; a < T; a++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) rsum[wave >> 1][wm0 + 16 * a + 4 * g + v] = rows[a][v];
+        }
+        __syncthreads();
+        // one partial slot per 64-column block: with BT = 128 each wave
+        // column is one block; with BT = 64 the two 32-wide wave columns
+        // make one block
+        if (t < BT) {
+            const int row = i0 + t;
+            if (BT == 128) {
+                sc.part[(size_t)(2 * it.tn + 0) * d + row] = rsum[0][t];
+                sc.part[(size_t)(2 * it.tn + 1) * d + row] = rsum[1][t];
+            } else {
+                sc.part[(size_t)it.tn * d + row] = rsum[0][t] + rsum[1][t];
+            }
+        }
+    }
+}
+
+// diagonal correction: W'_ii -= sum_j W'_ij (fixed summation order)
+template <bool BF, bool LO>
+__global__ void k_wfix(const WScen *__restrict__ scen, int nscen, const int *__restrict__ nsq,
+                       int round) {
+    using N = Num<BF>;
+    const int s = blockIdx.y;
+    if (s >= nscen) return;
+    if (nsq && round >= nsq[s]) return;  // this lattice has finished its squarings
+    const WScen sc = scen[s];
+    const int d = sc.d;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d) return;
+    const int nslots = d / 64;
+    double rs = 0.0;
+    for (int q = 0; q < nslots; q++) rs += sc.part[(size_t)q * d + i];
+    const size_t e = (size_t)i + (size_t)d * i;
+    const float w = (float)((double)sc.Wn[e] - rs);
+    sc.Wn[e] = w;
+    const float ws = w * N::S;
+    const uint16_t h = N::enc(ws);
+    sc.Hcn[e] = h;
+    sc.Hrn[e] = h;  // the diagonal sits at the same offset in both layouts
+    if (LO) {
+        const uint16_t l = N::enc((ws - N::dec(h)) * N::G);
+        sc.Lcn[e] = l;
+        sc.Lrn[e] = l;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+// C-ABI.  A W-form squaring state for one lattice lives in one arena
+// allocation laid out by ophip_wsq_layout; ophip_wsq_* drive it.
+// ---------------------------------------------------------------------
+
+// mode: 0 = FP16 one product, 1 = FP16 three products (hi/lo),
+//       2 = BF16 three products
+struct WsqLayout {
+    size_t W0, W1, H[2][4], part, total;  // byte offsets; H[buf][Hr,Hc,Lr,Lc]
+};
+
+static WsqLayout wsq_layout(int d, int mode) {
+    WsqLayout L{};
+    size_t dd = (size_t)d * d, off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    L.W0 = take(4 * dd);
+    L.W1 = take(4 * dd);
+    int narr = mode == 0 ? 2 : 4;
+    for (int b = 0; b < 2; b++)
+        for (int a = 0; a < 4; a++) L.H[b][a] = a < narr ? take(2 * dd) : (size_t)-1;
+    L.part = take(8 * (size_t)(d / 64) * d);
+    L.total = off;
+    return L;
+}
+
+// bytes of device state for one lattice of size d (d % 128 == 0)
+OPHIP_EXPORT unsigned long ophip_wsq_bytes(int d, int mode) {
+    if (d <= 0 || d % 64 || mode < 0 || mode > 2) return 0;
+    return (unsigned long)wsq_layout(d, mode).total;
+}
+
+namespace {
+
+WScen make_scen(const char *base, unsigned state, int d, int mode, int src) {
+    WsqLayout L = wsq_layout(d, mode);
+    const char *p = base + state;
+    WScen s{};
+    s.d = d;
+    const int dst = src ^ 1;
+    s.W = (const float *)(p + (src ? L.W1 : L.W0));
+    s.Wn = (float *)(p + (dst ? L.W1 : L.W0));
+    s.Hr = (const uint16_t *)(p + L.H[src][0]);
+    s.Hc = (const uint16_t *)(p + L.H[src][1]);
+    s.Hrn = (uint16_t *)(p + L.H[dst][0]);
+    s.Hcn = (uint16_t *)(p + L.H[dst][1]);
+    if (mode != 0) {
+        s.Lr = (const uint16_t *)(p + L.H[src][2]);
+        s.Lc = (const uint16_t *)(p + L.H[src][3]);
+        s.Lrn = (uint16_t *)(p + L.H[dst][2]);
+        s.Lcn = (uint16_t *)(p + L.H[dst][3]);
+    }
+    s.part = (double *)(p + L.part);
+    return s;
+}
+
+constexpr int kMaxScen = 4096;
+
+template <class F> unsigned launch_mode(int mode, F f) {
+    switch (mode) {
+    case 0: f(std::integral_constant<bool, false>(), std::integral_constant<bool, false>()); break;
+    case 1: f(std::integral_constant<bool, false>(), std::integral_constant<bool, true>()); break;
+    default: f(std::integral_constant<bool, true>(), std::integral_constant<bool, true>()); break;
+    }
+    return check_launch();
+}
+
+}  // namespace
+
+// A plan holds the device-side scenario and work tables of one batch of
+// lattices (built once, outside any graph capture); executing it only
+// launches kernels, so it can be captured into a HIP graph.  Lattices may
+// need different numbers of squarings (the reference planner gives each
+// kernel its own niter, CModel Common.cs:289-307): round r squares only
+// the lattices with nsq > r, as the reference's njtodo_r tables do
+// (CModel Common.cs:508-545, Device.cs:222-224).
+struct WsqPlan {
+    int n = 0, mode = 0, tile = 128, grid = 0, dmax = 0, rounds = 0;
+    std::vector<int> nsq;
+    int n_split = 0;
+    std::vector<int> n_round;       // items per round (per-lattice counts)
+    int n_all[2] = {0, 0};          // items per parity (uniform count)
+    void *dev = nullptr;            // one hipMalloc: tables below
+    WScen *sc = nullptr, *sc_split = nullptr;
+    int *nsq_dev = nullptr;
+    WItem *it_split = nullptr, *it_all[2] = {nullptr, nullptr};
+    std::vector<WItem *> it_round;
+};
+
+// Plan the W-form pipeline for `n` lattices (states[i]: arena handle of a
+// state of ophip_wsq_bytes(ds[i], mode) bytes, ds[i] % 128 == 0; nsqs[i]:
+// its number of squarings, or null for a count given at exec time).
+// tile: 128 or 64 (0 = choose by the number of 128-tiles); grid: persistent
+// workgroups per squaring launch (0 = 2 per CU).  Work items of all
+// lattices are interleaved tile-major so uneven sizes balance.
+OPHIP_EXPORT unsigned long ophip_wsq_plan(int n, const unsigned *states, const int *ds,
+                                          const int *nsqs, int mode, int tile, int grid) {
+    Device &D = dev();
+    if (n <= 0 || n > kMaxScen || mode < 0 || mode > 2) { set_status(kErrBadArg); return 0; }
+    for (int i = 0; i < n; i++) {
+        if (ds[i] <= 0 || ds[i] % 128 || !ptr(states[i]) || (nsqs && (nsqs[i] < 0 || nsqs[i] > 64)) ||
+            !handle_range_ok(states[i], wsq_layout(ds[i], mode).total)) {
+            set_status(kErrBadArg);
+            return 0;
+        }
+    }
+    const int ncu = D.n_cu > 0 ? D.n_cu : 256;
+    if (tile == 0) {
+        long t128 = 0;
+        for (int i = 0; i < n; i++) t128 += (long)(ds[i] / 128) * (ds[i] / 128);
+        tile = (t128 >= (long)ncu) ? 128 : 64;
+    }
+    if (tile != 64 && tile != 128) { set_status(kErrBadArg); return 0; }
+    WsqPlan *P = new WsqPlan();
+    P->n = n;
+    P->mode = mode;
+    P->tile = tile;
+    P->grid = grid > 0 ? grid : 2 * ncu;
+    P->nsq.assign(n, 0);
+    if (nsqs)
+        for (int i = 0; i < n; i++) {
+            P->nsq[i] = nsqs[i];
+            P->rounds = std::max(P->rounds, nsqs[i]);
+        }
+    std::vector<WScen> sc(2 * n), sc_split(n);
+    for (int i = 0; i < n; i++) {
+        sc[i] = make_scen(D.base, states[i], ds[i], mode, 0);
+        sc[n + i] = make_scen(D.base, states[i], ds[i], mode, 1);
+        // the split reads W0 and writes buffer 0's copies and diagonal
+        sc_split[i] = make_scen(D.base, states[i], ds[i], mode, 1);
+        sc_split[i].W = sc[i].W;
+        P->dmax = std::max(P->dmax, ds[i]);
+    }
+    auto items_for = [&](int T, int dir, int round) {
+        std::vector<WItem> v;
+        const int maxt = P->dmax / T;
+        for (int tm = 0; tm < maxt; tm++)
+            for (int tn = 0; tn < maxt; tn++)
+                for (int i = 0; i < n; i++)
+                    if (tm < ds[i] / T && tn < ds[i] / T && (round < 0 || P->nsq[i] > round))
+                        v.push_back(WItem{dir * n + i, tm, tn, 0});
+        return v;
+    };
+    std::vector<std::vector<WItem>> lists;
+    lists.push_back(items_for(64, 0, -1));
+    lists.push_back(items_for(tile, 0, -1));
+    lists.push_back(items_for(tile, 1, -1));
+    for (int r = 0; r < P->rounds; r++) lists.push_back(items_for(tile, r & 1, r));
+    size_t nitems = 0;
+    for (auto &l : lists) nitems += l.size();
+    size_t bytes = sizeof(WScen) * 3 * n + sizeof(int) * (n + 4) + sizeof(WItem) * nitems;
+    if (hipMalloc(&P->dev, bytes) != hipSuccess) {
+        delete P;
+        set_status(kErrOutOfArena);
+        return 0;
+    }
+    char *p = (char *)P->dev;
+    P->sc = (WScen *)p;
+    P->sc_split = P->sc + 2 * n;
+    P->nsq_dev = (int *)(P->sc_split + n);
+    WItem *cur = (WItem *)(P->nsq_dev + n + 4);
+    std::vector<WItem *> where;
+    for (auto &l : lists) {
+        where.push_back(cur);
+        cur += l.size();
+    }
+    P->it_split = where[0];
+    P->n_split = (int)lists[0].size();
+    P->it_all[0] = where[1];
+    P->n_all[0] = (int)lists[1].size();
+    P->it_all[1] = where[2];
+    P->n_all[1] = (int)lists[2].size();
+    for (int r = 0; r < P->rounds; r++) {
+        P->it_round.push_back(where[3 + r]);
+        P->n_round.push_back((int)lists[3 + r].size());
+    }
+    hipError_t e = hipMemcpy(P->sc, sc.data(), sizeof(WScen) * 2 * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(P->sc_split, sc_split.data(), sizeof(WScen) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(P->nsq_dev, P->nsq.data(), sizeof(int) * n, hipMemcpyHostToDevice);
+    for (size_t l = 0; l < lists.size() && e == hipSuccess; l++)
+        if (!lists[l].empty())
+            e = hipMemcpy(where[l], lists[l].data(), sizeof(WItem) * lists[l].size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(P->dev);
+        delete P;
+        set_status(hip_status(e));
+        return 0;
+    }
+    set_status(kOk);
+    return (unsigned long)(uintptr_t)P;
+}
+
+OPHIP_EXPORT void ophip_wsq_plan_free(unsigned long plan) {
+    WsqPlan *P = (WsqPlan *)(uintptr_t)plan;
+    if (!P) return;
+    hipStreamSynchronize(dev().stream);
+    hipFree(P->dev);
+    delete P;
+}
+
+// Split + diagonal fix of W0 (buffer 0, written by ophip_sek_rs out_kind
+// 2), then the squarings, each followed by the diagonal fix.  nsq >= 0:
+// every lattice squares nsq times; nsq < 0: the per-lattice counts given
+// to the plan.  Lattice i ends in buffer (its count) & 1.  Kernel launches
+// only.
+OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
+    Device &D = dev();
+    WsqPlan *P = (WsqPlan *)(uintptr_t)plan;
+    if (!P) return (int)set_status(kErrBadArg);
+    const bool per = nsq < 0;
+    const int rounds = per ? P->rounds : nsq;
+    const dim3 fixgrid(cdiv(P->dmax, 256), P->n);
+    return (int)launch_mode(P->mode, [&](auto bf, auto lo) {
+        constexpr bool BF = decltype(bf)::value, LO = decltype(lo)::value;
+        hipLaunchKernelGGL((k_wsplit<BF, LO>), dim3(P->n_split), dim3(64), 0, D.stream,
+                           P->sc_split, P->it_split);
+        hipLaunchKernelGGL((k_wfix<BF, LO>), fixgrid, dim3(256), 0, D.stream, P->sc_split, P->n,
+                           (const int *)nullptr, 0);
+        for (int rr = 0; rr < rounds; rr++) {
+            const int src = rr & 1;
+            const int nit = per ? P->n_round[rr] : P->n_all[src];
+            const WItem *its = per ? P->it_round[rr] : P->it_all[src];
+            if (nit == 0) continue;
+            const int g = nit < P->grid ? nit : P->grid;
+            // work items carry s = src*n + i (the full table); the fix kernel
+            // indexes lattices 0..n-1 of the table half for this direction
+            const WScen *scn = P->sc + (src ? P->n : 0);
+            if (P->tile == 128)
+                hipLaunchKernelGGL((k_wsq<BF, LO, 128>), dim3(g), dim3(256), 0, D.stream, P->sc, its, nit);
+            else
+                hipLaunchKernelGGL((k_wsq<BF, LO, 64>), dim3(g), dim3(256), 0, D.stream, P->sc, its, nit);
+            hipLaunchKernelGGL((k_wfix<BF, LO>), fixgrid, dim3(256), 0, D.stream, scn, P->n,
+                               per ? (const int *)P->nsq_dev : (const int *)nullptr, rr);
+        }
+    });
+}
+
+// handle of the FP32 W buffer `buf` (0/1) inside a state
+OPHIP_EXPORT unsigned ophip_wsq_w(unsigned state, int d, int mode, int buf) {
+    WsqLayout L = wsq_layout(d, mode);
+    return state + (unsigned)(buf ? L.W1 : L.W0);
+}
+
+// handle of one array of a state: which = 0 W (FP32), 1 Hr, 2 Hc, 3 Lr,
+// 4 Lc (16-bit copies, row-/column-major), 5 partial row sums (FP64);
+// 0 for arrays the mode does not have
+OPHIP_EXPORT unsigned ophip_wsq_array(unsigned state, int d, int mode, int buf, int which) {
+    if (d <= 0 || d % 64 || mode < 0 || mode > 2 || buf < 0 || buf > 1 || which < 0 || which > 5)
+        return 0;
+    WsqLayout L = wsq_layout(d, mode);
+    size_t off;
+    if (which == 0) off = buf ? L.W1 : L.W0;
+    else if (which == 5) off = L.part;
+    else off = L.H[buf][which - 1];
+    if (off == (size_t)-1) return 0;
+    return state + (unsigned)off;
+}

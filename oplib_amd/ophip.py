@@ -76,6 +76,21 @@ SIGNATURES = {
     "ophip_sgemm_strided_batched": (I32, [I32, I32, I32, F32, U32, I32, LL, U32, I32, LL, F32,
                                           U32, I32, LL, I32]),
     "ophip_sgemm_skinny": (I32, [I32, I32, I32, F32, U32, I32, U32, I32, U32, I32]),
+    "ophip_sck_rows_ex": (I32, [I32, I32, U32, U32, I32]),
+    "ophip_sgemm_skinny_ex": (I32, [I32, I32, I32, F32, U32, I32, U32, I32, U32, I32, I32]),
+    "ophip_saxpby": (I32, [I32, F32, U32, F32, U32, U32]),
+    "ophip_wsq_bytes": (ULONG, [I32, I32]),
+    "ophip_wsq_w": (U32, [U32, I32, I32, I32]),
+    "ophip_wsq_array": (U32, [U32, I32, I32, I32, I32]),
+    "ophip_wsq_plan": (ULONG, [I32, C.POINTER(U32), C.POINTER(I32), C.POINTER(I32), I32, I32,
+                               I32]),
+    "ophip_wsq_exec": (I32, [ULONG, I32]),
+    "ophip_wsq_plan_free": (V, [ULONG]),
+    "ophip_fwd_plan": (ULONG, [I32, C.POINTER(I32), C.POINTER(U32), C.POINTER(I32),
+                               C.POINTER(U32), C.POINTER(U32), C.POINTER(U32), C.POINTER(F32), F32,
+                               I32, C.POINTER(F32), I32]),
+    "ophip_fwd_exec": (I32, [ULONG]),
+    "ophip_fwd_plan_free": (V, [ULONG]),
     "ophip_mc_set_nrng": (I32, [I32]),
     "ophip_mc1f_rows": (I32, [U32, I32, U32, I32, I32, I32, U32, U32, ULONG]),
     "ophip_reserve_workspace": (I32, [ULONG]),

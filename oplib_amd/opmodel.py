@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(_HERE, "libopmodel.so")
 
 OPM_VOL_REGIME, OPM_VOL_C1, OPM_VOL_CEV = 0, 1, 2
 OPM_PRICE_BACKWARD, OPM_PRICE_PROPAGATE = 0, 1
-OPM_PREC_FP32 = 0
+OPM_PREC_FP32, OPM_PREC_F16, OPM_PREC_F16X3, OPM_PREC_BF16X3 = 0, 1, 2, 3
 
 
 class Config(C.Structure):
@@ -78,6 +78,20 @@ def _load():
 
 
 lib = _load()
+lib.opm_max_rate.argtypes = [C.POINTER(Config)]
+lib.opm_max_rate.restype = C.c_double
+lib.opm_plan_nsq.argtypes = [C.POINTER(Config), C.c_double]
+lib.opm_plan_nsq.restype = C.c_int
+
+
+def max_rate(cfg):
+    return lib.opm_max_rate(C.byref(cfg))
+
+
+def plan_nsq(cfg, dt0_cap):
+    """niter = ceil(log2(DeltaT / min(0.5/max|L_yy|, dt0_cap)))
+    (CModel Common.cs:289-307, in years)"""
+    return lib.opm_plan_nsq(C.byref(cfg), dt0_cap)
 
 
 # ---------------------------------------------------------------------------
@@ -244,3 +258,111 @@ def flops(cfg: Config) -> float:
 
 def df(cfg: Config, t: float) -> float:
     return math.exp(-cfg.rate * t)
+
+
+# ---------------------------------------------------------------------------
+# calibration sweep
+# ---------------------------------------------------------------------------
+
+OPM_PRICE_FORWARD = 2
+SWEEP_SIZES = (384, 512, 768, 1024, 1536)
+
+
+def _mt_uniforms(seed, n):
+    """n uniforms x/2^32 from MT19937 init_genrand(seed) (numpy's legacy
+    RandomState is that generator; its raw 32-bit outputs are the same
+    stream as std::mt19937(seed))."""
+    rs = np.random.RandomState(seed)
+    return rs.randint(0, 2**32, size=n, dtype=np.uint64).astype(np.float64) / 2.0**32
+
+
+def config_sweep(n=64, seed=5489, precision=OPM_PREC_F16X3):
+    """Calibration sweep: n one-regime CEV lattices.
+
+    Per scenario, four MT19937(seed) uniforms in this order: lattice size
+    d = SWEEP_SIZES[floor(5u)], sigma0 = 0.1 + 0.5u, beta = 0.3 + 0.7u,
+    r = 0.05u.  sigma(x) = sigma0 (x/100)^(beta-1) on [0,400] (spot on a
+    node).  16 squarings to U(0.1 yr), dt0 = 0.1 2^-16; 21 strikes
+    K = 80..120 step 2 at the 10 maturities 0.1, 0.2, ..., 1.0 yr."""
+    u = _mt_uniforms(seed, 4 * n).reshape(n, 4)
+    cfgs = []
+    for i in range(n):
+        c = _base(SWEEP_SIZES[int(u[i, 0] * len(SWEEP_SIZES))], 1, 0.05 * u[i, 3])
+        c.vol_kind = OPM_VOL_CEV
+        c.sigma0 = 0.1 + 0.5 * u[i, 1]
+        c.beta = 0.3 + 0.7 * u[i, 2]
+        c.skew = c.curv = 0.0
+        c.DeltaT = 0.1
+        # the reference planner with the 2^-20 yr cap on dt0 (at least the
+        # 16 squarings of the 2048-state workload): 17 or 18 here
+        c.nsq = max(16, plan_nsq(c, 2.0**-20))
+        c.price_kind = OPM_PRICE_FORWARD
+        c.nsteps = 10
+        ks = [80.0 + 2.0 * j for j in range(21)]
+        c.nstrikes = len(ks)
+        for j, k in enumerate(ks):
+            c.strikes[j] = k
+        c.precision = precision
+        cfgs.append(c)
+    return cfgs
+
+
+lib.opm_batch_create.argtypes = [C.POINTER(Config), C.c_int, C.c_int]
+lib.opm_batch_create.restype = C.c_void_p
+lib.opm_batch_destroy.argtypes = [C.c_void_p]
+lib.opm_batch_destroy.restype = None
+for _n in ("opm_batch_solve", "opm_batch_capture", "opm_batch_launch"):
+    getattr(lib, _n).argtypes = [C.c_void_p]
+    getattr(lib, _n).restype = C.c_int
+lib.opm_batch_prices.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+lib.opm_batch_prices.restype = C.c_int
+
+
+class Batch:
+    """Many one-regime lattices solved together (opm_batch)."""
+
+    def __init__(self, cfgs, tile=0):
+        ophip.init()
+        self.cfgs = list(cfgs)
+        arr = (Config * len(self.cfgs))(*self.cfgs)
+        self.p = lib.opm_batch_create(arr, len(self.cfgs), tile)
+        if not self.p:
+            raise ophip.OphipError(
+                f"opm_batch_create failed: 0x{ophip.lib.opcuda_get_status():x}")
+
+    def solve(self):
+        ophip.check(lib.opm_batch_solve(self.p), "opm_batch_solve")
+
+    def capture(self):
+        ophip.check(lib.opm_batch_capture(self.p), "opm_batch_capture")
+
+    def launch(self):
+        ophip.check(lib.opm_batch_launch(self.p), "opm_batch_launch")
+
+    def prices(self):
+        c = self.cfgs[0]
+        out = np.zeros(len(self.cfgs) * c.nsteps * c.nstrikes)
+        ophip.check(lib.opm_batch_prices(self.p, out.ctypes.data_as(C.POINTER(C.c_double))),
+                    "prices")
+        return out.reshape(len(self.cfgs), c.nsteps, c.nstrikes)
+
+    def close(self):
+        if self.p:
+            lib.opm_batch_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sweep_flops(cfgs):
+    """algorithmic FLOPs of a sweep: 2d^3 per squaring, 2d^2 per forward
+    step, 2d per strike price"""
+    f = 0.0
+    for c in cfgs:
+        d = c.d
+        f += c.nsq * 2.0 * d ** 3 + c.nsteps * 2.0 * d * d + c.nsteps * c.nstrikes * 2.0 * d
+    return f

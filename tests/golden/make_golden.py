@@ -62,6 +62,12 @@ def main():
     write_prices("rs2048", M.config_rs2048())
     write_prices("rs_monthly", M.config_rs_monthly())
     write_prices("small_bi", M.config_small(nx=64, nr=2))
+    cfgs = M.config_sweep()
+    ps = np.stack([O.solve_fp64(c)[0] for c in cfgs])  # [64][10][21]
+    np.savez(os.path.join(HERE, "prices_sweep.npz"), prices=ps,
+             d=np.array([c.nx for c in cfgs]), sigma0=np.array([c.sigma0 for c in cfgs]),
+             beta=np.array([c.beta for c in cfgs]), rate=np.array([c.rate for c in cfgs]))
+    print("sweep", ps.shape, float(ps.min()), float(ps.max()))
     write_prices("small_prop", M.config_small(nx=64, nr=2, price_kind=M.OPM_PRICE_PROPAGATE,
                                               nsteps=6))
 

@@ -120,7 +120,8 @@ def test_sglvg_vs_fp64(H):
     dr, vo = O.lv_params_c1(x, 0.02)
     pars = np.concatenate([dr, 1.3 * dr, vo, 0.9 * vo]).astype(np.float32)
     gen = H.DeviceArray.empty(ni * d * d)
-    H.lib.opcuda_sglvg(gen.h, ni, d, dev(H, inv.astype(np.float32)).h, dev(H, pars).h)
+    invd, parsd = dev(H, inv.astype(np.float32)), dev(H, pars)  # keep both alive
+    H.lib.opcuda_sglvg(gen.h, ni, d, invd.h, parsd.h)
     H.check()
     got = gen.download().reshape(ni, d * d)
     for i, (a, b) in enumerate(((dr, vo), (1.3 * dr, 0.9 * vo))):
@@ -303,7 +304,8 @@ def test_sgemv3_gather_and_sentinel(H):
     col0[17], col1[17] = 5 * d, 14 * d
     col1[18:] = sentinel + 5
     Zd = dev(H, Z)
-    H.lib.opcuda_sgemv3(d, nz, ncol, dev(H, A).h, Zd.h, dev(H, col0).h, dev(H, col1).h)
+    Ad, c0d, c1d = dev(H, A), dev(H, col0), dev(H, col1)  # keep alive over the call
+    H.lib.opcuda_sgemv3(d, nz, ncol, Ad.h, Zd.h, c0d.h, c1d.h)
     H.check()
     got = Zd.download()
     want = Z.copy().reshape(-1, order="F")
@@ -319,15 +321,21 @@ def test_sgemv3_gather_and_sentinel(H):
 def test_skinny_backward_step(H):
     banner("skinny")
     rng = np.random.default_rng(9)
-    for m, ncp in ((2048, 48), (256, 32), (384, 64)):
+    for m, ncp in ((2048, 48), (256, 32), (384, 64), (128, 16)):
         A = (rng.random((m, m)) / m).astype(np.float32)
         X = rng.random((m, ncp)).astype(np.float32)
-        Y = H.DeviceArray.empty(m * ncp)
-        H.check(H.lib.ophip_sgemm_skinny(m, m, ncp, np.float32(0.99), dev(H, A).h, m,
-                                          dev(H, X, order="C").h, ncp, Y.h, ncp))
-        got = Y.download().reshape(m, ncp)
-        want = 0.99 * A.astype(np.float64) @ X.astype(np.float64)
-        assert np.abs(got - want).max() < 2e-6, (m, ncp)
+        Ad, Xd = dev(H, A), dev(H, X, order="C")
+        for plus in (0, 1):
+            Y = H.DeviceArray.empty(m * ncp)
+            H.check(H.lib.ophip_sgemm_skinny_ex(m, m, ncp, np.float32(0.99), Ad.h, m, Xd.h, ncp,
+                                                 Y.h, ncp, plus))
+            got = Y.download().reshape(m, ncp)
+            want = 0.99 * (A.astype(np.float64) @ X.astype(np.float64) + plus * X)
+            assert np.abs(got - want).max() < 2e-6, (m, ncp, plus)
+            # the same launch twice gives the same bits (fixed-order reduction)
+            H.check(H.lib.ophip_sgemm_skinny_ex(m, m, ncp, np.float32(0.99), Ad.h, m, Xd.h, ncp,
+                                                 Y.h, ncp, plus))
+            assert np.array_equal(Y.download().reshape(m, ncp), got)
 
 
 # ---- tables and Monte Carlo ---------------------------------------------------------

@@ -1,0 +1,214 @@
+"""GPU tests of the U = I + W squaring kernels (ophip_wsq_*): every stage
+against a numpy restatement of the same arithmetic, and the batched plan
+against single-lattice plans.
+
+Bars: the 16-bit copies are bit-exact (one IEEE round-to-nearest-even of
+the FP32 value times the scale); one squaring agrees with the numpy
+restatement to FP32 accumulation-order rounding (1e-6 relative to the
+largest entry); the diagonal fix leaves FP64 row sums of 0 up to FP32
+rounding of the diagonal; a batch gives the same bits as the lattices
+squared alone.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+S16 = 2.0**15  # FP16 scale of the kernels
+G16 = 2048.0
+G_BF = 256.0
+
+
+def banner(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def bf16_round(x):
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def enc(x, mode):
+    """16-bit copy (as float32 values) and the low part, as the kernels do"""
+    x = np.asarray(x, dtype=np.float32)
+    if mode == 2:
+        h = bf16_round(x)
+        lo = bf16_round((x - h) * np.float32(G_BF))
+        return h, lo
+    xs = (x * np.float32(S16)).astype(np.float32)
+    h = xs.astype(np.float16).astype(np.float32)
+    lo = ((xs - h) * np.float32(G16)).astype(np.float16).astype(np.float32)
+    return h, lo
+
+
+def bits16(x, mode):
+    if mode == 2:
+        return (np.ascontiguousarray(x, dtype=np.float32).view(np.uint32) >> 16).astype(np.uint16)
+    return np.asarray(x, dtype=np.float32).astype(np.float16).view(np.uint16)
+
+
+class Lattice:
+    def __init__(self, H, nx, nr, mode, seed=0):
+        from oplib_amd import opmodel as M
+        self.H = H
+        c = M.config_small(nx=nx, nr=nr, nsq=12)
+        self.cfg, self.mode = c, mode
+        self.d = nx * nr
+        x, _ = M.grid(c)
+        dr, vo = M.params(c)
+        self.xd = H.DeviceArray.from_numpy(x)
+        self.dd = H.DeviceArray.from_numpy(dr)
+        self.vd = H.DeviceArray.from_numpy(vo)
+        nbytes = H.lib.ophip_wsq_bytes(self.d, mode)
+        assert nbytes > 0
+        self.state = H.DeviceArray(nbytes, np.uint8)
+        self.dt = math.ldexp(c.DeltaT, -c.nsq)
+        _, _, g = O.model_generator(c)
+        self.W0 = (self.dt * g).astype(np.float32)  # what out_kind 2 writes (one rounding)
+
+    def first_step(self):
+        H, c = self.H, self.cfg
+        w0 = H.lib.ophip_wsq_array(self.state.h, self.d, self.mode, 0, 0)
+        H.check(H.lib.ophip_sek_rs(c.nx, c.nr, self.xd.h, self.dd.h, self.vd.h, c.q_off,
+                                   self.dt, 2, w0))
+
+    def arr(self, buf, which, dtype, shape_order="F"):
+        H, d = self.H, self.d
+        h = H.lib.ophip_wsq_array(self.state.h, d, self.mode, buf, which)
+        out = np.empty(d * d, dtype=dtype)
+        H.check(H.lib.opcuda_memcpy_d2h(h, out.ctypes.data, out.nbytes))
+        return out.reshape(d, d, order=shape_order)
+
+
+def plan(H, lats, tile, nsqs=None):
+    import ctypes as C
+    n = len(lats)
+    st = (C.c_uint * n)(*[l.state.h for l in lats])
+    ds = (C.c_int * n)(*[l.d for l in lats])
+    ns = (C.c_int * n)(*nsqs) if nsqs is not None else None
+    p = H.lib.ophip_wsq_plan(n, st, ds, ns, lats[0].mode, tile, 0)
+    assert p, hex(H.lib.opcuda_get_status())
+    return p
+
+
+def fix_diag(W):
+    W = W.astype(np.float32).copy()
+    rs = W.astype(np.float64).sum(axis=1)
+    idx = np.arange(W.shape[0])
+    W[idx, idx] = (W[idx, idx].astype(np.float64) - rs).astype(np.float32)
+    return W
+
+
+def square_ref(W, mode):
+    """2W + W*W with the kernels' operand rounding, FP64 accumulation"""
+    h, lo = enc(W, mode)
+    h64, lo64 = h.astype(np.float64), lo.astype(np.float64)
+    if mode == 2:
+        P = h64 @ h64 + (h64 @ lo64 + lo64 @ h64) / G_BF
+    else:
+        P = h64 @ h64
+        if mode == 1:
+            P = P + (h64 @ lo64 + lo64 @ h64) / G16
+        P = P / (S16 * S16)
+    return (2.0 * W.astype(np.float64) + P)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("tile", [64, 128])
+def test_split_and_one_squaring(hip, mode, tile):
+    banner(f"wsq stages mode={mode} tile={tile}")
+    H = hip
+    L = Lattice(H, 64, 2, mode)
+    d = L.d
+    p = plan(H, [L], tile)
+    try:
+        # exec with 0 squarings: split + diagonal fix of W0
+        L.first_step()
+        H.check(H.lib.ophip_wsq_exec(p, 0))
+        H.sync()
+        W0 = L.arr(0, 0, np.float32)
+        assert np.abs(W0 - L.W0).max() <= 1e-6 * np.abs(L.W0).max() + 1e-12  # diag fix only
+        assert np.abs(W0.astype(np.float64).sum(axis=1)).max() < 1e-6 * np.abs(W0).max()
+        hr = L.arr(0, 1, np.uint16, "C")  # row-major copy: element (i,j) at i*d + j
+        hc = L.arr(0, 2, np.uint16, "F")
+        want_h, want_l = enc(W0, mode)
+        assert np.array_equal(hc, bits16(want_h if mode == 2 else want_h, mode))
+        assert np.array_equal(hr, hc)
+        if mode > 0:
+            lr, lc = L.arr(0, 3, np.uint16, "C"), L.arr(0, 4, np.uint16, "F")
+            assert np.array_equal(lc, bits16(want_l, mode))
+            assert np.array_equal(lr, lc)
+        # one squaring
+        L.first_step()
+        H.check(H.lib.ophip_wsq_exec(p, 1))
+        H.sync()
+        W1 = L.arr(1, 0, np.float32)
+        ref = square_ref(fix_diag(L.W0), mode)
+        # the GPU diagonal is then corrected: compare off-diagonal entries
+        off = ~np.eye(d, dtype=bool)
+        err = np.abs(W1[off] - ref[off]).max()
+        print(f"[err] one squaring mode={mode} tile={tile}: {err:.3e} (max |W1| "
+              f"{np.abs(ref).max():.3e})", flush=True)
+        assert err <= 1e-6 * np.abs(ref).max()
+        assert np.abs(W1.astype(np.float64).sum(axis=1)).max() < 1e-6 * np.abs(W1).max()
+        # two squarings land back in buffer 0
+        L.first_step()
+        H.check(H.lib.ophip_wsq_exec(p, 2))
+        H.sync()
+        W2 = L.arr(0, 0, np.float32)
+        ref2 = square_ref(fix_diag(W1), mode)
+        err2 = np.abs(W2[off] - ref2[off]).max()
+        assert err2 <= 1e-6 * np.abs(ref2).max()
+    finally:
+        H.lib.ophip_wsq_plan_free(p)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_batch_equals_single(hip, mode):
+    banner(f"wsq batch mode={mode}")
+    H = hip
+    A = Lattice(H, 64, 2, mode)       # d = 128
+    B = Lattice(H, 128, 2, mode)      # d = 256
+    C = Lattice(H, 64, 4, mode)       # d = 256
+    singles = []
+    for L in (A, B, C):
+        p = plan(H, [L], 64)
+        L.first_step()
+        H.check(H.lib.ophip_wsq_exec(p, 5))
+        H.sync()
+        singles.append(L.arr(1, 0, np.float32))
+        H.lib.ophip_wsq_plan_free(p)
+    p = plan(H, [A, B, C], 64)
+    for L in (A, B, C):
+        L.first_step()
+    H.check(H.lib.ophip_wsq_exec(p, 5))
+    H.sync()
+    for L, s in zip((A, B, C), singles):
+        assert np.array_equal(L.arr(1, 0, np.float32), s)
+    H.lib.ophip_wsq_plan_free(p)
+    # per-lattice counts (A squares 3 times, B 5, C 4) give what the
+    # lattices give squared alone that many times
+    want = {}
+    for L, k in ((A, 3), (C, 4)):
+        q = plan(H, [L], 64)
+        L.first_step()
+        H.check(H.lib.ophip_wsq_exec(q, k))
+        H.sync()
+        want[id(L)] = L.arr(k & 1, 0, np.float32)
+        H.lib.ophip_wsq_plan_free(q)
+    p = plan(H, [A, B, C], 64, nsqs=[3, 5, 4])
+    for L in (A, B, C):
+        L.first_step()
+    H.check(H.lib.ophip_wsq_exec(p, -1))
+    H.sync()
+    assert np.array_equal(A.arr(1, 0, np.float32), want[id(A)])
+    assert np.array_equal(B.arr(1, 0, np.float32), singles[1])
+    assert np.array_equal(C.arr(0, 0, np.float32), want[id(C)])
+    H.lib.ophip_wsq_plan_free(p)
