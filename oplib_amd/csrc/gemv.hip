@@ -149,116 +149,85 @@ __global__ __launch_bounds__(64) void k_sgemv4(const char *__restrict__ base,
 
 // ---------------------------------------------------------------------
 // skinny GEMM: Y[m x NCP] = alpha * (A[m x K] * X[K x NCP] [+ X]), X and Y
-// row-major.  Grid (m/16 strips, KS K-splits); 8 waves per workgroup split
-// the workgroup's K range; each wave keeps 4 k16-blocks of loads in flight.
-// With KS > 1 every workgroup writes its partial tile, and the workgroup
-// that finishes a strip last (per-strip arrival counter) adds the KS
-// partials in index order, so the result does not depend on which
-// workgroup came last.  The counter is reset by that workgroup.
+// row-major.  Grid (m/64 row blocks, KS K-chunks); each workgroup stages
+// its X chunk (kc x NCP) in LDS once and its 4 waves each stream 16 rows
+// of A over the chunk on the matrix cores, 4 k16-blocks of loads in
+// flight.  With KS > 1 the partial tiles go to a workspace and a second
+// kernel adds them in chunk order (deterministic, no atomics).
 // ---------------------------------------------------------------------
-constexpr int kSkWaves = 8;
-
 template <int NT>
-__global__ __launch_bounds__(512) void k_skinny(int m, int K, int kc, const float *__restrict__ A,
+__global__ __launch_bounds__(256) void k_skinny(int m, int K, int kc, const float *__restrict__ A,
                                                 int lda, const float *__restrict__ X, int ldx,
-                                                float *__restrict__ Y, int ldy, float alpha,
-                                                int plus_x, int KS, float *__restrict__ part,
-                                                int *__restrict__ counters) {
-    __shared__ f4 red[kSkWaves][NT][64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+                                                float *__restrict__ out, int ldo, float alpha,
+                                                int plus_x) {
+    constexpr int XP = 16 * NT + 4;  // LDS row pitch (floats)
+    extern __shared__ float xs[];    // kc x XP
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int r = lane & 15, grp = lane >> 4;
-    const int strip = blockIdx.x, row0 = strip * 16;
-    const int kbeg = blockIdx.y * kc, kend = min(K, kbeg + kc);
+    const int row0 = blockIdx.x * 64 + 16 * wave;
+    const int kbeg = blockIdx.y * kc, kn = min(K, kbeg + kc) - kbeg;
+    // stage X[kbeg : kbeg+kn, 0 : 16NT] (rows of 16NT floats, float4 loads)
+    constexpr int Q = 4 * NT;  // float4 per row
+    for (int i = t; i < kc * Q; i += 256) {
+        const int k = i / Q, c4 = i - k * Q;
+        f4 v = {0.f, 0.f, 0.f, 0.f};
+        if (k < kn) v = *(const f4 *)(X + (size_t)(kbeg + k) * ldx + 4 * c4);
+        *(f4 *)&xs[k * XP + 4 * c4] = v;
+    }
+    __syncthreads();
     f4 acc[NT];
 #pragma unroll
     for (int j = 0; j < NT; j++) acc[j] = f4{0.f, 0.f, 0.f, 0.f};
-    const float *Ar = A + row0 + r;
-    constexpr int U = 4;  // k16-blocks per iteration
-    for (int kb0 = kbeg + 16 * wave; kb0 < kend; kb0 += 16 * kSkWaves * U) {
+    const float *Ar = A + row0 + r + (size_t)lda * kbeg;
+    constexpr int U = 4;
+    for (int kb0 = 0; kb0 < kn; kb0 += 16 * U) {
         float a[U][4];
-        float xb[U][NT][4];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int kb = kb0 + 16 * kSkWaves * u;
-#pragma unroll
-            for (int s = 0; s < 4; s++) {
-                // lane group g of MFMA step s takes k = kb + 4g + s
-                const int k = kb + 4 * grp + s;
-                const bool ok = k < kend;
-                a[u][s] = ok ? Ar[(size_t)lda * k] : 0.0f;
-#pragma unroll
-                for (int j = 0; j < NT; j++) xb[u][j][s] = ok ? X[(size_t)k * ldx + 16 * j + r] : 0.0f;
-            }
-        }
 #pragma unroll
         for (int u = 0; u < U; u++)
 #pragma unroll
-            for (int s = 0; s < 4; s++)
+            for (int s = 0; s < 4; s++) {
+                // lane group g of MFMA step s takes k = kb + 4g + s
+                const int k = kb0 + 16 * u + 4 * grp + s;
+                a[u][s] = (k < kn) ? Ar[(size_t)lda * k] : 0.0f;
+            }
 #pragma unroll
-                for (int j = 0; j < NT; j++) acc[j] = mfma4(a[u][s], xb[u][j][s], acc[j]);
-    }
+        for (int u = 0; u < U; u++)
 #pragma unroll
-    for (int j = 0; j < NT; j++) red[wave][j][lane] = acc[j];
-    __syncthreads();
-    if (wave != 0) return;
-    f4 v[NT];
+            for (int s = 0; s < 4; s++) {
+                const int k = kb0 + 16 * u + 4 * grp + s;
 #pragma unroll
-    for (int j = 0; j < NT; j++) {
-        v[j] = red[0][j][lane];
-#pragma unroll
-        for (int w = 1; w < kSkWaves; w++) v[j] += red[w][j][lane];
+                for (int j = 0; j < NT; j++)
+                    acc[j] = mfma4(a[u][s], xs[k * XP + 16 * j + r], acc[j]);
+            }
     }
     // lane holds rows 4*grp+e of column 16j+r
-    if (KS > 1) {
-        float *pp = part + (size_t)blockIdx.y * m * 16 * NT;
-#pragma unroll
-        for (int j = 0; j < NT; j++)
-#pragma unroll
-            for (int e = 0; e < 4; e++)
-                pp[(size_t)(row0 + 4 * grp + e) * 16 * NT + 16 * j + r] = v[j][e];
-        __threadfence();  // release the partial before counting in
-        int old = 0;
-        if (lane == 0) old = atomicAdd(&counters[strip], 1);
-        old = __shfl(old, 0, 64);
-        if (old != KS - 1) return;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // see the other partials
-        // this workgroup completes the strip: sum the partials in order
-#pragma unroll
-        for (int j = 0; j < NT; j++)
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const size_t off = (size_t)(row0 + 4 * grp + e) * 16 * NT + 16 * j + r;
-                float sum = part[off];
-                for (int q = 1; q < KS; q++) sum += part[(size_t)q * m * 16 * NT + off];
-                v[j][e] = sum;
-            }
-        if (lane == 0) atomicExch(&counters[strip], 0);
-    }
+    float *o = out + (size_t)blockIdx.y * m * ldo;
+    const bool direct = gridDim.y == 1;
 #pragma unroll
     for (int j = 0; j < NT; j++)
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             const int row = row0 + 4 * grp + e;
-            float y = alpha * v[j][e];
-            if (plus_x) y += alpha * X[(size_t)row * ldx + 16 * j + r];
-            Y[(size_t)row * ldy + 16 * j + r] = y;
+            float y = acc[j][e];
+            if (direct) {
+                y *= alpha;
+                if (plus_x) y += alpha * X[(size_t)row * ldx + 16 * j + r];
+            }
+            o[(size_t)row * ldo + 16 * j + r] = y;
         }
 }
 
-constexpr int kMaxStrips = 1 << 16;
-
-// per-strip arrival counters of k_skinny: zeroed once, and every use leaves
-// them zero again (the last workgroup of a strip resets its counter)
-int *strip_counters() {
-    static int *c[16] = {};
-    Device &D = dev();
-    int id = D.id < 0 ? 0 : D.id;
-    if (!c[id]) {
-        if (hipMalloc(&c[id], sizeof(int) * kMaxStrips) != hipSuccess) return nullptr;
-        if (hipMemset(c[id], 0, sizeof(int) * kMaxStrips) != hipSuccess) return nullptr;
-        if (hipDeviceSynchronize() != hipSuccess) return nullptr;
-    }
-    return c[id];
+__global__ void k_skinny_combine(int m, int ncp, int KS, const float *__restrict__ part,
+                                 float *__restrict__ Y, int ldy, float alpha,
+                                 const float *__restrict__ X, int ldx, int plus_x) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= m * ncp) return;
+    const int i = idx / ncp, c = idx - i * ncp;
+    float s = part[(size_t)i * ncp + c];
+    for (int q = 1; q < KS; q++) s += part[(size_t)q * m * ncp + (size_t)i * ncp + c];
+    float y = alpha * s;
+    if (plus_x) y += alpha * X[(size_t)i * ldx + c];
+    Y[(size_t)i * ldy + c] = y;
 }
 
 }  // namespace
@@ -341,32 +310,39 @@ OPHIP_EXPORT void opcuda_sgemv4(int nblocks, unsigned argptr_bid) {
 static int skinny(int m, int K, int ncp, float alpha, unsigned A, int lda, unsigned X, int ldx,
                   unsigned Y, int ldy, int plus_x) {
     Device &D = dev();
-    if (m <= 0 || K <= 0 || (m % 16) || ncp <= 0 || ncp > 64 || (ncp % 16) || ldx < ncp ||
-        ldy < ncp || lda < m || (plus_x && m != K) || m / 16 > kMaxStrips)
+    if (m <= 0 || K <= 0 || (m % 64) || ncp <= 0 || ncp > 64 || (ncp % 16) || ldx < ncp ||
+        (ldx % 4) || ldy < ncp || lda < m || (plus_x && m != K))
         return (int)set_status(kErrBadArg);
     const float *Ap = P<const float>(A), *Xp = P<const float>(X);
     float *Yp = P<float>(Y);
-    if (!Ap || !Xp || !Yp) return (int)set_status(kErrBadHandle);
-    const int strips = m / 16;
+    if (!Ap || !Xp || !Yp || (((uintptr_t)Xp) & 15)) return (int)set_status(kErrBadHandle);
+    const int NT = ncp / 16;
+    const int kc_max = NT <= 3 ? 256 : 192;  // LDS: kc * (16 NT + 4) floats <= 52 KB
+    const int rb = m / 64;
     const int ncu = D.n_cu > 0 ? D.n_cu : 256;
-    int KS = cdiv(ncu, strips);
-    if (KS > 16) KS = 16;
+    int KS = cdiv(ncu, rb);
+    if (KS < cdiv(K, kc_max)) KS = cdiv(K, kc_max);
     int kc = cdiv(cdiv(K, KS), 16) * 16;
+    if (kc > kc_max) kc = kc_max;
     KS = cdiv(K, kc);
-    float *part = nullptr;
+    float *out = Yp;
+    int ldo = ldy;
     if (KS > 1) {
-        part = (float *)workspace((size_t)KS * m * ncp * sizeof(float));
-        if (!part) return (int)kErrOutOfArena;
+        out = (float *)workspace((size_t)KS * m * ncp * sizeof(float));
+        if (!out) return (int)kErrOutOfArena;
+        ldo = ncp;
     }
-    int *counters = strip_counters();
-    if (!counters) return (int)kErrOutOfArena;
-    dim3 grid(strips, KS), block(64 * kSkWaves);
-    switch (ncp / 16) {
-    case 1: hipLaunchKernelGGL(k_skinny<1>, grid, block, 0, D.stream, m, K, kc, Ap, lda, Xp, ldx, Yp, ldy, alpha, plus_x, KS, part, counters); break;
-    case 2: hipLaunchKernelGGL(k_skinny<2>, grid, block, 0, D.stream, m, K, kc, Ap, lda, Xp, ldx, Yp, ldy, alpha, plus_x, KS, part, counters); break;
-    case 3: hipLaunchKernelGGL(k_skinny<3>, grid, block, 0, D.stream, m, K, kc, Ap, lda, Xp, ldx, Yp, ldy, alpha, plus_x, KS, part, counters); break;
-    default: hipLaunchKernelGGL(k_skinny<4>, grid, block, 0, D.stream, m, K, kc, Ap, lda, Xp, ldx, Yp, ldy, alpha, plus_x, KS, part, counters); break;
+    const size_t lds = sizeof(float) * (size_t)kc * (16 * NT + 4);
+    dim3 grid(rb, KS), block(256);
+    switch (NT) {
+    case 1: hipLaunchKernelGGL(k_skinny<1>, grid, block, lds, D.stream, m, K, kc, Ap, lda, Xp, ldx, out, ldo, alpha, plus_x); break;
+    case 2: hipLaunchKernelGGL(k_skinny<2>, grid, block, lds, D.stream, m, K, kc, Ap, lda, Xp, ldx, out, ldo, alpha, plus_x); break;
+    case 3: hipLaunchKernelGGL(k_skinny<3>, grid, block, lds, D.stream, m, K, kc, Ap, lda, Xp, ldx, out, ldo, alpha, plus_x); break;
+    default: hipLaunchKernelGGL(k_skinny<4>, grid, block, lds, D.stream, m, K, kc, Ap, lda, Xp, ldx, out, ldo, alpha, plus_x); break;
     }
+    if (KS > 1)
+        hipLaunchKernelGGL(k_skinny_combine, dim3(cdiv(m * ncp, 256)), dim3(256), 0, D.stream, m,
+                           ncp, KS, out, Yp, ldy, alpha, Xp, ldx, plus_x);
     return (int)check_launch();
 }
 

@@ -91,7 +91,7 @@ void make_params(const opm_config *c, const std::vector<double> &x, std::vector<
 bool valid(const opm_config *c) {
     return c && c->nx >= 3 && c->nr >= 1 && c->nr <= 16 && c->xhi > c->xlo && c->DeltaT > 0 &&
            c->nsq >= 0 && c->nsq <= 40 && c->nsteps >= 1 && c->nstrikes >= 1 &&
-           c->nstrikes <= 64 && ((c->nx * c->nr) % 16) == 0;
+           c->nstrikes <= 64 && ((c->nx * c->nr) % 64) == 0;
 }
 
 }  // namespace

@@ -137,6 +137,7 @@ __global__ __launch_bounds__(256) void k_wsq(const WScen *__restrict__ scen,
     constexpr int WT = BT / 2, T = WT / 16;  // wave tile, MFMA tiles per side
     constexpr int NOP = LO ? 2 : 1;
     constexpr int LD16 = BT * BK / 8 / 256;  // 16-byte loads per thread per operand array
+    constexpr int PF = LO ? 3 : 4;           // K-steps of loads in flight
     static_assert(LD16 >= 1, "tile");
     __shared__ __attribute__((aligned(16))) uint16_t As[2][NOP][BT * PITCH];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[2][NOP][BT * PITCH];
@@ -161,43 +162,37 @@ __global__ __launch_bounds__(256) void k_wsq(const WScen *__restrict__ scen,
                 cor[a][b] = f4{0.f, 0.f, 0.f, 0.f};
             }
 
-        uint4 ra[NOP][LD16], rb[NOP][LD16];
-        auto gload = [&](int k0) {
+        // register ring: the global loads of K-step kt + PF are issued at
+        // K-step kt, so PF steps of MFMA work cover their latency
+        uint4 ra[PF][NOP][LD16], rb[PF][NOP][LD16];
+        auto gload = [&](int slot, int k0) {
 #pragma unroll
             for (int q = 0; q < LD16; q++) {
                 const int idx = t + 256 * q;
                 const int row = idx >> 2, part = idx & 3;  // 4 x 16 B per 32-element row
                 const size_t ea = (size_t)(i0 + row) * d + k0 + 8 * part;  // row-major A
                 const size_t eb = (size_t)(j0 + row) * d + k0 + 8 * part;  // column-major B
-                ra[0][q] = *(const uint4 *)(sc.Hr + ea);
-                rb[0][q] = *(const uint4 *)(sc.Hc + eb);
+                ra[slot][0][q] = *(const uint4 *)(sc.Hr + ea);
+                rb[slot][0][q] = *(const uint4 *)(sc.Hc + eb);
                 if (LO) {
-                    ra[NOP - 1][q] = *(const uint4 *)(sc.Lr + ea);
-                    rb[NOP - 1][q] = *(const uint4 *)(sc.Lc + eb);
+                    ra[slot][NOP - 1][q] = *(const uint4 *)(sc.Lr + ea);
+                    rb[slot][NOP - 1][q] = *(const uint4 *)(sc.Lc + eb);
                 }
             }
         };
-        auto sstore = [&](int buf) {
+        auto sstore = [&](int slot, int buf) {
 #pragma unroll
             for (int q = 0; q < LD16; q++) {
                 const int idx = t + 256 * q;
                 const int row = idx >> 2, part = idx & 3;
 #pragma unroll
                 for (int o = 0; o < NOP; o++) {
-                    *(uint4 *)&As[buf][o][row * PITCH + 8 * part] = ra[o][q];
-                    *(uint4 *)&Bs[buf][o][row * PITCH + 8 * part] = rb[o][q];
+                    *(uint4 *)&As[buf][o][row * PITCH + 8 * part] = ra[slot][o][q];
+                    *(uint4 *)&Bs[buf][o][row * PITCH + 8 * part] = rb[slot][o][q];
                 }
             }
         };
-
-        const int nk = d / BK;
-        gload(0);
-        __syncthreads();  // previous item's readers of LDS are done
-        sstore(0);
-        __syncthreads();
-        for (int kt = 0; kt < nk; kt++) {
-            const int buf = kt & 1;
-            if (kt + 1 < nk) gload((kt + 1) * BK);
+        auto compute = [&](int buf) {
             s8 fa[NOP][T], fb[NOP][T];
 #pragma unroll
             for (int o = 0; o < NOP; o++)
@@ -216,9 +211,27 @@ __global__ __launch_bounds__(256) void k_wsq(const WScen *__restrict__ scen,
                         cor[a][b] = N::mfma(fa[1][a], fb[0][b], cor[a][b]);
                     }
                 }
-            if (kt + 1 < nk) sstore(buf ^ 1);
-            __syncthreads();
+        };
+
+        const int nk = d / BK;
+#pragma unroll
+        for (int p = 0; p < PF; p++)
+            if (p < nk) gload(p, p * BK);
+        for (int kt0 = 0; kt0 < nk; kt0 += PF) {
+#pragma unroll
+            for (int p = 0; p < PF; p++) {
+                const int kt = kt0 + p;
+                if (kt < nk) {
+                    // LDS buffer kt&1 was last read by compute(kt-2), which every
+                    // thread finished before the barrier of step kt-1
+                    sstore(p, kt & 1);
+                    __syncthreads();
+                    if (kt + PF < nk) gload(p, (kt + PF) * BK);
+                    compute(kt & 1);
+                }
+            }
         }
+        __syncthreads();  // all LDS reads done before the next item's stores
 
         // epilogue: lane holds rows 4g+v of column r of every 16x16 tile
         const float inv = N::IS * N::IS;

@@ -80,7 +80,8 @@ def test_plain_fp32_vs_fp64(M, name, builder, bound):
 WFORM_BOUNDS = {
     ("rs2048", 1): 1.5e-4, ("rs2048", 2): 1e-5, ("rs2048", 3): 3e-5,
     ("lv1d", 1): 1e-3, ("lv1d", 2): 1e-5, ("lv1d", 3): 3e-5,
-    ("rs_monthly", 1): 1e-3, ("rs_monthly", 2): 1e-5, ("rs_monthly", 3): 3e-5,
+    # rs_monthly adds 11 FP32 propagation products: ~2e-5 measured for FP16x3
+    ("rs_monthly", 1): 1e-3, ("rs_monthly", 2): 5e-5, ("rs_monthly", 3): 5e-5,
 }
 
 
