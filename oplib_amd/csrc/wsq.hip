@@ -40,24 +40,32 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef short s8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));  // native vector: stays in VGPRs
 
 namespace {
 
-// one scenario (lattice) of a batched W-form squaring round
+// One scenario (lattice) of a batched W-form squaring round.  Every array
+// is a byte offset from the arena base, which the kernels receive as an
+// argument: pointers derived from it stay in the global address space, so
+// the compiler emits global (not flat) memory instructions and can overlap
+// them with LDS traffic.
 struct WScen {
     int d;
     int pad;
-    const float *W;    // FP32 W, column-major (input)
-    float *Wn;         // FP32 W' (output)
-    const uint16_t *Hr, *Hc, *Lr, *Lc;  // 16-bit copies of W (row-/column-major)
-    uint16_t *Hrn, *Hcn, *Lrn, *Lcn;    // copies of W' (outputs)
-    double *part;      // partial row sums of W': [d/64 column blocks][d]
+    uint64_t W, Wn;                // FP32 W (input), W' (output), column-major
+    uint64_t Hr, Hc, Lr, Lc;       // 16-bit copies of W (row-/column-major)
+    uint64_t Hrn, Hcn, Lrn, Lcn;   // copies of W' (outputs)
+    uint64_t part;                 // partial row sums of W': [d/64 column blocks][d]
 };
 
 // work item = (scenario, tile)
 struct WItem {
     int s, tm, tn, pad;
 };
+
+template <class T> __device__ inline T *at(char *base, uint64_t off) {
+    return reinterpret_cast<T *>(base + off);
+}
 
 template <bool BF> struct Num;
 template <> struct Num<false> {
@@ -91,53 +99,60 @@ template <> struct Num<true> {
 // k_wsplit: 64x64 tile per workgroup (64 threads), LDS transpose
 // ---------------------------------------------------------------------
 template <bool BF, bool LO>
-__global__ __launch_bounds__(64) void k_wsplit(const WScen *__restrict__ scen,
+__global__ __launch_bounds__(64) void k_wsplit(char *__restrict__ base,
+                                               const WScen *__restrict__ scen,
                                                const WItem *__restrict__ items) {
     using N = Num<BF>;
     __shared__ uint16_t th[64][66], tl[64][66];
     const WItem it = items[blockIdx.x];
     const WScen sc = scen[it.s];
+    const float *W = at<const float>(base, sc.W);
+    uint16_t *Hcn = at<uint16_t>(base, sc.Hcn), *Hrn = at<uint16_t>(base, sc.Hrn);
+    uint16_t *Lcn = at<uint16_t>(base, sc.Lcn), *Lrn = at<uint16_t>(base, sc.Lrn);
     const int d = sc.d, lane = threadIdx.x;
     const int i0 = it.tm * 64, j0 = it.tn * 64;
     const int i = i0 + lane;
     double rs = 0.0;
     for (int c = 0; c < 64; c++) {
         const size_t e = (size_t)i + (size_t)d * (j0 + c);
-        float w = sc.W[e];
+        float w = W[e];
         rs += (double)w;
         float ws = w * N::S;
         uint16_t h = N::enc(ws);
-        sc.Hcn[e] = h;
+        Hcn[e] = h;
         th[lane][c] = h;
         if (LO) {
             uint16_t l = N::enc((ws - N::dec(h)) * N::G);
-            sc.Lcn[e] = l;
+            Lcn[e] = l;
             tl[lane][c] = l;
         }
     }
-    sc.part[(size_t)it.tn * d + i] = rs;  // slot = 64-column block
+    at<double>(base, sc.part)[(size_t)it.tn * d + i] = rs;  // slot = 64-column block
     __syncthreads();
     // row-major copies: row r of the tile, lanes over columns
     for (int r = 0; r < 64; r++) {
         const size_t e = (size_t)(i0 + r) * d + j0 + lane;
-        sc.Hrn[e] = th[r][lane];
-        if (LO) sc.Lrn[e] = tl[r][lane];
+        Hrn[e] = th[r][lane];
+        if (LO) Lrn[e] = tl[r][lane];
     }
 }
 
 // ---------------------------------------------------------------------
 // k_wsq: W' = 2W + W*W.  Workgroup tile BT x BT (BT = 128: 4 waves of
-// 64x64; BT = 64: 4 waves of 32x32), K-step 32.
+// 64x64; BT = 64: 4 waves of 32x32), K-step 32, four K-steps of global
+// loads in flight in named register sets (no scratch), LDS double buffer.
+// Requires d % 128 == 0 (whole tiles, K-steps a multiple of 4).
 // ---------------------------------------------------------------------
 template <bool BF, bool LO, int BT>
-__global__ __launch_bounds__(256) void k_wsq(const WScen *__restrict__ scen,
+__global__ __launch_bounds__(256) void k_wsq(char *__restrict__ base,
+                                             const WScen *__restrict__ scen,
                                              const WItem *__restrict__ items, int nitems) {
     using N = Num<BF>;
     constexpr int BK = 32, PITCH = BK + 8;  // 16-bit elements per LDS row
     constexpr int WT = BT / 2, T = WT / 16;  // wave tile, MFMA tiles per side
     constexpr int NOP = LO ? 2 : 1;
     constexpr int LD16 = BT * BK / 8 / 256;  // 16-byte loads per thread per operand array
-    constexpr int PF = LO ? 3 : 4;           // K-steps of loads in flight
+    constexpr int PF = 4;                    // K-steps of loads in flight
     static_assert(LD16 >= 1, "tile");
     __shared__ __attribute__((aligned(16))) uint16_t As[2][NOP][BT * PITCH];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[2][NOP][BT * PITCH];
@@ -152,6 +167,8 @@ __global__ __launch_bounds__(256) void k_wsq(const WScen *__restrict__ scen,
         const WScen sc = scen[it.s];
         const int d = sc.d;
         const int i0 = it.tm * BT, j0 = it.tn * BT;
+        const uint16_t *Hr = at<const uint16_t>(base, sc.Hr), *Hc = at<const uint16_t>(base, sc.Hc);
+        const uint16_t *Lr = at<const uint16_t>(base, sc.Lr), *Lc = at<const uint16_t>(base, sc.Lc);
 
         f4 acc[T][T], cor[T][T];
 #pragma unroll
@@ -162,33 +179,31 @@ __global__ __launch_bounds__(256) void k_wsq(const WScen *__restrict__ scen,
                 cor[a][b] = f4{0.f, 0.f, 0.f, 0.f};
             }
 
-        // register ring: the global loads of K-step kt + PF are issued at
-        // K-step kt, so PF steps of MFMA work cover their latency
-        uint4 ra[PF][NOP][LD16], rb[PF][NOP][LD16];
-        auto gload = [&](int slot, int k0) {
+        typedef u4v Stage[2][NOP][LD16];  // [A/B][hi/lo][load]
+        auto gload = [&](Stage &x, int k0) {
 #pragma unroll
             for (int q = 0; q < LD16; q++) {
                 const int idx = t + 256 * q;
                 const int row = idx >> 2, part = idx & 3;  // 4 x 16 B per 32-element row
                 const size_t ea = (size_t)(i0 + row) * d + k0 + 8 * part;  // row-major A
                 const size_t eb = (size_t)(j0 + row) * d + k0 + 8 * part;  // column-major B
-                ra[slot][0][q] = *(const uint4 *)(sc.Hr + ea);
-                rb[slot][0][q] = *(const uint4 *)(sc.Hc + eb);
+                x[0][0][q] = *(const u4v *)(Hr + ea);
+                x[1][0][q] = *(const u4v *)(Hc + eb);
                 if (LO) {
-                    ra[slot][NOP - 1][q] = *(const uint4 *)(sc.Lr + ea);
-                    rb[slot][NOP - 1][q] = *(const uint4 *)(sc.Lc + eb);
+                    x[0][NOP - 1][q] = *(const u4v *)(Lr + ea);
+                    x[1][NOP - 1][q] = *(const u4v *)(Lc + eb);
                 }
             }
         };
-        auto sstore = [&](int slot, int buf) {
+        auto sstore = [&](const Stage &x, int buf) {
 #pragma unroll
             for (int q = 0; q < LD16; q++) {
                 const int idx = t + 256 * q;
                 const int row = idx >> 2, part = idx & 3;
 #pragma unroll
                 for (int o = 0; o < NOP; o++) {
-                    *(uint4 *)&As[buf][o][row * PITCH + 8 * part] = ra[slot][o][q];
-                    *(uint4 *)&Bs[buf][o][row * PITCH + 8 * part] = rb[slot][o][q];
+                    *(u4v *)&As[buf][o][row * PITCH + 8 * part] = x[0][o][q];
+                    *(u4v *)&Bs[buf][o][row * PITCH + 8 * part] = x[1][o][q];
                 }
             }
         };
@@ -204,36 +219,50 @@ __global__ __launch_bounds__(256) void k_wsq(const WScen *__restrict__ scen,
 #pragma unroll
             for (int b = 0; b < T; b++)
 #pragma unroll
-                for (int a = 0; a < T; a++) {
-                    acc[a][b] = N::mfma(fa[0][a], fb[0][b], acc[a][b]);
-                    if (LO) {
-                        cor[a][b] = N::mfma(fa[0][a], fb[1][b], cor[a][b]);
-                        cor[a][b] = N::mfma(fa[1][a], fb[0][b], cor[a][b]);
-                    }
-                }
+                for (int a = 0; a < T; a++) acc[a][b] = N::mfma(fa[0][a], fb[0][b], acc[a][b]);
+            if (LO) {
+#pragma unroll
+                for (int b = 0; b < T; b++)
+#pragma unroll
+                    for (int a = 0; a < T; a++) cor[a][b] = N::mfma(fa[0][a], fb[1][b], cor[a][b]);
+#pragma unroll
+                for (int b = 0; b < T; b++)
+#pragma unroll
+                    for (int a = 0; a < T; a++) cor[a][b] = N::mfma(fa[1][a], fb[0][b], cor[a][b]);
+            }
         };
 
-        const int nk = d / BK;
-#pragma unroll
-        for (int p = 0; p < PF; p++)
-            if (p < nk) gload(p, p * BK);
-        for (int kt0 = 0; kt0 < nk; kt0 += PF) {
-#pragma unroll
-            for (int p = 0; p < PF; p++) {
-                const int kt = kt0 + p;
-                if (kt < nk) {
-                    // LDS buffer kt&1 was last read by compute(kt-2), which every
-                    // thread finished before the barrier of step kt-1
-                    sstore(p, kt & 1);
-                    __syncthreads();
-                    if (kt + PF < nk) gload(p, (kt + PF) * BK);
-                    compute(kt & 1);
-                }
-            }
+        const int nk = d / BK;  // a multiple of PF
+        Stage s0, s1, s2, s3;
+        gload(s0, 0);
+        gload(s1, BK);
+        gload(s2, 2 * BK);
+        gload(s3, 3 * BK);
+        // stage kt: its registers go to LDS buffer kt&1 (last read by
+        // compute(kt-2), finished by every thread before the barrier of
+        // step kt-1), the registers are refilled with step kt+4 (clamped to
+        // the last step, so every load is unconditional), then compute
+#define OPHIP_WSQ_STAGE(S, KT)                                 \
+    {                                                          \
+        sstore(S, (KT) & 1);                                   \
+        __syncthreads();                                       \
+        gload(S, min((KT) + PF, nk - 1) * BK);                 \
+        compute((KT) & 1);                                     \
+    }
+        for (int kt = 0; kt < nk; kt += PF) {
+            OPHIP_WSQ_STAGE(s0, kt)
+            OPHIP_WSQ_STAGE(s1, kt + 1)
+            OPHIP_WSQ_STAGE(s2, kt + 2)
+            OPHIP_WSQ_STAGE(s3, kt + 3)
         }
+#undef OPHIP_WSQ_STAGE
         __syncthreads();  // all LDS reads done before the next item's stores
 
         // epilogue: lane holds rows 4g+v of column r of every 16x16 tile
+        const float *W = at<const float>(base, sc.W);
+        float *Wn = at<float>(base, sc.Wn);
+        uint16_t *Hrn = at<uint16_t>(base, sc.Hrn), *Hcn = at<uint16_t>(base, sc.Hcn);
+        uint16_t *Lrn = at<uint16_t>(base, sc.Lrn), *Lcn = at<uint16_t>(base, sc.Lcn);
         const float inv = N::IS * N::IS;
         double rows[T][4];
 #pragma unroll
@@ -247,11 +276,11 @@ __global__ __launch_bounds__(256) void k_wsq(const WScen *__restrict__ scen,
             for (int b = 0; b < T; b++) {
                 const int col = j0 + wn0 + 16 * b + r;
                 const size_t e = (size_t)row + (size_t)d * col;
-                f4 w = *(const f4 *)(sc.W + e);
+                f4 w = *(const f4 *)(W + e);
                 f4 p = acc[a][b];
                 if (LO) p += cor[a][b] * N::IG;
                 f4 o = 2.0f * w + p * inv;
-                *(f4 *)(sc.Wn + e) = o;
+                *(f4 *)(Wn + e) = o;
                 uint16_t hc[4], lc[4];
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
@@ -259,17 +288,17 @@ __global__ __launch_bounds__(256) void k_wsq(const WScen *__restrict__ scen,
                     float os = o[v] * N::S;
                     hc[v] = N::enc(os);
                     if (LO) lc[v] = N::enc((os - N::dec(hc[v])) * N::G);
-                    sc.Hrn[(size_t)(row + v) * d + col] = hc[v];
-                    if (LO) sc.Lrn[(size_t)(row + v) * d + col] = lc[v];
+                    Hrn[(size_t)(row + v) * d + col] = hc[v];
+                    if (LO) Lrn[(size_t)(row + v) * d + col] = lc[v];
                 }
                 uint2 ph, pl;
                 ph.x = (uint32_t)hc[0] | ((uint32_t)hc[1] << 16);
                 ph.y = (uint32_t)hc[2] | ((uint32_t)hc[3] << 16);
-                *(uint2 *)(sc.Hcn + e) = ph;
+                *(uint2 *)(Hcn + e) = ph;
                 if (LO) {
                     pl.x = (uint32_t)lc[0] | ((uint32_t)lc[1] << 16);
                     pl.y = (uint32_t)lc[2] | ((uint32_t)lc[3] << 16);
-                    *(uint2 *)(sc.Lcn + e) = pl;
+                    *(uint2 *)(Lcn + e) = pl;
                 }
             }
         }
@@ -295,22 +324,25 @@ __global__ __launch_bounds__(256) void k_wsq(const WScen *__restrict__ scen,
         // one partial slot per 64-column block: with BT = 128 each wave
         // column is one block; with BT = 64 the two 32-wide wave columns
         // make one block
+        double *part = at<double>(base, sc.part);
         if (t < BT) {
             const int row = i0 + t;
             if (BT == 128) {
-                sc.part[(size_t)(2 * it.tn + 0) * d + row] = rsum[0][t];
-                sc.part[(size_t)(2 * it.tn + 1) * d + row] = rsum[1][t];
+                part[(size_t)(2 * it.tn + 0) * d + row] = rsum[0][t];
+                part[(size_t)(2 * it.tn + 1) * d + row] = rsum[1][t];
             } else {
-                sc.part[(size_t)it.tn * d + row] = rsum[0][t] + rsum[1][t];
+                part[(size_t)it.tn * d + row] = rsum[0][t] + rsum[1][t];
             }
         }
     }
 }
 
-// diagonal correction: W'_ii -= sum_j W'_ij (fixed summation order)
+// diagonal correction: W'_ii -= sum_j W'_ij (fixed summation order).
+// One wavefront per 64 rows; the lanes of a row group read the d/64
+// partial slots of their rows in a fixed order.
 template <bool BF, bool LO>
-__global__ void k_wfix(const WScen *__restrict__ scen, int nscen, const int *__restrict__ nsq,
-                       int round) {
+__global__ void k_wfix(char *__restrict__ base, const WScen *__restrict__ scen, int nscen,
+                       const int *__restrict__ nsq, int round) {
     using N = Num<BF>;
     const int s = blockIdx.y;
     if (s >= nscen) return;
@@ -319,20 +351,22 @@ __global__ void k_wfix(const WScen *__restrict__ scen, int nscen, const int *__r
     const int d = sc.d;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d) return;
+    const double *part = at<const double>(base, sc.part);
     const int nslots = d / 64;
     double rs = 0.0;
-    for (int q = 0; q < nslots; q++) rs += sc.part[(size_t)q * d + i];
+    for (int q = 0; q < nslots; q++) rs += part[(size_t)q * d + i];
     const size_t e = (size_t)i + (size_t)d * i;
-    const float w = (float)((double)sc.Wn[e] - rs);
-    sc.Wn[e] = w;
+    float *Wn = at<float>(base, sc.Wn);
+    const float w = (float)((double)Wn[e] - rs);
+    Wn[e] = w;
     const float ws = w * N::S;
     const uint16_t h = N::enc(ws);
-    sc.Hcn[e] = h;
-    sc.Hrn[e] = h;  // the diagonal sits at the same offset in both layouts
+    at<uint16_t>(base, sc.Hcn)[e] = h;
+    at<uint16_t>(base, sc.Hrn)[e] = h;  // the diagonal sits at the same offset in both layouts
     if (LO) {
         const uint16_t l = N::enc((ws - N::dec(h)) * N::G);
-        sc.Lcn[e] = l;
-        sc.Lrn[e] = l;
+        at<uint16_t>(base, sc.Lcn)[e] = l;
+        at<uint16_t>(base, sc.Lrn)[e] = l;
     }
 }
 
@@ -375,25 +409,25 @@ OPHIP_EXPORT unsigned long ophip_wsq_bytes(int d, int mode) {
 
 namespace {
 
-WScen make_scen(const char *base, unsigned state, int d, int mode, int src) {
+WScen make_scen(unsigned state, int d, int mode, int src) {
     WsqLayout L = wsq_layout(d, mode);
-    const char *p = base + state;
+    const uint64_t p = state;
     WScen s{};
     s.d = d;
     const int dst = src ^ 1;
-    s.W = (const float *)(p + (src ? L.W1 : L.W0));
-    s.Wn = (float *)(p + (dst ? L.W1 : L.W0));
-    s.Hr = (const uint16_t *)(p + L.H[src][0]);
-    s.Hc = (const uint16_t *)(p + L.H[src][1]);
-    s.Hrn = (uint16_t *)(p + L.H[dst][0]);
-    s.Hcn = (uint16_t *)(p + L.H[dst][1]);
-    if (mode != 0) {
-        s.Lr = (const uint16_t *)(p + L.H[src][2]);
-        s.Lc = (const uint16_t *)(p + L.H[src][3]);
-        s.Lrn = (uint16_t *)(p + L.H[dst][2]);
-        s.Lcn = (uint16_t *)(p + L.H[dst][3]);
-    }
-    s.part = (double *)(p + L.part);
+    s.W = p + (src ? L.W1 : L.W0);
+    s.Wn = p + (dst ? L.W1 : L.W0);
+    s.Hr = p + L.H[src][0];
+    s.Hc = p + L.H[src][1];
+    s.Hrn = p + L.H[dst][0];
+    s.Hcn = p + L.H[dst][1];
+    // the one-product mode has no low parts: point them at the high parts
+    // (never read or written when LO is false)
+    s.Lr = mode ? p + L.H[src][2] : s.Hr;
+    s.Lc = mode ? p + L.H[src][3] : s.Hc;
+    s.Lrn = mode ? p + L.H[dst][2] : s.Hrn;
+    s.Lcn = mode ? p + L.H[dst][3] : s.Hcn;
+    s.part = p + L.part;
     return s;
 }
 
@@ -467,10 +501,10 @@ OPHIP_EXPORT unsigned long ophip_wsq_plan(int n, const unsigned *states, const i
         }
     std::vector<WScen> sc(2 * n), sc_split(n);
     for (int i = 0; i < n; i++) {
-        sc[i] = make_scen(D.base, states[i], ds[i], mode, 0);
-        sc[n + i] = make_scen(D.base, states[i], ds[i], mode, 1);
+        sc[i] = make_scen(states[i], ds[i], mode, 0);
+        sc[n + i] = make_scen(states[i], ds[i], mode, 1);
         // the split reads W0 and writes buffer 0's copies and diagonal
-        sc_split[i] = make_scen(D.base, states[i], ds[i], mode, 1);
+        sc_split[i] = make_scen(states[i], ds[i], mode, 1);
         sc_split[i].W = sc[i].W;
         P->dmax = std::max(P->dmax, ds[i]);
     }
@@ -555,10 +589,10 @@ OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
     const dim3 fixgrid(cdiv(P->dmax, 256), P->n);
     return (int)launch_mode(P->mode, [&](auto bf, auto lo) {
         constexpr bool BF = decltype(bf)::value, LO = decltype(lo)::value;
-        hipLaunchKernelGGL((k_wsplit<BF, LO>), dim3(P->n_split), dim3(64), 0, D.stream,
+        hipLaunchKernelGGL((k_wsplit<BF, LO>), dim3(P->n_split), dim3(64), 0, D.stream, D.base,
                            P->sc_split, P->it_split);
-        hipLaunchKernelGGL((k_wfix<BF, LO>), fixgrid, dim3(256), 0, D.stream, P->sc_split, P->n,
-                           (const int *)nullptr, 0);
+        hipLaunchKernelGGL((k_wfix<BF, LO>), fixgrid, dim3(256), 0, D.stream, D.base, P->sc_split,
+                           P->n, (const int *)nullptr, 0);
         for (int rr = 0; rr < rounds; rr++) {
             const int src = rr & 1;
             const int nit = per ? P->n_round[rr] : P->n_all[src];
@@ -569,10 +603,12 @@ OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
             // indexes lattices 0..n-1 of the table half for this direction
             const WScen *scn = P->sc + (src ? P->n : 0);
             if (P->tile == 128)
-                hipLaunchKernelGGL((k_wsq<BF, LO, 128>), dim3(g), dim3(256), 0, D.stream, P->sc, its, nit);
+                hipLaunchKernelGGL((k_wsq<BF, LO, 128>), dim3(g), dim3(256), 0, D.stream, D.base,
+                                   P->sc, its, nit);
             else
-                hipLaunchKernelGGL((k_wsq<BF, LO, 64>), dim3(g), dim3(256), 0, D.stream, P->sc, its, nit);
-            hipLaunchKernelGGL((k_wfix<BF, LO>), fixgrid, dim3(256), 0, D.stream, scn, P->n,
+                hipLaunchKernelGGL((k_wsq<BF, LO, 64>), dim3(g), dim3(256), 0, D.stream, D.base,
+                                   P->sc, its, nit);
+            hipLaunchKernelGGL((k_wfix<BF, LO>), fixgrid, dim3(256), 0, D.stream, D.base, scn, P->n,
                                per ? (const int *)P->nsq_dev : (const int *)nullptr, rr);
         }
     });
