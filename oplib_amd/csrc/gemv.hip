@@ -179,17 +179,24 @@ __global__ __launch_bounds__(256) void k_skinny(int m, int K, int kc, const floa
 #pragma unroll
     for (int j = 0; j < NT; j++) acc[j] = f4{0.f, 0.f, 0.f, 0.f};
     const float *Ar = A + row0 + r + (size_t)lda * kbeg;
+    // two register stages of 4 k16-blocks: the A loads of the next 64 k
+    // are in flight while the current 64 k go through the matrix cores.
+    // kc is a multiple of 128; k beyond this chunk reads a clamped address
+    // and contributes zero (X rows past the chunk are zero-filled).
     constexpr int U = 4;
-    for (int kb0 = 0; kb0 < kn; kb0 += 16 * U) {
-        float a[U][4];
+    float sa[U][4], sb[U][4];
+    auto load = [&](float (&a)[U][4], int kb0) {
 #pragma unroll
         for (int u = 0; u < U; u++)
 #pragma unroll
             for (int s = 0; s < 4; s++) {
                 // lane group g of MFMA step s takes k = kb + 4g + s
                 const int k = kb0 + 16 * u + 4 * grp + s;
-                a[u][s] = (k < kn) ? Ar[(size_t)lda * k] : 0.0f;
+                const float v = Ar[(size_t)lda * (k < kn ? k : kn - 1)];
+                a[u][s] = k < kn ? v : 0.0f;
             }
+    };
+    auto mma = [&](const float (&a)[U][4], int kb0) {
 #pragma unroll
         for (int u = 0; u < U; u++)
 #pragma unroll
@@ -199,6 +206,13 @@ __global__ __launch_bounds__(256) void k_skinny(int m, int K, int kc, const floa
                 for (int j = 0; j < NT; j++)
                     acc[j] = mfma4(a[u][s], xs[k * XP + 16 * j + r], acc[j]);
             }
+    };
+    load(sa, 0);
+    for (int kb0 = 0; kb0 < kn; kb0 += 32 * U) {
+        load(sb, kb0 + 16 * U);
+        mma(sa, kb0);
+        load(sa, kb0 + 32 * U);
+        mma(sb, kb0 + 16 * U);
     }
     // lane holds rows 4*grp+e of column 16j+r
     float *o = out + (size_t)blockIdx.y * m * ldo;
@@ -220,14 +234,16 @@ __global__ __launch_bounds__(256) void k_skinny(int m, int K, int kc, const floa
 __global__ void k_skinny_combine(int m, int ncp, int KS, const float *__restrict__ part,
                                  float *__restrict__ Y, int ldy, float alpha,
                                  const float *__restrict__ X, int ldx, int plus_x) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    // four outputs per thread (ncp % 16 == 0, ldx % 4 == 0, ldy % 4 == 0)
+    const int idx = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
     if (idx >= m * ncp) return;
     const int i = idx / ncp, c = idx - i * ncp;
-    float s = part[(size_t)i * ncp + c];
-    for (int q = 1; q < KS; q++) s += part[(size_t)q * m * ncp + (size_t)i * ncp + c];
-    float y = alpha * s;
-    if (plus_x) y += alpha * X[(size_t)i * ldx + c];
-    Y[(size_t)i * ldy + c] = y;
+    const size_t mn = (size_t)m * ncp;
+    f4 s = *(const f4 *)(part + idx);
+    for (int q = 1; q < KS; q++) s += *(const f4 *)(part + q * mn + idx);
+    f4 y = alpha * s;
+    if (plus_x) y += alpha * *(const f4 *)(X + (size_t)i * ldx + c);
+    *(f4 *)(Y + (size_t)i * ldy + c) = y;
 }
 
 }  // namespace
@@ -311,18 +327,18 @@ static int skinny(int m, int K, int ncp, float alpha, unsigned A, int lda, unsig
                   unsigned Y, int ldy, int plus_x) {
     Device &D = dev();
     if (m <= 0 || K <= 0 || (m % 64) || ncp <= 0 || ncp > 64 || (ncp % 16) || ldx < ncp ||
-        (ldx % 4) || ldy < ncp || lda < m || (plus_x && m != K))
+        (ldx % 4) || (ldy % 4) || ldy < ncp || lda < m || (plus_x && m != K))
         return (int)set_status(kErrBadArg);
     const float *Ap = P<const float>(A), *Xp = P<const float>(X);
     float *Yp = P<float>(Y);
     if (!Ap || !Xp || !Yp || (((uintptr_t)Xp) & 15)) return (int)set_status(kErrBadHandle);
     const int NT = ncp / 16;
-    const int kc_max = NT <= 3 ? 256 : 192;  // LDS: kc * (16 NT + 4) floats <= 52 KB
+    const int kc_max = NT <= 3 ? 256 : 128;  // LDS: kc * (16 NT + 4) floats <= 52 KB
     const int rb = m / 64;
     const int ncu = D.n_cu > 0 ? D.n_cu : 256;
-    int KS = cdiv(ncu, rb);
+    int KS = cdiv(2 * ncu, rb);  // about two workgroups per CU
     if (KS < cdiv(K, kc_max)) KS = cdiv(K, kc_max);
-    int kc = cdiv(cdiv(K, KS), 16) * 16;
+    int kc = cdiv(cdiv(K, KS), 128) * 128;  // whole pairs of 64-k stages
     if (kc > kc_max) kc = kc_max;
     KS = cdiv(K, kc);
     float *out = Yp;
@@ -341,8 +357,8 @@ static int skinny(int m, int K, int ncp, float alpha, unsigned A, int lda, unsig
     default: hipLaunchKernelGGL(k_skinny<4>, grid, block, lds, D.stream, m, K, kc, Ap, lda, Xp, ldx, out, ldo, alpha, plus_x); break;
     }
     if (KS > 1)
-        hipLaunchKernelGGL(k_skinny_combine, dim3(cdiv(m * ncp, 256)), dim3(256), 0, D.stream, m,
-                           ncp, KS, out, Yp, ldy, alpha, Xp, ldx, plus_x);
+        hipLaunchKernelGGL(k_skinny_combine, dim3(cdiv(m * ncp / 4, 256)), dim3(256), 0,
+                           D.stream, m, ncp, KS, out, Yp, ldy, alpha, Xp, ldx, plus_x);
     return (int)check_launch();
 }
 
