@@ -352,9 +352,18 @@ __global__ void k_wfix(char *__restrict__ base, const WScen *__restrict__ scen, 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d) return;
     const double *part = at<const double>(base, sc.part);
-    const int nslots = d / 64;
+    const int nslots = d / 64;  // a multiple of 2 (d % 128 == 0)
     double rs = 0.0;
-    for (int q = 0; q < nslots; q++) rs += part[(size_t)q * d + i];
+    // loads issued eight at a time, added in slot order
+    int q = 0;
+    for (; q + 8 <= nslots; q += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = part[(size_t)(q + u) * d + i];
+#pragma unroll
+        for (int u = 0; u < 8; u++) rs += v[u];
+    }
+    for (; q < nslots; q++) rs += part[(size_t)q * d + i];
     const size_t e = (size_t)i + (size_t)d * i;
     float *Wn = at<float>(base, sc.Wn);
     const float w = (float)((double)Wn[e] - rs);
@@ -586,12 +595,12 @@ OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
     if (!P) return (int)set_status(kErrBadArg);
     const bool per = nsq < 0;
     const int rounds = per ? P->rounds : nsq;
-    const dim3 fixgrid(cdiv(P->dmax, 256), P->n);
+    const dim3 fixgrid(cdiv(P->dmax, 64), P->n);
     return (int)launch_mode(P->mode, [&](auto bf, auto lo) {
         constexpr bool BF = decltype(bf)::value, LO = decltype(lo)::value;
         hipLaunchKernelGGL((k_wsplit<BF, LO>), dim3(P->n_split), dim3(64), 0, D.stream, D.base,
                            P->sc_split, P->it_split);
-        hipLaunchKernelGGL((k_wfix<BF, LO>), fixgrid, dim3(256), 0, D.stream, D.base, P->sc_split,
+        hipLaunchKernelGGL((k_wfix<BF, LO>), fixgrid, dim3(64), 0, D.stream, D.base, P->sc_split,
                            P->n, (const int *)nullptr, 0);
         for (int rr = 0; rr < rounds; rr++) {
             const int src = rr & 1;
@@ -608,7 +617,7 @@ OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
             else
                 hipLaunchKernelGGL((k_wsq<BF, LO, 64>), dim3(g), dim3(256), 0, D.stream, D.base,
                                    P->sc, its, nit);
-            hipLaunchKernelGGL((k_wfix<BF, LO>), fixgrid, dim3(256), 0, D.stream, D.base, scn, P->n,
+            hipLaunchKernelGGL((k_wfix<BF, LO>), fixgrid, dim3(64), 0, D.stream, D.base, scn, P->n,
                                per ? (const int *)P->nsq_dev : (const int *)nullptr, rr);
         }
     });
