@@ -181,8 +181,9 @@ __global__ __launch_bounds__(256) void k_skinny(int m, int K, int kc, const floa
     const float *Ar = A + row0 + r + (size_t)lda * kbeg;
     // two register stages of 4 k16-blocks: the A loads of the next 64 k
     // are in flight while the current 64 k go through the matrix cores.
-    // kc is a multiple of 128; k beyond this chunk reads a clamped address
-    // and contributes zero (X rows past the chunk are zero-filled).
+    // kc is a multiple of 128; k beyond this chunk reads a clamped (valid,
+    // finite) address and meets a zero row of the staged X, so no select
+    // follows the load and all loads of a stage issue back to back.
     constexpr int U = 4;
     float sa[U][4], sb[U][4];
     auto load = [&](float (&a)[U][4], int kb0) {
@@ -192,20 +193,25 @@ __global__ __launch_bounds__(256) void k_skinny(int m, int K, int kc, const floa
             for (int s = 0; s < 4; s++) {
                 // lane group g of MFMA step s takes k = kb + 4g + s
                 const int k = kb0 + 16 * u + 4 * grp + s;
-                const float v = Ar[(size_t)lda * (k < kn ? k : kn - 1)];
-                a[u][s] = k < kn ? v : 0.0f;
+                a[u][s] = Ar[(size_t)lda * (k < kn ? k : kn - 1)];
             }
     };
     auto mma = [&](const float (&a)[U][4], int kb0) {
+        float xv[U][4][NT];  // all X operands of the stage first, then the MFMAs
 #pragma unroll
         for (int u = 0; u < U; u++)
 #pragma unroll
             for (int s = 0; s < 4; s++) {
                 const int k = kb0 + 16 * u + 4 * grp + s;
 #pragma unroll
-                for (int j = 0; j < NT; j++)
-                    acc[j] = mfma4(a[u][s], xs[k * XP + 16 * j + r], acc[j]);
+                for (int j = 0; j < NT; j++) xv[u][s][j] = xs[k * XP + 16 * j + r];
             }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int s = 0; s < 4; s++)
+#pragma unroll
+                for (int j = 0; j < NT; j++) acc[j] = mfma4(a[u][s], xv[u][s][j], acc[j]);
     };
     load(sa, 0);
     for (int kb0 = 0; kb0 < kn; kb0 += 32 * U) {
