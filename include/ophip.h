@@ -132,7 +132,7 @@ int ophip_sgemm_strided_batched(int m, int n, int k, float alpha, unsigned A, in
                                 long long strideA, unsigned B, int ldb, long long strideB,
                                 float beta, unsigned C, int ldc, long long strideC, int batch);
 /* Y = alpha*A*X, A m x K column-major, X (K x ncp) and Y (m x ncp)
- * row-major, ncp in {16,32,48,64}, m % 16 == 0: one backward-induction
+ * row-major, ncp in {16,32,48}, m % 64 == 0: one backward-induction
  * step over many payoff columns (the role of sgemv3/sgemv4) */
 int ophip_sgemm_skinny(int m, int K, int ncp, float alpha, unsigned A, int lda, unsigned X,
                        int ldx, unsigned Y, int ldy);

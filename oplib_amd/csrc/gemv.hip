@@ -149,23 +149,41 @@ __global__ __launch_bounds__(64) void k_sgemv4(const char *__restrict__ base,
 
 // ---------------------------------------------------------------------
 // skinny GEMM: Y[m x NCP] = alpha * (A[m x K] * X[K x NCP] [+ X]), X and Y
-// row-major.  Grid (m/64 row blocks, KS K-chunks); each workgroup stages
-// its X chunk (kc x NCP) in LDS once and its 4 waves each stream 16 rows
-// of A over the chunk on the matrix cores, 4 k16-blocks of loads in
-// flight.  With KS > 1 the partial tiles go to a workspace and a second
-// kernel adds them in chunk order (deterministic, no atomics).
+// row-major.  Grid (m/64 row blocks, KS K-chunks of kc); workgroup = 4
+// waves on the same 64 rows, splitting the chunk's k16-blocks between them
+// (wave w takes blocks w, w+4, ...).  The X chunk is staged in LDS once.
+// A wave covers all 64 rows at once: lane (r, g) loads the float4
+// A[4r .. 4r+3][k] (256 contiguous bytes per k across the 16 lanes) and
+// feeds component v to MFMA tile v, so tile v holds rows 4i + v; one X
+// operand from LDS serves four MFMAs.  All A loads of a wave are issued
+// before its first MFMA.  The 4 waves' partial tiles are added in wave
+// order through LDS; with KS > 1 the workgroup partials go to a workspace
+// and k_skinny_combine adds them in chunk order (deterministic).
 // ---------------------------------------------------------------------
+constexpr int kSkKB = 4;  // k16-blocks per wave per chunk (kc = 16 * 4 * kSkKB = 256)
+
 template <int NT>
 __global__ __launch_bounds__(256) void k_skinny(int m, int K, int kc, const float *__restrict__ A,
                                                 int lda, const float *__restrict__ X, int ldx,
                                                 float *__restrict__ out, int ldo, float alpha,
                                                 int plus_x) {
     constexpr int XP = 16 * NT + 4;  // LDS row pitch (floats)
-    extern __shared__ float xs[];    // kc x XP
+    extern __shared__ float xs[];    // kc x XP, reused for the wave reduction
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int r = lane & 15, grp = lane >> 4;
-    const int row0 = blockIdx.x * 64 + 16 * wave;
+    const int row0 = blockIdx.x * 64;
     const int kbeg = blockIdx.y * kc, kn = min(K, kbeg + kc) - kbeg;
+    // A loads first (independent of the staging): k16-block b = wave + 4u
+    const float *Ar = A + row0 + 4 * r + (size_t)lda * kbeg;
+    f4 av[kSkKB][4];
+#pragma unroll
+    for (int u = 0; u < kSkKB; u++)
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const int k = 16 * (wave + 4 * u) + 4 * grp + s;
+            // past the chunk: a clamped (valid, finite) column times a zero X row
+            av[u][s] = *(const f4 *)(Ar + (size_t)lda * (k < kn ? k : kn - 1));
+        }
     // stage X[kbeg : kbeg+kn, 0 : 16NT] (rows of 16NT floats, float4 loads)
     constexpr int Q = 4 * NT;  // float4 per row
     for (int i = t; i < kc * Q; i += 256) {
@@ -175,78 +193,80 @@ __global__ __launch_bounds__(256) void k_skinny(int m, int K, int kc, const floa
         *(f4 *)&xs[k * XP + 4 * c4] = v;
     }
     __syncthreads();
-    f4 acc[NT];
+    f4 acc[4][NT];
 #pragma unroll
-    for (int j = 0; j < NT; j++) acc[j] = f4{0.f, 0.f, 0.f, 0.f};
-    const float *Ar = A + row0 + r + (size_t)lda * kbeg;
-    // two register stages of 4 k16-blocks: the A loads of the next 64 k
-    // are in flight while the current 64 k go through the matrix cores.
-    // kc is a multiple of 128; k beyond this chunk reads a clamped (valid,
-    // finite) address and meets a zero row of the staged X, so no select
-    // follows the load and all loads of a stage issue back to back.
-    constexpr int U = 4;
-    float sa[U][4], sb[U][4];
-    auto load = [&](float (&a)[U][4], int kb0) {
+    for (int v = 0; v < 4; v++)
 #pragma unroll
-        for (int u = 0; u < U; u++)
+        for (int j = 0; j < NT; j++) acc[v][j] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int s = 0; s < 4; s++) {
-                // lane group g of MFMA step s takes k = kb + 4g + s
-                const int k = kb0 + 16 * u + 4 * grp + s;
-                a[u][s] = Ar[(size_t)lda * (k < kn ? k : kn - 1)];
-            }
-    };
-    auto mma = [&](const float (&a)[U][4], int kb0) {
-        float xv[U][4][NT];  // all X operands of the stage first, then the MFMAs
+    for (int u = 0; u < kSkKB; u++) {
+        float xv[4][NT];
 #pragma unroll
-        for (int u = 0; u < U; u++)
+        for (int s = 0; s < 4; s++) {
+            const int k = 16 * (wave + 4 * u) + 4 * grp + s;
 #pragma unroll
-            for (int s = 0; s < 4; s++) {
-                const int k = kb0 + 16 * u + 4 * grp + s;
+            for (int j = 0; j < NT; j++) xv[s][j] = xs[k * XP + 16 * j + r];
+        }
 #pragma unroll
-                for (int j = 0; j < NT; j++) xv[u][s][j] = xs[k * XP + 16 * j + r];
-            }
+        for (int s = 0; s < 4; s++)
 #pragma unroll
-        for (int u = 0; u < U; u++)
+            for (int j = 0; j < NT; j++)
 #pragma unroll
-            for (int s = 0; s < 4; s++)
-#pragma unroll
-                for (int j = 0; j < NT; j++) acc[j] = mfma4(a[u][s], xv[u][s][j], acc[j]);
-    };
-    load(sa, 0);
-    for (int kb0 = 0; kb0 < kn; kb0 += 32 * U) {
-        load(sb, kb0 + 16 * U);
-        mma(sa, kb0);
-        load(sa, kb0 + 32 * U);
-        mma(sb, kb0 + 16 * U);
+                for (int v = 0; v < 4; v++) acc[v][j] = mfma4(av[u][s][v], xv[s][j], acc[v][j]);
     }
-    // lane holds rows 4*grp+e of column 16j+r
+    // add the 4 waves' tiles in wave order (xs is free once every wave is here)
+    __syncthreads();
+    f4 *red = (f4 *)xs;  // [wave][v][j][lane]
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+#pragma unroll
+        for (int j = 0; j < NT; j++) red[((wave * 4 + v) * NT + j) * 64 + lane] = acc[v][j];
+    __syncthreads();
+    // wave w finishes tile v = w: rows 4i + w, i = 4g + e
+    const int v = wave;
     float *o = out + (size_t)blockIdx.y * m * ldo;
     const bool direct = gridDim.y == 1;
 #pragma unroll
-    for (int j = 0; j < NT; j++)
+    for (int j = 0; j < NT; j++) {
+        f4 s = red[((0 * 4 + v) * NT + j) * 64 + lane];
+        s += red[((1 * 4 + v) * NT + j) * 64 + lane];
+        s += red[((2 * 4 + v) * NT + j) * 64 + lane];
+        s += red[((3 * 4 + v) * NT + j) * 64 + lane];
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-            const int row = row0 + 4 * grp + e;
-            float y = acc[j][e];
+            const int row = row0 + 4 * (4 * grp + e) + v;
+            float y = s[e];
             if (direct) {
                 y *= alpha;
                 if (plus_x) y += alpha * X[(size_t)row * ldx + 16 * j + r];
             }
             o[(size_t)row * ldo + 16 * j + r] = y;
         }
+    }
 }
 
 __global__ void k_skinny_combine(int m, int ncp, int KS, const float *__restrict__ part,
                                  float *__restrict__ Y, int ldy, float alpha,
                                  const float *__restrict__ X, int ldx, int plus_x) {
-    // four outputs per thread (ncp % 16 == 0, ldx % 4 == 0, ldy % 4 == 0)
+    // four outputs per thread (ncp % 16 == 0, ldx % 4 == 0, ldy % 4 == 0);
+    // partials loaded four at a time, added in chunk order
     const int idx = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
     if (idx >= m * ncp) return;
     const int i = idx / ncp, c = idx - i * ncp;
     const size_t mn = (size_t)m * ncp;
     f4 s = *(const f4 *)(part + idx);
-    for (int q = 1; q < KS; q++) s += *(const f4 *)(part + q * mn + idx);
+    int q = 1;
+    for (; q + 4 <= KS; q += 4) {
+        f4 p0 = *(const f4 *)(part + (q + 0) * mn + idx);
+        f4 p1 = *(const f4 *)(part + (q + 1) * mn + idx);
+        f4 p2 = *(const f4 *)(part + (q + 2) * mn + idx);
+        f4 p3 = *(const f4 *)(part + (q + 3) * mn + idx);
+        s += p0;
+        s += p1;
+        s += p2;
+        s += p3;
+    }
+    for (; q < KS; q++) s += *(const f4 *)(part + q * mn + idx);
     f4 y = alpha * s;
     if (plus_x) y += alpha * *(const f4 *)(X + (size_t)i * ldx + c);
     *(f4 *)(Y + (size_t)i * ldy + c) = y;
@@ -332,21 +352,16 @@ OPHIP_EXPORT void opcuda_sgemv4(int nblocks, unsigned argptr_bid) {
 static int skinny(int m, int K, int ncp, float alpha, unsigned A, int lda, unsigned X, int ldx,
                   unsigned Y, int ldy, int plus_x) {
     Device &D = dev();
-    if (m <= 0 || K <= 0 || (m % 64) || ncp <= 0 || ncp > 64 || (ncp % 16) || ldx < ncp ||
-        (ldx % 4) || (ldy % 4) || ldy < ncp || lda < m || (plus_x && m != K))
+    if (m <= 0 || K <= 0 || (m % 64) || ncp <= 0 || ncp > 64 || (ncp % 16) || ncp > 48 || ldx < ncp ||
+        (ldx % 4) || (ldy % 4) || ldy < ncp || lda < m || (lda % 4) || (plus_x && m != K))
         return (int)set_status(kErrBadArg);
     const float *Ap = P<const float>(A), *Xp = P<const float>(X);
     float *Yp = P<float>(Y);
-    if (!Ap || !Xp || !Yp || (((uintptr_t)Xp) & 15)) return (int)set_status(kErrBadHandle);
+    if (!Ap || !Xp || !Yp || (((uintptr_t)Xp | (uintptr_t)Ap) & 15))
+        return (int)set_status(kErrBadHandle);
     const int NT = ncp / 16;
-    const int kc_max = NT <= 3 ? 256 : 128;  // LDS: kc * (16 NT + 4) floats <= 52 KB
-    const int rb = m / 64;
-    const int ncu = D.n_cu > 0 ? D.n_cu : 256;
-    int KS = cdiv(2 * ncu, rb);  // about two workgroups per CU
-    if (KS < cdiv(K, kc_max)) KS = cdiv(K, kc_max);
-    int kc = cdiv(cdiv(K, KS), 128) * 128;  // whole pairs of 64-k stages
-    if (kc > kc_max) kc = kc_max;
-    KS = cdiv(K, kc);
+    const int kc = 64 * kSkKB;  // 4 waves x kSkKB k16-blocks
+    const int KS = cdiv(K, kc);
     float *out = Yp;
     int ldo = ldy;
     if (KS > 1) {
@@ -354,13 +369,16 @@ static int skinny(int m, int K, int ncp, float alpha, unsigned A, int lda, unsig
         if (!out) return (int)kErrOutOfArena;
         ldo = ncp;
     }
-    const size_t lds = sizeof(float) * (size_t)kc * (16 * NT + 4);
-    dim3 grid(rb, KS), block(256);
+    // LDS: the X chunk, or the 4 waves' partial tiles, whichever is larger
+    const size_t lds_x = sizeof(float) * (size_t)kc * (16 * NT + 4);
+    const size_t lds_r = sizeof(float) * 4 * 4 * NT * 64 * 4;
+    const size_t lds = lds_x > lds_r ? lds_x : lds_r;
+    dim3 grid(m / 64, KS), block(256);
     switch (NT) {
     case 1: hipLaunchKernelGGL(k_skinny<1>, grid, block, lds, D.stream, m, K, kc, Ap, lda, Xp, ldx, out, ldo, alpha, plus_x); break;
     case 2: hipLaunchKernelGGL(k_skinny<2>, grid, block, lds, D.stream, m, K, kc, Ap, lda, Xp, ldx, out, ldo, alpha, plus_x); break;
     case 3: hipLaunchKernelGGL(k_skinny<3>, grid, block, lds, D.stream, m, K, kc, Ap, lda, Xp, ldx, out, ldo, alpha, plus_x); break;
-    default: hipLaunchKernelGGL(k_skinny<4>, grid, block, lds, D.stream, m, K, kc, Ap, lda, Xp, ldx, out, ldo, alpha, plus_x); break;
+    default: break;
     }
     if (KS > 1)
         hipLaunchKernelGGL(k_skinny_combine, dim3(cdiv(m * ncp / 4, 256)), dim3(256), 0,

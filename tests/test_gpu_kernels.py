@@ -321,7 +321,7 @@ def test_sgemv3_gather_and_sentinel(H):
 def test_skinny_backward_step(H):
     banner("skinny")
     rng = np.random.default_rng(9)
-    for m, ncp in ((2048, 48), (256, 32), (384, 64), (128, 16)):
+    for m, ncp in ((2048, 48), (256, 32), (384, 48), (128, 16), (320, 16)):
         A = (rng.random((m, m)) / m).astype(np.float32)
         X = rng.random((m, ncp)).astype(np.float32)
         Ad, Xd = dev(H, A), dev(H, X, order="C")
