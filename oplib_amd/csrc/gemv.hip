@@ -173,7 +173,20 @@ __global__ __launch_bounds__(256) void k_skinny(int m, int K, int kc, const floa
     const int r = lane & 15, grp = lane >> 4;
     const int row0 = blockIdx.x * 64;
     const int kbeg = blockIdx.y * kc, kn = min(K, kbeg + kc) - kbeg;
-    // A loads first (independent of the staging): k16-block b = wave + 4u
+    // stage X[kbeg : kbeg+kn, 0 : 16NT] first (rows of 16NT floats, float4
+    // loads): the barrier below then waits for these loads only, and the A
+    // loads issued after them stay in flight while the first k16-blocks
+    // are multiplied (in-order completion counting)
+    constexpr int Q = 4 * NT;             // float4 per row
+    constexpr int KC = 64 * kSkKB;        // chunk rows (kc == KC)
+    constexpr int NX = KC * Q / 256;      // float4 per thread
+    f4 xl[NX];
+#pragma unroll
+    for (int q = 0; q < NX; q++) {        // all loads first, then the LDS stores
+        const int i = t + 256 * q, k = i / Q, c4 = i - k * Q;
+        xl[q] = *(const f4 *)(X + (size_t)(kbeg + (k < kn ? k : kn - 1)) * ldx + 4 * c4);
+    }
+    // A loads: k16-block b = wave + 4u
     const float *Ar = A + row0 + 4 * r + (size_t)lda * kbeg;
     f4 av[kSkKB][4];
 #pragma unroll
@@ -184,13 +197,11 @@ __global__ __launch_bounds__(256) void k_skinny(int m, int K, int kc, const floa
             // past the chunk: a clamped (valid, finite) column times a zero X row
             av[u][s] = *(const f4 *)(Ar + (size_t)lda * (k < kn ? k : kn - 1));
         }
-    // stage X[kbeg : kbeg+kn, 0 : 16NT] (rows of 16NT floats, float4 loads)
-    constexpr int Q = 4 * NT;  // float4 per row
-    for (int i = t; i < kc * Q; i += 256) {
-        const int k = i / Q, c4 = i - k * Q;
-        f4 v = {0.f, 0.f, 0.f, 0.f};
-        if (k < kn) v = *(const f4 *)(X + (size_t)(kbeg + k) * ldx + 4 * c4);
-        *(f4 *)&xs[k * XP + 4 * c4] = v;
+    // the X chunk to LDS: waits only for the X loads, issued before A
+#pragma unroll
+    for (int q = 0; q < NX; q++) {
+        const int i = t + 256 * q, k = i / Q, c4 = i - k * Q;
+        *(f4 *)&xs[k * XP + 4 * c4] = (k < kn) ? xl[q] : f4{0.f, 0.f, 0.f, 0.f};
     }
     __syncthreads();
     f4 acc[4][NT];
