@@ -1,24 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark: priced scenarios/s of the operator-method pricing path on one
-MI355X, with the error against FP64 stated.
+MI355X at <= 1e-4 relative price error against FP64.
 
 Workload (one "step" = one priced scenario): the 2048-state regime-
 switching local-vol lattice (256 price nodes x 8 vol regimes 0.1..0.8,
 regime rate 0.5/yr, r = 0.02), FP64 generator assembly fused with the first
-step on the GPU, 16 FP32 squarings of the 2048 x 2048 kernel to the daily
-step, then 252 backward-induction steps for 41 strikes.  The whole solve is
-one HIP-graph launch; inputs are resident in HBM before timing starts.
+step on the GPU, 16 squarings of the 2048 x 2048 kernel to the daily step,
+then 252 backward-induction steps for 41 strikes.  Each method's whole
+solve is one HIP-graph launch; inputs are resident in HBM before timing.
+`value` is the fastest method whose prices are within 1e-4 of the FP64
+oracle (the plain FP32 method, the drop-in, is timed beside it and fails
+that bar at this size).
 
 Also measured, in the same run:
-  roofline      the dominant kernel (the squaring GEMM) timed with HIP events
-                on the library stream, against the FP32 matrix-core peak;
-  gemv          the backward-induction step (skinny GEMM) in GB/s;
+  roofline      the headline method's dominant kernel, the backward step
+                (skinny GEMM), timed with HIP events on the library stream,
+                in GB/s against HBM; the FP32 squaring GEMM against the
+                FP32 matrix-core peak;
+  methods       every method's scenarios/s and error;
+  sweep         the 64-lattice calibration sweep, plain vs batched W form;
   mc            the MT19937 chain sampler on the daily table,
                 2^20 paths x 252 steps, in transitions/s;
+  surrogate     the learned surrogate with solver fallback (training
+                labels from the solver, held-out judgement, 2^16 draws);
   cpu_baseline  the CPU oracle's FP64 solve of the same scenario (the
                 reference's host method restated in C), on the host cores.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-surrogate ...]
 """
 from __future__ import annotations
 
@@ -59,6 +67,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mc", action="store_true", help="skip the Monte Carlo leg")
     ap.add_argument("--no-sweep", action="store_true", help="skip the calibration-sweep leg")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate leg")
+    ap.add_argument("--surr-train", type=int, default=65536, help="surrogate training draws")
+    ap.add_argument("--surr-steps", type=int, default=40000, help="surrogate Adam steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
@@ -146,43 +157,62 @@ def main():
     if not args.no_mc:
         mc = bench_mc(H, U, d, cfg)
 
+    # ---- learned surrogate with fallback ----
+    surrogate = None
+    if not args.no_surrogate:
+        surrogate = bench_surrogate(H, M, args.surr_train, args.surr_steps, log)
+
     # ---- CPU baseline: the oracle's FP64 solve of the same scenario ----
     cpu = None
     if not args.no_cpu:
         cpu = bench_cpu(cfg)
 
+    # the headline: the fastest method on this workload whose prices are
+    # within 1e-4 of FP64 (the plain FP32 method is reported beside it)
+    ok = {k: v for k, v in methods.items() if v["meets_1e-4"]}
+    best = max(ok, key=lambda k: ok[k]["scenarios_per_s"]) if ok else "plain_fp32"
+    bm = methods[best]
+    names = {"plain_fp32": "plain FP32", "wform_fp16x1": "U = I + W, FP16 x1 squarings",
+             "wform_fp16x3": "U = I + W, FP16 x3 (hi/lo) squarings",
+             "wform_bf16x3": "U = I + W, BF16 x3 (hi/lo) squarings"}
     line = {
-        "metric": "priced scenarios/s at <=1e-4 rel err vs FP64 (plain FP32 method)",
-        "value": value,
+        "metric": "priced scenarios/s at <=1e-4 rel err vs FP64",
+        "value": bm["scenarios_per_s"],
         "unit": "scenarios/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": bm["ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "fp32" if best == "plain_fp32" else best.split("_")[1][:4] + " (MFMA, fp32 accumulate) + fp32",
         "data": "synthetic",
         "config": {"workload": "regime-switching local vol, 256 x 8 = 2048 states, "
-                               "16 FP32 squarings to U(1/252), 252 backward steps, 41 strikes",
+                               "16 squarings to U(1/252), 252 backward steps, 41 strikes",
                    "d": d, "squarings": cfg.nsq, "backward_steps": cfg.nsteps,
-                   "strikes": cfg.nstrikes, "method": "plain FP32"},
-        "error_vs_fp64": {"max_rel": float(rel.max()),
-                          "rms_rel": float(np.sqrt(np.mean(rel ** 2))),
+                   "strikes": cfg.nstrikes, "method": names[best],
+                   "plain_fp32_scenarios_per_s": value,
+                   "speedup_vs_plain_fp32": bm["scenarios_per_s"] / value},
+        "error_vs_fp64": {"max_rel": bm["max_rel_err"],
+                          "rms_rel": bm.get("rms_rel_err"),
                           "denominator": "max(|p64|, 1e-3*max|p64|)",
                           "prices": int(rel.size),
-                          "meets_1e-4": bool(rel.max() <= 1e-4)},
-        "roofline": {"bound": "mfma", "kernel": "k_sgemm<128,128> (squaring)",
-                     "achieved": gemm_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": gemm_tflops / FP32_PEAK_TFLOPS, "traffic": None,
-                     "flop_per_launch": gemm_flop, "avg_launch_us": gemm_ms * 1e3},
-        "gemv": {"kernel": "k_skinny<3> + combine (backward step)", "avg_us": bi_us,
-                 "algorithmic_GBps": bi_gbs, "peak_GBps": HBM_PEAK_GBS,
-                 "frac": bi_gbs / HBM_PEAK_GBS},
+                          "meets_1e-4": bool(bm["meets_1e-4"])},
+        # dominant kernel of the headline method: the backward-induction
+        # step (252 per scenario) streams the 2048 x 2048 FP32 matrix
+        "roofline": {"bound": "hbm", "kernel": "backward step: k_skinny<3> + k_skinny_combine",
+                     "achieved": bi_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": bi_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bi_bytes, "avg_launch_us": bi_us},
+        "roofline_squaring": {"bound": "mfma", "kernel": "k_sgemm<128,128> (plain FP32 squaring)",
+                              "achieved": gemm_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                              "frac": gemm_tflops / FP32_PEAK_TFLOPS,
+                              "flop_per_launch": gemm_flop, "avg_launch_us": gemm_ms * 1e3},
         "methods": methods,
         "sweep": sweep,
         "mc": mc,
+        "surrogate": surrogate,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -276,16 +306,98 @@ def bench_mc(H, U, d, cfg):
             "streams": nrng, "d": d}
 
 
+def bench_surrogate(H, M, ntrain, steps, log_fn):
+    """Learned surrogate with fallback on the CEV-with-skew family: labels
+    from the mixed-precision batched solver, a 5-member ensemble trained on
+    the GPU, the spread threshold calibrated on separate draws, judged on
+    4096 held-out draws, then 2^16 inference draws priced with fallback."""
+    from oplib_amd import surrogate as S
+    out = {"workload": "1024-node CEV lattice with skew/curvature, 20 squarings to U(0.1), "
+                       "21 strikes x 10 maturities; params (sigma0, beta, r, a, b) uniform on the box",
+           "members": 5, "train_draws": ntrain, "train_steps": steps}
+    sol = S.PriceSolver(nb=64)
+    t0 = time.perf_counter()
+    ytr = sol.solve(S.draw_params(ntrain, S.SEED_TRAIN))
+    t_lab = time.perf_counter() - t0
+    out["label_solves_per_s"] = ntrain / t_lab
+    log_fn(f"[bench] surrogate labels: {ntrain} solves in {t_lab:.1f} s")
+    pho = S.draw_params(4096, S.SEED_HELDOUT)
+    yho = sol.solve(pho)
+    pcal = S.draw_params(2048, S.SEED_CAL)
+    ycal = sol.solve(pcal)
+    net = S.Surrogate(members=5, cap=4096, seed=7)
+    net.fit(S.draw_params(ntrain, S.SEED_TRAIN), ytr, steps=steps, batch=4096, lr=3e-3,
+            lr_end=1.5e-5)
+    out["train_seconds"] = net.train_seconds
+    log_fn(f"[bench] surrogate trained in {net.train_seconds:.1f} s")
+    sstar = net.calibrate(pcal, ycal, S.TOL)
+    out["spread_threshold"] = sstar
+    # held-out judgement against the solver
+    pr, sp, fl = net.evaluate(pho, tol=sstar)
+    e = S.rel_err(pr, yho)
+    served = fl == 0
+    e_cv = S.rel_err(S.control_variate(pho), yho)
+    out["heldout"] = {
+        "draws": len(pho), "served_fraction": float(served.mean()),
+        "served_max_rel_err": float(e[served].max()) if served.any() else None,
+        "served_over_1e-4": int((e[served] > S.TOL).sum()),
+        "network_median_rel_err": float(np.median(e)), "network_max_rel_err": float(e.max()),
+        "control_variate_median_rel_err": float(np.median(e_cv)),
+        "judge": "the FP16x3 W-form solver (within 1e-4 of FP64 on the committed fixture)"}
+    # inference: 2^16 resident draws, surrogate + fallback solves
+    n = 1 << 16
+    pin = S.draw_params(n, S.SEED_INFER)
+    prm = H.DeviceArray.from_numpy(pin.astype(np.float32), order="C")
+    prd, spd = H.DeviceArray.empty(n * S.NOUT), H.DeviceArray.empty(n)
+    fld = H.DeviceArray.empty(n, dtype=np.int32)
+    net.eval_into(n, prm, prd, spd, fld, sstar)
+    H.sync()
+    t0 = time.perf_counter()
+    net.eval_into(n, prm, prd, spd, fld, sstar)
+    H.sync()
+    t_eval = time.perf_counter() - t0
+    flags = fld.download()
+    bad = np.nonzero(flags)[0]
+    t0 = time.perf_counter()
+    if len(bad):
+        sol.solve(pin[bad])
+    t_fb = time.perf_counter() - t0
+    rate = n / (t_eval + t_fb)
+    out["inference"] = {"draws": n, "eval_ms": t_eval * 1e3, "network_draws_per_s": n / t_eval,
+                        "fallback_draws": int(len(bad)), "fallback_seconds": t_fb,
+                        "scenarios_per_s": rate}
+    # the methods without the network on the same family
+    mixed = len(bad) / t_fb if len(bad) >= 64 else None
+    sol.close()
+    plain = S.PriceSolver(nb=64, precision=M.OPM_PREC_FP32)
+    plain.solve(pin[:64])
+    t0 = time.perf_counter()
+    pp = plain.solve(pin[:256])
+    t_pl = time.perf_counter() - t0
+    plain.close()
+    out["plain_fp32_scenarios_per_s"] = 256 / t_pl
+    out["mixed_batched_scenarios_per_s"] = mixed if mixed else out["label_solves_per_s"]
+    out["speedup_vs_plain"] = rate / out["plain_fp32_scenarios_per_s"]
+    net.close()
+    log_fn(f"[bench] surrogate: {out}")
+    return out
+
+
 def bench_cpu(cfg):
     from oracle import oracle as O
     nth = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
     O.set_threads(nth)
     t0 = time.perf_counter()
-    p = O.solve_fp64(cfg)
-    t = time.perf_counter() - t0
-    log(f"[bench] CPU FP64 oracle solve {t:.2f} s on {nth} threads")
-    return {"value": 1.0 / t, "unit": "scenarios/s", "cores": nth, "kind": "port",
-            "sample": "one full FP64 solve of the same scenario (assembly, first step, "
+    k = 0
+    while True:  # whole solves until about 10 s (at most 8)
+        O.solve_fp64(cfg)
+        k += 1
+        t = time.perf_counter() - t0
+        if t >= 10.0 or k >= 8:
+            break
+    log(f"[bench] CPU FP64 oracle: {k} solves in {t:.2f} s on {nth} threads")
+    return {"value": k / t, "unit": "scenarios/s", "cores": nth, "kind": "port",
+            "sample": f"{k} full FP64 solves of the same scenario (assembly, first step, "
                       "16 DGEMM squarings at d=2048, 252 backward steps x 41 strikes) by the "
                       "C oracle with OpenMP", "seconds": t}
 

@@ -166,6 +166,17 @@ unsigned long ophip_wsq_plan(int n, const unsigned *states, const int *ds, const
                              int mode, int tile, int grid);
 int           ophip_wsq_exec(unsigned long plan, int nsq);
 void          ophip_wsq_plan_free(unsigned long plan);
+/* the same plan with a tile per lattice (64 or 128 each): one persistent
+ * launch per tile class and round */
+unsigned long ophip_wsq_plan_tiles(int n, const unsigned *states, const int *ds, const int *nsqs,
+                                   int mode, const int *tiles, int grid);
+/* autotuner: per size bucket the faster of 64- and 128-tiles (each bucket
+ * timed alone), then the fastest persistent grid (1, 2, 4 workgroups per
+ * CU) for the combined plan; cached per (mode, d, count) and per size
+ * signature.  Runs kernels on the states (contents are overwritten) and
+ * synchronizes: call it before writing W0, never under capture. */
+int           ophip_wsq_autotune(int n, const unsigned *states, const int *ds, const int *nsqs,
+                                 int mode, int *tiles, int *grid);
 
 /* Forward pricing of many lattices in one launch per step (calibration
  * sweep): p_k = p_{k-1} U, or p_{k-1}(I + W) when plus_identity[i]; prices
@@ -178,6 +189,44 @@ unsigned long ophip_fwd_plan(int n, const int *d, const unsigned *K, const int *
                              int nstrikes);
 int  ophip_fwd_exec(unsigned long plan);
 void ophip_fwd_plan_free(unsigned long plan);
+/* new rates for every lattice of a plan (waits for the library stream) */
+int  ophip_fwd_set_rates(unsigned long plan, const float *rate);
+
+/* Learned surrogate (no reference counterpart): an ensemble of MLPs
+ * 5 -> 256 -> 256 -> 256 -> 210, exact-erf GELU on the hidden layers, BF16
+ * operands on the matrix cores with FP32 accumulation, FP32 master
+ * weights and Adam moments.  Batches B % 64 == 0, B <= capacity.
+ * Parameter arrays (ophip_mlp_param_count() floats per member):
+ * W1[256][32] b1[256] W2[256][256] b2[256] W3[256][256] b3[256]
+ * W4[256][256] b4[256] with W_l[out][in]; W1 columns 5..31 and W4/b4 rows
+ * 210..255 are padding and stay 0. */
+unsigned long ophip_mlp_create(int members, int batch_cap);
+void ophip_mlp_free(unsigned long net);
+int  ophip_mlp_param_count(void);
+/* which: 0 weights, 1 gradient, 2/3 Adam moments (member's kPar floats),
+ * 4/5 output mean/scale (256 floats); uploading weights resets Adam's t */
+int  ophip_mlp_params(unsigned long net, int member, int which, float *host, int to_device);
+/* Y[e][b][256] for every member e from normalized inputs X (B x 5) */
+int  ophip_mlp_forward(unsigned long net, int B, unsigned X, unsigned Y);
+/* gradients: member e trains on rows idx[e*B + b] of X (rows x 5) and T
+ * (rows x 210), loss sum (y - T)^2 / (B * 210); or, with G != 0, the
+ * gradient of sum(y .* G), G members x B x 256 */
+int  ophip_mlp_backward(unsigned long net, int B, unsigned idx, unsigned X, unsigned T,
+                        unsigned G);
+int  ophip_mlp_adam(unsigned long net, float lr, float beta1, float beta2, float eps);
+/* squared-error sums of the last MSE backward, one double per member */
+int  ophip_mlp_loss(unsigned long net, int B, double *out);
+/* surrogate evaluation of parameter draws prm (B x 5: sigma0, beta, r, a,
+ * b): prices (B x 210, maturity-major) = Black-Scholes control variate at
+ * the local vol of sqrt(spot K) + mean + scale * ensemble mean; spread =
+ * max over outputs of the ensemble std / max(|price|, 1e-3 max|price|);
+ * flags = spread > tol (draws for the operator solver) */
+int  ophip_surr_eval(unsigned long net, int B, unsigned prm, const float *lo, const float *hi,
+                     float spot, float dT, int nmat, const float *strikes, int nstrikes,
+                     float tol, unsigned prices, unsigned spread, unsigned flags);
+/* the control variate alone (B x 210) */
+int  ophip_surr_base(int B, unsigned prm, float spot, float dT, int nmat, const float *strikes,
+                     int nstrikes, unsigned base);
 
 /* Monte Carlo */
 int ophip_mc_set_nrng(int nstreams);

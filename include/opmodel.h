@@ -98,6 +98,9 @@ int opm_batch_capture(opm_batch *b);
 int opm_batch_launch(opm_batch *b);
 /* prices[n][nsteps][nstrikes] */
 int opm_batch_prices(opm_batch *b, double *prices);
+/* new drift/vol/rate for every lattice, same grids, squaring counts,
+ * maturities and strikes: plans and the captured graph stay valid */
+int opm_batch_update(opm_batch *b, const opm_config *cfgs);
 
 #ifdef __cplusplus
 }

@@ -56,6 +56,7 @@ struct FwdPlan {
     void *dev = nullptr;
     FwdScen *sc = nullptr;
     float *strikes = nullptr;
+    FwdScen *host = nullptr;  // host mirror of the scenario table
 };
 
 }  // namespace
@@ -108,9 +109,10 @@ OPHIP_EXPORT unsigned long ophip_fwd_plan(int n, const int *d, const unsigned *K
         e = hipMemcpy(pl->sc, h, sizeof(FwdScen) * n, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(pl->strikes, strikes, sizeof(float) * nstrikes, hipMemcpyHostToDevice);
     }
-    delete[] h;
+    pl->host = h;
     if (e != hipSuccess) {
-        if (pl->dev) hipFree(pl->dev);
+        if (pl->dev) (void)hipFree(pl->dev);
+        delete[] pl->host;
         delete pl;
         set_status(hip_status(e));
         return 0;
@@ -136,7 +138,21 @@ OPHIP_EXPORT int ophip_fwd_exec(unsigned long plan) {
 OPHIP_EXPORT void ophip_fwd_plan_free(unsigned long plan) {
     FwdPlan *pl = (FwdPlan *)(uintptr_t)plan;
     if (!pl) return;
-    hipStreamSynchronize(dev().stream);
-    hipFree(pl->dev);
+    (void)hipStreamSynchronize(dev().stream);
+    (void)hipFree(pl->dev);
+    delete[] pl->host;
     delete pl;
+}
+
+// new discount rates for every lattice of a plan (same plan, same graph):
+// waits for the library stream, then rewrites the device table
+OPHIP_EXPORT int ophip_fwd_set_rates(unsigned long plan, const float *rate) {
+    FwdPlan *pl = (FwdPlan *)(uintptr_t)plan;
+    if (!pl || !rate) return (int)set_status(kErrBadArg);
+    for (int i = 0; i < pl->n; i++) pl->host[i].rate = rate[i];
+    hipError_t e = hipStreamSynchronize(dev().stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(pl->sc, pl->host, sizeof(FwdScen) * pl->n, hipMemcpyHostToDevice, dev().stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(dev().stream);
+    return (int)set_status(hip_status(e));
 }

@@ -425,7 +425,12 @@ struct opm_batch {
     std::vector<int> nsq;
     std::vector<int> d, x0;
     std::vector<double> dt0;
+    // x, drift, vol, x32, P and out are views into one block `blk`:
+    // [drift_0 vol_0 drift_1 vol_1 ...][out_0 out_1 ...][x_i x32_i P_i ...]
+    // so that new parameters are one upload and all prices one download
     std::vector<Buf> x, drift, vol, x32, P, out, state, U1;  // U1: plain path ping-pong partner
+    Buf blk;
+    size_t dv_bytes = 0, out_off = 0;
     unsigned long wplan = 0, fplan = 0, graph = 0;
 };
 
@@ -436,8 +441,9 @@ void opm_batch_destroy(opm_batch *b) {
     if (b->graph) ophip_graph_destroy(b->graph);
     if (b->wplan) ophip_wsq_plan_free(b->wplan);
     if (b->fplan) ophip_fwd_plan_free(b->fplan);
-    for (auto *v : {&b->x, &b->drift, &b->vol, &b->x32, &b->P, &b->out, &b->state, &b->U1})
+    for (auto *v : {&b->state, &b->U1})
         for (Buf &q : *v) release(q);
+    release(b->blk);
     delete b;
 }
 
@@ -473,6 +479,37 @@ opm_batch *opm_batch_create(const opm_config *cfgs, int n, int tile) {
     std::vector<unsigned> Kh(n), Ph(n), xh(n), oh(n);
     std::vector<int> plus(n, b->wmode >= 0 ? 1 : 0);
     std::vector<float> rate(n);
+    {  // the block of small arrays
+        size_t off = 0;
+        for (int i = 0; i < n; i++) off += 16 * (size_t)cfgs[i].nx;
+        b->dv_bytes = off;
+        off = (off + 255) & ~(size_t)255;
+        b->out_off = off;
+        off += 4 * (size_t)n * b->nmat * b->nstrikes;
+        for (int i = 0; i < n; i++) {
+            off = (off + 255) & ~(size_t)255;
+            off += 12 * (size_t)cfgs[i].nx + 4 * (size_t)(b->nmat + 1) * cfgs[i].nx + 256;
+        }
+        ok = alloc(b->blk, off);
+        size_t dv = 0, tail = b->out_off + 4 * (size_t)n * b->nmat * b->nstrikes;
+        for (int i = 0; i < n && ok; i++) {
+            const size_t d = (size_t)cfgs[i].nx;
+            auto view = [&](Buf &q, size_t at, size_t bytes) {
+                q.h = b->blk.h + (unsigned)at;
+                q.bytes = bytes;
+            };
+            view(b->drift[i], dv, 8 * d);
+            view(b->vol[i], dv + 8 * d, 8 * d);
+            dv += 16 * d;
+            view(b->out[i], b->out_off + 4 * (size_t)i * b->nmat * b->nstrikes,
+                 4 * (size_t)b->nmat * b->nstrikes);
+            tail = (tail + 255) & ~(size_t)255;
+            view(b->x[i], tail, 8 * d);
+            view(b->x32[i], tail + 8 * d, 4 * d);
+            view(b->P[i], tail + 12 * d + 256 - (12 * d) % 256, 4 * (size_t)(b->nmat + 1) * d);
+            tail += 12 * d + 4 * (size_t)(b->nmat + 1) * d + 256;
+        }
+    }
     for (int i = 0; i < n && ok; i++) {
         const opm_config *c = &b->cfg[i];
         const int d = c->nx;
@@ -482,10 +519,6 @@ opm_batch *opm_batch_create(const opm_config *cfgs, int n, int tile) {
         make_grid(c, x, b->x0[i]);
         make_params(c, x, dr, vo);
         const size_t dd = (size_t)d * d;
-        ok = alloc(b->x[i], 8 * (size_t)d) && alloc(b->drift[i], 8 * (size_t)d) &&
-             alloc(b->vol[i], 8 * (size_t)d) && alloc(b->x32[i], 4 * (size_t)d) &&
-             alloc(b->P[i], 4 * (size_t)(b->nmat + 1) * d) &&
-             alloc(b->out[i], 4 * (size_t)b->nmat * b->nstrikes);
         if (ok && b->wmode >= 0) ok = alloc(b->state[i], ophip_wsq_bytes(d, b->wmode));
         if (ok && b->wmode < 0) ok = alloc(b->state[i], 4 * dd) && alloc(b->U1[i], 4 * dd);
         if (!ok) break;
@@ -504,7 +537,17 @@ opm_batch *opm_batch_create(const opm_config *cfgs, int n, int tile) {
     if (ok && b->wmode >= 0) {
         std::vector<unsigned> st(n);
         for (int i = 0; i < n; i++) st[i] = b->state[i].h;
-        b->wplan = ophip_wsq_plan(n, st.data(), b->d.data(), b->nsq.data(), b->wmode, tile, 0);
+        if (tile == 0) {  // the autotuner's tile per size bucket and grid
+            std::vector<int> tiles(n, 128);
+            int grid = 0;
+            ok = ophip_wsq_autotune(n, st.data(), b->d.data(), b->nsq.data(), b->wmode, tiles.data(),
+                                    &grid) == 0;
+            if (ok)
+                b->wplan = ophip_wsq_plan_tiles(n, st.data(), b->d.data(), b->nsq.data(), b->wmode,
+                                                tiles.data(), grid);
+        } else {
+            b->wplan = ophip_wsq_plan(n, st.data(), b->d.data(), b->nsq.data(), b->wmode, tile, 0);
+        }
         ok = b->wplan != 0;
     }
     if (ok) {
@@ -572,16 +615,47 @@ int opm_batch_capture(opm_batch *b) {
 
 int opm_batch_launch(opm_batch *b) { return (b && b->graph) ? ophip_graph_launch(b->graph) : 1; }
 
+/* new model parameters for every lattice of a batch, keeping its plans
+ * and captured graph: each cfgs[i] must match the lattice it replaces in
+ * grid (nx, nr, xlo, xhi, spot), squaring count, maturities and strikes;
+ * drift, vol and rate may change */
+int opm_batch_update(opm_batch *b, const opm_config *cfgs) {
+    if (!b || !cfgs) return 1;
+    std::vector<float> rate(b->n);
+    for (int i = 0; i < b->n; i++) {
+        const opm_config &c = cfgs[i], &o = b->cfg[i];
+        if (!valid(&c) || c.nx != o.nx || c.nr != o.nr || c.xlo != o.xlo || c.xhi != o.xhi ||
+            c.spot != o.spot || c.nsq != o.nsq || c.DeltaT != o.DeltaT || c.nsteps != o.nsteps ||
+            c.precision != o.precision || c.nstrikes != o.nstrikes ||
+            std::memcmp(c.strikes, o.strikes, sizeof(double) * c.nstrikes) != 0)
+            return 0x10003;  /* bad argument (the library's status code) */
+    }
+    std::vector<double> dv(b->dv_bytes / 8);
+    size_t o = 0;
+    for (int i = 0; i < b->n; i++) {
+        const opm_config *c = &cfgs[i];
+        std::vector<double> x, dr, vo;
+        int x0 = 0;
+        make_grid(c, x, x0);
+        make_params(c, x, dr, vo);
+        std::copy(dr.begin(), dr.end(), dv.begin() + o);
+        std::copy(vo.begin(), vo.end(), dv.begin() + o + dr.size());
+        o += dr.size() + vo.size();
+        b->cfg[i] = *c;
+        rate[i] = (float)c->rate;
+    }
+    int st = ophip_fwd_set_rates(b->fplan, rate.data());  // waits for the stream first
+    if (st) return st;
+    return opcuda_memcpy_h2d(b->blk.h, (unsigned long)(uintptr_t)dv.data(), (unsigned)b->dv_bytes);
+}
+
 int opm_batch_prices(opm_batch *b, double *prices) {
     if (!b) return 1;
-    opcuda_thread_synchronize();
-    std::vector<float> o((size_t)b->nmat * b->nstrikes);
-    for (int i = 0; i < b->n; i++) {
-        int st = opcuda_memcpy_d2h(b->out[i].h, (unsigned long)(uintptr_t)o.data(),
-                                   (unsigned)(4 * o.size()));
-        if (st) return st;
-        for (size_t e = 0; e < o.size(); e++) prices[(size_t)i * o.size() + e] = o[e];
-    }
+    std::vector<float> o((size_t)b->n * b->nmat * b->nstrikes);
+    int st = opcuda_memcpy_d2h(b->blk.h + (unsigned)b->out_off, (unsigned long)(uintptr_t)o.data(),
+                               (unsigned)(4 * o.size()));
+    if (st) return st;
+    for (size_t e = 0; e < o.size(); e++) prices[e] = o[e];
     return 0;
 }
 

@@ -31,8 +31,13 @@
 #include "ophip_internal.h"
 
 #include <algorithm>
+#include <array>
+#include <map>
+#include <mutex>
 #include <type_traits>
 #include <vector>
+
+extern "C" unsigned opcuda_get_status(void);
 
 using namespace ophip;
 
@@ -460,48 +465,46 @@ template <class F> unsigned launch_mode(int mode, F f) {
 // kernel its own niter, CModel Common.cs:289-307): round r squares only
 // the lattices with nsq > r, as the reference's njtodo_r tables do
 // (CModel Common.cs:508-545, Device.cs:222-224).
+// Work items are kept per tile class (c = 0: 64x64 tiles, c = 1: 128x128):
+// each lattice squares with the tile its plan gives it, and a round
+// launches one persistent kernel per non-empty class.
 struct WsqPlan {
-    int n = 0, mode = 0, tile = 128, grid = 0, dmax = 0, rounds = 0;
-    std::vector<int> nsq;
+    int n = 0, mode = 0, grid = 0, dmax = 0, rounds = 0;
+    std::vector<int> nsq, tiles;
     int n_split = 0;
-    std::vector<int> n_round;       // items per round (per-lattice counts)
-    int n_all[2] = {0, 0};          // items per parity (uniform count)
+    std::vector<int> n_round[2];    // items per round and class (per-lattice counts)
+    int n_all[2][2] = {{0, 0}, {0, 0}};  // items per class and parity (uniform count)
     void *dev = nullptr;            // one hipMalloc: tables below
     WScen *sc = nullptr, *sc_split = nullptr;
     int *nsq_dev = nullptr;
-    WItem *it_split = nullptr, *it_all[2] = {nullptr, nullptr};
-    std::vector<WItem *> it_round;
+    WItem *it_split = nullptr, *it_all[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    std::vector<WItem *> it_round[2];
 };
 
 // Plan the W-form pipeline for `n` lattices (states[i]: arena handle of a
 // state of ophip_wsq_bytes(ds[i], mode) bytes, ds[i] % 128 == 0; nsqs[i]:
 // its number of squarings, or null for a count given at exec time).
-// tile: 128 or 64 (0 = choose by the number of 128-tiles); grid: persistent
+// tiles[i]: 64 or 128, the tile lattice i squares with; grid: persistent
 // workgroups per squaring launch (0 = 2 per CU).  Work items of all
-// lattices are interleaved tile-major so uneven sizes balance.
-OPHIP_EXPORT unsigned long ophip_wsq_plan(int n, const unsigned *states, const int *ds,
-                                          const int *nsqs, int mode, int tile, int grid) {
+// lattices of a class are interleaved tile-major so uneven sizes balance.
+static WsqPlan *make_plan(int n, const unsigned *states, const int *ds, const int *nsqs, int mode,
+                          const int *tiles, int grid) {
     Device &D = dev();
-    if (n <= 0 || n > kMaxScen || mode < 0 || mode > 2) { set_status(kErrBadArg); return 0; }
+    if (n <= 0 || n > kMaxScen || mode < 0 || mode > 2) { set_status(kErrBadArg); return nullptr; }
     for (int i = 0; i < n; i++) {
         if (ds[i] <= 0 || ds[i] % 128 || !ptr(states[i]) || (nsqs && (nsqs[i] < 0 || nsqs[i] > 64)) ||
-            !handle_range_ok(states[i], wsq_layout(ds[i], mode).total)) {
+            !handle_range_ok(states[i], wsq_layout(ds[i], mode).total) ||
+            (tiles[i] != 64 && tiles[i] != 128)) {
             set_status(kErrBadArg);
-            return 0;
+            return nullptr;
         }
     }
     const int ncu = D.n_cu > 0 ? D.n_cu : 256;
-    if (tile == 0) {
-        long t128 = 0;
-        for (int i = 0; i < n; i++) t128 += (long)(ds[i] / 128) * (ds[i] / 128);
-        tile = (t128 >= (long)ncu) ? 128 : 64;
-    }
-    if (tile != 64 && tile != 128) { set_status(kErrBadArg); return 0; }
     WsqPlan *P = new WsqPlan();
     P->n = n;
     P->mode = mode;
-    P->tile = tile;
     P->grid = grid > 0 ? grid : 2 * ncu;
+    P->tiles.assign(tiles, tiles + n);
     P->nsq.assign(n, 0);
     if (nsqs)
         for (int i = 0; i < n; i++) {
@@ -517,28 +520,33 @@ OPHIP_EXPORT unsigned long ophip_wsq_plan(int n, const unsigned *states, const i
         sc_split[i].W = sc[i].W;
         P->dmax = std::max(P->dmax, ds[i]);
     }
-    auto items_for = [&](int T, int dir, int round) {
+    // T: tile of the items; cls: lattices of tile class cls (-1: all)
+    auto items_for = [&](int T, int cls, int dir, int round) {
         std::vector<WItem> v;
         const int maxt = P->dmax / T;
         for (int tm = 0; tm < maxt; tm++)
             for (int tn = 0; tn < maxt; tn++)
                 for (int i = 0; i < n; i++)
-                    if (tm < ds[i] / T && tn < ds[i] / T && (round < 0 || P->nsq[i] > round))
+                    if (tm < ds[i] / T && tn < ds[i] / T && (round < 0 || P->nsq[i] > round) &&
+                        (cls < 0 || (tiles[i] == 128) == (cls == 1)))
                         v.push_back(WItem{dir * n + i, tm, tn, 0});
         return v;
     };
     std::vector<std::vector<WItem>> lists;
-    lists.push_back(items_for(64, 0, -1));
-    lists.push_back(items_for(tile, 0, -1));
-    lists.push_back(items_for(tile, 1, -1));
-    for (int r = 0; r < P->rounds; r++) lists.push_back(items_for(tile, r & 1, r));
+    lists.push_back(items_for(64, -1, 0, -1));
+    for (int c = 0; c < 2; c++) {
+        const int T = c ? 128 : 64;
+        lists.push_back(items_for(T, c, 0, -1));
+        lists.push_back(items_for(T, c, 1, -1));
+        for (int r = 0; r < P->rounds; r++) lists.push_back(items_for(T, c, r & 1, r));
+    }
     size_t nitems = 0;
     for (auto &l : lists) nitems += l.size();
     size_t bytes = sizeof(WScen) * 3 * n + sizeof(int) * (n + 4) + sizeof(WItem) * nitems;
     if (hipMalloc(&P->dev, bytes) != hipSuccess) {
         delete P;
         set_status(kErrOutOfArena);
-        return 0;
+        return nullptr;
     }
     char *p = (char *)P->dev;
     P->sc = (WScen *)p;
@@ -552,13 +560,16 @@ OPHIP_EXPORT unsigned long ophip_wsq_plan(int n, const unsigned *states, const i
     }
     P->it_split = where[0];
     P->n_split = (int)lists[0].size();
-    P->it_all[0] = where[1];
-    P->n_all[0] = (int)lists[1].size();
-    P->it_all[1] = where[2];
-    P->n_all[1] = (int)lists[2].size();
-    for (int r = 0; r < P->rounds; r++) {
-        P->it_round.push_back(where[3 + r]);
-        P->n_round.push_back((int)lists[3 + r].size());
+    size_t w = 1;
+    for (int c = 0; c < 2; c++) {
+        for (int src = 0; src < 2; src++, w++) {
+            P->it_all[c][src] = where[w];
+            P->n_all[c][src] = (int)lists[w].size();
+        }
+        for (int r = 0; r < P->rounds; r++, w++) {
+            P->it_round[c].push_back(where[w]);
+            P->n_round[c].push_back((int)lists[w].size());
+        }
     }
     hipError_t e = hipMemcpy(P->sc, sc.data(), sizeof(WScen) * 2 * n, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(P->sc_split, sc_split.data(), sizeof(WScen) * n, hipMemcpyHostToDevice);
@@ -567,20 +578,43 @@ OPHIP_EXPORT unsigned long ophip_wsq_plan(int n, const unsigned *states, const i
         if (!lists[l].empty())
             e = hipMemcpy(where[l], lists[l].data(), sizeof(WItem) * lists[l].size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        hipFree(P->dev);
+        (void)hipFree(P->dev);
         delete P;
         set_status(hip_status(e));
-        return 0;
+        return nullptr;
     }
     set_status(kOk);
-    return (unsigned long)(uintptr_t)P;
+    return P;
+}
+
+// tile: 128 or 64 for every lattice (0 = 128 when the lattices hold at
+// least one 128-tile per CU, else 64)
+OPHIP_EXPORT unsigned long ophip_wsq_plan(int n, const unsigned *states, const int *ds,
+                                          const int *nsqs, int mode, int tile, int grid) {
+    if (n <= 0 || n > kMaxScen || !ds) { set_status(kErrBadArg); return 0; }
+    if (tile == 0) {
+        const int ncu = dev().n_cu > 0 ? dev().n_cu : 256;
+        long t128 = 0;
+        for (int i = 0; i < n; i++) t128 += (long)(ds[i] / 128) * (ds[i] / 128);
+        tile = (t128 >= (long)ncu) ? 128 : 64;
+    }
+    std::vector<int> tiles(n, tile);
+    return (unsigned long)(uintptr_t)make_plan(n, states, ds, nsqs, mode, tiles.data(), grid);
+}
+
+// the same with a tile per lattice (64 or 128 each)
+OPHIP_EXPORT unsigned long ophip_wsq_plan_tiles(int n, const unsigned *states, const int *ds,
+                                                const int *nsqs, int mode, const int *tiles,
+                                                int grid) {
+    if (!tiles) { set_status(kErrBadArg); return 0; }
+    return (unsigned long)(uintptr_t)make_plan(n, states, ds, nsqs, mode, tiles, grid);
 }
 
 OPHIP_EXPORT void ophip_wsq_plan_free(unsigned long plan) {
     WsqPlan *P = (WsqPlan *)(uintptr_t)plan;
     if (!P) return;
-    hipStreamSynchronize(dev().stream);
-    hipFree(P->dev);
+    (void)hipStreamSynchronize(dev().stream);
+    (void)hipFree(P->dev);
     delete P;
 }
 
@@ -604,19 +638,25 @@ OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
                            P->n, (const int *)nullptr, 0);
         for (int rr = 0; rr < rounds; rr++) {
             const int src = rr & 1;
-            const int nit = per ? P->n_round[rr] : P->n_all[src];
-            const WItem *its = per ? P->it_round[rr] : P->it_all[src];
-            if (nit == 0) continue;
-            const int g = nit < P->grid ? nit : P->grid;
-            // work items carry s = src*n + i (the full table); the fix kernel
-            // indexes lattices 0..n-1 of the table half for this direction
+            bool any = false;
+            for (int c = 1; c >= 0; c--) {
+                const int nit = per ? P->n_round[c][rr] : P->n_all[c][src];
+                const WItem *its = per ? P->it_round[c][rr] : P->it_all[c][src];
+                if (nit == 0) continue;
+                any = true;
+                const int g = nit < P->grid ? nit : P->grid;
+                // work items carry s = src*n + i (the full table)
+                if (c == 1)
+                    hipLaunchKernelGGL((k_wsq<BF, LO, 128>), dim3(g), dim3(256), 0, D.stream, D.base,
+                                       P->sc, its, nit);
+                else
+                    hipLaunchKernelGGL((k_wsq<BF, LO, 64>), dim3(g), dim3(256), 0, D.stream, D.base,
+                                       P->sc, its, nit);
+            }
+            if (!any) continue;
+            // the fix kernel indexes lattices 0..n-1 of the table half for
+            // this direction
             const WScen *scn = P->sc + (src ? P->n : 0);
-            if (P->tile == 128)
-                hipLaunchKernelGGL((k_wsq<BF, LO, 128>), dim3(g), dim3(256), 0, D.stream, D.base,
-                                   P->sc, its, nit);
-            else
-                hipLaunchKernelGGL((k_wsq<BF, LO, 64>), dim3(g), dim3(256), 0, D.stream, D.base,
-                                   P->sc, its, nit);
             hipLaunchKernelGGL((k_wfix<BF, LO>), fixgrid, dim3(64), 0, D.stream, D.base, scn, P->n,
                                per ? (const int *)P->nsq_dev : (const int *)nullptr, rr);
         }
@@ -642,4 +682,112 @@ OPHIP_EXPORT unsigned ophip_wsq_array(unsigned state, int d, int mode, int buf, 
     else off = L.H[buf][which - 1];
     if (off == (size_t)-1) return 0;
     return state + (unsigned)off;
+}
+
+// ---------------------------------------------------------------------
+// Autotuner: a tile per size bucket, then the persistent grid.
+// Each bucket (the lattices of one size d) is planned alone with 64- and
+// with 128-tiles and the faster one is kept; the combined plan then tries
+// 1, 2 and 4 workgroups per CU.  Timing runs the plan's split and two
+// squarings on the states as they are (their contents do not matter to
+// the timing and are overwritten by the caller's next W0).  Results are
+// cached per (mode, d, lattice count) and per full size signature.
+// ---------------------------------------------------------------------
+namespace {
+
+std::mutex g_tune_mu;
+std::map<std::array<int, 3>, int> g_tile_cache;                 // (mode, d, count) -> tile
+std::map<std::vector<int>, int> g_grid_cache;                   // (mode, d..., tile...) -> grid
+
+float time_plan(WsqPlan *P, int reps) {
+    const unsigned long h = (unsigned long)(uintptr_t)P;
+    if (ophip_wsq_exec(h, 2)) return -1.0f;  // warm-up
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return -1.0f;
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return -1.0f;
+    }
+    float ms = -1.0f;
+    hipStream_t s = dev().stream;
+    if (hipEventRecord(a, s) == hipSuccess) {
+        bool ok = true;
+        for (int r = 0; r < reps && ok; r++) ok = ophip_wsq_exec(h, 2) == 0;
+        if (ok && hipEventRecord(b, s) == hipSuccess && hipEventSynchronize(b) == hipSuccess)
+            (void)hipEventElapsedTime(&ms, a, b);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms;
+}
+
+}  // namespace
+
+// tiles[i] (64/128) and *grid for a plan of these lattices; returns 0 or
+// a status.  Not graph-capturable (it synchronizes).
+OPHIP_EXPORT int ophip_wsq_autotune(int n, const unsigned *states, const int *ds, const int *nsqs,
+                                    int mode, int *tiles, int *grid) {
+    if (n <= 0 || n > kMaxScen || !states || !ds || !tiles || !grid || mode < 0 || mode > 2)
+        return (int)set_status(kErrBadArg);
+    (void)nsqs;
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    const int ncu = dev().n_cu > 0 ? dev().n_cu : 256;
+    std::map<int, std::vector<int>> buckets;
+    for (int i = 0; i < n; i++) buckets[ds[i]].push_back(i);
+    for (auto &kv : buckets) {
+        const int d = kv.first;
+        const std::vector<int> &ids = kv.second;
+        const std::array<int, 3> key = {mode, d, (int)ids.size()};
+        int best = 0;
+        auto it = g_tile_cache.find(key);
+        if (it != g_tile_cache.end()) {
+            best = it->second;
+        } else {
+            std::vector<unsigned> st;
+            std::vector<int> dd;
+            for (int i : ids) {
+                st.push_back(states[i]);
+                dd.push_back(ds[i]);
+            }
+            float tbest = -1.0f;
+            for (int T : {128, 64}) {
+                std::vector<int> tt(ids.size(), T);
+                WsqPlan *P = make_plan((int)ids.size(), st.data(), dd.data(), nullptr, mode, tt.data(), 0);
+                if (!P) return (int)opcuda_get_status();
+                const float ms = time_plan(P, 3);
+                ophip_wsq_plan_free((unsigned long)(uintptr_t)P);
+                if (ms < 0.0f) return (int)set_status(kErrUnsupported);
+                if (tbest < 0.0f || ms < tbest) {
+                    tbest = ms;
+                    best = T;
+                }
+            }
+            g_tile_cache[key] = best;
+        }
+        for (int i : ids) tiles[i] = best;
+    }
+    std::vector<int> sig = {mode};
+    for (int i = 0; i < n; i++) sig.push_back(ds[i]);
+    for (int i = 0; i < n; i++) sig.push_back(tiles[i]);
+    auto git = g_grid_cache.find(sig);
+    if (git != g_grid_cache.end()) {
+        *grid = git->second;
+        return (int)set_status(kOk);
+    }
+    float gbest = -1.0f;
+    int g_best = 2 * ncu;
+    for (int g : {2 * ncu, ncu, 4 * ncu}) {
+        WsqPlan *P = make_plan(n, states, ds, nullptr, mode, tiles, g);
+        if (!P) return (int)opcuda_get_status();
+        const float ms = time_plan(P, 3);
+        ophip_wsq_plan_free((unsigned long)(uintptr_t)P);
+        if (ms < 0.0f) return (int)set_status(kErrUnsupported);
+        if (gbest < 0.0f || ms < gbest) {
+            gbest = ms;
+            g_best = g;
+        }
+    }
+    g_grid_cache[sig] = g_best;
+    *grid = g_best;
+    return (int)set_status(kOk);
 }

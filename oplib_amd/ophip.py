@@ -86,11 +86,27 @@ SIGNATURES = {
                                I32]),
     "ophip_wsq_exec": (I32, [ULONG, I32]),
     "ophip_wsq_plan_free": (V, [ULONG]),
+    "ophip_wsq_plan_tiles": (ULONG, [I32, C.POINTER(U32), C.POINTER(I32), C.POINTER(I32), I32,
+                                     C.POINTER(I32), I32]),
+    "ophip_wsq_autotune": (I32, [I32, C.POINTER(U32), C.POINTER(I32), C.POINTER(I32), I32,
+                                 C.POINTER(I32), C.POINTER(I32)]),
     "ophip_fwd_plan": (ULONG, [I32, C.POINTER(I32), C.POINTER(U32), C.POINTER(I32),
                                C.POINTER(U32), C.POINTER(U32), C.POINTER(U32), C.POINTER(F32), F32,
                                I32, C.POINTER(F32), I32]),
     "ophip_fwd_exec": (I32, [ULONG]),
     "ophip_fwd_plan_free": (V, [ULONG]),
+    "ophip_fwd_set_rates": (I32, [ULONG, C.POINTER(F32)]),
+    "ophip_mlp_create": (ULONG, [I32, I32]),
+    "ophip_mlp_free": (V, [ULONG]),
+    "ophip_mlp_param_count": (I32, []),
+    "ophip_mlp_params": (I32, [ULONG, I32, I32, C.POINTER(F32), I32]),
+    "ophip_mlp_forward": (I32, [ULONG, I32, U32, U32]),
+    "ophip_mlp_backward": (I32, [ULONG, I32, U32, U32, U32, U32]),
+    "ophip_mlp_adam": (I32, [ULONG, F32, F32, F32, F32]),
+    "ophip_mlp_loss": (I32, [ULONG, I32, C.POINTER(F64)]),
+    "ophip_surr_eval": (I32, [ULONG, I32, U32, C.POINTER(F32), C.POINTER(F32), F32, F32, I32,
+                              C.POINTER(F32), I32, F32, U32, U32, U32]),
+    "ophip_surr_base": (I32, [I32, U32, F32, F32, I32, C.POINTER(F32), I32, U32]),
     "ophip_mc_set_nrng": (I32, [I32]),
     "ophip_mc1f_rows": (I32, [U32, I32, U32, I32, I32, I32, U32, U32, ULONG]),
     "ophip_reserve_workspace": (I32, [ULONG]),

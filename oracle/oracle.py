@@ -262,3 +262,90 @@ def rel_err(a, b, floor=None):
     if floor is None:
         floor = 1e-3 * float(np.max(np.abs(b)))
     return np.abs(a - b) / np.maximum(np.abs(b), floor)
+
+
+# ---------------------------------------------------------------------------
+# surrogate control variate (no reference counterpart)
+# ---------------------------------------------------------------------------
+
+def local_vol_cev(x, sigma0, beta, a, b):
+    """sigma0 (x/100)^(beta-1) (1 + a l + b l^2), l = ln(x/100), floored at
+    1e-4 (the lattice's CEV family, lv_params_cev)"""
+    l = np.log(np.asarray(x, dtype=np.float64) / 100.0)
+    return np.maximum(sigma0 * np.exp((beta - 1.0) * l) * (1.0 + a * l + b * l * l), 1e-4)
+
+
+def bbf_vol(p, S, K):
+    """Berestycki-Busca-Florent short-maturity implied vol: the harmonic
+    mean of the local vol along the log path, 1/sigma = int_0^1 ds /
+    sigma_loc(S (K/S)^s), by 8-point Gauss-Legendre on [0, 1]"""
+    xg, wg = np.polynomial.legendre.leggauss(8)
+    s = 0.5 * (1.0 + xg)
+    xs = S * np.exp(s * math.log(K / S))
+    inv = np.sum(0.5 * wg / local_vol_cev(xs, p[0], p[1], p[3], p[4]))
+    return 1.0 / inv
+
+
+def bs_call_np(S, K, r, sigma, T):
+    from scipy.special import erfc
+    sq = sigma * math.sqrt(T)
+    d1 = (math.log(S / K) + (r + 0.5 * sigma * sigma) * T) / sq
+    d2 = d1 - sq
+    return S * 0.5 * erfc(-d1 / math.sqrt(2.0)) - K * math.exp(-r * T) * 0.5 * erfc(-d2 / math.sqrt(2.0))
+
+
+def surr_control_variate(params, strikes, nmat, dT, spot):
+    """[n][nmat * nstrikes] Black-Scholes calls at the BBF vol, maturity-major"""
+    params = np.atleast_2d(np.asarray(params, dtype=np.float64))
+    out = np.zeros((len(params), nmat * len(strikes)))
+    for i, p in enumerate(params):
+        for k in range(nmat):
+            for c, K in enumerate(strikes):
+                out[i, k * len(strikes) + c] = bs_call_np(spot, K, p[2], bbf_vol(p, spot, K), (k + 1) * dT)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# MLP (or_mlp_forward / or_mlp_backward): W_l [out][in] row-major
+# ---------------------------------------------------------------------------
+
+_PP = C.POINTER(C.POINTER(C.c_double))
+
+
+def _ptrs(arrs):
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in arrs]
+    tab = (C.POINTER(C.c_double) * len(arrs))(*[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs])
+    return arrs, tab
+
+
+def mlp_forward(Ws, bs, x):
+    """outputs [batch][out] and the pre-activations of every layer"""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    dims = (C.c_int * (len(Ws) + 1))(*([Ws[0].shape[1]] + [W.shape[0] for W in Ws]))
+    Wk, Wt = _ptrs(Ws)
+    bk, bt = _ptrs(bs)
+    n = x.shape[0]
+    out = np.zeros((n, Ws[-1].shape[0]))
+    acts = np.zeros(n * sum(W.shape[0] for W in Ws))
+    lib.or_mlp_forward(len(Ws), dims, Wt, bt, n, x.ctypes.data_as(C.POINTER(C.c_double)),
+                       out.ctypes.data_as(C.POINTER(C.c_double)),
+                       acts.ctypes.data_as(C.POINTER(C.c_double)))
+    return out, acts
+
+
+def mlp_backward(Ws, bs, x, dout):
+    """gradients (dW list, db list) of sum(out .* dout) over the batch"""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    dout = np.ascontiguousarray(dout, dtype=np.float64)
+    _, acts = mlp_forward(Ws, bs, x)
+    dims = (C.c_int * (len(Ws) + 1))(*([Ws[0].shape[1]] + [W.shape[0] for W in Ws]))
+    Wk, Wt = _ptrs(Ws)
+    bk, bt = _ptrs(bs)
+    dW = [np.zeros(W.shape) for W in Ws]
+    db = [np.zeros(W.shape[0]) for W in Ws]
+    dWk, dWt = _ptrs(dW)
+    dbk, dbt = _ptrs(db)
+    lib.or_mlp_backward(len(Ws), dims, Wt, bt, x.shape[0], x.ctypes.data_as(C.POINTER(C.c_double)),
+                        acts.ctypes.data_as(C.POINTER(C.c_double)),
+                        dout.ctypes.data_as(C.POINTER(C.c_double)), dWt, dbt)
+    return dWk, dbk

@@ -1,6 +1,6 @@
 """Writes the committed golden fixtures under tests/golden/.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [surrogate]
 
 * prices_*.npz   FP64 prices of the named workloads from the CPU oracle
                  (the reference's FP64 host method restated: generator,
@@ -43,7 +43,21 @@ def write_prices(name, c):
     print(name, p.shape, float(p.min()), float(p.max()))
 
 
+def write_surrogate():
+    """FP64 prices of the first 8 held-out surrogate draws (1024-node CEV
+    lattice with skew/curvature, 20 squarings to U(0.1), forward chain)"""
+    from oplib_amd import surrogate as S  # configuration builders only
+    prm = S.draw_params(8, S.SEED_HELDOUT)
+    ps = np.stack([O.solve_fp64(S.config(p, M.OPM_PREC_FP32))[0] for p in prm])  # [8][10][21]
+    np.savez(os.path.join(HERE, "prices_surrogate.npz"), prices=ps, params=prm,
+             strikes=S.STRIKES, nsq=S.NSQ, d=S.D_LATTICE)
+    print("surrogate", ps.shape, float(ps.min()), float(ps.max()))
+
+
 def main():
+    if sys.argv[1:] == ["surrogate"]:
+        write_surrogate()
+        return
     # MT19937 known answers (C++ [rand.predef]: 10000th output of the
     # default-seeded mt19937 is 4123659995)
     r = O.mt_raw(5489, 10000)
@@ -70,6 +84,7 @@ def main():
     print("sweep", ps.shape, float(ps.min()), float(ps.max()))
     write_prices("small_prop", M.config_small(nx=64, nr=2, price_kind=M.OPM_PRICE_PROPAGATE,
                                               nsteps=6))
+    write_surrogate()
 
 
 if __name__ == "__main__":

@@ -1,0 +1,340 @@
+"""GPU tests of the surrogate kernels (ophip_mlp_*, ophip_surr_*) through
+the C-ABI.
+
+Bars:
+* forward and gradients against a numpy restatement of the same
+  arithmetic (BF16 weights, BF16 activations / output gradients, FP32 or
+  better accumulation): 1e-3 of the largest value, with a few elements
+  allowed up to 2e-2 where one BF16 rounding lands on the other side of a
+  tie after a different accumulation order;
+* gradients against the FP64 oracle (or_mlp_backward) without any BF16
+  rounding: 3e-2 relative Frobenius error per layer (the BF16 storage
+  error), which a dropped term or a transposed operand exceeds by far;
+* Adam against the oracle's closed-form update (or_adam_step) in FP64
+  with the same FP32 hyperparameters: FP32 rounding (1e-5 relative; the
+  first moment's bound scales with its two terms, which can cancel);
+* the control variate against the oracle's FP64 restatement: 1e-4
+  relative (FP32 transcendental functions and the in-the-money
+  cancellation S N(d1) - K exp(-rT) N(d2); measured 3.6e-5);
+* the label solver against the FP64 oracle on the committed held-out
+  draws: 1e-4.
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+LAYERS = ((256, 32, 5, 256), (256, 256, 256, 256), (256, 256, 256, 256), (256, 256, 256, 210))
+
+
+def banner(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def bf16(x):
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def gelu(z):
+    from scipy.special import erf
+    return 0.5 * z * (1.0 + erf(z / math.sqrt(2.0)))
+
+
+def gelu_grad(z):
+    from scipy.special import erf
+    return 0.5 * (1.0 + erf(z / math.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / math.sqrt(2 * math.pi)
+
+
+def unpack(flat):
+    out, o = [], 0
+    for (po, pi, li, lo) in LAYERS:
+        W = flat[o:o + po * pi].reshape(po, pi)
+        o += po * pi
+        b = flat[o:o + po]
+        o += po
+        out.append((W, b, li, lo))
+    return out
+
+
+def emulate_forward(flat, xn):
+    """the kernels' arithmetic in numpy: returns y [B][256] and the cache"""
+    B = xn.shape[0]
+    a = np.zeros((B, 32), np.float32)
+    a[:, :5] = bf16(xn)
+    cache = {"a": [a], "z": []}
+    L = unpack(flat)
+    for l, (W, b, li, lo) in enumerate(L):
+        z = a.astype(np.float64) @ bf16(W).astype(np.float64).T + b
+        if l < 3:
+            cache["z"].append(z)
+            a = bf16(gelu(z).astype(np.float32))
+            cache["a"].append(a)
+        else:
+            z[:, 210:] = 0.0
+            y = z
+    return y, cache
+
+
+def emulate_backward(flat, cache, G):
+    """gradients of sum(y .* G) with the kernels' BF16 roundings"""
+    L = unpack(flat)
+    grads = [None] * 4
+    d = bf16(np.where(np.arange(256)[None, :] < 210, G, 0.0).astype(np.float32)).astype(np.float64)
+    for l in range(3, -1, -1):
+        W, b, li, lo = L[l]
+        a = cache["a"][l].astype(np.float64)
+        grads[l] = (d.T @ a, d.sum(0))
+        if l > 0:
+            dh = d @ bf16(W).astype(np.float64)
+            d = bf16((dh * gelu_grad(cache["z"][l - 1])).astype(np.float32)).astype(np.float64)
+    return grads
+
+
+def grads_of(flat):
+    return [(W, b) for (W, b, li, lo) in unpack(flat)]
+
+
+def close_mostly(a, b, scale, tight=1e-3, loose=2e-2, frac=0.995):
+    err = np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64)) / scale
+    return (err <= loose).all() and (err <= tight).mean() >= frac, float(err.max()), float((err <= tight).mean())
+
+
+@pytest.fixture(scope="module")
+def S(hip):
+    from oplib_amd import surrogate
+    return surrogate
+
+
+def make_net(S, members, cap, seed=3):
+    net = S.Surrogate(members=members, cap=cap, seed=seed)
+    rng = np.random.RandomState(seed)
+    # non-zero biases so that the bias paths are exercised
+    for e in range(members):
+        flat = net.get(e, 0)
+        for (W, b, li, lo) in unpack(flat):
+            b[:lo] = rng.uniform(-0.2, 0.2, size=lo)
+        net.set_weights(e, flat)
+    return net
+
+
+def test_forward_matches_emulation_and_oracle(S, hip):
+    banner("mlp forward E=2 B=128")
+    E, B = 2, 128
+    net = make_net(S, E, B)
+    xn = np.random.RandomState(5).uniform(-1, 1, size=(B, 5)).astype(np.float32)
+    X = hip.DeviceArray.from_numpy(xn, order="C")
+    Y = hip.DeviceArray.empty(E * B * 256)
+    hip.check(hip.lib.ophip_mlp_forward(net.net, B, X.h, Y.h), "forward")
+    y = Y.download().reshape(E, B, 256)
+    for e in range(E):
+        flat = net.get(e, 0)
+        ye, _ = emulate_forward(flat, xn)
+        assert np.all(y[e][:, 210:] == 0.0)
+        ok, mx, fr = close_mostly(y[e][:, :210], ye[:, :210], np.abs(ye).max())
+        assert ok, (e, mx, fr)
+        # the FP64 oracle without BF16 storage
+        Ws = [W[:lo, :li].astype(np.float64) for (W, b, li, lo) in unpack(flat)]
+        bs = [b[:lo].astype(np.float64) for (W, b, li, lo) in unpack(flat)]
+        yo, _ = O.mlp_forward(Ws, bs, xn.astype(np.float64))
+        rel = np.linalg.norm(y[e][:, :210] - yo) / np.linalg.norm(yo)
+        assert rel < 3e-2, rel
+    net.close()
+
+
+def test_backward_matches_emulation_and_oracle(S, hip):
+    banner("mlp backward E=2 B=128 (given output gradient)")
+    E, B = 2, 128
+    net = make_net(S, E, B, seed=9)
+    rng = np.random.RandomState(6)
+    xn = rng.uniform(-1, 1, size=(B, 5)).astype(np.float32)
+    G = rng.randn(E, B, 256).astype(np.float32)
+    X = hip.DeviceArray.from_numpy(xn, order="C")
+    Gd = hip.DeviceArray.from_numpy(G, order="C")
+    hip.check(hip.lib.ophip_mlp_backward(net.net, B, 0, X.h, 0, Gd.h), "backward")
+    for e in range(E):
+        flat = net.get(e, 0)
+        g = net.get(e, 1)
+        gk = grads_of(g)
+        _, cache = emulate_forward(flat, xn)
+        ge = emulate_backward(flat, cache, G[e].astype(np.float64))
+        Ws = [W[:lo, :li].astype(np.float64) for (W, b, li, lo) in unpack(flat)]
+        bs = [b[:lo].astype(np.float64) for (W, b, li, lo) in unpack(flat)]
+        dWo, dbo = O.mlp_backward(Ws, bs, xn.astype(np.float64), G[e][:, :210].astype(np.float64))
+        for l, (po, pi, li, lo) in enumerate(LAYERS):
+            Wg, bg = gk[l]
+            # padding stays exactly zero
+            assert np.all(Wg[lo:, :] == 0) and np.all(Wg[:, li:] == 0) and np.all(bg[lo:] == 0)
+            ok, mx, fr = close_mostly(Wg[:lo, :li], ge[l][0][:lo, :li], np.abs(ge[l][0]).max())
+            assert ok, ("W", e, l, mx, fr)
+            ok, mx, fr = close_mostly(bg[:lo], ge[l][1][:lo], np.abs(ge[l][1]).max())
+            assert ok, ("b", e, l, mx, fr)
+            rw = np.linalg.norm(Wg[:lo, :li] - dWo[l]) / np.linalg.norm(dWo[l])
+            rb = np.linalg.norm(bg[:lo] - dbo[l]) / np.linalg.norm(dbo[l])
+            assert rw < 3e-2 and rb < 3e-2, (e, l, rw, rb)
+    # the same call again gives the same bits
+    g0 = net.get(1, 1)
+    hip.check(hip.lib.ophip_mlp_backward(net.net, B, 0, X.h, 0, Gd.h), "backward")
+    assert np.array_equal(g0, net.get(1, 1))
+    net.close()
+
+
+def test_adam_matches_closed_form(S, hip):
+    banner("mlp adam two steps")
+    E, B = 1, 64
+    net = make_net(S, E, B, seed=4)
+    rng = np.random.RandomState(8)
+    xn = rng.uniform(-1, 1, size=(B, 5)).astype(np.float32)
+    X = hip.DeviceArray.from_numpy(xn, order="C")
+    p = net.get(0, 0).astype(np.float64)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    for t in (1, 2):
+        G = rng.randn(E, B, 256).astype(np.float32)
+        Gd = hip.DeviceArray.from_numpy(G, order="C")
+        hip.check(hip.lib.ophip_mlp_backward(net.net, B, 0, X.h, 0, Gd.h), "backward")
+        g = net.get(0, 1).astype(np.float64)
+        m0 = m.copy()
+        hip.check(hip.lib.ophip_mlp_adam(net.net, 1e-3, 0.9, 0.999, 1e-8), "adam")
+        # the kernel receives the hyperparameters as FP32 values
+        f = lambda x: float(np.float32(x))  # noqa: E731
+        O.lib.or_adam_step(len(p), p, g, m, v, f(1e-3), f(0.9), f(0.999), f(1e-8), t)
+        pk, mk, vk = net.get(0, 0), net.get(0, 2), net.get(0, 3)
+
+        def worst(a, b, atol):
+            r = np.abs(a - b) / (np.abs(b) * 1e-5 + atol)
+            i = int(np.argmax(r))
+            return t, i, float(a[i]), float(b[i]), float(g[i])
+        # FP32 rounding of each term: m = b1 m + (1-b1) g cancels when the
+        # two terms have opposite signs, so its bound scales with the terms
+        assert np.all(np.abs(mk - m) <= 1e-6 * (np.abs(m0) + np.abs(g)) + 1e-12), worst(mk, m, 1e-9)
+        assert np.allclose(vk, v, rtol=1e-5, atol=1e-12), worst(vk, v, 1e-12)
+        assert np.allclose(pk, p, rtol=1e-5, atol=1e-7), worst(pk, p, 1e-7)
+        p = pk.astype(np.float64)  # continue from the device's FP32 state
+        m, v = mk.astype(np.float64), vk.astype(np.float64)
+    # the bf16 copies follow the master weights: forward equals the emulation
+    Y = hip.DeviceArray.empty(B * 256)
+    hip.check(hip.lib.ophip_mlp_forward(net.net, B, X.h, Y.h), "forward")
+    ye, _ = emulate_forward(net.get(0, 0), xn)
+    ok, mx, fr = close_mostly(Y.download().reshape(B, 256)[:, :210], ye[:, :210], np.abs(ye).max())
+    assert ok, (mx, fr)
+    net.close()
+
+
+def test_mse_gradient_and_loss(S, hip):
+    banner("mlp MSE backward with gathered rows")
+    E, B, N = 2, 128, 300
+    net = make_net(S, E, B, seed=12)
+    rng = np.random.RandomState(9)
+    xa = rng.uniform(-1, 1, size=(N, 5)).astype(np.float32)
+    T = rng.randn(N, 210).astype(np.float32)
+    idx = np.stack([rng.permutation(N)[:B] for _ in range(E)]).astype(np.int32)
+    Xd = hip.DeviceArray.from_numpy(xa, order="C")
+    Td = hip.DeviceArray.from_numpy(T, order="C")
+    Id = hip.DeviceArray.from_numpy(idx, order="C")
+    hip.check(hip.lib.ophip_mlp_backward(net.net, B, Id.h, Xd.h, Td.h, 0), "backward")
+    loss = np.zeros(E)
+    import ctypes as C
+    hip.check(hip.lib.ophip_mlp_loss(net.net, B, loss.ctypes.data_as(C.POINTER(C.c_double))), "loss")
+    for e in range(E):
+        flat = net.get(e, 0)
+        ye, cache = emulate_forward(flat, xa[idx[e]])
+        diff = ye[:, :210] - T[idx[e]]
+        assert loss[e] == pytest.approx(float((diff ** 2).sum()), rel=1e-3)
+        G = np.zeros((B, 256))
+        G[:, :210] = 2.0 * diff / (B * 210)
+        ge = emulate_backward(flat, cache, G)
+        gk = grads_of(net.get(e, 1))
+        for l, (po, pi, li, lo) in enumerate(LAYERS):
+            rw = np.linalg.norm(gk[l][0][:lo, :li] - ge[l][0][:lo, :li]) / np.linalg.norm(ge[l][0])
+            assert rw < 1e-2, (e, l, rw)
+    net.close()
+
+
+def test_control_variate_matches_oracle(S, hip):
+    banner("surrogate control variate, 64 draws")
+    prm = S.draw_params(64, 77)
+    got = S.control_variate(prm)
+    ref = O.surr_control_variate(prm, S.STRIKES, S.NMAT, S.DT, S.SPOT)
+    den = np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max(axis=1, keepdims=True))
+    assert (np.abs(got - ref) / den).max() < 1e-4
+
+
+def test_eval_head_matches_members(S, hip):
+    banner("surrogate head: ensemble mean, spread, flags")
+    E, B = 3, 128
+    net = make_net(S, E, B, seed=21)
+    rng = np.random.RandomState(2)
+    net.mu[:210] = rng.uniform(-0.1, 0.1, 210)
+    net.sd[:210] = rng.uniform(1e-3, 1e-2, 210)
+    net._upload_stats()
+    prm = S.draw_params(B, 99)
+    pr, sp, fl = net.evaluate(prm, tol=2e-3)
+    X = hip.DeviceArray.from_numpy(S.normalize(prm), order="C")
+    Y = hip.DeviceArray.empty(E * B * 256)
+    hip.check(hip.lib.ophip_mlp_forward(net.net, B, X.h, Y.h), "forward")
+    y = Y.download().reshape(E, B, 256)[:, :, :210].astype(np.float64)
+    base = O.surr_control_variate(prm, S.STRIKES, S.NMAT, S.DT, S.SPOT)
+    w = np.maximum(np.abs(base), 1e-3 * np.abs(base).max(axis=1, keepdims=True))
+    mean = y.mean(0)
+    ref = base + w * (net.mu[:210] + net.sd[:210] * mean)
+    assert np.abs(pr - ref).max() <= 1e-4 * np.abs(ref).max()
+    std = w * net.sd[:210] * y.std(0)
+    den = np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max(axis=1, keepdims=True))
+    spr = (std / den).max(axis=1)
+    assert np.allclose(sp, spr, rtol=1e-3, atol=1e-7)
+    near = np.abs(spr - 2e-3) < 1e-5
+    assert np.array_equal(fl[~near], (spr[~near] > 2e-3).astype(np.int32))
+    net.close()
+
+
+def test_label_solver_matches_fp64(S, hip):
+    banner("surrogate label solver (1024 nodes, FP16x3 W form) vs FP64 oracle")
+    g = np.load(os.path.join(GOLD, "prices_surrogate.npz"))
+    prm = g["params"]
+    assert np.allclose(prm, S.draw_params(8, S.SEED_HELDOUT))
+    sol = S.PriceSolver(nb=8)
+    got = sol.solve(prm)
+    ref = g["prices"].reshape(8, -1)
+    err = S.rel_err(got, ref)
+    print("label solver max rel err", err.max())
+    assert err.max() < 1e-4
+    sol.close()
+
+
+def test_surrogate_gain_and_fallback(S, hip):
+    """trained on solver prices, the network beats the control variate
+    alone (the method without it) on held-out draws by a wide margin, and
+    every flagged draw gets the solver's own prices"""
+    banner("surrogate: train 4096 draws, judge 256 held-out draws")
+    sol = S.PriceSolver(nb=64)
+    ptr = S.draw_params(4096, S.SEED_TRAIN)
+    ytr = sol.solve(ptr)
+    pho = S.draw_params(256, S.SEED_HELDOUT)
+    yho = sol.solve(pho)
+    net = S.Surrogate(members=5, cap=1024, seed=7)
+    hist = net.fit(ptr, ytr, steps=1500, batch=1024, lr=2e-3, lr_end=1e-5)
+    assert hist[-1][1].max() < 1e-2, hist[-1]
+    pr, sp, fl = net.evaluate(pho)
+    e_net = S.rel_err(pr, yho)
+    e_cv = S.rel_err(S.control_variate(pho), yho)
+    print(f"[surrogate] median err: control variate {np.median(e_cv):.3e}, "
+          f"network {np.median(e_net):.3e}; served {np.mean(fl == 0):.3f}")
+    assert np.median(e_net) < np.median(e_cv) / 10
+    # the spread ranks the errors: flagged draws are the worse ones
+    if 0 < fl.sum() < len(fl):
+        assert np.median(e_net[fl == 1]) > np.median(e_net[fl == 0])
+    out, served, _ = net.predict(pho, sol)
+    assert np.array_equal(out[~served], yho[~served])
+    assert np.array_equal(out[served], pr[served].astype(np.float64))
+    net.close()
+    sol.close()

@@ -212,3 +212,46 @@ def test_batch_equals_single(hip, mode):
     assert np.array_equal(B.arr(1, 0, np.float32), singles[1])
     assert np.array_equal(C.arr(0, 0, np.float32), want[id(C)])
     H.lib.ophip_wsq_plan_free(p)
+
+
+@pytest.mark.parametrize("mode", [1])
+def test_mixed_tiles_and_autotune(hip, mode):
+    """a plan with a tile per lattice gives each lattice the bits of its
+    own single-lattice plan with that tile; the autotuner returns valid,
+    cached choices"""
+    import ctypes as C
+    banner(f"wsq mixed tiles + autotune mode={mode}")
+    H = hip
+    A = Lattice(H, 64, 2, mode)       # d = 128
+    B = Lattice(H, 128, 2, mode)      # d = 256
+    lats, tl = (A, B), (128, 64)
+    singles = []
+    for L, t in zip(lats, tl):
+        p = plan(H, [L], t)
+        L.first_step()
+        H.check(H.lib.ophip_wsq_exec(p, 4))
+        H.sync()
+        singles.append(L.arr(0, 0, np.float32))
+        H.lib.ophip_wsq_plan_free(p)
+    st = (C.c_uint * 2)(*[L.state.h for L in lats])
+    ds = (C.c_int * 2)(*[L.d for L in lats])
+    tiles = (C.c_int * 2)(*tl)
+    p = H.lib.ophip_wsq_plan_tiles(2, st, ds, None, mode, tiles, 0)
+    assert p, hex(H.lib.opcuda_get_status())
+    for L in lats:
+        L.first_step()
+    H.check(H.lib.ophip_wsq_exec(p, 4))
+    H.sync()
+    for L, s in zip(lats, singles):
+        assert np.array_equal(L.arr(0, 0, np.float32), s)
+    H.lib.ophip_wsq_plan_free(p)
+    # autotuner
+    got = (C.c_int * 2)(0, 0)
+    grid = C.c_int(0)
+    H.check(H.lib.ophip_wsq_autotune(2, st, ds, None, mode, got, C.byref(grid)))
+    assert all(t in (64, 128) for t in got) and grid.value > 0
+    again = (C.c_int * 2)(0, 0)
+    grid2 = C.c_int(0)
+    H.check(H.lib.ophip_wsq_autotune(2, st, ds, None, mode, again, C.byref(grid2)))
+    assert list(got) == list(again) and grid.value == grid2.value
+    print(f"[autotune] tiles {list(got)} grid {grid.value}")

@@ -286,3 +286,65 @@ def test_adam_step_closed_form():
     O.lib.or_adam_step(n, p, g, m, v, 1e-3, 0.9, 0.999, 1e-8, 1)
     # first step: m_hat = g, v_hat = g^2 -> p -= lr * g/(|g| + eps)
     assert np.allclose(p, p0 - 1e-3 * g / (np.abs(g) + 1e-8), atol=1e-15)
+
+
+def test_mlp_forward_closed_form():
+    # one linear layer is W x + b; two layers put an exact-erf GELU between
+    rng = np.random.RandomState(3)
+    W1, b1 = rng.randn(4, 3), rng.randn(4)
+    W2, b2 = rng.randn(2, 4), rng.randn(2)
+    x = rng.randn(6, 3)
+    out, _ = O.mlp_forward([W1], [b1], x)
+    assert np.allclose(out, x @ W1.T + b1, atol=1e-13)
+    from scipy.special import erf
+    h = x @ W1.T + b1
+    h = 0.5 * h * (1.0 + erf(h / math.sqrt(2.0)))
+    out, _ = O.mlp_forward([W1, W2], [b1, b2], x)
+    assert np.allclose(out, h @ W2.T + b2, atol=1e-12)
+
+
+def test_mlp_backward_finite_differences():
+    # d/dtheta sum(out .* dout) against central differences in FP64
+    rng = np.random.RandomState(4)
+    dims = [5, 7, 6, 3]
+    Ws = [rng.randn(dims[i + 1], dims[i]) * 0.7 for i in range(3)]
+    bs = [rng.randn(dims[i + 1]) * 0.3 for i in range(3)]
+    x = rng.randn(4, 5)
+    dout = rng.randn(4, 3)
+    dW, db = O.mlp_backward(Ws, bs, x, dout)
+
+    def f():
+        return float(np.sum(O.mlp_forward(Ws, bs, x)[0] * dout))
+    h = 1e-6
+    for l in range(3):
+        for (i, j) in [(0, 0), (dims[l + 1] - 1, dims[l] - 1), (1, 2)]:
+            old = Ws[l][i, j]
+            Ws[l][i, j] = old + h
+            fp = f()
+            Ws[l][i, j] = old - h
+            fm = f()
+            Ws[l][i, j] = old
+            assert dW[l][i, j] == pytest.approx((fp - fm) / (2 * h), rel=1e-6, abs=1e-8)
+        for i in (0, dims[l + 1] - 1):
+            old = bs[l][i]
+            bs[l][i] = old + h
+            fp = f()
+            bs[l][i] = old - h
+            fm = f()
+            bs[l][i] = old
+            assert db[l][i] == pytest.approx((fp - fm) / (2 * h), rel=1e-6, abs=1e-8)
+
+
+def test_bbf_vol_closed_forms():
+    # constant local vol: the implied vol is that vol, and the control
+    # variate is the Black-Scholes price (bstest: 12.336)
+    p = [0.25, 1.0, 0.05, 0.0, 0.0]
+    assert O.bbf_vol(p, 100.0, 120.0) == pytest.approx(0.25, abs=1e-15)
+    cv = O.surr_control_variate([p], [100.0], 10, 0.1, 100.0)
+    assert cv[0, 9] == pytest.approx(O.bs_call(100, 100, 0.05, 0.25, 1.0), abs=1e-12)
+    assert cv[0, 9] == pytest.approx(12.336, abs=5e-4)
+    # CEV: 1/sigma = int_0^1 ds / (s0 (K/S)^{s(beta-1)}) = ((K/S)^(1-beta) - 1) / (s0 (1-beta) ln(K/S))
+    for beta, K in ((0.5, 130.0), (0.3, 80.0), (0.8, 101.0)):
+        p = [0.3, beta, 0.0, 0.0, 0.0]
+        inv = ((K / 100.0) ** (1 - beta) - 1.0) / (0.3 * (1 - beta) * math.log(K / 100.0))
+        assert O.bbf_vol(p, 100.0, K) == pytest.approx(1.0 / inv, rel=1e-12)
