@@ -49,6 +49,31 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/counters_*.md: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of
+    `bench.py --probe`, scaled by the copy kernel of known size), or None"""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "counters_*.md"))):
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch") and kernel in line:
+                try:
+                    return float(line.split()[-1])
+                except ValueError:
+                    return None
+    return None
+
+
+def roofline_backward(best, wb_gbs, wb_us, wb_bytes, bi_gbs, bi_us, bi_bytes):
+    if best in ("wform_fp16x3",):
+        k, gbs, us, by = "k_wbwd1 (W-form FP16 x3 backward step)", wb_gbs, wb_us, wb_bytes
+    else:
+        k, gbs, us, by = "backward step: k_skinny<3> + k_skinny_combine", bi_gbs, bi_us, bi_bytes
+    return {"bound": "hbm", "kernel": k, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": gbs / HBM_PEAK_GBS, "traffic": traffic_from_profiles(k.split()[0]),
+            "bytes_per_launch": by, "avg_launch_us": us}
+
+
 def timed_events(H, fn, reps, a=0, b=1):
     """average ms per call of fn() between two events on the library stream"""
     H.check(H.lib.ophip_event_record(a))
@@ -70,6 +95,9 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate leg")
     ap.add_argument("--surr-train", type=int, default=65536, help="surrogate training draws")
     ap.add_argument("--surr-steps", type=int, default=40000, help="surrogate Adam steps")
+    ap.add_argument("--probe", action="store_true",
+                    help="only launch the dominant kernel (W-form backward step) 20 times and a "
+                         "64 MiB copy kernel 20 times, for counter collection")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
@@ -83,6 +111,18 @@ def main():
     H.init()
     cfg = M.config_rs2048()
     d = cfg.d
+    if args.probe:
+        wb = WformBackward(H, M, cfg)
+        for _ in range(20):
+            wb.step()
+        n = 16 << 20  # floats: 64 MiB copied by k_copy4 (opcuda_scopy1)
+        A, B = H.DeviceArray.empty(n), H.DeviceArray.empty(n)
+        for _ in range(20):
+            H.check(H.lib.opcuda_scopy1(B.h, A.h, n), "scopy1")
+        H.sync()
+        print(json.dumps({"probe": "k_wbwd1 x20, k_copy4 (64 MiB) x20", "step_bytes": wb.bytes,
+                          "copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
+        return
     log(f"[bench] workload rs2048 d={d} nsq={cfg.nsq} steps={cfg.nsteps} strikes={cfg.nstrikes}")
     s = M.Solver(cfg)
     s.solve()
@@ -123,6 +163,15 @@ def main():
     gemm_flop = 2.0 * d ** 3
     gemm_tflops = gemm_flop / (gemm_ms * 1e-3) / 1e12
     log(f"[bench] squaring GEMM {gemm_ms * 1e3:.1f} us, {gemm_tflops:.1f} TFLOP/s")
+
+    # ---- the W-form backward step (dominant kernel of the x3 methods) ----
+    wb = WformBackward(H, M, cfg)
+    for _ in range(4):
+        wb.step()
+    wb_us = timed_events(H, wb.step, 100) * 1e3
+    H.check()
+    wb_gbs = wb.bytes / (wb_us * 1e-6) / 1e9
+    log(f"[bench] W-form backward step {wb_us:.2f} us, {wb_gbs:.0f} GB/s algorithmic")
 
     # ---- backward-induction step: skinny GEMM, GB/s ----
     ncp = 48
@@ -200,15 +249,18 @@ def main():
                           "prices": int(rel.size),
                           "meets_1e-4": bool(bm["meets_1e-4"])},
         # dominant kernel of the headline method: the backward-induction
-        # step (252 per scenario) streams the 2048 x 2048 FP32 matrix
-        "roofline": {"bound": "hbm", "kernel": "backward step: k_skinny<3> + k_skinny_combine",
-                     "achieved": bi_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": bi_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": bi_bytes, "avg_launch_us": bi_us},
+        # step (252 per scenario) streams the 2048 x 2048 matrix's hi/lo
+        # copies (FP16 x3 W form; the FP32 methods' step is in "gemv")
+        "roofline": roofline_backward(best, wb_gbs, wb_us, wb.bytes, bi_gbs, bi_us, bi_bytes),
         "roofline_squaring": {"bound": "mfma", "kernel": "k_sgemm<128,128> (plain FP32 squaring)",
                               "achieved": gemm_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                               "frac": gemm_tflops / FP32_PEAK_TFLOPS,
                               "flop_per_launch": gemm_flop, "avg_launch_us": gemm_ms * 1e3},
+        "wform_backward": {"kernel": "k_wbwd1 (FP16 x3 backward step)", "avg_us": wb_us,
+                           "algorithmic_GBps": wb_gbs, "frac": wb_gbs / HBM_PEAK_GBS},
+        "gemv": {"kernel": "k_skinny<3> + combine (FP32 backward step)", "avg_us": bi_us,
+                 "algorithmic_GBps": bi_gbs, "peak_GBps": HBM_PEAK_GBS,
+                 "frac": bi_gbs / HBM_PEAK_GBS},
         "methods": methods,
         "sweep": sweep,
         "mc": mc,
@@ -216,6 +268,49 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+class WformBackward:
+    """the headline method's dominant kernel on its real operands: the
+    rs2048 lattice squared 16 times in the FP16 x3 W form, then the
+    backward-induction step (k_wbwd1) alternating two value buffers"""
+
+    def __init__(self, H, M, cfg, ncp=48):
+        import math
+        self.H, self.d, self.ncp = H, cfg.d, ncp
+        d, mode = cfg.d, 1
+        x, _ = M.grid(cfg)
+        dr, vo = M.params(cfg)
+        self.keep = [H.DeviceArray.from_numpy(a) for a in (x, dr, vo)]
+        self.state = H.DeviceArray(H.lib.ophip_wsq_bytes(d, mode), np.uint8)
+        w0 = H.lib.ophip_wsq_w(self.state.h, d, mode, 0)
+        H.check(H.lib.ophip_sek_rs(cfg.nx, cfg.nr, self.keep[0].h, self.keep[1].h, self.keep[2].h,
+                                   cfg.q_off, math.ldexp(cfg.DeltaT, -cfg.nsq), 2, w0))
+        import ctypes as C
+        st, ds = (C.c_uint * 1)(self.state.h), (C.c_int * 1)(d)
+        plan = H.lib.ophip_wsq_plan(1, st, ds, None, mode, 0, 0)
+        H.check(H.lib.ophip_wsq_exec(plan, cfg.nsq))
+        self.buf = cfg.nsq & 1
+        H.check(H.lib.ophip_wsq_bwd_layout(self.state.h, d, mode, self.buf))
+        H.sync()
+        H.lib.ophip_wsq_plan_free(plan)
+        xs = np.tile(x, cfg.nr)
+        pay = np.zeros((d, ncp), np.float32)
+        for k in range(cfg.nstrikes):
+            pay[:, k] = np.maximum(xs - cfg.strikes[k], 0.0)
+        self.V = [H.DeviceArray.from_numpy(pay, order="C"), H.DeviceArray.empty(d * ncp)]
+        self.VT = [H.DeviceArray.empty(2 * d * ncp, np.uint16) for _ in range(2)]
+        H.check(H.lib.ophip_wsq_bwd_prep(d, ncp, mode, self.V[0].h, self.VT[0].h))
+        self.t = 0
+        # algorithmic bytes per step: the matrix (hi + lo = 4 d^2) and the
+        # 41 live value columns read and written
+        self.bytes = 4.0 * d * d + 4.0 * d * cfg.nstrikes * 2
+
+    def step(self):
+        a, b = self.t & 1, (self.t + 1) & 1
+        self.H.lib.ophip_wsq_bwd_step(self.state.h, self.d, 1, self.buf, self.ncp, np.float32(0.9999),
+                                      self.V[a].h, self.VT[a].h, self.V[b].h, self.VT[b].h)
+        self.t += 1
 
 
 def rel_err(p, gold):

@@ -166,6 +166,18 @@ unsigned long ophip_wsq_plan(int n, const unsigned *states, const int *ds, const
                              int mode, int tile, int grid);
 int           ophip_wsq_exec(unsigned long plan, int nsq);
 void          ophip_wsq_plan_free(unsigned long plan);
+/* Backward induction in the W form on the matrix cores (modes 1, 2):
+ * Y = df (X + W X).  ophip_wsq_bwd_layout copies the hi/lo copies of W
+ * (buffer `buf`, the one the squarings ended in) into an MFMA
+ * fragment-major layout kept in the other buffer; X/Y are d x ncp
+ * row-major FP32 (ncp 16, 32 or 48) with 16-bit hi/lo copies XT/YT
+ * (2 * ncp * d values, fragment-major); ophip_wsq_bwd_prep writes XT from
+ * X.  The reference's step is the FP32 SGEMV of U (sgemv.cu:44-51, driven
+ * per step by CModel Device.cs). */
+int ophip_wsq_bwd_layout(unsigned state, int d, int mode, int buf);
+int ophip_wsq_bwd_prep(int d, int ncp, int mode, unsigned X, unsigned XT);
+int ophip_wsq_bwd_step(unsigned state, int d, int mode, int buf, int ncp, float df, unsigned X,
+                       unsigned XT, unsigned Y, unsigned YT);
 /* the same plan with a tile per lattice (64 or 128 each): one persistent
  * launch per tile class and round */
 unsigned long ophip_wsq_plan_tiles(int n, const unsigned *states, const int *ds, const int *nsqs,

@@ -265,19 +265,16 @@ __global__ void k_skinny_combine(int m, int ncp, int KS, const float *__restrict
     if (idx >= m * ncp) return;
     const int i = idx / ncp, c = idx - i * ncp;
     const size_t mn = (size_t)m * ncp;
-    f4 s = *(const f4 *)(part + idx);
-    int q = 1;
-    for (; q + 4 <= KS; q += 4) {
-        f4 p0 = *(const f4 *)(part + (q + 0) * mn + idx);
-        f4 p1 = *(const f4 *)(part + (q + 1) * mn + idx);
-        f4 p2 = *(const f4 *)(part + (q + 2) * mn + idx);
-        f4 p3 = *(const f4 *)(part + (q + 3) * mn + idx);
-        s += p0;
-        s += p1;
-        s += p2;
-        s += p3;
-    }
-    for (; q < KS; q++) s += *(const f4 *)(part + q * mn + idx);
+    // every partial load is issued before the first add (KS <= 16)
+    f4 p[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+        if (q < KS) p[q] = *(const f4 *)(part + q * mn + idx);
+    f4 s = p[0];
+#pragma unroll
+    for (int q = 1; q < 16; q++)
+        if (q < KS) s += p[q];
+    for (int q = 16; q < KS; q++) s += *(const f4 *)(part + q * mn + idx);
     f4 y = alpha * s;
     if (plus_x) y += alpha * *(const f4 *)(X + (size_t)i * ldx + c);
     *(f4 *)(Y + (size_t)i * ldy + c) = y;

@@ -103,6 +103,7 @@ struct opm_solver {
     Buf x, drift, vol;   // FP64 lattice inputs
     Buf U[2];            // FP32 d x d ping-pong for the squarings
     Buf V[2], pay0;      // backward induction: d x ncp row-major
+    Buf VT[2];           // W form (x3 modes): hi/lo copies of V for the matrix-core step
     Buf Uk[2], payT, out;  // propagation: U_k, payoff (nstrikes x d), prices
     int u_final = 0, v_final = 0;
     unsigned long graph = 0;
@@ -199,6 +200,7 @@ opm_solver *opm_create(const opm_config *cfg) {
         s->n_mat = 1;
         size_t vb = 4 * (size_t)d * s->ncp;
         ok = alloc(s->V[0], vb) && alloc(s->V[1], vb) && alloc(s->pay0, vb);
+        if (ok && s->wmode == 1) ok = alloc(s->VT[0], vb) && alloc(s->VT[1], vb);
         if (ok) {
             std::vector<float> p((size_t)d * s->ncp, 0.0f);
             for (int y = 0; y < d; y++)
@@ -236,7 +238,8 @@ void opm_destroy(opm_solver *s) {
     if (s->graph) ophip_graph_destroy(s->graph);
     if (s->wplan) ophip_wsq_plan_free(s->wplan);
     Buf *all[] = {&s->x, &s->drift, &s->vol, &s->U[0], &s->U[1], &s->V[0], &s->V[1],
-                  &s->pay0, &s->Uk[0], &s->Uk[1], &s->payT, &s->out, &s->wstate};
+                  &s->VT[0], &s->VT[1], &s->pay0, &s->Uk[0], &s->Uk[1], &s->payT, &s->out,
+                  &s->wstate};
     for (Buf *b : all) release(*b);
     delete s;
 }
@@ -275,11 +278,27 @@ static int solve_body(opm_solver *s) {
         const float df = (float)std::exp(-c->rate * c->DeltaT);
         st = opcuda_scopy1(s->V[0].h, s->pay0.h, (unsigned)((size_t)d * s->ncp));
         if (st) return st;
-        for (int t = 0; t < c->nsteps; t++) {
-            // V <- df*U*V, or df*(V + W*V) in the W form
-            st = ophip_sgemm_skinny_ex(d, d, s->ncp, df, U, d, s->V[t & 1].h, s->ncp,
-                                       s->V[(t + 1) & 1].h, s->ncp, wform ? 1 : 0);
+        if (wform && s->wmode == 1) {
+            // V <- df*(V + W*V) on the FP16 matrix cores from W's hi/lo
+            // copies (a BF16 hi/lo pair of V keeps 16 bits, too few over 252
+            // steps: the BF16 method steps in FP32)
+            st = ophip_wsq_bwd_layout(s->wstate.h, d, s->wmode, s->u_final);
             if (st) return st;
+            st = ophip_wsq_bwd_prep(d, s->ncp, s->wmode, s->V[0].h, s->VT[0].h);
+            if (st) return st;
+            for (int t = 0; t < c->nsteps; t++) {
+                const int a = t & 1, b = (t + 1) & 1;
+                st = ophip_wsq_bwd_step(s->wstate.h, d, s->wmode, s->u_final, s->ncp, df, s->V[a].h,
+                                        s->VT[a].h, s->V[b].h, s->VT[b].h);
+                if (st) return st;
+            }
+        } else {
+            for (int t = 0; t < c->nsteps; t++) {
+                // V <- df*U*V, or df*(V + W*V) in the W form
+                st = ophip_sgemm_skinny_ex(d, d, s->ncp, df, U, d, s->V[t & 1].h, s->ncp,
+                                           s->V[(t + 1) & 1].h, s->ncp, wform ? 1 : 0);
+                if (st) return st;
+            }
         }
         s->v_final = c->nsteps & 1;
     } else {

@@ -290,7 +290,11 @@ __global__ __launch_bounds__(256) void k_wsq(char *__restrict__ base,
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
                     rows[a][v] += (double)o[v];
-                    float os = o[v] * N::S;
+                    // split the stored FP32 value: no fused multiply-add-
+                    // convert, whose single rounding differs at FP16 ties
+                    float ov = o[v];
+                    asm volatile("" : "+v"(ov));
+                    float os = ov * N::S;
                     hc[v] = N::enc(os);
                     if (LO) lc[v] = N::enc((os - N::dec(hc[v])) * N::G);
                     Hrn[(size_t)(row + v) * d + col] = hc[v];
@@ -381,6 +385,167 @@ __global__ void k_wfix(char *__restrict__ base, const WScen *__restrict__ scen, 
         const uint16_t l = N::enc((ws - N::dec(h)) * N::G);
         at<uint16_t>(base, sc.Lcn)[e] = l;
         at<uint16_t>(base, sc.Lrn)[e] = l;
+    }
+}
+
+// ---------------------------------------------------------------------
+// Backward induction in the W form on the matrix cores:
+//     Y = df (X + W X),  X, Y: d x ncp row-major FP32.
+// W enters through its 16-bit hi/lo copies (H = rnd(W 2^S), L = rnd((W 2^S
+// - H) 2^G)), X through its own (Xh = rnd(X), Xl = rnd((X - Xh) 2^G)):
+//     W X ~ 2^-S (H Xh + 2^-G (H Xl + L Xh)).
+// Operands are stored fragment-major: a matrix M (rows i, reduction index
+// k) is cut into 16 x 32 tiles (it, kt), tile-row-major, and within a tile
+// lane l = 16 g + r holds M[16 it + r][32 kt + 8 g .. +7], so one wave's
+// MFMA operand is one contiguous kilobyte.  W's copies are put in that
+// layout once per solve (k_wfrag); each step writes the next step's X
+// copies in it.
+// ---------------------------------------------------------------------
+__device__ inline size_t frag_off(int it, int kt, int l, int nkt) {
+    return ((size_t)(it * nkt + kt) * 64 + l) * 8;
+}
+
+// row-major H (and L) of a d x d matrix -> fragment-major; one thread per
+// 16-byte fragment
+template <bool LO>
+__global__ __launch_bounds__(256) void k_wfrag(int d, const uint16_t *__restrict__ Hr,
+                                               const uint16_t *__restrict__ Lr,
+                                               uint16_t *__restrict__ Hf, uint16_t *__restrict__ Lf) {
+    const int nkt = d / 32;
+    const size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // fragment index
+    if (f >= (size_t)d * d / 8) return;
+    const int l = (int)(f & 63);
+    const size_t tile = f >> 6;
+    const int it = (int)(tile / nkt), kt = (int)(tile % nkt);
+    const int r = l & 15, g = l >> 4;
+    const size_t src = (size_t)(16 * it + r) * d + 32 * kt + 8 * g;
+    *(uint4 *)(Hf + f * 8) = *(const uint4 *)(Hr + src);
+    if (LO) *(uint4 *)(Lf + f * 8) = *(const uint4 *)(Lr + src);
+}
+
+// X (d x 16NT row-major FP32) -> its hi/lo copies, fragment-major with the
+// columns as rows: thread per (column, 8 consecutive k)
+template <bool BF, int NT>
+__global__ __launch_bounds__(256) void k_wbwd_prep(int d, const float *__restrict__ X,
+                                                   uint16_t *__restrict__ XhT,
+                                                   uint16_t *__restrict__ XlT) {
+    using N = Num<BF>;
+    constexpr int NC = 16 * NT;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (k8 group, column)
+    if (i >= d / 8 * NC) return;
+    const int c = i % NC, k8 = i / NC;
+    const int k0 = 8 * k8, kt = k0 / 32, g = (k0 % 32) / 8;
+    uint16_t h[8], lo[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const float x = X[(size_t)(k0 + e) * NC + c];
+        h[e] = N::enc(x);
+        lo[e] = N::enc((x - N::dec(h[e])) * N::G);
+    }
+    const size_t o = frag_off(c / 16, kt, 16 * g + (c % 16), d / 32);
+    *(uint4 *)(XhT + o) = *(const uint4 *)h;
+    *(uint4 *)(XlT + o) = *(const uint4 *)lo;
+}
+
+// One kernel per step, no split-K: workgroup = 16 rows x all of K, NW
+// waves each taking K/NW (groups of two k32 blocks, two groups of loads in
+// flight), the waves' tiles added in wave order through LDS, then Y =
+// df (X + W X) and Y's hi/lo copies for the next step.  d / 16 workgroups.
+template <bool BF, int NT, int NW>
+__global__ __launch_bounds__(64 * NW) void k_wbwd1(int d, float df, const uint16_t *__restrict__ Hf,
+                                                   const uint16_t *__restrict__ Lf,
+                                                   const uint16_t *__restrict__ XhT,
+                                                   const uint16_t *__restrict__ XlT,
+                                                   const float *__restrict__ X, float *__restrict__ Y,
+                                                   uint16_t *__restrict__ YhT,
+                                                   uint16_t *__restrict__ YlT) {
+    using N = Num<BF>;
+    constexpr int NC = 16 * NT, GB = 2, NOP = 2 + 2 * NT;
+    __shared__ f4 red[NW][NT][64];
+    __shared__ uint16_t th[NC][16], tl[NC][16];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int it = blockIdx.x, row0 = 16 * it;
+    const int nkt = d / 32, nblk = nkt / NW, ngrp = (nblk + GB - 1) / GB;
+    const int kt0 = wave * nblk;
+    // fragment of tile (it, kt) for this lane: base + kt * 512
+    const uint16_t *ha = Hf + frag_off(it, 0, lane, nkt);
+    const uint16_t *la = Lf + frag_off(it, 0, lane, nkt);
+    const uint16_t *xh = XhT + frag_off(0, 0, lane, nkt);
+    const uint16_t *xl = XlT + frag_off(0, 0, lane, nkt);
+    const size_t ctile = (size_t)nkt * 512;  // elements per 16-column strip
+    typedef s8 Stage[GB][NOP];
+    auto load = [&](Stage &S, int grp) {
+#pragma unroll
+        for (int b = 0; b < GB; b++) {
+            const size_t k = (size_t)512 * (kt0 + min(grp * GB + b, nblk - 1));  // clamped: unused
+            S[b][0] = *(const s8 *)(ha + k);
+            S[b][1] = *(const s8 *)(la + k);
+#pragma unroll
+            for (int j = 0; j < NT; j++) {
+                S[b][2 + j] = *(const s8 *)(xh + j * ctile + k);
+                S[b][2 + NT + j] = *(const s8 *)(xl + j * ctile + k);
+            }
+        }
+    };
+    f4 acc[NT], cor[NT];
+#pragma unroll
+    for (int j = 0; j < NT; j++) {
+        acc[j] = f4{0.f, 0.f, 0.f, 0.f};
+        cor[j] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+    auto comp = [&](const Stage &S, int grp) {
+#pragma unroll
+        for (int b = 0; b < GB; b++) {
+            if (grp * GB + b >= nblk) continue;  // uniform
+#pragma unroll
+            for (int j = 0; j < NT; j++) {
+                acc[j] = N::mfma(S[b][0], S[b][2 + j], acc[j]);
+                cor[j] = N::mfma(S[b][0], S[b][2 + NT + j], cor[j]);
+                cor[j] = N::mfma(S[b][1], S[b][2 + j], cor[j]);
+            }
+        }
+    };
+    Stage s0, s1;
+    load(s0, 0);
+    for (int grp = 0; grp < ngrp; grp += 2) {
+        load(s1, min(grp + 1, ngrp - 1));
+        comp(s0, grp);
+        load(s0, min(grp + 2, ngrp - 1));
+        if (grp + 1 < ngrp) comp(s1, grp + 1);
+    }
+#pragma unroll
+    for (int j = 0; j < NT; j++) red[wave][j][lane] = (acc[j] + cor[j] * N::IG) * N::IS;
+    __syncthreads();
+    // wave j < NT finishes column tile j: rows row0 + 4g + e, column 16j + r
+    if (wave < NT) {
+        const int j = wave, col = 16 * j + r;
+        f4 sum = red[0][j][lane];
+#pragma unroll
+        for (int w = 1; w < NW; w++) sum += red[w][j][lane];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const size_t o = (size_t)(row0 + 4 * g + e) * NC + col;
+            float y = df * (X[o] + sum[e]);
+            // the copies split the rounded FP32 y (no fused multiply-convert,
+            // whose single rounding differs from y's at FP16 ties)
+            asm volatile("" : "+v"(y));
+            Y[o] = y;
+            const uint16_t h = N::enc(y);
+            th[col][4 * g + e] = h;
+            tl[col][4 * g + e] = N::enc((y - N::dec(h)) * N::G);
+        }
+    }
+    __syncthreads();
+    // Y's copies: rows row0 .. row0+15 are k = 32 kt + 8 g .. of tile kt =
+    // row0 / 32, groups g0 and g0 + 1 (g0 = (row0 % 32) / 8): two 16-byte
+    // fragments per column and array
+    const int kt = row0 / 32, g0 = (row0 % 32) / 8;
+    for (int i = t; i < NC * 4; i += 64 * NW) {
+        const int arr = i / (NC * 2), c = (i % (NC * 2)) >> 1, half = i & 1;
+        const size_t o = frag_off(c / 16, kt, 16 * (g0 + half) + (c % 16), nkt);
+        *(uint4 *)((arr ? YlT : YhT) + o) =
+            arr ? *(const uint4 *)&tl[c][8 * half] : *(const uint4 *)&th[c][8 * half];
     }
 }
 
@@ -790,4 +955,90 @@ OPHIP_EXPORT int ophip_wsq_autotune(int n, const unsigned *states, const int *ds
     g_grid_cache[sig] = g_best;
     *grid = g_best;
     return (int)set_status(kOk);
+}
+
+// ---------------------------------------------------------------------
+// W-form backward induction (modes 1 and 2: the states hold L copies)
+// ---------------------------------------------------------------------
+namespace {
+template <class F> unsigned launch_bwd(int mode, int ncp, F f) {
+    auto nt = [&](auto bf) {
+        switch (ncp) {
+        case 16: f(bf, std::integral_constant<int, 1>()); break;
+        case 32: f(bf, std::integral_constant<int, 2>()); break;
+        default: f(bf, std::integral_constant<int, 3>()); break;
+        }
+    };
+    if (mode == 1) nt(std::integral_constant<bool, false>());
+    else nt(std::integral_constant<bool, true>());
+    return check_launch();
+}
+bool bwd_args_ok(int d, int ncp, int mode) {
+    return d > 0 && d % 128 == 0 && d <= 4096 && (mode == 1 || mode == 2) &&
+           (ncp == 16 || ncp == 32 || ncp == 48);
+}
+}  // namespace
+
+// XT (2 * ncp * d 16-bit values: hi then lo, fragment-major) from X
+// (d x ncp row-major FP32): the first step's operand copies
+OPHIP_EXPORT int ophip_wsq_bwd_prep(int d, int ncp, int mode, unsigned X, unsigned XT) {
+    if (!bwd_args_ok(d, ncp, mode)) return (int)set_status(kErrBadArg);
+    const size_t n = (size_t)d * ncp;
+    if (!handle_range_ok(X, 4 * n) || !handle_range_ok(XT, 4 * n)) return (int)set_status(kErrBadHandle);
+    Device &D = dev();
+    const float *x = P<const float>(X);
+    uint16_t *xt = P<uint16_t>(XT);
+    return (int)launch_bwd(mode, ncp, [&](auto bf, auto nt) {
+        constexpr bool BF = decltype(bf)::value;
+        constexpr int NT = decltype(nt)::value;
+        hipLaunchKernelGGL((k_wbwd_prep<BF, NT>), dim3(cdiv(d / 8 * 16 * NT, 256)), dim3(256), 0,
+                           D.stream, d, x, xt, xt + n);
+    });
+}
+
+// W's hi/lo copies of buffer `buf`, fragment-major, into the column-major
+// slots of the other buffer (free once the squarings are done); once per
+// solve, before the steps
+OPHIP_EXPORT int ophip_wsq_bwd_layout(unsigned state, int d, int mode, int buf) {
+    if (!bwd_args_ok(d, 16, mode) || buf < 0 || buf > 1) return (int)set_status(kErrBadArg);
+    const WsqLayout L = wsq_layout(d, mode);
+    if (!handle_range_ok(state, L.total)) return (int)set_status(kErrBadHandle);
+    Device &D = dev();
+    const uint16_t *Hr = P<const uint16_t>(state + (unsigned)L.H[buf][0]);
+    const uint16_t *Lr = P<const uint16_t>(state + (unsigned)L.H[buf][2]);
+    uint16_t *Hf = P<uint16_t>(state + (unsigned)L.H[1 - buf][1]);
+    uint16_t *Lf = P<uint16_t>(state + (unsigned)L.H[1 - buf][3]);
+    const size_t nf = (size_t)d * d / 8;
+    hipLaunchKernelGGL((k_wfrag<true>), dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, D.stream, d, Hr,
+                       Lr, Hf, Lf);
+    return (int)check_launch();
+}
+
+// Y = df (X + W X) with W laid out by ophip_wsq_bwd_layout(state, .., buf),
+// and Y's copies YT for the next step.  X/XT and Y/YT must not overlap.
+OPHIP_EXPORT int ophip_wsq_bwd_step(unsigned state, int d, int mode, int buf, int ncp, float df,
+                                    unsigned X, unsigned XT, unsigned Y, unsigned YT) {
+    if (!bwd_args_ok(d, ncp, mode) || buf < 0 || buf > 1) return (int)set_status(kErrBadArg);
+    const WsqLayout L = wsq_layout(d, mode);
+    const size_t n = (size_t)d * ncp;
+    if (!handle_range_ok(state, L.total) || !handle_range_ok(X, 4 * n) || !handle_range_ok(XT, 4 * n) ||
+        !handle_range_ok(Y, 4 * n) || !handle_range_ok(YT, 4 * n))
+        return (int)set_status(kErrBadHandle);
+    Device &D = dev();
+    const uint16_t *Hf = P<const uint16_t>(state + (unsigned)L.H[1 - buf][1]);
+    const uint16_t *Lf = P<const uint16_t>(state + (unsigned)L.H[1 - buf][3]);
+    const uint16_t *xt = P<const uint16_t>(XT);
+    const float *x = P<const float>(X);
+    float *y = P<float>(Y);
+    uint16_t *yt = P<uint16_t>(YT);
+    return (int)launch_bwd(mode, ncp, [&](auto bf, auto nt) {
+        constexpr bool BF = decltype(bf)::value;
+        constexpr int NT = decltype(nt)::value;
+        if (d % 256 == 0)
+            hipLaunchKernelGGL((k_wbwd1<BF, NT, 8>), dim3(d / 16), dim3(512), 0, D.stream, d, df, Hf, Lf,
+                               xt, xt + n, x, y, yt, yt + n);
+        else
+            hipLaunchKernelGGL((k_wbwd1<BF, NT, 4>), dim3(d / 16), dim3(256), 0, D.stream, d, df, Hf, Lf,
+                               xt, xt + n, x, y, yt, yt + n);
+    });
 }

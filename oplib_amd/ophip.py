@@ -86,6 +86,9 @@ SIGNATURES = {
                                I32]),
     "ophip_wsq_exec": (I32, [ULONG, I32]),
     "ophip_wsq_plan_free": (V, [ULONG]),
+    "ophip_wsq_bwd_layout": (I32, [U32, I32, I32, I32]),
+    "ophip_wsq_bwd_prep": (I32, [I32, I32, I32, U32, U32]),
+    "ophip_wsq_bwd_step": (I32, [U32, I32, I32, I32, I32, F32, U32, U32, U32, U32]),
     "ophip_wsq_plan_tiles": (ULONG, [I32, C.POINTER(U32), C.POINTER(I32), C.POINTER(I32), I32,
                                      C.POINTER(I32), I32]),
     "ophip_wsq_autotune": (I32, [I32, C.POINTER(U32), C.POINTER(I32), C.POINTER(I32), I32,

@@ -255,3 +255,75 @@ def test_mixed_tiles_and_autotune(hip, mode):
     H.check(H.lib.ophip_wsq_autotune(2, st, ds, None, mode, again, C.byref(grid2)))
     assert list(got) == list(again) and grid.value == grid2.value
     print(f"[autotune] tiles {list(got)} grid {grid.value}")
+
+
+def dec16(u, mode):
+    u = np.ascontiguousarray(u, dtype=np.uint16)
+    if mode == 2:
+        return (u.astype(np.uint32) << 16).view(np.float32)
+    return u.view(np.float16).astype(np.float32)
+
+
+def enc16(x, mode):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    if mode == 2:
+        return (bf16_round(x).view(np.uint32) >> 16).astype(np.uint16)
+    return x.astype(np.float16).view(np.uint16)
+
+
+def unfrag(a, nrows, K):
+    """fragment-major (16 x 32 tiles, lane l = 16 g + r holds row r, k 8g..8g+7)
+    -> [row][k]"""
+    a = np.asarray(a).reshape(nrows // 16, K // 32, 4, 16, 8)  # it, kt, g, r, e
+    return a.transpose(0, 3, 1, 2, 4).reshape(nrows, K)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_wform_backward_step(hip, mode):
+    """Y = df (X + W X) from the state's hi/lo copies (fragment-major): against
+    the same arithmetic in numpy (FP64 accumulation, 1e-6 of max|Y|),
+    against df (X + W X) with the FP32 W (2e-5), and X's / Y's 16-bit
+    copies bit-exact"""
+    banner(f"wsq backward step mode={mode}")
+    H = hip
+    L = Lattice(H, 128, 2, mode)  # d = 256
+    p = plan(H, [L], 128)
+    L.first_step()
+    H.check(H.lib.ophip_wsq_exec(p, 3))
+    H.sync()
+    H.lib.ophip_wsq_plan_free(p)
+    d, ncp, buf = L.d, 48, 1
+    W = L.arr(buf, 0, np.float32).astype(np.float64)
+    Hr = dec16(L.arr(buf, 1, np.uint16, "C"), mode).astype(np.float64)
+    Lr = dec16(L.arr(buf, 3, np.uint16, "C"), mode).astype(np.float64)
+    S, G = (1.0, G_BF) if mode == 2 else (S16, G16)
+    X = np.random.RandomState(3).uniform(0.0, 300.0, size=(d, ncp)).astype(np.float32)
+    Xd = H.DeviceArray.from_numpy(X, order="C")
+    XT = H.DeviceArray.empty(2 * ncp * d, dtype=np.uint16)
+    Yd = H.DeviceArray.empty(d * ncp)
+    YT = H.DeviceArray.empty(2 * ncp * d, dtype=np.uint16)
+    H.check(H.lib.ophip_wsq_bwd_layout(L.state.h, d, mode, buf))
+    H.check(H.lib.ophip_wsq_bwd_prep(d, ncp, mode, Xd.h, XT.h))
+    df = 0.9999
+    H.check(H.lib.ophip_wsq_bwd_step(L.state.h, d, mode, buf, ncp, np.float32(df), Xd.h, XT.h,
+                                     Yd.h, YT.h))
+    H.sync()
+    xt = XT.download().reshape(2, -1)
+    xt = np.stack([unfrag(xt[0], ncp, d), unfrag(xt[1], ncp, d)])
+    xh = enc16(X.T, mode)
+    xl = enc16((X.T - dec16(xh, mode)) * G, mode)
+    assert np.array_equal(xt[0], xh) and np.array_equal(xt[1], xl)
+    Y = Yd.download().reshape(d, ncp)
+    Xh, Xl = dec16(xh, mode).T.astype(np.float64), dec16(xl, mode).T.astype(np.float64)
+    ref = df * (X.astype(np.float64) + (Hr @ Xh + (Hr @ Xl + Lr @ Xh) / G) / S)
+    assert np.abs(Y - ref).max() <= 1e-6 * np.abs(ref).max()
+    exact = df * (X.astype(np.float64) + W @ X.astype(np.float64))
+    assert np.abs(Y - exact).max() <= 2e-5 * np.abs(exact).max()
+    yt = YT.download().reshape(2, -1)
+    yt = np.stack([unfrag(yt[0], ncp, d), unfrag(yt[1], ncp, d)])
+    yh = enc16(Y.T, mode)
+    assert np.array_equal(yt[0], yh)
+    yl = enc16((Y.T - dec16(yh, mode)) * G, mode)
+    bad = np.argwhere(yt[1] != yl)
+    assert len(bad) == 0, (len(bad), [(float(Y.T[tuple(b)]), int(yh[tuple(b)]), int(yt[1][tuple(b)]),
+                                       int(yl[tuple(b)])) for b in bad[:4]])
