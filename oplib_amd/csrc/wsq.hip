@@ -646,6 +646,53 @@ struct WsqPlan {
     std::vector<WItem *> it_round[2];
 };
 
+// XCD-aware order of a squaring round's work items.  Workgroup b of a
+// launch runs on XCD b % 8 and the persistent grid is a multiple of 8, so
+// item j runs on XCD j % 8.  Tiles that share operand panels should share
+// an XCD (its L2): a lattice with at least 32 tiles is cut into a 2 x 4
+// grid of tile blocks, one per XCD; smaller lattices go whole to the
+// least-loaded XCD.  The per-XCD lists are then balanced (moving items off
+// the longest lists) and interleaved.
+static std::vector<WItem> xcd_order(const std::vector<WItem> &v, const int *ds, int n, int T) {
+    constexpr int X = 8;
+    if (v.size() < 2 * X) return v;
+    std::vector<std::vector<WItem>> per(X);
+    std::vector<std::vector<WItem>> by_lat(n);
+    for (const WItem &w : v) by_lat[w.s % n].push_back(w);
+    std::vector<size_t> load(X, 0);
+    for (int i = 0; i < n; i++) {
+        if (by_lat[i].empty()) continue;
+        const int nt = ds[i] / T;
+        if ((int)by_lat[i].size() >= 4 * X && nt >= 4) {
+            for (const WItem &w : by_lat[i]) {
+                const int x = (w.tm * 2 / nt) * 4 + (w.tn * 4 / nt);
+                per[x].push_back(w);
+                load[x]++;
+            }
+        } else {
+            const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            for (const WItem &w : by_lat[i]) per[x].push_back(w);
+            load[x] += by_lat[i].size();
+        }
+    }
+    const size_t total = v.size(), target = (total + X - 1) / X;
+    for (int x = 0; x < X; x++)
+        while (per[x].size() < target) {
+            int big = 0;
+            for (int y = 1; y < X; y++)
+                if (per[y].size() > per[big].size()) big = y;
+            if (per[big].size() <= target) break;
+            per[x].push_back(per[big].back());
+            per[big].pop_back();
+        }
+    std::vector<WItem> out;
+    out.reserve(total);
+    for (size_t k = 0; out.size() < total; k++)
+        for (int x = 0; x < X; x++)
+            if (k < per[x].size()) out.push_back(per[x][k]);
+    return out;
+}
+
 // Plan the W-form pipeline for `n` lattices (states[i]: arena handle of a
 // state of ophip_wsq_bytes(ds[i], mode) bytes, ds[i] % 128 == 0; nsqs[i]:
 // its number of squarings, or null for a count given at exec time).
@@ -701,9 +748,9 @@ static WsqPlan *make_plan(int n, const unsigned *states, const int *ds, const in
     lists.push_back(items_for(64, -1, 0, -1));
     for (int c = 0; c < 2; c++) {
         const int T = c ? 128 : 64;
-        lists.push_back(items_for(T, c, 0, -1));
-        lists.push_back(items_for(T, c, 1, -1));
-        for (int r = 0; r < P->rounds; r++) lists.push_back(items_for(T, c, r & 1, r));
+        lists.push_back(xcd_order(items_for(T, c, 0, -1), ds, n, T));
+        lists.push_back(xcd_order(items_for(T, c, 1, -1), ds, n, T));
+        for (int r = 0; r < P->rounds; r++) lists.push_back(xcd_order(items_for(T, c, r & 1, r), ds, n, T));
     }
     size_t nitems = 0;
     for (auto &l : lists) nitems += l.size();
@@ -809,7 +856,9 @@ OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
                 const WItem *its = per ? P->it_round[c][rr] : P->it_all[c][src];
                 if (nit == 0) continue;
                 any = true;
-                const int g = nit < P->grid ? nit : P->grid;
+                // a multiple of 8 keeps item j on XCD j % 8 (xcd_order)
+                int g = nit < P->grid ? nit : P->grid;
+                if (g > 8) g -= g % 8;
                 // work items carry s = src*n + i (the full table)
                 if (c == 1)
                     hipLaunchKernelGGL((k_wsq<BF, LO, 128>), dim3(g), dim3(256), 0, D.stream, D.base,
