@@ -192,7 +192,8 @@ def main():
     methods = {"plain_fp32": {"scenarios_per_s": value, "ms": ms_per_step,
                               "max_rel_err": float(rel.max()),
                               "meets_1e-4": bool(rel.max() <= 1e-4)}}
-    for prec, name in ((1, "wform_fp16x1"), (2, "wform_fp16x3"), (3, "wform_bf16x3")):
+    for prec, name in ((1, "wform_fp16x1"), (2, "wform_fp16x3"), (3, "wform_bf16x3"),
+                       (4, "wform_fp8x3")):
         methods[name] = bench_method(H, M, prec, args.steps, args.warmup, gold)
         log(f"[bench] {name}: {methods[name]}")
 
@@ -223,7 +224,8 @@ def main():
     bm = methods[best]
     names = {"plain_fp32": "plain FP32", "wform_fp16x1": "U = I + W, FP16 x1 squarings",
              "wform_fp16x3": "U = I + W, FP16 x3 (hi/lo) squarings",
-             "wform_bf16x3": "U = I + W, BF16 x3 (hi/lo) squarings"}
+             "wform_bf16x3": "U = I + W, BF16 x3 (hi/lo) squarings",
+             "wform_fp8x3": "U = I + W, FP8 e4m3 x3 (hi/lo, power-of-two scales) squarings"}
     line = {
         "metric": "priced scenarios/s at <=1e-4 rel err vs FP64",
         "value": bm["scenarios_per_s"],

@@ -166,6 +166,17 @@ unsigned long ophip_wsq_plan(int n, const unsigned *states, const int *ds, const
                              int mode, int tile, int grid);
 int           ophip_wsq_exec(unsigned long plan, int nsq);
 void          ophip_wsq_plan_free(unsigned long plan);
+/* The W form with FP8 e4m3 operands: per round, row / column power-of-two
+ * scales, hi/lo e4m3 copies (fragment-major), W' = 2W + W*W on
+ * v_mfma_f32_16x16x32_fp8_fp8 (three products, FP32 accumulate) and the
+ * FP64 row-sum diagonal reset.  d % 64 == 0.  Write W0 = dt*L into buffer
+ * 0 (ophip_sek_rs out_kind 2); W(2^nsq dt) ends in buffer nsq & 1.
+ * which: 0 Ah, 1 Al, 2 Bh, 3 Bl, 4 row scales, 5 column scales. */
+unsigned long ophip_w8_bytes(int d);
+unsigned      ophip_w8_w(unsigned state, int d, int buf);
+unsigned      ophip_w8_array(unsigned state, int d, int which);
+int           ophip_w8_exec(unsigned state, int d, int nsq);
+
 /* Backward induction in the W form on the matrix cores (modes 1, 2):
  * Y = df (X + W X).  ophip_wsq_bwd_layout copies the hi/lo copies of W
  * (buffer `buf`, the one the squarings ended in) into an MFMA

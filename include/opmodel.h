@@ -36,6 +36,8 @@ extern "C" {
 #define OPM_PREC_F16    1     /* U = I + W, FP16 one product, row-sum renormalised */
 #define OPM_PREC_F16X3  2     /* U = I + W, FP16 hi/lo three products, renormalised */
 #define OPM_PREC_BF16X3 3     /* U = I + W, BF16 hi/lo three products, renormalised */
+#define OPM_PREC_F8X3   4     /* U = I + W, FP8 e4m3 hi/lo three products with power-of-two
+                                 row/column scales, renormalised (single lattices only) */
 
 typedef struct {
     int nx, nr;

@@ -91,7 +91,8 @@ void make_params(const opm_config *c, const std::vector<double> &x, std::vector<
 bool valid(const opm_config *c) {
     return c && c->nx >= 3 && c->nr >= 1 && c->nr <= 16 && c->xhi > c->xlo && c->DeltaT > 0 &&
            c->nsq >= 0 && c->nsq <= 40 && c->nsteps >= 1 && c->nstrikes >= 1 &&
-           c->nstrikes <= 64 && ((c->nx * c->nr) % 64) == 0;
+           c->nstrikes <= 64 && ((c->nx * c->nr) % 64) == 0 && c->precision >= OPM_PREC_FP32 &&
+           c->precision <= OPM_PREC_F8X3;
 }
 
 }  // namespace
@@ -185,10 +186,10 @@ opm_solver *opm_create(const opm_config *cfg) {
     if (c->precision == OPM_PREC_FP32) {
         ok = ok && alloc(s->U[0], 4 * dd) && alloc(s->U[1], 4 * dd);
     } else {
-        s->wmode = c->precision - 1;  // OPM_PREC_F16.. -> wsq modes 0..2
-        unsigned long wb = ophip_wsq_bytes(d, s->wmode);
-        ok = ok && wb && (d % 128) == 0 && alloc(s->wstate, wb);
-        if (ok) {
+        s->wmode = c->precision - 1;  // OPM_PREC_F16.. -> wsq modes 0..2, 3 = FP8
+        unsigned long wb = s->wmode == 3 ? ophip_w8_bytes(d) : ophip_wsq_bytes(d, s->wmode);
+        ok = ok && wb && (d % (s->wmode == 3 ? 64 : 128)) == 0 && alloc(s->wstate, wb);
+        if (ok && s->wmode != 3) {
             unsigned st = s->wstate.h;
             s->wplan = ophip_wsq_plan(1, &st, &d, nullptr, s->wmode, 0, 0);
             ok = s->wplan != 0;
@@ -265,13 +266,15 @@ static int solve_body(opm_solver *s) {
     } else {
         // W0 = dt*L in FP32 straight from the FP64 assembly, then the
         // W-form squarings with renormalisation
-        const unsigned w0 = ophip_wsq_w(s->wstate.h, d, s->wmode, 0);
+        const bool f8 = s->wmode == 3;
+        const unsigned w0 = f8 ? ophip_w8_w(s->wstate.h, d, 0) : ophip_wsq_w(s->wstate.h, d, s->wmode, 0);
         st = ophip_sek_rs(nx, nr, s->x.h, s->drift.h, s->vol.h, c->q_off, s->dt0, 2, w0);
         if (st) return st;
-        st = ophip_wsq_exec(s->wplan, c->nsq);
+        st = f8 ? ophip_w8_exec(s->wstate.h, d, c->nsq) : ophip_wsq_exec(s->wplan, c->nsq);
         if (st) return st;
         s->u_final = c->nsq & 1;
-        s->kernel_h = ophip_wsq_w(s->wstate.h, d, s->wmode, s->u_final);
+        s->kernel_h = f8 ? ophip_w8_w(s->wstate.h, d, s->u_final)
+                         : ophip_wsq_w(s->wstate.h, d, s->wmode, s->u_final);
     }
     const unsigned U = s->kernel_h;
     if (c->price_kind == OPM_PRICE_BACKWARD) {
@@ -477,6 +480,7 @@ opm_batch *opm_batch_create(const opm_config *cfgs, int n, int tile) {
             std::memcmp(c.strikes, c0.strikes, sizeof(double) * c.nstrikes) != 0)
             return nullptr;
         if (c.precision != OPM_PREC_FP32 && c.nx % 128) return nullptr;
+        if (c.precision == OPM_PREC_F8X3) return nullptr;  // FP8: single lattices only
     }
     opm_batch *b = new opm_batch();
     b->n = n;

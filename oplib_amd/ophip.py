@@ -86,6 +86,10 @@ SIGNATURES = {
                                I32]),
     "ophip_wsq_exec": (I32, [ULONG, I32]),
     "ophip_wsq_plan_free": (V, [ULONG]),
+    "ophip_w8_bytes": (ULONG, [I32]),
+    "ophip_w8_w": (U32, [U32, I32, I32]),
+    "ophip_w8_array": (U32, [U32, I32, I32]),
+    "ophip_w8_exec": (I32, [U32, I32, I32]),
     "ophip_wsq_bwd_layout": (I32, [U32, I32, I32, I32]),
     "ophip_wsq_bwd_prep": (I32, [I32, I32, I32, U32, U32]),
     "ophip_wsq_bwd_step": (I32, [U32, I32, I32, I32, I32, F32, U32, U32, U32, U32]),

@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(_HERE, "libopmodel.so")
 
 OPM_VOL_REGIME, OPM_VOL_C1, OPM_VOL_CEV = 0, 1, 2
 OPM_PRICE_BACKWARD, OPM_PRICE_PROPAGATE = 0, 1
-OPM_PREC_FP32, OPM_PREC_F16, OPM_PREC_F16X3, OPM_PREC_BF16X3 = 0, 1, 2, 3
+OPM_PREC_FP32, OPM_PREC_F16, OPM_PREC_F16X3, OPM_PREC_BF16X3, OPM_PREC_F8X3 = 0, 1, 2, 3, 4
 
 
 class Config(C.Structure):

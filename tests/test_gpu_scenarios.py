@@ -82,6 +82,9 @@ WFORM_BOUNDS = {
     ("lv1d", 1): 1e-3, ("lv1d", 2): 1e-5, ("lv1d", 3): 3e-5,
     # rs_monthly adds 11 FP32 propagation products: ~2e-5 measured for FP16x3
     ("rs_monthly", 1): 1e-3, ("rs_monthly", 2): 5e-5, ("rs_monthly", 3): 5e-5,
+    # FP8 e4m3 hi/lo with power-of-two scales (about 8 bits a pair): measured
+    # 8.0e-4 / 2.1e-3 / 1.5e-3, above the 1e-4 bar; bounds at ~2.5x
+    ("rs2048", 4): 2e-3, ("lv1d", 4): 5e-3, ("rs_monthly", 4): 4e-3,
 }
 
 

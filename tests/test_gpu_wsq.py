@@ -327,3 +327,76 @@ def test_wform_backward_step(hip, mode):
     bad = np.argwhere(yt[1] != yl)
     assert len(bad) == 0, (len(bad), [(float(Y.T[tuple(b)]), int(yh[tuple(b)]), int(yt[1][tuple(b)]),
                                        int(yl[tuple(b)])) for b in bad[:4]])
+
+
+def e4m3_rne(x):
+    """round to OCP FP8 e4m3 (bias 7, 3 mantissa bits, subnormals below
+    2^-6, max 448), nearest even"""
+    x = np.asarray(x, dtype=np.float64)
+    a = np.abs(x)
+    e = np.maximum(np.floor(np.log2(np.maximum(a, 1e-300))), -6.0)
+    step = 2.0 ** (e - 3)
+    return np.sign(x) * np.minimum(np.round(a / step) * step, 448.0)
+
+
+def e4m3_dec(b):
+    b = np.asarray(b, dtype=np.int64)
+    s = np.where(b >> 7, -1.0, 1.0)
+    ex, man = (b >> 3) & 15, b & 7
+    return s * np.where(ex == 0, man * 2.0**-9, (1.0 + man / 8.0) * 2.0 ** (ex - 7.0))
+
+
+def test_fp8_split_and_one_squaring(hip):
+    """FP8 W form: the e4m3 copies are the round-to-nearest-even of the
+    scaled FP32 values (bit-exact), scales are the powers of two that put
+    each row / column maximum in [64, 128), and one squaring equals the
+    same arithmetic in numpy (FP64 accumulation) to 1e-6 of max|W'|"""
+    banner("fp8 W form: split + one squaring")
+    from oplib_amd import opmodel as M
+    H = hip
+    c = M.config_small(nx=128, nr=2, nsq=12)
+    d = c.nx * c.nr
+    x, _ = M.grid(c)
+    dr, vo = M.params(c)
+    xd, ddv, vd = (H.DeviceArray.from_numpy(a) for a in (x, dr, vo))
+    st = H.DeviceArray(H.lib.ophip_w8_bytes(d), np.uint8)
+    dt = math.ldexp(c.DeltaT, -c.nsq)
+    H.check(H.lib.ophip_sek_rs(c.nx, c.nr, xd.h, ddv.h, vd.h, c.q_off, dt, 2, H.lib.ophip_w8_w(st.h, d, 0)))
+    H.check(H.lib.ophip_w8_exec(st.h, d, 1))
+    H.sync()
+
+    def rd(h, n, dt_):
+        out = np.empty(n, dtype=dt_)
+        H.check(H.lib.opcuda_memcpy_d2h(h, out.ctypes.data, out.nbytes))
+        return out
+    W0 = rd(H.lib.ophip_w8_w(st.h, d, 0), d * d, np.float32).reshape(d, d, order="F").astype(np.float64)
+    W1 = rd(H.lib.ophip_w8_w(st.h, d, 1), d * d, np.float32).reshape(d, d, order="F").astype(np.float64)
+    rs = rd(H.lib.ophip_w8_array(st.h, d, 4), d, np.float32).astype(np.float64)
+    cs = rd(H.lib.ophip_w8_array(st.h, d, 5), d, np.float32).astype(np.float64)
+    arrs = [unfrag(rd(H.lib.ophip_w8_array(st.h, d, k), d * d, np.uint8), d, d) for k in range(4)]
+    Ah, Al, Bh, Bl = (e4m3_dec(a) for a in arrs)
+    # scales: powers of two with max * scale in [64, 128)
+    for sc, mx in ((rs, np.abs(W0).max(axis=1)), (cs, np.abs(W0).max(axis=0))):
+        assert np.all(np.log2(sc) == np.round(np.log2(sc)))
+        assert np.all((mx * sc >= 64) & (mx * sc < 128))
+    # copies: hi = rne(W s), lo = rne((W s - hi) 16), bit-exact
+    va, vb = W0 * rs[:, None], (W0 * cs[None, :]).T
+    for v, hi, lo in ((va, Ah, Al), (vb, Bh, Bl)):
+        v32 = v.astype(np.float32).astype(np.float64)
+        assert np.array_equal(hi, e4m3_rne(v32))
+        assert np.array_equal(lo, e4m3_rne(((v32 - hi).astype(np.float32) * 16.0).astype(np.float64)))
+    # one squaring with the same arithmetic, then the FP64 diagonal reset
+    P = (Ah @ Bh.T + (Ah @ Bl.T + Al @ Bh.T) / 16.0) / (rs[:, None] * cs[None, :])
+    ref = (2.0 * W0 + P).astype(np.float32).astype(np.float64)
+    idx = np.arange(d)
+    off = ref.copy()
+    off[idx, idx] = 0.0
+    ref[idx, idx] = -off.sum(axis=1)
+    err = np.abs(W1 - ref).max() / np.abs(ref).max()
+    print(f"[err] fp8 one squaring vs emulation: {err:.3e}")
+    assert err < 1e-6
+    # and against the exact square: the e4m3 pair's ~2^-8 of |W|^2
+    ex = 2.0 * W0 + W0 @ W0
+    rel = np.abs(W1 - ex).max() / np.abs(W0 @ W0).max()
+    print(f"[err] fp8 product vs exact: {rel:.3e} of max|W*W|")
+    assert rel < 2e-2
