@@ -148,24 +148,29 @@ __global__ __launch_bounds__(64) void k_wsplit(char *__restrict__ base,
 // loads in flight in named register sets (no scratch), LDS double buffer.
 // Requires d % 128 == 0 (whole tiles, K-steps a multiple of 4).
 // ---------------------------------------------------------------------
-template <bool BF, bool LO, int BT>
-__global__ __launch_bounds__(256) void k_wsq(char *__restrict__ base,
+template <bool BF, bool LO, int BT, int NW>
+__global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
                                              const WScen *__restrict__ scen,
                                              const WItem *__restrict__ items, int nitems) {
     using N = Num<BF>;
-    constexpr int BK = 32, PITCH = BK + 8;  // 16-bit elements per LDS row
-    constexpr int WT = BT / 2, T = WT / 16;  // wave tile, MFMA tiles per side
+    // LDS rows of BK = 32 16-bit elements (64 bytes, no padding); the four
+    // 16-byte chunks of row R sit at chunk c ^ ((R >> 1) & 3), so that the
+    // stores (8 lanes = 2 rows x 4 chunks) and the fragment reads (8 lanes =
+    // 8 rows, one chunk) each touch 8 distinct 16-byte bank groups
+    constexpr int BK = 32, PITCH = BK;
+    // NW waves as 2 (rows) x NW/2 (columns); wave tile WTM x WTN of TM x TN MFMA tiles
+    constexpr int WGN = NW / 2, WTM = BT / 2, WTN = BT / WGN, TM = WTM / 16, TN = WTN / 16;
     constexpr int NOP = LO ? 2 : 1;
-    constexpr int LD16 = BT * BK / 8 / 256;  // 16-byte loads per thread per operand array
+    constexpr int LD16 = BT * BK / 8 / (64 * NW);  // 16-byte loads per thread per operand array
     constexpr int PF = 4;                    // K-steps of loads in flight
     static_assert(LD16 >= 1, "tile");
     __shared__ __attribute__((aligned(16))) uint16_t As[2][NOP][BT * PITCH];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[2][NOP][BT * PITCH];
-    __shared__ double rsum[2][BT];
+    __shared__ double rsum[WGN][BT];
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int r = lane & 15, g = lane >> 4;
-    const int wm0 = (wave & 1) * WT, wn0 = (wave >> 1) * WT;
+    const int wm0 = (wave & 1) * WTM, wn0 = (wave >> 1) * WTN;
 
     for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
         const WItem it = items[item];
@@ -175,11 +180,11 @@ __global__ __launch_bounds__(256) void k_wsq(char *__restrict__ base,
         const uint16_t *Hr = at<const uint16_t>(base, sc.Hr), *Hc = at<const uint16_t>(base, sc.Hc);
         const uint16_t *Lr = at<const uint16_t>(base, sc.Lr), *Lc = at<const uint16_t>(base, sc.Lc);
 
-        f4 acc[T][T], cor[T][T];
+        f4 acc[TM][TN], cor[TM][TN];
 #pragma unroll
-        for (int a = 0; a < T; a++)
+        for (int a = 0; a < TM; a++)
 #pragma unroll
-            for (int b = 0; b < T; b++) {
+            for (int b = 0; b < TN; b++) {
                 acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
                 cor[a][b] = f4{0.f, 0.f, 0.f, 0.f};
             }
@@ -188,7 +193,7 @@ __global__ __launch_bounds__(256) void k_wsq(char *__restrict__ base,
         auto gload = [&](Stage &x, int k0) {
 #pragma unroll
             for (int q = 0; q < LD16; q++) {
-                const int idx = t + 256 * q;
+                const int idx = t + 64 * NW * q;
                 const int row = idx >> 2, part = idx & 3;  // 4 x 16 B per 32-element row
                 const size_t ea = (size_t)(i0 + row) * d + k0 + 8 * part;  // row-major A
                 const size_t eb = (size_t)(j0 + row) * d + k0 + 8 * part;  // column-major B
@@ -203,37 +208,40 @@ __global__ __launch_bounds__(256) void k_wsq(char *__restrict__ base,
         auto sstore = [&](const Stage &x, int buf) {
 #pragma unroll
             for (int q = 0; q < LD16; q++) {
-                const int idx = t + 256 * q;
+                const int idx = t + 64 * NW * q;
                 const int row = idx >> 2, part = idx & 3;
 #pragma unroll
                 for (int o = 0; o < NOP; o++) {
-                    *(u4v *)&As[buf][o][row * PITCH + 8 * part] = x[0][o][q];
-                    *(u4v *)&Bs[buf][o][row * PITCH + 8 * part] = x[1][o][q];
+                    const int sw = row * PITCH + 8 * (part ^ ((row >> 1) & 3));
+                    *(u4v *)&As[buf][o][sw] = x[0][o][q];
+                    *(u4v *)&Bs[buf][o][sw] = x[1][o][q];
                 }
             }
         };
         auto compute = [&](int buf) {
-            s8 fa[NOP][T], fb[NOP][T];
+            s8 fa[NOP][TM], fb[NOP][TN];
 #pragma unroll
             for (int o = 0; o < NOP; o++)
 #pragma unroll
-                for (int a = 0; a < T; a++) {
-                    fa[o][a] = *(const s8 *)&As[buf][o][(wm0 + 16 * a + r) * PITCH + 8 * g];
-                    fb[o][a] = *(const s8 *)&Bs[buf][o][(wn0 + 16 * a + r) * PITCH + 8 * g];
+                for (int a = 0; a < (TM > TN ? TM : TN); a++) {
+                    // wm0, wn0 and 16a are multiples of 16: the row's swizzle is (r >> 1) & 3
+                    const int cg = 8 * (g ^ ((r >> 1) & 3));
+                    if (a < TM) fa[o][a] = *(const s8 *)&As[buf][o][(wm0 + 16 * a + r) * PITCH + cg];
+                    if (a < TN) fb[o][a] = *(const s8 *)&Bs[buf][o][(wn0 + 16 * a + r) * PITCH + cg];
                 }
 #pragma unroll
-            for (int b = 0; b < T; b++)
+            for (int b = 0; b < TN; b++)
 #pragma unroll
-                for (int a = 0; a < T; a++) acc[a][b] = N::mfma(fa[0][a], fb[0][b], acc[a][b]);
+                for (int a = 0; a < TM; a++) acc[a][b] = N::mfma(fa[0][a], fb[0][b], acc[a][b]);
             if (LO) {
 #pragma unroll
-                for (int b = 0; b < T; b++)
+                for (int b = 0; b < TN; b++)
 #pragma unroll
-                    for (int a = 0; a < T; a++) cor[a][b] = N::mfma(fa[0][a], fb[1][b], cor[a][b]);
+                    for (int a = 0; a < TM; a++) cor[a][b] = N::mfma(fa[0][a], fb[1][b], cor[a][b]);
 #pragma unroll
-                for (int b = 0; b < T; b++)
+                for (int b = 0; b < TN; b++)
 #pragma unroll
-                    for (int a = 0; a < T; a++) cor[a][b] = N::mfma(fa[1][a], fb[0][b], cor[a][b]);
+                    for (int a = 0; a < TM; a++) cor[a][b] = N::mfma(fa[1][a], fb[0][b], cor[a][b]);
             }
         };
 
@@ -269,16 +277,16 @@ __global__ __launch_bounds__(256) void k_wsq(char *__restrict__ base,
         uint16_t *Hrn = at<uint16_t>(base, sc.Hrn), *Hcn = at<uint16_t>(base, sc.Hcn);
         uint16_t *Lrn = at<uint16_t>(base, sc.Lrn), *Lcn = at<uint16_t>(base, sc.Lcn);
         const float inv = N::IS * N::IS;
-        double rows[T][4];
+        double rows[TM][4];
 #pragma unroll
-        for (int a = 0; a < T; a++)
+        for (int a = 0; a < TM; a++)
 #pragma unroll
             for (int v = 0; v < 4; v++) rows[a][v] = 0.0;
 #pragma unroll
-        for (int a = 0; a < T; a++) {
+        for (int a = 0; a < TM; a++) {
             const int row = i0 + wm0 + 16 * a + 4 * g;
 #pragma unroll
-            for (int b = 0; b < T; b++) {
+            for (int b = 0; b < TN; b++) {
                 const int col = j0 + wn0 + 16 * b + r;
                 const size_t e = (size_t)row + (size_t)d * col;
                 f4 w = *(const f4 *)(W + e);
@@ -311,9 +319,9 @@ __global__ __launch_bounds__(256) void k_wsq(char *__restrict__ base,
                 }
             }
         }
-        // row sums over this wave's WT columns: reduce across the 16 lanes r
+        // row sums over this wave's WTN columns: reduce across the 16 lanes r
 #pragma unroll
-        for (int a = 0; a < T; a++)
+        for (int a = 0; a < TM; a++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 double x = rows[a][v];
@@ -325,22 +333,23 @@ __global__ __launch_bounds__(256) void k_wsq(char *__restrict__ base,
             }
         if (r == 0) {
 #pragma unroll
-            for (int a = 0; a < T; a++)
+            for (int a = 0; a < TM; a++)
 #pragma unroll
                 for (int v = 0; v < 4; v++) rsum[wave >> 1][wm0 + 16 * a + 4 * g + v] = rows[a][v];
         }
         __syncthreads();
-        // one partial slot per 64-column block: with BT = 128 each wave
-        // column is one block; with BT = 64 the two 32-wide wave columns
-        // make one block
+        // one partial slot per 64-column block, the sum of its CPS wave
+        // columns in order (CPS = 64 / WTN)
         double *part = at<double>(base, sc.part);
         if (t < BT) {
+            constexpr int CPS = 64 / WTN;
             const int row = i0 + t;
-            if (BT == 128) {
-                part[(size_t)(2 * it.tn + 0) * d + row] = rsum[0][t];
-                part[(size_t)(2 * it.tn + 1) * d + row] = rsum[1][t];
-            } else {
-                part[(size_t)it.tn * d + row] = rsum[0][t] + rsum[1][t];
+#pragma unroll
+            for (int h = 0; h < BT / 64; h++) {
+                double x = rsum[h * CPS][t];
+#pragma unroll
+                for (int c = 1; c < CPS; c++) x += rsum[h * CPS + c][t];
+                part[(size_t)((BT / 64) * it.tn + h) * d + row] = x;
             }
         }
     }
@@ -861,10 +870,10 @@ OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
                 if (g > 8) g -= g % 8;
                 // work items carry s = src*n + i (the full table)
                 if (c == 1)
-                    hipLaunchKernelGGL((k_wsq<BF, LO, 128>), dim3(g), dim3(256), 0, D.stream, D.base,
+                    hipLaunchKernelGGL((k_wsq<BF, LO, 128, 8>), dim3(g), dim3(512), 0, D.stream, D.base,
                                        P->sc, its, nit);
                 else
-                    hipLaunchKernelGGL((k_wsq<BF, LO, 64>), dim3(g), dim3(256), 0, D.stream, D.base,
+                    hipLaunchKernelGGL((k_wsq<BF, LO, 64, 4>), dim3(g), dim3(256), 0, D.stream, D.base,
                                        P->sc, its, nit);
             }
             if (!any) continue;
