@@ -93,8 +93,8 @@ def main():
     ap.add_argument("--no-mc", action="store_true", help="skip the Monte Carlo leg")
     ap.add_argument("--no-sweep", action="store_true", help="skip the calibration-sweep leg")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate leg")
-    ap.add_argument("--surr-train", type=int, default=65536, help="surrogate training draws")
-    ap.add_argument("--surr-steps", type=int, default=40000, help="surrogate Adam steps")
+    ap.add_argument("--surr-train", type=int, default=131072, help="surrogate training draws")
+    ap.add_argument("--surr-steps", type=int, default=80000, help="surrogate Adam steps")
     ap.add_argument("--probe", action="store_true",
                     help="only launch the dominant kernel (W-form backward step) 20 times and a "
                          "64 MiB copy kernel 20 times, for counter collection")
