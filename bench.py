@@ -93,6 +93,7 @@ def main():
     ap.add_argument("--no-mc", action="store_true", help="skip the Monte Carlo leg")
     ap.add_argument("--no-sweep", action="store_true", help="skip the calibration-sweep leg")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate leg")
+    ap.add_argument("--no-fp8", action="store_true", help="skip the FP8 W-form method")
     ap.add_argument("--surr-train", type=int, default=131072, help="surrogate training draws")
     ap.add_argument("--surr-steps", type=int, default=80000, help="surrogate Adam steps")
     ap.add_argument("--probe", action="store_true",
@@ -194,8 +195,23 @@ def main():
                               "meets_1e-4": bool(rel.max() <= 1e-4)}}
     for prec, name in ((1, "wform_fp16x1"), (2, "wform_fp16x3"), (3, "wform_bf16x3"),
                        (4, "wform_fp8x3")):
+        if prec == 4 and args.no_fp8:
+            continue
         methods[name] = bench_method(H, M, prec, args.steps, args.warmup, gold)
         log(f"[bench] {name}: {methods[name]}")
+
+    # ---- precision study: the same generator to the 12 monthly maturities
+    # (41 strikes x 12 maturities x 8 regimes), every method's error ----
+    gold_m = np.load(os.path.join(ROOT, "tests", "golden", "prices_rs_monthly.npz"))["prices"]
+    precision = {"workload": "2048-state regime lattice, 16 squarings to U(1/12), propagation to "
+                             "12 monthly maturities, 41 strikes, prices at every regime's spot node"}
+    for prec, name in ((0, "plain_fp32"), (1, "wform_fp16x1"), (2, "wform_fp16x3"),
+                       (3, "wform_bf16x3"), (4, "wform_fp8x3")):
+        if prec == 4 and args.no_fp8:
+            continue
+        precision[name] = bench_method(H, M, prec, max(3, args.steps // 4), 1, gold_m,
+                                       M.config_rs_monthly)
+        log(f"[bench] precision study {name}: {precision[name]}")
 
     # ---- calibration sweep: 64 lattices, plain FP32 one by one vs batched W form ----
     sweep = None
@@ -264,6 +280,7 @@ def main():
                  "algorithmic_GBps": bi_gbs, "peak_GBps": HBM_PEAK_GBS,
                  "frac": bi_gbs / HBM_PEAK_GBS},
         "methods": methods,
+        "precision_study": precision,
         "sweep": sweep,
         "mc": mc,
         "surrogate": surrogate,
@@ -320,8 +337,8 @@ def rel_err(p, gold):
     return np.abs(p - gold) / den
 
 
-def bench_method(H, M, prec, steps, warmup, gold):
-    cfg = M.config_rs2048()
+def bench_method(H, M, prec, steps, warmup, gold, make=None):
+    cfg = (make or M.config_rs2048)()
     cfg.precision = prec
     s = M.Solver(cfg)
     s.solve()

@@ -212,6 +212,12 @@ unsigned long ophip_fwd_plan(int n, const int *d, const unsigned *K, const int *
                              int nstrikes);
 int  ophip_fwd_exec(unsigned long plan);
 void ophip_fwd_plan_free(unsigned long plan);
+/* prices at up to 16 spot rows of one kernel (U, or W with plus_identity):
+ * out[r*nstrikes + c] = df (plus_identity pay(y0_r, c) + sum_j U[y0_r][j]
+ * pay(j, c)), pay(j, c) = payT[c + nstrikes*j]: the payoff GEMV of each
+ * row (OPPricer.cs:143-146) in one launch */
+int  ophip_spot_prices(int d, unsigned U, int nrow, const int *rows, int plus_identity,
+                       unsigned payT, int nstrikes, float df, unsigned out);
 /* new rates for every lattice of a plan (waits for the library stream) */
 int  ophip_fwd_set_rates(unsigned long plan, const float *rate);
 

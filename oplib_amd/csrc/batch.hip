@@ -51,6 +51,47 @@ __global__ __launch_bounds__(256) void k_price(const FwdScen *__restrict__ sc,
     if (lane == 0) s.out[(size_t)kk * nstrikes + c] = expf(-s.rate * s.dT * (kk + 1)) * acc;
 }
 
+// prices of a few spot rows of one kernel: out[r][c] = df (plus * pay(y0_r,
+// c) + sum_j U[y0_r][j] pay(j, c)) with U column-major and pay(j, c) =
+// payT[c + nstrikes j].  One workgroup per row; each thread accumulates
+// every strike for its columns j, then the 4 waves are reduced in order.
+struct SpotRows {
+    int n, pad;
+    int y0[16];
+};
+
+__global__ __launch_bounds__(256) void k_spot_prices(int d, const float *__restrict__ U, SpotRows rows,
+                                                     int plus_identity, const float *__restrict__ payT,
+                                                     int nstrikes, float df, float *__restrict__ out) {
+    __shared__ float red[4][64];
+    const int r = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int y0 = rows.y0[r];
+    float acc[64];
+#pragma unroll
+    for (int c = 0; c < 64; c++) acc[c] = 0.0f;
+    for (int j = t; j < d; j += 256) {
+        const float u = U[(size_t)y0 + (size_t)d * j];
+        const float *p = payT + (size_t)nstrikes * j;
+#pragma unroll
+        for (int c = 0; c < 64; c++)
+            if (c < nstrikes) acc[c] += u * p[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 64; c++) {
+        if (c < nstrikes) {  // uniform
+            float x = acc[c];
+            for (int w = 32; w >= 1; w >>= 1) x += __shfl_xor(x, w, 64);
+            if (lane == 0) red[wave][c] = x;
+        }
+    }
+    __syncthreads();
+    if (t < nstrikes) {
+        float x = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
+        if (plus_identity) x += payT[(size_t)nstrikes * y0 + t];
+        out[(size_t)r * nstrikes + t] = df * x;
+    }
+}
+
 struct FwdPlan {
     int n = 0, nmat = 0, nstrikes = 0, dmax = 0;
     void *dev = nullptr;
@@ -155,4 +196,24 @@ OPHIP_EXPORT int ophip_fwd_set_rates(unsigned long plan, const float *rate) {
         e = hipMemcpyAsync(pl->sc, pl->host, sizeof(FwdScen) * pl->n, hipMemcpyHostToDevice, dev().stream);
     if (e == hipSuccess) e = hipStreamSynchronize(dev().stream);
     return (int)set_status(hip_status(e));
+}
+
+// prices at up to 16 spot rows of a kernel (U, or W when plus_identity):
+// out[r * nstrikes + c], see k_spot_prices; kernel launches only
+OPHIP_EXPORT int ophip_spot_prices(int d, unsigned U, int nrow, const int *rows, int plus_identity,
+                                   unsigned payT, int nstrikes, float df, unsigned out) {
+    if (d <= 0 || nrow <= 0 || nrow > 16 || !rows || nstrikes <= 0 || nstrikes > 64)
+        return (int)set_status(kErrBadArg);
+    if (!handle_range_ok(U, 4 * (size_t)d * d) || !handle_range_ok(payT, 4 * (size_t)d * nstrikes) ||
+        !handle_range_ok(out, 4 * (size_t)nrow * nstrikes))
+        return (int)set_status(kErrBadHandle);
+    SpotRows sr{};
+    sr.n = nrow;
+    for (int r = 0; r < nrow; r++) {
+        if (rows[r] < 0 || rows[r] >= d) return (int)set_status(kErrBadArg);
+        sr.y0[r] = rows[r];
+    }
+    hipLaunchKernelGGL(k_spot_prices, dim3(nrow), dim3(256), 0, dev().stream, d, P<const float>(U), sr,
+                       plus_identity ? 1 : 0, P<const float>(payT), nstrikes, df, P<float>(out));
+    return (int)check_launch();
 }

@@ -105,6 +105,9 @@ struct opm_solver {
     Buf U[2];            // FP32 d x d ping-pong for the squarings
     Buf V[2], pay0;      // backward induction: d x ncp row-major
     Buf VT[2];           // W form (x3 modes): hi/lo copies of V for the matrix-core step
+    // W form, propagation: the spot rows' forward chain p_k = p_{k-1} (I + W)
+    Buf fP, fx, fout;
+    unsigned long fplan = 0;
     Buf Uk[2], payT, out;  // propagation: U_k, payoff (nstrikes x d), prices
     int u_final = 0, v_final = 0;
     unsigned long graph = 0;
@@ -211,6 +214,35 @@ opm_solver *opm_create(const opm_config *cfg) {
                 }
             ok = !upload(s->pay0, p.data(), vb);
         }
+    } else if (ok && s->wmode >= 0) {
+        // W form: the forward chain of each regime's spot row, p_k =
+        // p_{k-1} (I + W), and its prices (batch.hip), instead of forming
+        // every W_k (only the spot rows are priced)
+        s->n_mat = c->nsteps;
+        const size_t prow = (size_t)(c->nsteps + 1) * d;
+        ok = alloc(s->fP, 4 * prow * nr) && alloc(s->fx, 4 * (size_t)d) &&
+             alloc(s->fout, 4 * (size_t)nr * c->nsteps * c->nstrikes);
+        if (ok) {
+            std::vector<float> P0(prow * nr, 0.0f), xs((size_t)d);
+            for (int r = 0; r < nr; r++) P0[prow * r + s->x0 + (size_t)nx * r] = 1.0f;
+            for (int y = 0; y < d; y++) xs[y] = (float)x[y % nx];
+            ok = !upload(s->fP, P0.data(), 4 * prow * nr) && !upload(s->fx, xs.data(), 4 * (size_t)d);
+        }
+        if (ok) {
+            const unsigned W = s->wmode == 3 ? ophip_w8_w(s->wstate.h, d, c->nsq & 1)
+                                             : ophip_wsq_w(s->wstate.h, d, s->wmode, c->nsq & 1);
+            std::vector<int> ds(nr, d), plus(nr, 1);
+            std::vector<unsigned> K(nr, W), Ph(nr), xh(nr, s->fx.h), oh(nr);
+            std::vector<float> rate(nr, (float)c->rate), ks(c->strikes, c->strikes + c->nstrikes);
+            for (int r = 0; r < nr; r++) {
+                Ph[r] = s->fP.h + (unsigned)(4 * prow * r);
+                oh[r] = s->fout.h + (unsigned)(4 * (size_t)r * c->nsteps * c->nstrikes);
+            }
+            s->fplan = ophip_fwd_plan(nr, ds.data(), K.data(), plus.data(), Ph.data(), xh.data(),
+                                      oh.data(), rate.data(), (float)c->DeltaT, c->nsteps, ks.data(),
+                                      c->nstrikes);
+            ok = s->fplan != 0;
+        }
     } else if (ok) {
         s->n_mat = c->nsteps;
         ok = alloc(s->Uk[0], 4 * dd) && alloc(s->Uk[1], 4 * dd) &&
@@ -238,9 +270,10 @@ void opm_destroy(opm_solver *s) {
     if (!s) return;
     if (s->graph) ophip_graph_destroy(s->graph);
     if (s->wplan) ophip_wsq_plan_free(s->wplan);
+    if (s->fplan) ophip_fwd_plan_free(s->fplan);
     Buf *all[] = {&s->x, &s->drift, &s->vol, &s->U[0], &s->U[1], &s->V[0], &s->V[1],
                   &s->VT[0], &s->VT[1], &s->pay0, &s->Uk[0], &s->Uk[1], &s->payT, &s->out,
-                  &s->wstate};
+                  &s->wstate, &s->fP, &s->fx, &s->fout};
     for (Buf *b : all) release(*b);
     delete s;
 }
@@ -304,6 +337,9 @@ static int solve_body(opm_solver *s) {
             }
         }
         s->v_final = c->nsteps & 1;
+    } else if (s->fplan) {
+        st = ophip_fwd_exec(s->fplan);
+        if (st) return st;
     } else {
         // U_1 = U; U_k = U_{k-1} * U (CModel Device.cs:269-279); in the W
         // form W_k = W_{k-1} + W + W_{k-1} * W
@@ -323,23 +359,13 @@ static int solve_body(opm_solver *s) {
                 if (st) return st;
             }
             const float df = (float)std::exp(-c->rate * c->DeltaT * k);
-            for (int r = 0; r < nr; r++) {
-                // row y0 of U_k is the vector at element offset y0, stride d
-                const int y0 = s->x0 + nx * r;
-                const unsigned row = cur + 4u * (unsigned)y0;
-                const unsigned dst =
-                    s->out.h + 4u * (unsigned)(((k - 1) * nr + r) * c->nstrikes);
-                float beta = 0.0f;
-                if (wform) {
-                    // the identity's row: the payoffs at y0 themselves
-                    opcuda_scopy(c->nstrikes, s->payT.h + 4u * (unsigned)(c->nstrikes * y0), 1,
-                                 dst, 1);
-                    beta = df;
-                }
-                opcuda_sgemv(c->nstrikes, d, df, s->payT.h, c->nstrikes, row, d, beta, dst, 1);
-                st = (int)opcuda_get_status();
-                if (st) return st;
-            }
+            // prices at the spot row of every regime (the payoff GEMV of
+            // each row, OPPricer.cs:143-146), one launch per maturity
+            std::vector<int> rows(nr);
+            for (int r = 0; r < nr; r++) rows[r] = s->x0 + nx * r;
+            st = ophip_spot_prices(d, cur, nr, rows.data(), wform ? 1 : 0, s->payT.h, c->nstrikes, df,
+                                   s->out.h + 4u * (unsigned)((k - 1) * nr * c->nstrikes));
+            if (st) return st;
             prev = cur;
         }
     }
@@ -390,6 +416,13 @@ int opm_prices(opm_solver *s, double *prices) {
         for (int r = 0; r < nr; r++)
             for (int k = 0; k < c->nstrikes; k++)
                 prices[(size_t)r * c->nstrikes + k] = v[(size_t)(s->x0 + nx * r) * s->ncp + k];
+    } else if (s->fplan) {
+        // forward chain: device order [r][k][strike], the output's
+        std::vector<float> o((size_t)c->nsteps * nr * c->nstrikes);
+        int st = opcuda_memcpy_d2h(s->fout.h, (unsigned long)(uintptr_t)o.data(),
+                                   (unsigned)(4 * o.size()));
+        if (st) return st;
+        for (size_t e = 0; e < o.size(); e++) prices[e] = o[e];
     } else {
         std::vector<float> o((size_t)c->nsteps * nr * c->nstrikes);
         int st = opcuda_memcpy_d2h(s->out.h, (unsigned long)(uintptr_t)o.data(),

@@ -102,6 +102,7 @@ SIGNATURES = {
                                I32, C.POINTER(F32), I32]),
     "ophip_fwd_exec": (I32, [ULONG]),
     "ophip_fwd_plan_free": (V, [ULONG]),
+    "ophip_spot_prices": (I32, [I32, U32, I32, C.POINTER(I32), I32, U32, I32, F32, U32]),
     "ophip_fwd_set_rates": (I32, [ULONG, C.POINTER(F32)]),
     "ophip_mlp_create": (ULONG, [I32, I32]),
     "ophip_mlp_free": (V, [ULONG]),
