@@ -435,3 +435,24 @@ def test_mc1f_rejects_bad_shapes(H):
         assert st != 0
     finally:
         H.lib.ophip_mc_set_nrng(4096)
+
+
+@pytest.mark.parametrize("plus", [0, 1])
+def test_spot_prices(H, plus):
+    """df (plus pay(y0, c) + sum_j U[y0][j] pay(j, c)) for a few rows of a
+    column-major U, against FP64 numpy (FP32 accumulation: 1e-5)"""
+    import ctypes as C
+    banner(f"spot prices plus_identity={plus}")
+    rng = np.random.RandomState(21)
+    d, ns = 512, 41
+    U = rng.uniform(0, 1.0 / d, size=(d, d)).astype(np.float32)
+    pay = np.maximum(rng.uniform(0, 400, size=d)[:, None] - (80.0 + np.arange(ns))[None, :], 0).astype(np.float32)
+    Ud = dev(H, U)                       # column-major
+    Pd = dev(H, pay.T)                   # payT[c + ns j] = pay[j][c]
+    rows = np.array([7, 64, 300, 511], dtype=np.int32)
+    out = H.DeviceArray.empty(len(rows) * ns)
+    H.check(H.lib.ophip_spot_prices(d, Ud.h, len(rows), rows.ctypes.data_as(C.POINTER(C.c_int)), plus,
+                                    Pd.h, ns, np.float32(0.97), out.h))
+    got = out.download().reshape(len(rows), ns)
+    ref = 0.97 * (U[rows].astype(np.float64) @ pay.astype(np.float64) + plus * pay[rows])
+    assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max()
