@@ -53,23 +53,28 @@ __global__ __launch_bounds__(256) void k_price(const FwdScen *__restrict__ sc,
 
 // prices of a few spot rows of one kernel: out[r][c] = df (plus * pay(y0_r,
 // c) + sum_j U[y0_r][j] pay(j, c)) with U column-major and pay(j, c) =
-// payT[c + nstrikes j].  One workgroup per row; each thread accumulates
-// every strike for its columns j, then the 4 waves are reduced in order.
+// payT[c + nstrikes j].  k_spot_part: workgroup (r, s) takes the columns
+// j of chunk s, one per thread, and writes the chunk's partial sums for
+// every strike (its 4 waves reduced in order); k_spot_sum adds the chunks
+// in order.
 struct SpotRows {
     int n, pad;
     int y0[16];
 };
+constexpr int kSpotChunks = 16;
 
-__global__ __launch_bounds__(256) void k_spot_prices(int d, const float *__restrict__ U, SpotRows rows,
-                                                     int plus_identity, const float *__restrict__ payT,
-                                                     int nstrikes, float df, float *__restrict__ out) {
+__global__ __launch_bounds__(256) void k_spot_part(int d, const float *__restrict__ U, SpotRows rows,
+                                                   const float *__restrict__ payT, int nstrikes,
+                                                   float *__restrict__ part) {
     __shared__ float red[4][64];
-    const int r = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int r = blockIdx.x, sc = blockIdx.y, t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int y0 = rows.y0[r];
+    const int len = (d + kSpotChunks - 1) / kSpotChunks;
+    const int jb = sc * len, je = min(d, jb + len);
     float acc[64];
 #pragma unroll
     for (int c = 0; c < 64; c++) acc[c] = 0.0f;
-    for (int j = t; j < d; j += 256) {
+    for (int j = jb + t; j < je; j += 256) {
         const float u = U[(size_t)y0 + (size_t)d * j];
         const float *p = payT + (size_t)nstrikes * j;
 #pragma unroll
@@ -85,11 +90,18 @@ __global__ __launch_bounds__(256) void k_spot_prices(int d, const float *__restr
         }
     }
     __syncthreads();
-    if (t < nstrikes) {
-        float x = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
-        if (plus_identity) x += payT[(size_t)nstrikes * y0 + t];
-        out[(size_t)r * nstrikes + t] = df * x;
-    }
+    if (t < nstrikes)
+        part[((size_t)r * kSpotChunks + sc) * 64 + t] = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
+}
+
+__global__ void k_spot_sum(SpotRows rows, int plus_identity, const float *__restrict__ payT,
+                           int nstrikes, float df, const float *__restrict__ part, float *__restrict__ out) {
+    const int r = blockIdx.x, c = threadIdx.x;
+    if (c >= nstrikes) return;
+    float x = 0.0f;
+    for (int sc = 0; sc < kSpotChunks; sc++) x += part[((size_t)r * kSpotChunks + sc) * 64 + c];
+    if (plus_identity) x += payT[(size_t)nstrikes * rows.y0[r] + c];
+    out[(size_t)r * nstrikes + c] = df * x;
 }
 
 struct FwdPlan {
@@ -199,7 +211,8 @@ OPHIP_EXPORT int ophip_fwd_set_rates(unsigned long plan, const float *rate) {
 }
 
 // prices at up to 16 spot rows of a kernel (U, or W when plus_identity):
-// out[r * nstrikes + c], see k_spot_prices; kernel launches only
+// out[r * nstrikes + c], see k_spot_part; kernel launches only (the
+// partials live in the library workspace)
 OPHIP_EXPORT int ophip_spot_prices(int d, unsigned U, int nrow, const int *rows, int plus_identity,
                                    unsigned payT, int nstrikes, float df, unsigned out) {
     if (d <= 0 || nrow <= 0 || nrow > 16 || !rows || nstrikes <= 0 || nstrikes > 64)
@@ -213,7 +226,11 @@ OPHIP_EXPORT int ophip_spot_prices(int d, unsigned U, int nrow, const int *rows,
         if (rows[r] < 0 || rows[r] >= d) return (int)set_status(kErrBadArg);
         sr.y0[r] = rows[r];
     }
-    hipLaunchKernelGGL(k_spot_prices, dim3(nrow), dim3(256), 0, dev().stream, d, P<const float>(U), sr,
-                       plus_identity ? 1 : 0, P<const float>(payT), nstrikes, df, P<float>(out));
+    float *part = (float *)workspace(sizeof(float) * 64 * kSpotChunks * nrow);
+    if (!part) return (int)kErrOutOfArena;
+    hipLaunchKernelGGL(k_spot_part, dim3(nrow, kSpotChunks), dim3(256), 0, dev().stream, d,
+                       P<const float>(U), sr, P<const float>(payT), nstrikes, part);
+    hipLaunchKernelGGL(k_spot_sum, dim3(nrow), dim3(64), 0, dev().stream, sr, plus_identity ? 1 : 0,
+                       P<const float>(payT), nstrikes, df, (const float *)part, P<float>(out));
     return (int)check_launch();
 }

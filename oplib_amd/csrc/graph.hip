@@ -31,8 +31,9 @@ OPHIP_EXPORT unsigned long ophip_capture_end(void) {
     if (e != hipSuccess || !g) { set_status(hip_status(e)); return 0; }
     hipGraphExec_t ex = nullptr;
     e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     if (e != hipSuccess) { set_status(hip_status(e)); return 0; }
+    D.live_graphs++;
     set_status(kOk);
     return (unsigned long)(uintptr_t)ex;
 }
@@ -44,7 +45,11 @@ OPHIP_EXPORT int ophip_graph_launch(unsigned long gx) {
 }
 
 OPHIP_EXPORT void ophip_graph_destroy(unsigned long gx) {
-    if (gx) hipGraphExecDestroy((hipGraphExec_t)(uintptr_t)gx);
+    if (!gx) return;
+    Device &D = dev();
+    (void)hipStreamSynchronize(D.stream);
+    (void)hipGraphExecDestroy((hipGraphExec_t)(uintptr_t)gx);
+    if (D.live_graphs > 0) D.live_graphs--;
 }
 
 // Record event `slot` on the library stream (the stream every kernel of
@@ -65,7 +70,7 @@ OPHIP_EXPORT double ophip_event_elapsed_ms(int a, int b) {
         set_status(kErrBadArg);
         return -1.0;
     }
-    hipEventSynchronize(g_ev[b]);
+    (void)hipEventSynchronize(g_ev[b]);
     float ms = 0.0f;
     hipError_t e = hipEventElapsedTime(&ms, g_ev[a], g_ev[b]);
     set_status(hip_status(e));

@@ -33,6 +33,7 @@ struct Device {
     int n_cu = 0;
     void *ws = nullptr;        // library workspace (split-K partials etc.)
     size_t ws_bytes = 0;
+    int live_graphs = 0;       // instantiated graphs: they hold the workspace address
 };
 
 // current device context (creates the arena and stream on first use)
@@ -48,7 +49,9 @@ unsigned check_launch();
 // true when [h, h+bytes) lies inside one live arena allocation
 bool handle_range_ok(uint32_t h, size_t bytes);
 // library-owned device workspace of at least `bytes` (grown on demand,
-// never shrunk; only call between kernels on the library stream)
+// never shrunk; only call between kernels on the library stream).  It is
+// not grown while a graph is being captured or any instantiated graph is
+// alive (those hold its address): reserve enough before capturing.
 void *workspace(size_t bytes);
 
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }

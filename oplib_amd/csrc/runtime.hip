@@ -59,7 +59,7 @@ bool init_device(int id) {
     e = hipMalloc(&p, bytes);
     if (e != hipSuccess) { set_status(hip_status(e)); return false; }
     e = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking);
-    if (e != hipSuccess) { hipFree(p); set_status(hip_status(e)); return false; }
+    if (e != hipSuccess) { (void)hipFree(p); set_status(hip_status(e)); return false; }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, id) == hipSuccess) d.n_cu = prop.multiProcessorCount;
     d.base = (char *)p;
@@ -106,9 +106,15 @@ unsigned check_launch() {
 void *workspace(size_t bytes) {
     Device &d = dev();
     if (d.ws_bytes >= bytes) return d.ws;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (d.live_graphs > 0 || (hipStreamIsCapturing(d.stream, &cs) == hipSuccess &&
+                              cs != hipStreamCaptureStatusNone)) {
+        set_status(kErrUnsupported);  // a graph holds the current address
+        return nullptr;
+    }
     if (d.ws) {
-        hipStreamSynchronize(d.stream);
-        hipFree(d.ws);
+        (void)hipStreamSynchronize(d.stream);
+        (void)hipFree(d.ws);
         d.ws = nullptr;
         d.ws_bytes = 0;
     }
@@ -192,7 +198,7 @@ OPHIP_EXPORT int opcuda_cublas_init(void) {
 
 OPHIP_EXPORT int opcuda_shutdown(void) {
     Device &d = dev();
-    if (d.stream) hipStreamSynchronize(d.stream);
+    if (d.stream) (void)hipStreamSynchronize(d.stream);
     return (int)set_status(kOk);
 }
 
@@ -250,7 +256,7 @@ OPHIP_EXPORT unsigned opcuda_mem_alloc(unsigned sz) {
 
 OPHIP_EXPORT void opcuda_mem_free_device(unsigned h) {
     Device &d = dev();
-    if (d.stream) hipStreamSynchronize(d.stream);  // util.cu:219 syncs first
+    if (d.stream) (void)hipStreamSynchronize(d.stream);  // util.cu:219 syncs first
     set_status(arena_free(h) ? kOk : kErrBadHandle);
 }
 
