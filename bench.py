@@ -395,6 +395,10 @@ def bench_mc(H, U, d, cfg):
     # cumulative table of the daily kernel, row-contiguous
     ct = H.DeviceArray.empty(d * d)
     H.check(H.lib.ophip_sck_rows(d, 1, U.h, ct.h))
+    sck_ms = timed_events(H, lambda: H.lib.ophip_sck_rows(d, 1, U.h, ct.h), 5)
+    H.check()
+    sck_gbs = 8.0 * d * d / (sck_ms * 1e-3) / 1e9  # 4 B read + 4 B written per entry
+    log(f"[bench] cumulative table {sck_ms * 1e3:.1f} us, {sck_gbs:.0f} GB/s")
     nk = 252
     nrng = 4096
     nscen = 1 << 20
@@ -416,6 +420,7 @@ def bench_mc(H, U, d, cfg):
     log(f"[bench] MC {ms:.2f} ms per 2^20 x 252, {trans / ms / 1e6:.2f} G transitions/s")
     ysk.free()
     return {"kernel": "k_mc1f<rows> (MT19937 per wavefront + bisection)", "paths": nscen,
+            "cumulative_table_us": sck_ms * 1e3, "cumulative_table_GBps": sck_gbs,
             "steps": nk, "ms": ms, "transitions_per_s": trans / (ms * 1e-3),
             "streams": nrng, "d": d}
 

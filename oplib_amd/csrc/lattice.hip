@@ -258,62 +258,78 @@ OPHIP_EXPORT void opcuda_sck(int d, int nm, unsigned tpk, unsigned ctpk) {
     check_launch();
 }
 
-// Row-contiguous table crow[m*d*d + row*d + y], same values bit for bit
-// (same sequential order per row).  Workgroup = 64 rows x one m; the
-// column-major input is read 64 states at a time through an LDS tile so
-// both the reads (64 consecutive rows) and the writes (64 consecutive y)
-// are coalesced.  Pass 1 forms the totals, pass 2 re-forms the running sums
-// in the same order and writes the normalised values.
-__global__ __launch_bounds__(64) void k_sck_rows(int d, const float *__restrict__ tpk,
-                                                  float *__restrict__ crow, int plus_identity) {
+// Row-contiguous table crow[m*d*d + row*d + y], the same values bit for bit
+// (the same sequential FP32 order per row): k_sck_scan forms each row's
+// running sums of the positive parts (its last one is the row total the
+// reference forms in a first pass with the same adds), one wavefront per
+// 64 rows; k_sck_norm divides by the total and clamps to 1 over the whole
+// table in parallel.
+__global__ __launch_bounds__(64) void k_sck_scan(int d, const float *__restrict__ tpk,
+                                                  float *__restrict__ crow, float *__restrict__ tot,
+                                                  int plus_identity) {
+    // lane = row: each 64-column step loads the lane's own 64 values with
+    // 64 coalesced loads (64 consecutive rows per column), the next step's
+    // loads issued before this step's sums; the running sums go through
+    // an LDS tile so that the row-contiguous stores are coalesced too.
+    // Rows and columns past d are clamped reads that are never used.
     __shared__ float tile[64][65];
     const int lane = threadIdx.x;
-    const int row0 = blockIdx.x * 64;
-    const int m = blockIdx.y;
-    const size_t mb = (size_t)m * d * d;
-    const int row = row0 + lane;
-    float total = 0.0f;
-    for (int pass = 0; pass < 2; pass++) {
-        float s = 0.0f;
-        for (int y0 = 0; y0 < d; y0 += 64) {
-            int ny = min(64, d - y0);
-            // load tile: element (r, c) = tpk[row0 + r, y0 + c]
-            for (int c = 0; c < ny; c++) {
-                float v = 0.0f;
-                if (row0 + lane < d) {
-                    v = tpk[mb + (size_t)(row0 + lane) + (size_t)d * (y0 + c)];
-                    if (plus_identity && row0 + lane == y0 + c) v += 1.0f;
-                }
-                tile[lane][c] = v;
-            }
-            __syncthreads();
-            // each lane scans its own row through the tile
-            for (int c = 0; c < ny; c++) {
-                float v = tile[lane][c];
+    const int row0 = blockIdx.x * 64, row = row0 + lane;
+    const size_t mb = (size_t)blockIdx.y * d * d;
+    const float *src = tpk + mb + min(row, d - 1);
+    const int last = (d - 1) / 64 * 64;  // first column of the last step
+    float cur[64], nxt[64];
+#pragma unroll
+    for (int c = 0; c < 64; c++) cur[c] = src[(size_t)d * min(c, d - 1)];
+    float s = 0.0f;
+    for (int y0 = 0; y0 < d; y0 += 64) {
+        const int yn = min(y0 + 64, last);  // the last step reloads itself (unused)
+#pragma unroll
+        for (int c = 0; c < 64; c++) nxt[c] = src[(size_t)d * min(yn + c, d - 1)];
+        const int ny = min(64, d - y0);
+#pragma unroll
+        for (int c = 0; c < 64; c++) {
+            if (c < ny) {
+                float v = cur[c];
+                if (plus_identity && row == y0 + c) v += 1.0f;
                 s += (v > 0.0f) ? v : 0.0f;
-                if (pass == 1) {
-                    float q = s / total;
-                    tile[lane][c] = (q < 1.0f) ? q : 1.0f;
-                }
             }
-            __syncthreads();
-            if (pass == 1) {
-                // write rows: for each row r of the tile, lanes cover y
-                for (int r = 0; r < 64 && row0 + r < d; r++)
-                    if (lane < ny) crow[mb + (size_t)(row0 + r) * d + y0 + lane] = tile[r][lane];
-            }
-            __syncthreads();
+            tile[lane][c] = s;
         }
-        total = s;
+        __syncthreads();
+        for (int r = 0; r < 64 && row0 + r < d; r++)
+            if (lane < ny) crow[mb + (size_t)(row0 + r) * d + y0 + lane] = tile[r][lane];
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 64; c++) cur[c] = nxt[c];
     }
-    (void)row;
+    if (row < d) tot[(size_t)blockIdx.y * d + row] = s;
+}
+
+// C = min(1, running sum / total), in place, one thread per 4 entries
+__global__ void k_sck_norm(int d, size_t n, float *__restrict__ crow, const float *__restrict__ tot) {
+    const size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        if (i + e < n) {
+            const float q = crow[i + e] / tot[(i + e) / d];
+            crow[i + e] = q < 1.0f ? q : 1.0f;
+        }
+    }
 }
 
 static int sck_rows(int d, int nm, unsigned tpk, unsigned crow, int plus_identity) {
     Device &D = dev();
     if (d <= 0 || nm <= 0 || !ptr(tpk) || !ptr(crow)) return (int)set_status(kErrBadArg);
-    hipLaunchKernelGGL(k_sck_rows, dim3(cdiv(d, 64), nm), dim3(64), 0, D.stream, d,
-                       P<const float>(tpk), P<float>(crow), plus_identity);
+    if (!handle_range_ok(tpk, 4 * (size_t)nm * d * d) || !handle_range_ok(crow, 4 * (size_t)nm * d * d))
+        return (int)set_status(kErrBadArg);
+    float *tot = (float *)workspace(sizeof(float) * (size_t)nm * d);
+    if (!tot) return (int)kErrOutOfArena;
+    hipLaunchKernelGGL(k_sck_scan, dim3(cdiv(d, 64), nm), dim3(64), 0, D.stream, d, P<const float>(tpk),
+                       P<float>(crow), tot, plus_identity);
+    const size_t n = (size_t)nm * d * d, n4 = (n + 3) / 4;
+    hipLaunchKernelGGL(k_sck_norm, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, D.stream, d, n,
+                       P<float>(crow), (const float *)tot);
     return (int)check_launch();
 }
 
