@@ -661,7 +661,7 @@ struct WsqPlan {
 // an XCD (its L2): a lattice with at least 32 tiles is cut into a 2 x 4
 // grid of tile blocks, one per XCD; smaller lattices go whole to the
 // least-loaded XCD.  The per-XCD lists are then balanced (moving items off
-// the longest lists) and interleaved.
+// the longest lists), sorted longest item first and interleaved.
 static std::vector<WItem> xcd_order(const std::vector<WItem> &v, const int *ds, int n, int T) {
     constexpr int X = 8;
     if (v.size() < 2 * X) return v;
@@ -694,6 +694,12 @@ static std::vector<WItem> xcd_order(const std::vector<WItem> &v, const int *ds, 
             per[x].push_back(per[big].back());
             per[big].pop_back();
         }
+    // within an XCD, longest items first (an item's cost is its lattice's d,
+    // the K length): the XCD's workgroups take items round-robin, so this is
+    // longest-processing-time-first assignment; equal sizes keep their order
+    for (int x = 0; x < X; x++)
+        std::stable_sort(per[x].begin(), per[x].end(),
+                         [&](const WItem &a, const WItem &b) { return ds[a.s % n] > ds[b.s % n]; });
     std::vector<WItem> out;
     out.reserve(total);
     for (size_t k = 0; out.size() < total; k++)
