@@ -153,16 +153,18 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
                                              const WScen *__restrict__ scen,
                                              const WItem *__restrict__ items, int nitems) {
     using N = Num<BF>;
-    // LDS rows of BK = 32 16-bit elements (64 bytes, no padding); the four
-    // 16-byte chunks of row R sit at chunk c ^ ((R >> 1) & 3), so that the
-    // stores (8 lanes = 2 rows x 4 chunks) and the fragment reads (8 lanes =
-    // 8 rows, one chunk) each touch 8 distinct 16-byte bank groups
-    constexpr int BK = 32, PITCH = BK;
+    // K-step BK = 64 for 128-tiles (two MFMA k-steps per LDS stage, two
+    // register stages in flight), 32 for 64-tiles (four stages).  LDS rows
+    // hold BK 16-bit elements, unpadded; the 16-byte chunk c of row R sits at
+    // c ^ swz(R) -- (R >> 1) & 3 for 4 chunks, R & 7 for 8 -- so that the
+    // stores (8 lanes = 8 consecutive chunks) and the fragment reads (8
+    // lanes = 8 rows, one chunk) each touch 8 distinct 16-byte bank groups
+    constexpr int BK = BT == 128 ? 64 : 32, PITCH = BK, CPR = BK / 8;
     // NW waves as 2 (rows) x NW/2 (columns); wave tile WTM x WTN of TM x TN MFMA tiles
     constexpr int WGN = NW / 2, WTM = BT / 2, WTN = BT / WGN, TM = WTM / 16, TN = WTN / 16;
     constexpr int NOP = LO ? 2 : 1;
     constexpr int LD16 = BT * BK / 8 / (64 * NW);  // 16-byte loads per thread per operand array
-    constexpr int PF = 4;                    // K-steps of loads in flight
+    constexpr int PF = BK == 64 ? 2 : 4;     // K-steps of loads in flight
     static_assert(LD16 >= 1, "tile");
     __shared__ __attribute__((aligned(16))) uint16_t As[2][NOP][BT * PITCH];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[2][NOP][BT * PITCH];
@@ -189,12 +191,13 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
                 cor[a][b] = f4{0.f, 0.f, 0.f, 0.f};
             }
 
+        auto swz = [](int row) { return CPR == 8 ? (row & 7) : ((row >> 1) & 3); };
         typedef u4v Stage[2][NOP][LD16];  // [A/B][hi/lo][load]
         auto gload = [&](Stage &x, int k0) {
 #pragma unroll
             for (int q = 0; q < LD16; q++) {
                 const int idx = t + 64 * NW * q;
-                const int row = idx >> 2, part = idx & 3;  // 4 x 16 B per 32-element row
+                const int row = idx / CPR, part = idx % CPR;  // CPR x 16 B per row
                 const size_t ea = (size_t)(i0 + row) * d + k0 + 8 * part;  // row-major A
                 const size_t eb = (size_t)(j0 + row) * d + k0 + 8 * part;  // column-major B
                 x[0][0][q] = *(const u4v *)(Hr + ea);
@@ -209,23 +212,25 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
 #pragma unroll
             for (int q = 0; q < LD16; q++) {
                 const int idx = t + 64 * NW * q;
-                const int row = idx >> 2, part = idx & 3;
+                const int row = idx / CPR, part = idx % CPR;
 #pragma unroll
                 for (int o = 0; o < NOP; o++) {
-                    const int sw = row * PITCH + 8 * (part ^ ((row >> 1) & 3));
+                    const int sw = row * PITCH + 8 * (part ^ swz(row));
                     *(u4v *)&As[buf][o][sw] = x[0][o][q];
                     *(u4v *)&Bs[buf][o][sw] = x[1][o][q];
                 }
             }
         };
         auto compute = [&](int buf) {
+#pragma unroll
+          for (int ks = 0; ks < BK / 32; ks++) {
             s8 fa[NOP][TM], fb[NOP][TN];
 #pragma unroll
             for (int o = 0; o < NOP; o++)
 #pragma unroll
                 for (int a = 0; a < (TM > TN ? TM : TN); a++) {
-                    // wm0, wn0 and 16a are multiples of 16: the row's swizzle is (r >> 1) & 3
-                    const int cg = 8 * (g ^ ((r >> 1) & 3));
+                    // wm0, wn0 and 16a are multiples of 16: the row's swizzle is swz(r)
+                    const int cg = 8 * ((4 * ks + g) ^ swz(r));
                     if (a < TM) fa[o][a] = *(const s8 *)&As[buf][o][(wm0 + 16 * a + r) * PITCH + cg];
                     if (a < TN) fb[o][a] = *(const s8 *)&Bs[buf][o][(wn0 + 16 * a + r) * PITCH + cg];
                 }
@@ -243,17 +248,20 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
 #pragma unroll
                     for (int a = 0; a < TM; a++) cor[a][b] = N::mfma(fa[1][a], fb[0][b], cor[a][b]);
             }
+          }
         };
 
-        const int nk = d / BK;  // a multiple of PF
+        const int nk = d / BK;  // a multiple of PF (d % 128 == 0)
         Stage s0, s1, s2, s3;
         gload(s0, 0);
         gload(s1, BK);
-        gload(s2, 2 * BK);
-        gload(s3, 3 * BK);
+        if (PF == 4) {
+            gload(s2, 2 * BK);
+            gload(s3, 3 * BK);
+        }
         // stage kt: its registers go to LDS buffer kt&1 (last read by
         // compute(kt-2), finished by every thread before the barrier of
-        // step kt-1), the registers are refilled with step kt+4 (clamped to
+        // step kt-1), the registers are refilled with step kt+PF (clamped to
         // the last step, so every load is unconditional), then compute
 #define OPHIP_WSQ_STAGE(S, KT)                                 \
     {                                                          \
@@ -265,8 +273,10 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
         for (int kt = 0; kt < nk; kt += PF) {
             OPHIP_WSQ_STAGE(s0, kt)
             OPHIP_WSQ_STAGE(s1, kt + 1)
-            OPHIP_WSQ_STAGE(s2, kt + 2)
-            OPHIP_WSQ_STAGE(s3, kt + 3)
+            if (PF == 4) {
+                OPHIP_WSQ_STAGE(s2, kt + 2)
+                OPHIP_WSQ_STAGE(s3, kt + 3)
+            }
         }
 #undef OPHIP_WSQ_STAGE
         __syncthreads();  // all LDS reads done before the next item's stores
