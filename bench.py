@@ -411,15 +411,28 @@ def bench_mc(H, U, d, cfg):
     ysk = H.DeviceArray.empty(nscen * nk, dtype=np.int16)
     y0 = 64 + 256 * 1
 
-    def run():
+    gd = H.DeviceArray.empty(d * 257, dtype=np.int16)
+    H.check(H.lib.ophip_mc_guide(d, 1, ct.h, gd.h))
+    guide_us = timed_events(H, lambda: H.lib.ophip_mc_guide(d, 1, ct.h, gd.h), 5) * 1e3
+    log(f"[bench] guide table {guide_us:.1f} us")
+
+    def run_bisect():
         H.lib.ophip_mc1f_rows(mt.h, y0, ysk.h, nscen, nk, d, ct.h, mk.h, 0)
+    run_bisect()
+    H.sync()
+    ms_bisect = timed_events(H, run_bisect, 3)
+
+    def run():
+        H.lib.ophip_mc1f_guided(mt.h, y0, ysk.h, nscen, nk, d, ct.h, gd.h, mk.h)
     run()
     H.sync()
     ms = timed_events(H, run, 3)
     trans = float(nscen) * nk
     log(f"[bench] MC {ms:.2f} ms per 2^20 x 252, {trans / ms / 1e6:.2f} G transitions/s")
     ysk.free()
-    return {"kernel": "k_mc1f<rows> (MT19937 per wavefront + bisection)", "paths": nscen,
+    log(f"[bench] MC full bisection {ms_bisect:.2f} ms")
+    return {"kernel": "k_mc1f_guided (MT19937 per wavefront + guide table + bisection)",
+            "paths": nscen, "ms_full_bisection": ms_bisect, "guide_table_us": guide_us,
             "cumulative_table_us": sck_ms * 1e3, "cumulative_table_GBps": sck_gbs,
             "steps": nk, "ms": ms, "transitions_per_s": trans / (ms * 1e-3),
             "streams": nrng, "d": d}

@@ -261,6 +261,13 @@ int  ophip_surr_base(int B, unsigned prm, float spot, float dT, int nmat, const 
 int ophip_mc_set_nrng(int nstreams);
 int ophip_mc1f_rows(unsigned mt, int y0, unsigned y_sk, int nscen, int nk, int d,
                     unsigned crow, unsigned m_k, unsigned long yhost);
+/* guide tables over row-contiguous cumulative tables (Chen & Asau
+ * indexed search, 256 buckets): guide = nm * d * 257 int16, guide[b] =
+ * min{j : b/256 < C[j]}; the guided sampler returns the same states as
+ * ophip_mc1f_rows, bit for bit, with fewer dependent table probes */
+int ophip_mc_guide(int d, int nm, unsigned crow, unsigned guide);
+int ophip_mc1f_guided(unsigned mt, int y0, unsigned y_sk, int nscen, int nk, int d, unsigned crow,
+                      unsigned guide, unsigned m_k);
 
 /* graphs, events, workspace */
 int           ophip_reserve_workspace(unsigned long bytes);

@@ -145,6 +145,66 @@ __global__ __launch_bounds__(64) void k_mc1f(uint32_t *__restrict__ mt, int G, i
     w.store(st, lane);
 }
 
+// Guide tables (indexed search, Chen & Asau 1974) over row-contiguous
+// cumulative tables: guide[m][row][b] = min{j : b/NB < C[j]} (clamped to
+// d-1), b = 0..NB.  For a uniform u in [b/NB, (b+1)/NB) the sampled state
+// min{j : u < C[j]} lies in [guide[b], guide[b+1]]: every j below guide[b]
+// has C[j] <= b/NB <= u, and C[guide[b+1]] > (b+1)/NB > u.  The search on
+// that range uses the same comparison, so the states are the same bits as
+// the full bisection.  u = m 2^-24 and NB = 2^8, so b = floor(u NB) is exact.
+constexpr int kGuideNB = 256;
+
+__global__ void k_mc_guide(int d, size_t nrows, const float *__restrict__ crow,
+                           int16_t *__restrict__ guide) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // (row, b)
+    if (i >= nrows * (kGuideNB + 1)) return;
+    const size_t rowg = i / (kGuideNB + 1);
+    const int b = (int)(i - rowg * (kGuideNB + 1));
+    const float t = (float)b * (1.0f / kGuideNB);
+    const float *row = crow + rowg * d;
+    int lo = -1, hi = d - 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (t < row[mid]) hi = mid;
+        else lo = mid;
+    }
+    guide[i] = (int16_t)hi;
+}
+
+__global__ __launch_bounds__(64) void k_mc1f_guided(uint32_t *__restrict__ mt, int G, int nk, int d,
+                                                    int y0, const float *__restrict__ crow,
+                                                    const int16_t *__restrict__ guide,
+                                                    const int *__restrict__ m_k,
+                                                    int16_t *__restrict__ y_sk, size_t nscen) {
+    __shared__ uint32_t b0[kMTN], b1[kMTN];
+    const int lane = threadIdx.x;
+    const int s = blockIdx.x;
+    WaveMT w{b0, b1, 0, 0};
+    uint32_t *st = mt + (size_t)s * kMTW;
+    w.load(st, lane);
+    const size_t d2 = (size_t)d * d, gstride = (size_t)d * (kGuideNB + 1);
+    for (int g = 0; g < G; g++) {
+        int y = y0;
+        const size_t scen = ((size_t)s * G + g) * 64 + lane;
+        for (int k = 0; k < nk; k++) {
+            const float u = u01(w.next64(lane));
+            const int m = m_k[k];
+            const float *row = crow + (size_t)m * d2 + (size_t)y * d;
+            const int16_t *gd = guide + (size_t)m * gstride + (size_t)y * (kGuideNB + 1);
+            const int b = (int)(u * (float)kGuideNB);
+            int lo = (int)gd[b] - 1, hi = gd[b + 1];
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (u < row[mid]) hi = mid;
+                else lo = mid;
+            }
+            y = hi;
+            y_sk[(size_t)k * nscen + scen] = (int16_t)y;
+        }
+    }
+    w.store(st, lane);
+}
+
 __global__ __launch_bounds__(64) void k_mt_bench(uint32_t *__restrict__ mt, int per_stream,
                                                  float *__restrict__ unif) {
     __shared__ uint32_t b0[kMTN], b1[kMTN];
@@ -272,5 +332,41 @@ OPHIP_EXPORT int opcuda_mt_benchmark(unsigned mtptr, unsigned unif, int nscen) {
         !handle_range_ok(mtptr, (size_t)nrng * kMTW * 4))
         return (int)set_status(kErrBadHandle);
     hipLaunchKernelGGL(k_mt_bench, dim3(nrng), dim3(64), 0, D.stream, mt, nscen / nrng, u);
+    return (int)check_launch();
+}
+
+// guide tables for nm row-contiguous cumulative tables (ophip_sck_rows):
+// guide = nm * d * 257 int16 (d <= 32767)
+OPHIP_EXPORT int ophip_mc_guide(int d, int nm, unsigned crow, unsigned guide) {
+    if (d <= 0 || d > 32767 || nm <= 0) return (int)set_status(kErrBadArg);
+    const size_t nrows = (size_t)nm * d;
+    if (!handle_range_ok(crow, 4 * nrows * d) || !handle_range_ok(guide, 2 * nrows * (kGuideNB + 1)))
+        return (int)set_status(kErrBadHandle);
+    const size_t n = nrows * (kGuideNB + 1);
+    hipLaunchKernelGGL(k_mc_guide, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, dev().stream, d, nrows,
+                       P<const float>(crow), P<int16_t>(guide));
+    return (int)check_launch();
+}
+
+// ophip_mc1f_rows with guide tables: the same states, bit for bit, in
+// fewer dependent probes
+OPHIP_EXPORT int ophip_mc1f_guided(unsigned mtptr, int y0, unsigned y_skp, int nscen, int nk, int d,
+                                   unsigned crow, unsigned guide, unsigned m_kp) {
+    Device &D = dev();
+    const int nrng = g_nrng;
+    if (nscen <= 0 || nk <= 0 || d <= 0 || d > 32767 || y0 < 0 || y0 >= d || nscen % (64 * nrng) != 0)
+        return (int)set_status(kErrBadArg);
+    uint32_t *mt = P<uint32_t>(mtptr);
+    int16_t *ysk = P<int16_t>(y_skp);
+    const float *tab = P<const float>(crow);
+    const int16_t *gd = P<const int16_t>(guide);
+    const int *mk = P<const int>(m_kp);
+    if (!mt || !ysk || !tab || !gd || !mk) return (int)set_status(kErrBadHandle);
+    if (!handle_range_ok(mtptr, (size_t)nrng * kMTW * 4) ||
+        !handle_range_ok(y_skp, (size_t)nscen * nk * 2) || !handle_range_ok(m_kp, (size_t)nk * 4))
+        return (int)set_status(kErrBadHandle);
+    const int G = nscen / (64 * nrng);
+    hipLaunchKernelGGL(k_mc1f_guided, dim3(nrng), dim3(64), 0, D.stream, mt, G, nk, d, y0, tab, gd, mk,
+                       ysk, (size_t)nscen);
     return (int)check_launch();
 }

@@ -423,6 +423,22 @@ def test_mc_chain_bit_exact(H, d, nrng, groups, nk):
             H.check(fn(mt.h, y0, ysk.h, nscen, nk, d, (cr if rows else ct).h, mkd.h,
                        host.ctypes.data))
             assert np.array_equal(host.reshape(nk, nscen), want), rows
+        # guide tables: the same states from the indexed search
+        gd = H.DeviceArray.empty(nm * d * 257, dtype=np.int16)
+        H.check(H.lib.ophip_mc_guide(d, nm, cr.h, gd.h))
+        crt = cr.download().reshape(nm, d, d)
+        g = gd.download().reshape(nm, d, 257)
+        for b in (0, 1, 128, 255, 256):  # guide[b] = min{j : b/256 < C[j]}, clamped
+            t_ = np.float32(b / 256.0)
+            exp = np.where((crt > t_).any(axis=2), (crt > t_).argmax(axis=2), d - 1)
+            assert np.array_equal(g[:, :, b], exp), b
+        mt = H.DeviceArray.empty(625 * nrng, dtype=np.uint32)
+        seeds = (777 + np.arange(nrng)).astype(np.uint32)
+        H.lib.opcuda_mc_setseed(seeds.ctypes.data, mt.h)
+        H.check()
+        ysk = H.DeviceArray.empty(nscen * nk, dtype=np.int16)
+        H.check(H.lib.ophip_mc1f_guided(mt.h, y0, ysk.h, nscen, nk, d, cr.h, gd.h, mkd.h))
+        assert np.array_equal(ysk.download().reshape(nk, nscen), want)
     finally:
         H.lib.ophip_mc_set_nrng(4096)
 
