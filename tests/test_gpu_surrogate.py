@@ -338,3 +338,26 @@ def test_surrogate_gain_and_fallback(S, hip):
     assert np.array_equal(out[served], pr[served].astype(np.float64))
     net.close()
     sol.close()
+
+
+def test_batch_update_matches_fresh_batch(S, hip):
+    """re-parameterising a captured batch (opm_batch_update) gives the bits a
+    batch created for those parameters gives; mismatched grids are refused"""
+    from oplib_amd import opmodel as M
+    banner("batch update = fresh batch")
+    prm = S.draw_params(8, 555)
+    sol = S.PriceSolver(nb=8)
+    got = sol.solve(prm)
+    fresh = M.Batch([S.config(p) for p in prm])
+    fresh.solve()
+    ref = fresh.prices().reshape(8, -1)
+    fresh.close()
+    assert np.array_equal(got, ref)
+    import ctypes as C
+    bad = (M.Config * 8)(*[S.config(p) for p in prm])
+    bad[3].nsq = S.NSQ + 1
+    assert M.lib.opm_batch_update(sol.batch.p, bad) != 0
+    bad[3].nsq = S.NSQ
+    bad[5].nx = 512
+    assert M.lib.opm_batch_update(sol.batch.p, bad) != 0
+    sol.close()
