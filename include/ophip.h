@@ -160,8 +160,14 @@ int ophip_saxpby(int n, float a, unsigned A, float b, unsigned B, unsigned C);
 unsigned long ophip_wsq_bytes(int d, int mode);
 unsigned      ophip_wsq_w(unsigned state, int d, int mode, int buf);
 /* any array of a state: which = 0 W, 1 Hr, 2 Hc, 3 Lr, 4 Lc, 5 partial
- * row sums (0 when the mode has no such array) */
+ * row sums, 6 cell flags (d/64 x d/64 bytes; 0 = every 16-bit copy value of
+ * that 64 x 64 cell is +-0) (0 when the mode has no such array) */
 unsigned      ophip_wsq_array(unsigned state, int d, int mode, int buf, int which);
+/* Zero-cell skipping: the squaring visits only the K cells whose A and B
+ * cells are flagged nonzero (the skipped products are exact zeros: same
+ * bits).  1 = on (default), 0 = every K-step; returns the previous value.
+ * No reference counterpart (the reference's ssqmm.cu:119-170 is dense). */
+int           ophip_wsq_set_skip(int on);
 unsigned long ophip_wsq_plan(int n, const unsigned *states, const int *ds, const int *nsqs,
                              int mode, int tile, int grid);
 int           ophip_wsq_exec(unsigned long plan, int nsq);

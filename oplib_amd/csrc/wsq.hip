@@ -61,11 +61,34 @@ struct WScen {
     uint64_t Hr, Hc, Lr, Lc;       // 16-bit copies of W (row-/column-major)
     uint64_t Hrn, Hcn, Lrn, Lcn;   // copies of W' (outputs)
     uint64_t part;                 // partial row sums of W': [d/64 column blocks][d]
+    uint64_t fl, fln;              // cell flags of the copies of W, W' (below)
 };
+
+// Cell flags.  The 16-bit copies are cut into 64 x 64 cells; flag byte
+// [cr][cc] (d/64 x d/64, row-major) is 0 only if every H (and L) value of
+// cell (rows 64 cr.., columns 64 cc..) is +-0.  FP16 H and L flush entries
+// |W| < 2^-51 to zero, and a banded generator keeps most cells there for
+// many squarings, so the product's K loop visits only the cells k with
+// A(rows, k) and B(k, columns) both flagged: the skipped MFMAs would add
+// exact zeros, the result is the same bits.  Nonzero flags are only a "may
+// be nonzero" (the diagonal fix sets its cells' flags unconditionally).
+// Flush rule: an element whose copies are both +-0 (|W| < 2^-51 for FP16
+// x3; H alone for x1) is stored as FP32 0 as well, so a zero cell is zero
+// in every array of its buffer.  A squaring item whose output cells are
+// then certainly zero (no K cells, zero input cells) and whose destination
+// cells already hold zeros (flags of the buffer's previous contents) writes
+// nothing at all, and the diagonal fix skips the partial sums of zero cells.
 
 // work item = (scenario, tile)
 struct WItem {
     int s, tm, tn, pad;
+};
+
+// a live work item of a sparse round: the item and its K cells (bit c =
+// cell column c of A and cell row c of B both flagged)
+struct WLive {
+    int s, tm, tn, pad;
+    uint64_t cells, pad2;
 };
 
 template <class T> __device__ inline T *at(char *base, uint64_t off) {
@@ -118,21 +141,39 @@ __global__ __launch_bounds__(64) void k_wsplit(char *__restrict__ base,
     const int i0 = it.tm * 64, j0 = it.tn * 64;
     const int i = i0 + lane;
     double rs = 0.0;
+    unsigned nz = 0;
+    float *Wm = at<float>(base, sc.W);  // W0 itself (flushed in place)
+    // all 64 loads issued before any store (the stores may alias W)
+    float wrow[64];
+#pragma unroll
+    for (int c = 0; c < 64; c++) wrow[c] = W[(size_t)i + (size_t)d * (j0 + c)];
+#pragma unroll
     for (int c = 0; c < 64; c++) {
         const size_t e = (size_t)i + (size_t)d * (j0 + c);
-        float w = W[e];
-        rs += (double)w;
+        float w = wrow[c];
         float ws = w * N::S;
         uint16_t h = N::enc(ws);
+        uint16_t l = 0;
+        if (LO) l = N::enc((ws - N::dec(h)) * N::G);
+        const unsigned nzv = (h | l) & 0x7fffu;
+        if (!nzv) {  // flush rule; zeros stored as +0 everywhere
+            h = l = 0;
+            if (__builtin_bit_cast(uint32_t, w) != 0u) Wm[e] = 0.0f;
+            w = 0.0f;
+        }
+        rs += (double)w;
         Hcn[e] = h;
         th[lane][c] = h;
+        nz |= nzv;
         if (LO) {
-            uint16_t l = N::enc((ws - N::dec(h)) * N::G);
             Lcn[e] = l;
             tl[lane][c] = l;
         }
     }
     at<double>(base, sc.part)[(size_t)it.tn * d + i] = rs;  // slot = 64-column block
+    // the tile is one cell
+    const bool any = __ballot(nz != 0) != 0;
+    if (lane == 0) at<uint8_t>(base, sc.fln)[(size_t)it.tm * (d / 64) + it.tn] = any ? 1 : 0;
     __syncthreads();
     // row-major copies: row r of the tile, lanes over columns
     for (int r = 0; r < 64; r++) {
@@ -151,7 +192,13 @@ __global__ __launch_bounds__(64) void k_wsplit(char *__restrict__ base,
 template <bool BF, bool LO, int BT, int NW>
 __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
                                              const WScen *__restrict__ scen,
-                                             const WItem *__restrict__ items, int nitems) {
+                                             const WItem *__restrict__ items, int nitems,
+                                             const WLive *__restrict__ live,
+                                             const int *__restrict__ counts, int cap) {
+    // Work: the static list items[0..nitems) (every K-step of every item),
+    // or, when live is given, the live items of the round (k_wlive): list x
+    // = live + x cap (counts[x] items) is taken by the workgroups on XCD x
+    // (b % 8 == x; the grid is a multiple of 8), with each item's K cells.
     using N = Num<BF>;
     // K-step BK = 64 for 128-tiles (two MFMA k-steps per LDS stage, two
     // register stages in flight), 32 for 64-tiles (four stages).  LDS rows
@@ -165,22 +212,50 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
     constexpr int NOP = LO ? 2 : 1;
     constexpr int LD16 = BT * BK / 8 / (64 * NW);  // 16-byte loads per thread per operand array
     constexpr int PF = BK == 64 ? 2 : 4;     // K-steps of loads in flight
+    constexpr int SPC = 64 / BK, TPC = BT / 64;  // K-steps per cell, cells per tile side
     static_assert(LD16 >= 1, "tile");
     __shared__ __attribute__((aligned(16))) uint16_t As[2][NOP][BT * PITCH];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[2][NOP][BT * PITCH];
     __shared__ double rsum[WGN][BT];
+    __shared__ int wnz[NW];
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int r = lane & 15, g = lane >> 4;
     const int wm0 = (wave & 1) * WTM, wn0 = (wave >> 1) * WTN;
 
-    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
-        const WItem it = items[item];
+    const WLive *list = live ? live + (size_t)(blockIdx.x & 7) * cap : nullptr;
+    const int nlist = live ? counts[blockIdx.x & 7] : nitems;
+    const int first = live ? (int)(blockIdx.x >> 3) : (int)blockIdx.x;
+    const int stride = live ? (int)(gridDim.x >> 3) : (int)gridDim.x;
+    for (int item = first; item < nlist; item += stride) {
+        WItem it;
+        uint64_t cells = 0;
+        if (live) {
+            const WLive lv = list[item];
+            it = WItem{lv.s, lv.tm, lv.tn, 0};
+            cells = lv.cells;
+        } else {
+            it = items[item];
+        }
         const WScen sc = scen[it.s];
         const int d = sc.d;
         const int i0 = it.tm * BT, j0 = it.tn * BT;
         const uint16_t *Hr = at<const uint16_t>(base, sc.Hr), *Hc = at<const uint16_t>(base, sc.Hc);
         const uint16_t *Lr = at<const uint16_t>(base, sc.Lr), *Lc = at<const uint16_t>(base, sc.Lc);
+
+        // the epilogue's W values, loaded before the K loop's first loads:
+        // their latency hides behind the loop, and the epilogue issues
+        // stores only (a load there would wait for every earlier store)
+        f4 wv[TM][TN];
+        {
+            const float *W = at<const float>(base, sc.W);
+#pragma unroll
+            for (int a = 0; a < TM; a++)
+#pragma unroll
+                for (int b = 0; b < TN; b++)
+                    wv[a][b] = *(const f4 *)(W + (size_t)(i0 + wm0 + 16 * a + 4 * g) +
+                                             (size_t)d * (j0 + wn0 + 16 * b + r));
+        }
 
         f4 acc[TM][TN], cor[TM][TN];
 #pragma unroll
@@ -251,43 +326,72 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
           }
         };
 
-        const int nk = d / BK;  // a multiple of PF (d % 128 == 0)
-        Stage s0, s1, s2, s3;
-        gload(s0, 0);
-        gload(s1, BK);
-        if (PF == 4) {
-            gload(s2, 2 * BK);
-            gload(s3, 3 * BK);
-        }
-        // stage kt: its registers go to LDS buffer kt&1 (last read by
-        // compute(kt-2), finished by every thread before the barrier of
-        // step kt-1), the registers are refilled with step kt+PF (clamped to
-        // the last step, so every load is unconditional), then compute
+        // K-steps to visit: every K-step (static list), or every K-step of
+        // the item's K cells (wave-uniform: scalar registers)
+        const int nc = d / 64;
+        const bool dense = live == nullptr;
+        const int ns = dense ? d / BK : __popcll(cells) * SPC;
+        // load cursor: the K offset of the next step to load; past the last
+        // step it repeats the last one (loads stay unconditional)
+        uint64_t lm = cells;
+        int lq = 0, lh = 0, klast = 0;
+        auto next_k = [&]() {
+            int k0;
+            if (dense) {
+                k0 = min(lq, ns - 1) * BK;
+                lq++;
+            } else if (lm) {
+                k0 = 64 * __builtin_ctzll(lm) + BK * lh;
+                if (SPC == 1 || lh) {
+                    lm &= lm - 1;
+                    lh = 0;
+                } else {
+                    lh = 1;
+                }
+                klast = k0;
+            } else {
+                k0 = klast;
+            }
+            return k0;
+        };
+        if (ns > 0) {
+            Stage s0, s1, s2, s3;
+            gload(s0, next_k());
+            gload(s1, next_k());
+            if (PF == 4) {
+                gload(s2, next_k());
+                gload(s3, next_k());
+            }
+            // step kt: its registers go to LDS buffer kt&1 (last read by
+            // compute(kt-2), finished by every thread before the barrier of
+            // step kt-1), the registers are refilled with step kt+PF, then
+            // compute.  ns is workgroup-uniform, so are the tail conditions.
 #define OPHIP_WSQ_STAGE(S, KT)                                 \
-    {                                                          \
+    if ((KT) < ns) {                                           \
         sstore(S, (KT) & 1);                                   \
         __syncthreads();                                       \
-        gload(S, min((KT) + PF, nk - 1) * BK);                 \
+        gload(S, next_k());                                    \
         compute((KT) & 1);                                     \
     }
-        for (int kt = 0; kt < nk; kt += PF) {
-            OPHIP_WSQ_STAGE(s0, kt)
-            OPHIP_WSQ_STAGE(s1, kt + 1)
-            if (PF == 4) {
-                OPHIP_WSQ_STAGE(s2, kt + 2)
-                OPHIP_WSQ_STAGE(s3, kt + 3)
+            for (int kt = 0; kt < ns; kt += PF) {
+                OPHIP_WSQ_STAGE(s0, kt)
+                OPHIP_WSQ_STAGE(s1, kt + 1)
+                if (PF == 4) {
+                    OPHIP_WSQ_STAGE(s2, kt + 2)
+                    OPHIP_WSQ_STAGE(s3, kt + 3)
+                }
             }
-        }
 #undef OPHIP_WSQ_STAGE
+        }
         __syncthreads();  // all LDS reads done before the next item's stores
 
         // epilogue: lane holds rows 4g+v of column r of every 16x16 tile
-        const float *W = at<const float>(base, sc.W);
         float *Wn = at<float>(base, sc.Wn);
         uint16_t *Hrn = at<uint16_t>(base, sc.Hrn), *Hcn = at<uint16_t>(base, sc.Hcn);
         uint16_t *Lrn = at<uint16_t>(base, sc.Lrn), *Lcn = at<uint16_t>(base, sc.Lcn);
         const float inv = N::IS * N::IS;
         double rows[TM][4];
+        unsigned nzl = 0;  // any nonzero copy among this lane's elements
 #pragma unroll
         for (int a = 0; a < TM; a++)
 #pragma unroll
@@ -299,25 +403,33 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
             for (int b = 0; b < TN; b++) {
                 const int col = j0 + wn0 + 16 * b + r;
                 const size_t e = (size_t)row + (size_t)d * col;
-                f4 w = *(const f4 *)(W + e);
+                const f4 w = wv[a][b];
                 f4 p = acc[a][b];
                 if (LO) p += cor[a][b] * N::IG;
                 f4 o = 2.0f * w + p * inv;
-                *(f4 *)(Wn + e) = o;
                 uint16_t hc[4], lc[4];
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
-                    rows[a][v] += (double)o[v];
                     // split the stored FP32 value: no fused multiply-add-
                     // convert, whose single rounding differs at FP16 ties
                     float ov = o[v];
                     asm volatile("" : "+v"(ov));
                     float os = ov * N::S;
                     hc[v] = N::enc(os);
+                    lc[v] = 0;
                     if (LO) lc[v] = N::enc((os - N::dec(hc[v])) * N::G);
+                    const unsigned nzv = (hc[v] | lc[v]) & 0x7fffu;
+                    if (!nzv) {  // flush rule; zeros stored as +0 everywhere
+                        o[v] = 0.0f;
+                        hc[v] = 0;
+                        lc[v] = 0;
+                    }
+                    nzl |= nzv;
+                    rows[a][v] += (double)o[v];
                     Hrn[(size_t)(row + v) * d + col] = hc[v];
                     if (LO) Lrn[(size_t)(row + v) * d + col] = lc[v];
                 }
+                *(f4 *)(Wn + e) = o;
                 uint2 ph, pl;
                 ph.x = (uint32_t)hc[0] | ((uint32_t)hc[1] << 16);
                 ph.y = (uint32_t)hc[2] | ((uint32_t)hc[3] << 16);
@@ -347,7 +459,20 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
 #pragma unroll
                 for (int v = 0; v < 4; v++) rsum[wave >> 1][wm0 + 16 * a + 4 * g + v] = rows[a][v];
         }
+        {
+            const bool wz = __ballot(nzl != 0) != 0;
+            if (lane == 0) wnz[wave] = wz ? 1 : 0;
+        }
         __syncthreads();
+        // flags of the tile's cells: a wave's WTM x WTN block lies in one cell
+        if (t < TPC * TPC) {
+            const int ca = t / TPC, cb = t % TPC;
+            int f = 0;
+#pragma unroll
+            for (int w = 0; w < NW; w++)
+                if ((w & 1) * WTM / 64 == ca && (w >> 1) * WTN / 64 == cb) f |= wnz[w];
+            at<uint8_t>(base, sc.fln)[(size_t)(it.tm * TPC + ca) * nc + it.tn * TPC + cb] = (uint8_t)f;
+        }
         // one partial slot per 64-column block, the sum of its CPS wave
         // columns in order (CPS = 64 / WTN)
         double *part = at<double>(base, sc.part);
@@ -365,13 +490,57 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
     }
 }
 
+// Live items of a sparse squaring round, one wavefront per work item of
+// the round's static list (lane c: cell column c of A / cell row c of B).
+// An item is live if it has K cells, if its own input cells are nonzero
+// (W' = 2W + ...) or, when wskip is 0 (a lattice's first round: the
+// destination's flags are not yet those of its contents) or its
+// destination cells are nonzero (they must be overwritten with zeros).
+// Live items are appended to the list of their XCD (static position j % 8:
+// the xcd_order placement); the list order within an XCD does not change
+// any result.  Requires d <= 4096 (64 cells).
+template <int TPC>
+__global__ __launch_bounds__(256) void k_wlive(char *__restrict__ base, const WScen *__restrict__ scen,
+                                               const WItem *__restrict__ items, int nitems,
+                                               WLive *__restrict__ live, int cap, int *__restrict__ counts,
+                                               int wskip) {
+    const int lane = threadIdx.x & 63;
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= nitems) return;  // wave-uniform
+    const WItem it = items[j];
+    const WScen sc = scen[it.s];
+    const int nc = sc.d / 64;
+    const uint8_t *fl = at<const uint8_t>(base, sc.fl);
+    int fa = 0, fb = 0;
+    if (lane < nc) {
+#pragma unroll
+        for (int q = 0; q < TPC; q++) {
+            fa |= fl[(size_t)(it.tm * TPC + q) * nc + lane];
+            fb |= fl[(size_t)lane * nc + it.tn * TPC + q];
+        }
+    }
+    const uint64_t cells = __ballot(fa != 0 && fb != 0);
+    bool need = (lane >= it.tn * TPC && lane < it.tn * TPC + TPC && fa != 0) || !wskip;
+    if (lane < TPC * TPC)
+        need = need || at<const uint8_t>(base, sc.fln)[(size_t)(it.tm * TPC + lane / TPC) * nc + it.tn * TPC +
+                                                       lane % TPC] != 0;
+    if (cells == 0 && __ballot(need) == 0) return;
+    if (lane == 0) {
+        const int x = j & 7;
+        const int p = atomicAdd(&counts[x], 1);
+        live[(size_t)x * cap + p] = WLive{it.s, it.tm, it.tn, 0, cells, 0};
+    }
+}
+
 // diagonal correction: W'_ii -= sum_j W'_ij (fixed summation order).
 // One wavefront per 64 rows; the lanes of a row group read the d/64
 // partial slots of their rows in a fixed order.
 template <bool BF, bool LO>
 __global__ void k_wfix(char *__restrict__ base, const WScen *__restrict__ scen, int nscen,
-                       const int *__restrict__ nsq, int round) {
+                       const int *__restrict__ nsq, int round, int *__restrict__ counts) {
     using N = Num<BF>;
+    // the live-list counters of the next round start at zero
+    if (counts && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 16) counts[threadIdx.x] = 0;
     const int s = blockIdx.y;
     if (s >= nscen) return;
     if (nsq && round >= nsq[s]) return;  // this lattice has finished its squarings
@@ -381,17 +550,27 @@ __global__ void k_wfix(char *__restrict__ base, const WScen *__restrict__ scen, 
     if (i >= d) return;
     const double *part = at<const double>(base, sc.part);
     const int nslots = d / 64;  // a multiple of 2 (d % 128 == 0)
+    // slots of zero cells (flag 0) hold no partial sum of this round (the
+    // squaring may have skipped them: stale values, selected away) and add 0
+    const uint8_t *fl = at<const uint8_t>(base, sc.fln) + (size_t)(i >> 6) * nslots;
     double rs = 0.0;
     // loads issued eight at a time, added in slot order
     int q = 0;
     for (; q + 8 <= nslots; q += 8) {
         double v[8];
+        uint8_t f[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) v[u] = part[(size_t)(q + u) * d + i];
+        for (int u = 0; u < 8; u++) {
+            v[u] = part[(size_t)(q + u) * d + i];
+            f[u] = fl[q + u];
+        }
 #pragma unroll
-        for (int u = 0; u < 8; u++) rs += v[u];
+        for (int u = 0; u < 8; u++) rs += f[u] ? v[u] : 0.0;
     }
-    for (; q < nslots; q++) rs += part[(size_t)q * d + i];
+    for (; q < nslots; q++) {
+        const double v = part[(size_t)q * d + i];
+        rs += fl[q] ? v : 0.0;
+    }
     const size_t e = (size_t)i + (size_t)d * i;
     float *Wn = at<float>(base, sc.Wn);
     const float w = (float)((double)Wn[e] - rs);
@@ -405,6 +584,8 @@ __global__ void k_wfix(char *__restrict__ base, const WScen *__restrict__ scen, 
         at<uint16_t>(base, sc.Lcn)[e] = l;
         at<uint16_t>(base, sc.Lrn)[e] = l;
     }
+    // the diagonal cells may have become nonzero
+    if ((i & 63) == 0) at<uint8_t>(base, sc.fln)[(size_t)(i >> 6) * (d / 64) + (i >> 6)] = 1;
 }
 
 // ---------------------------------------------------------------------
@@ -578,7 +759,7 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd1(int d, float df, const uint16
 // mode: 0 = FP16 one product, 1 = FP16 three products (hi/lo),
 //       2 = BF16 three products
 struct WsqLayout {
-    size_t W0, W1, H[2][4], part, total;  // byte offsets; H[buf][Hr,Hc,Lr,Lc]
+    size_t W0, W1, H[2][4], part, fl[2], total;  // byte offsets; H[buf][Hr,Hc,Lr,Lc]
 };
 
 static WsqLayout wsq_layout(int d, int mode) {
@@ -595,6 +776,7 @@ static WsqLayout wsq_layout(int d, int mode) {
     for (int b = 0; b < 2; b++)
         for (int a = 0; a < 4; a++) L.H[b][a] = a < narr ? take(2 * dd) : (size_t)-1;
     L.part = take(8 * (size_t)(d / 64) * d);
+    for (int b = 0; b < 2; b++) L.fl[b] = take((size_t)(d / 64) * (d / 64));  // cell flags
     L.total = off;
     return L;
 }
@@ -626,10 +808,16 @@ WScen make_scen(unsigned state, int d, int mode, int src) {
     s.Lrn = mode ? p + L.H[dst][2] : s.Hrn;
     s.Lcn = mode ? p + L.H[dst][3] : s.Hcn;
     s.part = p + L.part;
+    s.fl = p + L.fl[src];
+    s.fln = p + L.fl[dst];
     return s;
 }
 
 constexpr int kMaxScen = 4096;
+
+// zero-cell skipping in k_wsq (on by default; off gives the dense K loop,
+// the same bits -- for tests and timing)
+int g_wsq_skip = 1;
 
 template <class F> unsigned launch_mode(int mode, F f) {
     switch (mode) {
@@ -663,15 +851,24 @@ struct WsqPlan {
     int *nsq_dev = nullptr;
     WItem *it_split = nullptr, *it_all[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     std::vector<WItem *> it_round[2];
+    // sparse rounds (every lattice d <= 4096): per class, 8 XCD lists of
+    // cap live items and their counters (counts[8 c + x])
+    bool sparse_ok = false;
+    int cap[2] = {0, 0};
+    WLive *live[2] = {nullptr, nullptr};
+    int *counts = nullptr;
 };
 
 // XCD-aware order of a squaring round's work items.  Workgroup b of a
 // launch runs on XCD b % 8 and the persistent grid is a multiple of 8, so
 // item j runs on XCD j % 8.  Tiles that share operand panels should share
-// an XCD (its L2): a lattice with at least 32 tiles is cut into a 2 x 4
-// grid of tile blocks, one per XCD; smaller lattices go whole to the
-// least-loaded XCD.  The per-XCD lists are then balanced (moving items off
-// the longest lists), sorted longest item first and interleaved.
+// an XCD (its L2), and the work of a banded W (most of it near the
+// diagonal while zero cells are skipped) should spread evenly: a lattice
+// with at least 8 tile rows is cut into 8 blocks of consecutive tile rows,
+// one per XCD (each holds its share of the band; its A panels stay in its
+// L2); smaller lattices go whole to the least-loaded XCD.  The per-XCD
+// lists are then balanced (moving items off the longest lists), sorted
+// longest item first and interleaved.
 static std::vector<WItem> xcd_order(const std::vector<WItem> &v, const int *ds, int n, int T) {
     constexpr int X = 8;
     if (v.size() < 2 * X) return v;
@@ -682,9 +879,9 @@ static std::vector<WItem> xcd_order(const std::vector<WItem> &v, const int *ds, 
     for (int i = 0; i < n; i++) {
         if (by_lat[i].empty()) continue;
         const int nt = ds[i] / T;
-        if ((int)by_lat[i].size() >= 4 * X && nt >= 4) {
+        if (nt >= X) {
             for (const WItem &w : by_lat[i]) {
-                const int x = (w.tm * 2 / nt) * 4 + (w.tn * 4 / nt);
+                const int x = w.tm * X / nt;
                 per[x].push_back(w);
                 load[x]++;
             }
@@ -779,7 +976,12 @@ static WsqPlan *make_plan(int n, const unsigned *states, const int *ds, const in
     }
     size_t nitems = 0;
     for (auto &l : lists) nitems += l.size();
+    P->sparse_ok = P->dmax <= 4096;
+    for (int c = 0; c < 2; c++) P->cap[c] = (int)((lists[1 + c * (2 + P->rounds)].size() + 7) / 8);
     size_t bytes = sizeof(WScen) * 3 * n + sizeof(int) * (n + 4) + sizeof(WItem) * nitems;
+    bytes = (bytes + 255) & ~(size_t)255;
+    const size_t live_off = bytes;
+    bytes += sizeof(WLive) * 8 * (size_t)(P->cap[0] + P->cap[1]) + 16 * sizeof(int);
     if (hipMalloc(&P->dev, bytes) != hipSuccess) {
         delete P;
         set_status(kErrOutOfArena);
@@ -795,6 +997,9 @@ static WsqPlan *make_plan(int n, const unsigned *states, const int *ds, const in
         where.push_back(cur);
         cur += l.size();
     }
+    P->live[0] = (WLive *)(p + live_off);
+    P->live[1] = P->live[0] + 8 * (size_t)P->cap[0];
+    P->counts = (int *)(P->live[1] + 8 * (size_t)P->cap[1]);
     P->it_split = where[0];
     P->n_split = (int)lists[0].size();
     size_t w = 1;
@@ -814,6 +1019,7 @@ static WsqPlan *make_plan(int n, const unsigned *states, const int *ds, const in
     for (size_t l = 0; l < lists.size() && e == hipSuccess; l++)
         if (!lists[l].empty())
             e = hipMemcpy(where[l], lists[l].data(), sizeof(WItem) * lists[l].size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(P->counts, 0, 16 * sizeof(int));
     if (e != hipSuccess) {
         (void)hipFree(P->dev);
         delete P;
@@ -871,8 +1077,12 @@ OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
         constexpr bool BF = decltype(bf)::value, LO = decltype(lo)::value;
         hipLaunchKernelGGL((k_wsplit<BF, LO>), dim3(P->n_split), dim3(64), 0, D.stream, D.base,
                            P->sc_split, P->it_split);
+        // sparse rounds: live-item compaction before each squaring launch;
+        // every diagonal fix clears the counters for the next round
+        const bool sparse = g_wsq_skip && P->sparse_ok;
+        int *counts = sparse ? P->counts : nullptr;
         hipLaunchKernelGGL((k_wfix<BF, LO>), fixgrid, dim3(64), 0, D.stream, D.base, P->sc_split,
-                           P->n, (const int *)nullptr, 0);
+                           P->n, (const int *)nullptr, 0, counts);
         for (int rr = 0; rr < rounds; rr++) {
             const int src = rr & 1;
             bool any = false;
@@ -881,25 +1091,52 @@ OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
                 const WItem *its = per ? P->it_round[c][rr] : P->it_all[c][src];
                 if (nit == 0) continue;
                 any = true;
-                // a multiple of 8 keeps item j on XCD j % 8 (xcd_order)
+                // a multiple of 8 keeps item j on XCD j % 8 (xcd_order); the
+                // live lists need all 8 XCDs
                 int g = nit < P->grid ? nit : P->grid;
                 if (g > 8) g -= g % 8;
-                // work items carry s = src*n + i (the full table)
+                if (sparse) g = std::max(g, 8);
+                // work items carry s = src*n + i (the full table); the
+                // destination's flags describe its contents from round 1 on
+                const WLive *lv = nullptr;
+                const int *cn = nullptr;
+                if (sparse) {
+                    lv = P->live[c];
+                    cn = P->counts + 8 * c;
+                    if (c == 1)
+                        hipLaunchKernelGGL((k_wlive<2>), dim3(cdiv(nit, 4)), dim3(256), 0, D.stream, D.base,
+                                           P->sc, its, nit, P->live[c], P->cap[c], P->counts + 8 * c,
+                                           rr > 0 ? 1 : 0);
+                    else
+                        hipLaunchKernelGGL((k_wlive<1>), dim3(cdiv(nit, 4)), dim3(256), 0, D.stream, D.base,
+                                           P->sc, its, nit, P->live[c], P->cap[c], P->counts + 8 * c,
+                                           rr > 0 ? 1 : 0);
+                }
                 if (c == 1)
                     hipLaunchKernelGGL((k_wsq<BF, LO, 128, 8>), dim3(g), dim3(512), 0, D.stream, D.base,
-                                       P->sc, its, nit);
+                                       P->sc, its, nit, lv, cn, P->cap[c]);
                 else
                     hipLaunchKernelGGL((k_wsq<BF, LO, 64, 4>), dim3(g), dim3(256), 0, D.stream, D.base,
-                                       P->sc, its, nit);
+                                       P->sc, its, nit, lv, cn, P->cap[c]);
             }
             if (!any) continue;
             // the fix kernel indexes lattices 0..n-1 of the table half for
             // this direction
             const WScen *scn = P->sc + (src ? P->n : 0);
             hipLaunchKernelGGL((k_wfix<BF, LO>), fixgrid, dim3(64), 0, D.stream, D.base, scn, P->n,
-                               per ? (const int *)P->nsq_dev : (const int *)nullptr, rr);
+                               per ? (const int *)P->nsq_dev : (const int *)nullptr, rr, counts);
         }
     });
+}
+
+// zero-cell skipping of the squaring kernel: 1 on (default), 0 off (every
+// K-step visited; the results are the same bits).  Read when kernels are
+// launched: a captured graph keeps the setting it was captured with.
+// Returns the previous setting.
+OPHIP_EXPORT int ophip_wsq_set_skip(int on) {
+    const int old = g_wsq_skip;
+    g_wsq_skip = on ? 1 : 0;
+    return old;
 }
 
 // handle of the FP32 W buffer `buf` (0/1) inside a state
@@ -909,22 +1146,25 @@ OPHIP_EXPORT unsigned ophip_wsq_w(unsigned state, int d, int mode, int buf) {
 }
 
 // handle of one array of a state: which = 0 W (FP32), 1 Hr, 2 Hc, 3 Lr,
-// 4 Lc (16-bit copies, row-/column-major), 5 partial row sums (FP64);
-// 0 for arrays the mode does not have
+// 4 Lc (16-bit copies, row-/column-major), 5 partial row sums (FP64),
+// 6 cell flags of the copies (d/64 x d/64 bytes, row-major; 0 = every
+// value of the 64 x 64 cell is +-0); 0 for arrays the mode does not have
 OPHIP_EXPORT unsigned ophip_wsq_array(unsigned state, int d, int mode, int buf, int which) {
-    if (d <= 0 || d % 64 || mode < 0 || mode > 2 || buf < 0 || buf > 1 || which < 0 || which > 5)
+    if (d <= 0 || d % 64 || mode < 0 || mode > 2 || buf < 0 || buf > 1 || which < 0 || which > 6)
         return 0;
     WsqLayout L = wsq_layout(d, mode);
     size_t off;
     if (which == 0) off = buf ? L.W1 : L.W0;
     else if (which == 5) off = L.part;
+    else if (which == 6) off = L.fl[buf];
     else off = L.H[buf][which - 1];
     if (off == (size_t)-1) return 0;
     return state + (unsigned)off;
 }
 
 // ---------------------------------------------------------------------
-// Autotuner: a tile per size bucket, then the persistent grid.
+// Autotuner: a tile per size bucket, then the persistent grid (timed with
+// the dense K loop).
 // Each bucket (the lattices of one size d) is planned alone with 64- and
 // with 128-tiles and the faster one is kept; the combined plan then tries
 // 1, 2 and 4 workgroups per CU.  Timing runs the plan's split and two
@@ -970,6 +1210,12 @@ OPHIP_EXPORT int ophip_wsq_autotune(int n, const unsigned *states, const int *ds
         return (int)set_status(kErrBadArg);
     (void)nsqs;
     std::lock_guard<std::mutex> lock(g_tune_mu);
+    // time the dense K loop: the states' contents are arbitrary here, and
+    // their cell flags would make the timing depend on them
+    struct SkipOff {
+        int old = ophip_wsq_set_skip(0);
+        ~SkipOff() { ophip_wsq_set_skip(old); }
+    } skip_off;
     const int ncu = dev().n_cu > 0 ? dev().n_cu : 256;
     std::map<int, std::vector<int>> buckets;
     for (int i = 0; i < n; i++) buckets[ds[i]].push_back(i);

@@ -400,3 +400,65 @@ def test_fp8_split_and_one_squaring(hip):
     rel = np.abs(W1 - ex).max() / np.abs(W0 @ W0).max()
     print(f"[err] fp8 product vs exact: {rel:.3e} of max|W*W|")
     assert rel < 2e-2
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("tile", [64, 128])
+def test_zero_cell_skip_same_bits(hip, mode, tile):
+    """zero-cell skipping: the flags are exact (flag 0 <=> the cell's copies
+    are all +-0, checked cell by cell on every buffer), the skip is real on a
+    banded lattice (most cells are zero after the first squarings), and the
+    squarings give the same bits as the dense K loop"""
+    banner(f"wsq zero-cell skip mode={mode} tile={tile}")
+    H = hip
+    lats = [Lattice(H, 512, 1, mode), Lattice(H, 128, 3, mode)]   # d = 512, 384
+
+    def cells_zero(L, buf):
+        d, nc = L.d, L.d // 64
+        z = np.ones((nc, nc), dtype=bool)
+        for which in ((1, 2, 3, 4) if mode else (1, 2)):
+            a = L.arr(buf, which, np.uint16, "C" if which in (1, 3) else "F") & 0x7FFF
+            z &= ~a.reshape(nc, 64, nc, 64).any(axis=(1, 3))
+        return z
+
+    def flags(L, buf):
+        nc = L.d // 64
+        h = H.lib.ophip_wsq_array(L.state.h, L.d, mode, buf, 6)
+        out = np.empty(nc * nc, dtype=np.uint8)
+        H.check(H.lib.opcuda_memcpy_d2h(h, out.ctypes.data, out.nbytes))
+        return out.reshape(nc, nc)
+
+    p = plan(H, lats, tile)
+    old = H.lib.ophip_wsq_set_skip(1)
+    try:
+        for nsq in (1, 2, 7):
+            got = {}
+            for skip in (1, 0):
+                H.lib.ophip_wsq_set_skip(skip)
+                for L in lats:
+                    L.first_step()
+                H.check(H.lib.ophip_wsq_exec(p, nsq))
+                H.sync()
+                got[skip] = [[L.arr(b, w, np.uint32 if w == 0 else np.uint16) for w in
+                              ((0, 1, 2, 3, 4) if mode else (0, 1, 2))] for L in lats
+                             for b in (0, 1)]
+                if skip:
+                    for L in lats:
+                        for b in (0, 1):
+                            z = cells_zero(L, b)
+                            f = flags(L, b)
+                            # flag 0 only on all-zero cells; all-zero cells flagged 0
+                            # except diagonal cells (flagged by the fix)
+                            assert not np.any((f == 0) & ~z)
+                            off = ~np.eye(L.d // 64, dtype=bool)
+                            assert np.array_equal((f == 0)[off], z[off])
+                        zf = (flags(L, nsq & 1) == 0).mean()
+                        print(f"[skip] d={L.d} nsq={nsq}: zero cells {zf:.2f}", flush=True)
+                        if nsq <= 2 and L.cfg.nr == 1:
+                            assert zf > 0.3
+            for a, b in zip(got[1], got[0]):
+                for x, y in zip(a, b):
+                    assert np.array_equal(x, y)
+    finally:
+        H.lib.ophip_wsq_set_skip(old)
+        H.lib.ophip_wsq_plan_free(p)
