@@ -117,7 +117,8 @@ int ophip_dgen_rs(int nx, int nr, unsigned xval, unsigned drift, unsigned vol, d
                   unsigned gen);
 /* the same generator fused with the first step, evaluated in FP64:
  * out_kind 0: FP32 I+dt*L, 1: FP64 I+dt*L, 2: FP32 dt*L, 3: FP64 dt*L
- * (host_d_mc_ek, CModel Host Double.cs:89-121) */
+ * (host_d_mc_ek, CModel Host Double.cs:89-121); out_kind + 4: the same in
+ * node-major state order y = nr*x + r (one band; used by the W form) */
 int ophip_sek_rs(int nx, int nr, unsigned xval, unsigned drift, unsigned vol, double q_off,
                  double dt, int out_kind, unsigned out);
 /* cumulative tables stored row-contiguous: crow[m*d*d + row*d + y]; the
@@ -192,6 +193,11 @@ int           ophip_w8_exec(unsigned state, int d, int nsq);
  * X.  The reference's step is the FP32 SGEMV of U (sgemv.cu:44-51, driven
  * per step by CModel Device.cs). */
 int ophip_wsq_bwd_layout(unsigned state, int d, int mode, int buf);
+/* the same with the states reordered: fragment state i is the copies'
+ * state perm[i] (perm: handle of d ints, a permutation), e.g. a lattice
+ * squared in node-major order stepped in the reference's order (the step's
+ * FP32 accumulation depends on the order; its algebra does not) */
+int ophip_wsq_bwd_layout_perm(unsigned state, int d, int mode, int buf, unsigned perm);
 int ophip_wsq_bwd_prep(int d, int ncp, int mode, unsigned X, unsigned XT);
 int ophip_wsq_bwd_step(unsigned state, int d, int mode, int buf, int ncp, float df, unsigned X,
                        unsigned XT, unsigned Y, unsigned YT);

@@ -77,11 +77,12 @@ int opm_launch(opm_solver *s);
  * backward induction, nsteps for propagation) */
 int opm_prices(opm_solver *s, double *prices);
 /* copy U(DeltaT) (FP32, d*d column-major; I + W formed on the host in the
- * W form) */
+ * W form), states in the reference's order y = x + nx*r */
 int opm_kernel(opm_solver *s, float *U);
-/* the squared kernel as held on the device: U, or W when *is_w = 1 */
+/* the squared kernel: U, or W when *is_w = 1 (same state order) */
 int opm_kernel_raw(opm_solver *s, float *K, int *is_w);
-/* its device handle (valid after a solve) */
+/* its device handle (valid after a solve); on the device the W form keeps
+ * node-major states y = nr*x + r when nr > 1 */
 unsigned opm_kernel_handle(opm_solver *s);
 int opm_dims(opm_solver *s, int *d, int *n_mat);
 

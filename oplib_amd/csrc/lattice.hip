@@ -151,20 +151,25 @@ __device__ inline void rs_rates(const RSLattice &L, int y, double &am, double &a
     ap = i10 * mu + i11 * s2;
 }
 
-// MODE 0: generator (FP64 out); MODE 1: I + dt*L; MODE 2: dt*L (U0 - I)
-template <int MODE, typename OUT>
+// MODE 0: generator (FP64 out); MODE 1: I + dt*L; MODE 2: dt*L (U0 - I).
+// NM: node-major states y = nr*x + r (the same operator in another basis:
+// the 8 regimes of a node are neighbours, so the matrix is one band and
+// the W form's zero-cell skipping applies); drift/vol stay indexed by the
+// canonical state x + nx*r.
+template <int MODE, typename OUT, bool NM>
 __global__ void k_rs_assemble(RSLattice L, double dt, OUT *__restrict__ out) {
-    const int nx = L.nx, d = L.nx * L.nr;
+    const int nx = L.nx, nr = L.nr, d = L.nx * L.nr;
     size_t n = (size_t)d * d;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
          e += (size_t)gridDim.x * blockDim.x) {
         int i = (int)(e % d), j = (int)(e / d);  // row i (from), column j (to)
-        int xi = i % nx, ri = i / nx, xj = j % nx, rj = j / nx;
+        int xi = NM ? i / nr : i % nx, ri = NM ? i % nr : i / nx;
+        int xj = NM ? j / nr : j % nx, rj = NM ? j % nr : j / nx;
         double g = 0.0;
         if (ri == rj) {
             if (xj == xi - 1 || xj == xi + 1 || xj == xi) {
                 double am, ap;
-                rs_rates(L, i, am, ap);
+                rs_rates(L, xi + nx * ri, am, ap);
                 if (xj == xi - 1) g = am;
                 else if (xj == xi + 1) g = ap;
                 else g = -(am + ap + (double)(L.nr - 1) * L.q_off);
@@ -198,13 +203,14 @@ OPHIP_EXPORT int ophip_dgen_rs(int nx, int nr, unsigned xval, unsigned drift, un
     if (!rs_args(nx, nr, xval, drift, vol, L) || !ptr(gen)) return (int)set_status(kErrBadArg);
     L.q_off = q_off;
     size_t d = (size_t)nx * nr;
-    hipLaunchKernelGGL((k_rs_assemble<0, double>), dim3(grid1d(d * d, 256)), dim3(256), 0,
+    hipLaunchKernelGGL((k_rs_assemble<0, double, false>), dim3(grid1d(d * d, 256)), dim3(256), 0,
                        D.stream, L, 0.0, P<double>(gen));
     return (int)check_launch();
 }
 
 // Fused assembly + first step.  out_kind 0: FP32 I+dt*L; 1: FP64 I+dt*L;
-// 2: FP32 dt*L (the U-I form); 3: FP64 dt*L.
+// 2: FP32 dt*L (the U-I form); 3: FP64 dt*L; + 4: node-major states
+// (y = nr*x + r instead of x + nx*r).
 OPHIP_EXPORT int ophip_sek_rs(int nx, int nr, unsigned xval, unsigned drift, unsigned vol,
                               double q_off, double dt, int out_kind, unsigned out) {
     Device &D = dev();
@@ -213,11 +219,16 @@ OPHIP_EXPORT int ophip_sek_rs(int nx, int nr, unsigned xval, unsigned drift, uns
     L.q_off = q_off;
     size_t d = (size_t)nx * nr;
     dim3 g(grid1d(d * d, 256)), b(256);
+    hipStream_t st = D.stream;
     switch (out_kind) {
-    case 0: hipLaunchKernelGGL((k_rs_assemble<1, float>), g, b, 0, D.stream, L, dt, P<float>(out)); break;
-    case 1: hipLaunchKernelGGL((k_rs_assemble<1, double>), g, b, 0, D.stream, L, dt, P<double>(out)); break;
-    case 2: hipLaunchKernelGGL((k_rs_assemble<2, float>), g, b, 0, D.stream, L, dt, P<float>(out)); break;
-    case 3: hipLaunchKernelGGL((k_rs_assemble<2, double>), g, b, 0, D.stream, L, dt, P<double>(out)); break;
+    case 0: hipLaunchKernelGGL((k_rs_assemble<1, float, false>), g, b, 0, st, L, dt, P<float>(out)); break;
+    case 1: hipLaunchKernelGGL((k_rs_assemble<1, double, false>), g, b, 0, st, L, dt, P<double>(out)); break;
+    case 2: hipLaunchKernelGGL((k_rs_assemble<2, float, false>), g, b, 0, st, L, dt, P<float>(out)); break;
+    case 3: hipLaunchKernelGGL((k_rs_assemble<2, double, false>), g, b, 0, st, L, dt, P<double>(out)); break;
+    case 4: hipLaunchKernelGGL((k_rs_assemble<1, float, true>), g, b, 0, st, L, dt, P<float>(out)); break;
+    case 5: hipLaunchKernelGGL((k_rs_assemble<1, double, true>), g, b, 0, st, L, dt, P<double>(out)); break;
+    case 6: hipLaunchKernelGGL((k_rs_assemble<2, float, true>), g, b, 0, st, L, dt, P<float>(out)); break;
+    case 7: hipLaunchKernelGGL((k_rs_assemble<2, double, true>), g, b, 0, st, L, dt, P<double>(out)); break;
     default: return (int)set_status(kErrBadArg);
     }
     return (int)check_launch();

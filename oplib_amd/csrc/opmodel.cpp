@@ -116,6 +116,21 @@ struct opm_solver {
     Buf wstate;
     unsigned long wplan = 0;
     unsigned kernel_h = 0;  // handle of U (plain) or W (W form) after the squarings
+    // state order on the device: the W form keeps node-major states y =
+    // nr*x + r (one band: its zero-cell skipping applies), the plain method
+    // the reference's x + nx*r (S2DGrid.cs:89); prices and opm_kernel are
+    // in the reference's order either way
+    bool nm = false;
+    int state(int x, int r) const { return nm ? x * cfg.nr + r : x + cfg.nx * r; }
+    int node(int y) const { return nm ? y / cfg.nr : y % cfg.nx; }
+    // the FP16 x3 matrix-core backward steps in the reference's order (its
+    // FP32 accumulation is more accurate there): W's fragment copies are
+    // gathered through perm (reference state -> device state), and the
+    // value vectors V are in the reference's order
+    bool v_ref = false;
+    Buf perm;
+    int vstate(int x, int r) const { return v_ref ? x + cfg.nx * r : state(x, r); }
+    int vnode(int y) const { return v_ref ? y % cfg.nx : node(y); }
 };
 
 extern "C" {
@@ -190,6 +205,7 @@ opm_solver *opm_create(const opm_config *cfg) {
         ok = ok && alloc(s->U[0], 4 * dd) && alloc(s->U[1], 4 * dd);
     } else {
         s->wmode = c->precision - 1;  // OPM_PREC_F16.. -> wsq modes 0..2, 3 = FP8
+        s->nm = nr > 1;
         unsigned long wb = s->wmode == 3 ? ophip_w8_bytes(d) : ophip_wsq_bytes(d, s->wmode);
         ok = ok && wb && (d % (s->wmode == 3 ? 64 : 128)) == 0 && alloc(s->wstate, wb);
         if (ok && s->wmode != 3) {
@@ -205,11 +221,18 @@ opm_solver *opm_create(const opm_config *cfg) {
         size_t vb = 4 * (size_t)d * s->ncp;
         ok = alloc(s->V[0], vb) && alloc(s->V[1], vb) && alloc(s->pay0, vb);
         if (ok && s->wmode == 1) ok = alloc(s->VT[0], vb) && alloc(s->VT[1], vb);
+        if (ok && s->wmode == 1 && s->nm) {
+            s->v_ref = true;
+            std::vector<int> pm((size_t)d);
+            for (int r = 0; r < nr; r++)
+                for (int i = 0; i < nx; i++) pm[(size_t)i + (size_t)nx * r] = s->state(i, r);
+            ok = alloc(s->perm, 4 * (size_t)d) && !upload(s->perm, pm.data(), 4 * (size_t)d);
+        }
         if (ok) {
             std::vector<float> p((size_t)d * s->ncp, 0.0f);
             for (int y = 0; y < d; y++)
                 for (int k = 0; k < c->nstrikes; k++) {
-                    double v = x[y % nx] - c->strikes[k];
+                    double v = x[s->vnode(y)] - c->strikes[k];
                     p[(size_t)y * s->ncp + k] = (float)(v > 0.0 ? v : 0.0);
                 }
             ok = !upload(s->pay0, p.data(), vb);
@@ -224,8 +247,8 @@ opm_solver *opm_create(const opm_config *cfg) {
              alloc(s->fout, 4 * (size_t)nr * c->nsteps * c->nstrikes);
         if (ok) {
             std::vector<float> P0(prow * nr, 0.0f), xs((size_t)d);
-            for (int r = 0; r < nr; r++) P0[prow * r + s->x0 + (size_t)nx * r] = 1.0f;
-            for (int y = 0; y < d; y++) xs[y] = (float)x[y % nx];
+            for (int r = 0; r < nr; r++) P0[prow * r + s->state(s->x0, r)] = 1.0f;
+            for (int y = 0; y < d; y++) xs[y] = (float)x[s->node(y)];
             ok = !upload(s->fP, P0.data(), 4 * prow * nr) && !upload(s->fx, xs.data(), 4 * (size_t)d);
         }
         if (ok) {
@@ -273,7 +296,7 @@ void opm_destroy(opm_solver *s) {
     if (s->fplan) ophip_fwd_plan_free(s->fplan);
     Buf *all[] = {&s->x, &s->drift, &s->vol, &s->U[0], &s->U[1], &s->V[0], &s->V[1],
                   &s->VT[0], &s->VT[1], &s->pay0, &s->Uk[0], &s->Uk[1], &s->payT, &s->out,
-                  &s->wstate, &s->fP, &s->fx, &s->fout};
+                  &s->wstate, &s->fP, &s->fx, &s->fout, &s->perm};
     for (Buf *b : all) release(*b);
     delete s;
 }
@@ -301,7 +324,7 @@ static int solve_body(opm_solver *s) {
         // W-form squarings with renormalisation
         const bool f8 = s->wmode == 3;
         const unsigned w0 = f8 ? ophip_w8_w(s->wstate.h, d, 0) : ophip_wsq_w(s->wstate.h, d, s->wmode, 0);
-        st = ophip_sek_rs(nx, nr, s->x.h, s->drift.h, s->vol.h, c->q_off, s->dt0, 2, w0);
+        st = ophip_sek_rs(nx, nr, s->x.h, s->drift.h, s->vol.h, c->q_off, s->dt0, s->nm ? 6 : 2, w0);
         if (st) return st;
         st = f8 ? ophip_w8_exec(s->wstate.h, d, c->nsq) : ophip_wsq_exec(s->wplan, c->nsq);
         if (st) return st;
@@ -318,7 +341,8 @@ static int solve_body(opm_solver *s) {
             // V <- df*(V + W*V) on the FP16 matrix cores from W's hi/lo
             // copies (a BF16 hi/lo pair of V keeps 16 bits, too few over 252
             // steps: the BF16 method steps in FP32)
-            st = ophip_wsq_bwd_layout(s->wstate.h, d, s->wmode, s->u_final);
+            st = s->v_ref ? ophip_wsq_bwd_layout_perm(s->wstate.h, d, s->wmode, s->u_final, s->perm.h)
+                          : ophip_wsq_bwd_layout(s->wstate.h, d, s->wmode, s->u_final);
             if (st) return st;
             st = ophip_wsq_bwd_prep(d, s->ncp, s->wmode, s->V[0].h, s->VT[0].h);
             if (st) return st;
@@ -407,7 +431,7 @@ int opm_prices(opm_solver *s, double *prices) {
     if (!s) return 1;
     opcuda_thread_synchronize();
     const opm_config *c = &s->cfg;
-    const int d = s->d, nx = c->nx, nr = c->nr;
+    const int d = s->d, nr = c->nr;
     if (c->price_kind == OPM_PRICE_BACKWARD) {
         std::vector<float> v((size_t)d * s->ncp);
         int st = opcuda_memcpy_d2h(s->V[s->v_final].h, (unsigned long)(uintptr_t)v.data(),
@@ -415,7 +439,7 @@ int opm_prices(opm_solver *s, double *prices) {
         if (st) return st;
         for (int r = 0; r < nr; r++)
             for (int k = 0; k < c->nstrikes; k++)
-                prices[(size_t)r * c->nstrikes + k] = v[(size_t)(s->x0 + nx * r) * s->ncp + k];
+                prices[(size_t)r * c->nstrikes + k] = v[(size_t)s->vstate(s->x0, r) * s->ncp + k];
     } else if (s->fplan) {
         // forward chain: device order [r][k][strike], the output's
         std::vector<float> o((size_t)c->nsteps * nr * c->nstrikes);
@@ -438,25 +462,40 @@ int opm_prices(opm_solver *s, double *prices) {
     return 0;
 }
 
+// the squared kernel (column-major d x d) in the reference's state order
+static int kernel_canonical(opm_solver *s, float *K) {
+    const int d = s->d;
+    const size_t dd = (size_t)d * d;
+    if (!s->nm)
+        return opcuda_memcpy_d2h(s->kernel_h, (unsigned long)(uintptr_t)K, (unsigned)(4 * dd));
+    std::vector<float> t(dd);
+    int st = opcuda_memcpy_d2h(s->kernel_h, (unsigned long)(uintptr_t)t.data(), (unsigned)(4 * dd));
+    if (st) return st;
+    const int nx = s->cfg.nx;
+    std::vector<int> canon((size_t)d);  // device state -> reference state
+    for (int y = 0; y < d; y++) canon[y] = s->node(y) + nx * (y % s->cfg.nr);
+    for (int j = 0; j < d; j++)
+        for (int i = 0; i < d; i++) K[(size_t)canon[i] + (size_t)d * canon[j]] = t[(size_t)i + (size_t)d * j];
+    return 0;
+}
+
 // U(DeltaT) in FP32; in the W form U = I + W is formed on the host
 int opm_kernel(opm_solver *s, float *U) {
     if (!s || !s->kernel_h) return 1;
     opcuda_thread_synchronize();
     const int d = s->d;
-    int st = opcuda_memcpy_d2h(s->kernel_h, (unsigned long)(uintptr_t)U,
-                               (unsigned)(4 * (size_t)d * d));
+    int st = kernel_canonical(s, U);
     if (st || s->wmode < 0) return st;
     for (int i = 0; i < d; i++) U[(size_t)i * d + i] += 1.0f;
     return 0;
 }
 
-// the raw squared kernel as held on the device (U, or W in the W form)
+// the raw squared kernel (U, or W in the W form), reference state order
 int opm_kernel_raw(opm_solver *s, float *K, int *is_w) {
     if (!s || !s->kernel_h) return 1;
     opcuda_thread_synchronize();
     *is_w = s->wmode >= 0;
-    return opcuda_memcpy_d2h(s->kernel_h, (unsigned long)(uintptr_t)K,
-                             (unsigned)(4 * (size_t)s->d * s->d));
+    return kernel_canonical(s, K);
 }
 
 unsigned opm_kernel_handle(opm_solver *s) { return s ? s->kernel_h : 0; }

@@ -623,6 +623,31 @@ __global__ __launch_bounds__(256) void k_wfrag(int d, const uint16_t *__restrict
     if (LO) *(uint4 *)(Lf + f * 8) = *(const uint4 *)(Lr + src);
 }
 
+// the same in another state order: fragment rows/columns are states i of
+// the output order, read from the copies' state perm[i] (a permutation)
+__global__ __launch_bounds__(256) void k_wfragp(int d, const uint16_t *__restrict__ Hr,
+                                                const uint16_t *__restrict__ Lr,
+                                                const int *__restrict__ perm,
+                                                uint16_t *__restrict__ Hf, uint16_t *__restrict__ Lf) {
+    const int nkt = d / 32;
+    const size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= (size_t)d * d / 8) return;
+    const int l = (int)(f & 63);
+    const size_t tile = f >> 6;
+    const int it = (int)(tile / nkt), kt = (int)(tile % nkt);
+    const int r = l & 15, g = l >> 4;
+    const size_t row = (size_t)perm[16 * it + r] * d;
+    uint16_t h[8], lo[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int k = perm[32 * kt + 8 * g + e];
+        h[e] = Hr[row + k];
+        lo[e] = Lr[row + k];
+    }
+    *(uint4 *)(Hf + f * 8) = *(const uint4 *)h;
+    *(uint4 *)(Lf + f * 8) = *(const uint4 *)lo;
+}
+
 // X (d x 16NT row-major FP32) -> its hi/lo copies, fragment-major with the
 // columns as rows: thread per (column, 8 consecutive k)
 template <bool BF, int NT>
@@ -1331,6 +1356,27 @@ OPHIP_EXPORT int ophip_wsq_bwd_layout(unsigned state, int d, int mode, int buf) 
     const size_t nf = (size_t)d * d / 8;
     hipLaunchKernelGGL((k_wfrag<true>), dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, D.stream, d, Hr,
                        Lr, Hf, Lf);
+    return (int)check_launch();
+}
+
+// ophip_wsq_bwd_layout with the states reordered: fragment state i is the
+// copies' state perm[i] (perm: handle of d ints, a permutation of 0..d-1),
+// so the steps run in the order of perm's domain.  The step arithmetic is
+// the same in every order, its FP32 accumulation is not: a lattice squared
+// in a band order steps in its natural order.
+OPHIP_EXPORT int ophip_wsq_bwd_layout_perm(unsigned state, int d, int mode, int buf, unsigned perm) {
+    if (!bwd_args_ok(d, 16, mode) || buf < 0 || buf > 1) return (int)set_status(kErrBadArg);
+    const WsqLayout L = wsq_layout(d, mode);
+    if (!handle_range_ok(state, L.total) || !handle_range_ok(perm, 4 * (size_t)d))
+        return (int)set_status(kErrBadHandle);
+    Device &D = dev();
+    const uint16_t *Hr = P<const uint16_t>(state + (unsigned)L.H[buf][0]);
+    const uint16_t *Lr = P<const uint16_t>(state + (unsigned)L.H[buf][2]);
+    uint16_t *Hf = P<uint16_t>(state + (unsigned)L.H[1 - buf][1]);
+    uint16_t *Lf = P<uint16_t>(state + (unsigned)L.H[1 - buf][3]);
+    const size_t nf = (size_t)d * d / 8;
+    hipLaunchKernelGGL(k_wfragp, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, D.stream, d, Hr, Lr,
+                       P<const int>(perm), Hf, Lf);
     return (int)check_launch();
 }
 

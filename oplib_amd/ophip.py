@@ -92,6 +92,7 @@ SIGNATURES = {
     "ophip_w8_array": (U32, [U32, I32, I32]),
     "ophip_w8_exec": (I32, [U32, I32, I32]),
     "ophip_wsq_bwd_layout": (I32, [U32, I32, I32, I32]),
+    "ophip_wsq_bwd_layout_perm": (I32, [U32, I32, I32, I32, U32]),
     "ophip_wsq_bwd_prep": (I32, [I32, I32, I32, U32, U32]),
     "ophip_wsq_bwd_step": (I32, [U32, I32, I32, I32, I32, F32, U32, U32, U32, U32]),
     "ophip_wsq_plan_tiles": (ULONG, [I32, C.POINTER(U32), C.POINTER(I32), C.POINTER(I32), I32,

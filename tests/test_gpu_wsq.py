@@ -329,6 +329,40 @@ def test_wform_backward_step(hip, mode):
                                        int(yl[tuple(b)])) for b in bad[:4]])
 
 
+def test_wform_backward_layout_perm(hip):
+    """the permuted fragment layout (ophip_wsq_bwd_layout_perm) is the plain
+    layout of the permuted matrix P W P^T, bit for bit (fragments of the
+    other buffer's column-major slots, unpacked)"""
+    banner("wsq backward layout with a state permutation")
+    H = hip
+    mode, buf = 1, 1
+    L = Lattice(H, 128, 2, mode)  # d = 256
+    p = plan(H, [L], 128)
+    L.first_step()
+    H.check(H.lib.ophip_wsq_exec(p, 3))
+    H.sync()
+    H.lib.ophip_wsq_plan_free(p)
+    d = L.d
+    Hr = L.arr(buf, 1, np.uint16, "C")
+    Lr = L.arr(buf, 3, np.uint16, "C")
+    perm = np.random.RandomState(7).permutation(d).astype(np.int32)
+    pd = H.DeviceArray.from_numpy(perm)
+    H.check(H.lib.ophip_wsq_bwd_layout_perm(L.state.h, d, mode, buf, pd.h))
+    H.sync()
+    hf = L.arr(1 - buf, 2, np.uint16, "F").reshape(-1, order="F")
+    lf = L.arr(1 - buf, 4, np.uint16, "F").reshape(-1, order="F")
+    assert np.array_equal(unfrag(hf, d, d), Hr[np.ix_(perm, perm)])
+    assert np.array_equal(unfrag(lf, d, d), Lr[np.ix_(perm, perm)])
+    # the identity permutation is the plain layout
+    ident = H.DeviceArray.from_numpy(np.arange(d, dtype=np.int32))
+    H.check(H.lib.ophip_wsq_bwd_layout_perm(L.state.h, d, mode, buf, ident.h))
+    H.sync()
+    a = L.arr(1 - buf, 2, np.uint16, "F").copy()
+    H.check(H.lib.ophip_wsq_bwd_layout(L.state.h, d, mode, buf))
+    H.sync()
+    assert np.array_equal(L.arr(1 - buf, 2, np.uint16, "F"), a)
+
+
 def e4m3_rne(x):
     """round to OCP FP8 e4m3 (bias 7, 3 mantissa bits, subnormals below
     2^-6, max 448), nearest even"""
