@@ -64,14 +64,25 @@ def traffic_from_profiles(kernel):
     return None
 
 
-def roofline_backward(best, wb_gbs, wb_us, wb_bytes, bi_gbs, bi_us, bi_bytes):
-    if best in ("wform_fp16x3",):
-        k, gbs, us, by = "k_wbwd1 (W-form FP16 x3 backward step)", wb_gbs, wb_us, wb_bytes
-    else:
-        k, gbs, us, by = "backward step: k_skinny<3> + k_skinny_combine", bi_gbs, bi_us, bi_bytes
-    return {"bound": "hbm", "kernel": k, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": gbs / HBM_PEAK_GBS, "traffic": traffic_from_profiles(k.split()[0]),
-            "bytes_per_launch": by, "avg_launch_us": us}
+def roofline_backward(best, methods, wbk, bi_gbs, bi_us, bi_bytes):
+    """roofline entry of the headline method's dominant kernel: the FP16 x3
+    band-sparse backward step (per launch: the nonzero fragments of every
+    lattice's hi/lo copies and the live value columns), or the FP32
+    skinny step"""
+    if best.startswith("wform_fp16x3"):
+        nb = methods[best].get("group", 1)
+        w = wbk.get(nb) or wbk[1]
+        key = f"k_wbwd_sp/{nb}"
+        return {"bound": "hbm", "kernel": w["kernel"], "achieved": w["GBps"], "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": w["GBps"] / HBM_PEAK_GBS,
+                "traffic": traffic_from_profiles(key),
+                "bytes_per_launch": w["bytes_per_launch"], "avg_launch_us": w["avg_launch_us"],
+                "dense_bytes_per_launch": w["dense_bytes_per_launch"],
+                "nonzero_fragment_fraction": w["nonzero_fragment_fraction"]}
+    k = "backward step: k_skinny<3> + k_skinny_combine"
+    return {"bound": "hbm", "kernel": k, "achieved": bi_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": bi_gbs / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": bi_bytes,
+            "avg_launch_us": bi_us}
 
 
 def timed_events(H, fn, reps, a=0, b=1):
@@ -94,6 +105,10 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="skip the calibration-sweep leg")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate leg")
     ap.add_argument("--no-fp8", action="store_true", help="skip the FP8 W-form method")
+    ap.add_argument("--groups", type=int, nargs="*", default=[8, 16],
+                    help="group sizes of the FP16 x3 scenario-group method")
+    ap.add_argument("--probe-group", type=int, default=16,
+                    help="lattices per backward-step launch in --probe")
     ap.add_argument("--surr-train", type=int, default=131072, help="surrogate training draws")
     ap.add_argument("--surr-steps", type=int, default=80000, help="surrogate Adam steps")
     ap.add_argument("--probe", action="store_true",
@@ -113,7 +128,8 @@ def main():
     cfg = M.config_rs2048()
     d = cfg.d
     if args.probe:
-        wb = WformBackward(H, M, cfg)
+        nb = args.probe_group
+        wb = WformBackward(H, M, M.config_rs2048_scenarios(nb))
         for _ in range(20):
             wb.step()
         n = 16 << 20  # floats: 64 MiB copied by k_copy4 (opcuda_scopy1)
@@ -121,7 +137,9 @@ def main():
         for _ in range(20):
             H.check(H.lib.opcuda_scopy1(B.h, A.h, n), "scopy1")
         H.sync()
-        print(json.dumps({"probe": "k_wbwd1 x20, k_copy4 (64 MiB) x20", "step_bytes": wb.bytes,
+        print(json.dumps({"probe": f"k_wbwd_sp ({nb} lattices per launch) x20, k_copy4 (64 MiB) x20",
+                          "step_bytes_nonzero": wb.bytes, "step_bytes_dense": wb.bytes_dense,
+                          "nonzero_fragment_fraction": wb.nz_frac,
                           "copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
         return
     log(f"[bench] workload rs2048 d={d} nsq={cfg.nsq} steps={cfg.nsteps} strikes={cfg.nstrikes}")
@@ -165,15 +183,6 @@ def main():
     gemm_tflops = gemm_flop / (gemm_ms * 1e-3) / 1e12
     log(f"[bench] squaring GEMM {gemm_ms * 1e3:.1f} us, {gemm_tflops:.1f} TFLOP/s")
 
-    # ---- the W-form backward step (dominant kernel of the x3 methods) ----
-    wb = WformBackward(H, M, cfg)
-    for _ in range(4):
-        wb.step()
-    wb_us = timed_events(H, wb.step, 100) * 1e3
-    H.check()
-    wb_gbs = wb.bytes / (wb_us * 1e-6) / 1e9
-    log(f"[bench] W-form backward step {wb_us:.2f} us, {wb_gbs:.0f} GB/s algorithmic")
-
     # ---- backward-induction step: skinny GEMM, GB/s ----
     ncp = 48
     X = H.DeviceArray.empty(d * ncp)
@@ -199,6 +208,32 @@ def main():
             continue
         methods[name] = bench_method(H, M, prec, args.steps, args.warmup, gold)
         log(f"[bench] {name}: {methods[name]}")
+    # many scenarios of the same lattice priced together (FP16 x3)
+    for nb in args.groups:
+        name = f"wform_fp16x3_group{nb}"
+        methods[name] = bench_group(H, M, nb, max(2, args.steps // 4), 1)
+        log(f"[bench] {name}: {methods[name]}")
+
+    # ---- the FP16 x3 backward step (dominant kernel of the x3 methods), on
+    # one lattice and on the largest group, HIP events on the library stream
+    wbk = {}
+    for nb in sorted({1} | set(args.groups)):
+        wb = WformBackward(H, M, M.config_rs2048_scenarios(nb))
+        for _ in range(4):
+            wb.step()
+        us = timed_events(H, wb.step, 100) * 1e3
+        H.check()
+        wbk[nb] = {"kernel": f"k_wbwd_sp<FP16, 16 waves, 3> ({nb} lattice{'s' if nb > 1 else ''} "
+                             "per launch)",
+                   "avg_launch_us": us, "bytes_per_launch": wb.bytes,
+                   "dense_bytes_per_launch": wb.bytes_dense,
+                   "nonzero_fragment_fraction": wb.nz_frac,
+                   "GBps": wb.bytes / (us * 1e-6) / 1e9,
+                   "dense_equivalent_GBps": wb.bytes_dense / (us * 1e-6) / 1e9}
+        log(f"[bench] FP16x3 backward step x{nb}: {us:.2f} us, {wbk[nb]['GBps']:.0f} GB/s of nonzero "
+            f"fragments ({wb.nz_frac:.3f} of the matrix), {wbk[nb]['dense_equivalent_GBps']:.0f} GB/s "
+            "dense-equivalent")
+        del wb
 
     # ---- precision study: the same generator to the 12 monthly maturities
     # (41 strikes x 12 maturities x 8 regimes), every method's error ----
@@ -242,6 +277,11 @@ def main():
              "wform_fp16x3": "U = I + W, FP16 x3 (hi/lo) squarings",
              "wform_bf16x3": "U = I + W, BF16 x3 (hi/lo) squarings",
              "wform_fp8x3": "U = I + W, FP8 e4m3 x3 (hi/lo, power-of-two scales) squarings"}
+    for nb in args.groups:
+        names[f"wform_fp16x3_group{nb}"] = (f"U = I + W, FP16 x3 (hi/lo) squarings, {nb} scenarios "
+                                            "per graph launch (one squaring launch per round and one "
+                                            "band-sparse backward-step launch per step for all)")
+    grp = bm.get("group", 1)
     line = {
         "metric": "priced scenarios/s at <=1e-4 rel err vs FP64",
         "value": bm["scenarios_per_s"],
@@ -256,7 +296,10 @@ def main():
         "dtype": "fp32" if best == "plain_fp32" else best.split("_")[1][:4] + " (MFMA, fp32 accumulate) + fp32",
         "data": "synthetic",
         "config": {"workload": "regime-switching local vol, 256 x 8 = 2048 states, "
-                               "16 squarings to U(1/252), 252 backward steps, 41 strikes",
+                               "16 squarings to U(1/252), 252 backward steps, 41 strikes"
+                               + (f"; {grp} scenarios per launch (the workload's parameters, then "
+                                  "seeded vol levels, regime rates and rates)" if grp > 1 else ""),
+                   "scenarios_per_launch": grp,
                    "d": d, "squarings": cfg.nsq, "backward_steps": cfg.nsteps,
                    "strikes": cfg.nstrikes, "method": names[best],
                    "plain_fp32_scenarios_per_s": value,
@@ -264,18 +307,17 @@ def main():
         "error_vs_fp64": {"max_rel": bm["max_rel_err"],
                           "rms_rel": bm.get("rms_rel_err"),
                           "denominator": "max(|p64|, 1e-3*max|p64|)",
-                          "prices": int(rel.size),
+                          "prices": int(bm.get("group", 1) * rel.size),
                           "meets_1e-4": bool(bm["meets_1e-4"])},
         # dominant kernel of the headline method: the backward-induction
         # step (252 per scenario) streams the 2048 x 2048 matrix's hi/lo
         # copies (FP16 x3 W form; the FP32 methods' step is in "gemv")
-        "roofline": roofline_backward(best, wb_gbs, wb_us, wb.bytes, bi_gbs, bi_us, bi_bytes),
+        "roofline": roofline_backward(best, methods, wbk, bi_gbs, bi_us, bi_bytes),
         "roofline_squaring": {"bound": "mfma", "kernel": "k_sgemm<128,128> (plain FP32 squaring)",
                               "achieved": gemm_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                               "frac": gemm_tflops / FP32_PEAK_TFLOPS,
                               "flop_per_launch": gemm_flop, "avg_launch_us": gemm_ms * 1e3},
-        "wform_backward": {"kernel": "k_wbwd1 (FP16 x3 backward step)", "avg_us": wb_us,
-                           "algorithmic_GBps": wb_gbs, "frac": wb_gbs / HBM_PEAK_GBS},
+        "wform_backward": {str(k): v for k, v in wbk.items()},
         "gemv": {"kernel": "k_skinny<3> + combine (FP32 backward step)", "avg_us": bi_us,
                  "algorithmic_GBps": bi_gbs, "peak_GBps": HBM_PEAK_GBS,
                  "frac": bi_gbs / HBM_PEAK_GBS},
@@ -290,45 +332,71 @@ def main():
 
 
 class WformBackward:
-    """the headline method's dominant kernel on its real operands: the
-    rs2048 lattice squared 16 times in the FP16 x3 W form, then the
-    backward-induction step (k_wbwd1) alternating two value buffers"""
+    """the FP16 x3 method's dominant kernel on its real operands: each
+    lattice (a 2048-state regime scenario) squared 16 times in the W form,
+    W's fragment copies and tile lists laid out, then the band-sparse
+    backward step (k_wbwd_sp) of all of them in one launch, alternating two
+    value buffers per lattice"""
 
-    def __init__(self, H, M, cfg, ncp=48):
-        import math
-        self.H, self.d, self.ncp = H, cfg.d, ncp
-        d, mode = cfg.d, 1
-        x, _ = M.grid(cfg)
-        dr, vo = M.params(cfg)
-        self.keep = [H.DeviceArray.from_numpy(a) for a in (x, dr, vo)]
-        self.state = H.DeviceArray(H.lib.ophip_wsq_bytes(d, mode), np.uint8)
-        w0 = H.lib.ophip_wsq_w(self.state.h, d, mode, 0)
-        H.check(H.lib.ophip_sek_rs(cfg.nx, cfg.nr, self.keep[0].h, self.keep[1].h, self.keep[2].h,
-                                   cfg.q_off, math.ldexp(cfg.DeltaT, -cfg.nsq), 2, w0))
+    def __init__(self, H, M, cfgs, ncp=48):
         import ctypes as C
-        st, ds = (C.c_uint * 1)(self.state.h), (C.c_int * 1)(d)
-        plan = H.lib.ophip_wsq_plan(1, st, ds, None, mode, 0, 0)
-        H.check(H.lib.ophip_wsq_exec(plan, cfg.nsq))
+        import math
+        self.H, self.d, self.ncp, self.n = H, cfgs[0].d, ncp, len(cfgs)
+        d, mode = self.d, 1
+        self.keep, self.states = [], []
+        for cfg in cfgs:
+            x, _ = M.grid(cfg)
+            dr, vo = M.params(cfg)
+            arrs = [H.DeviceArray.from_numpy(a) for a in (x, dr, vo)]
+            st = H.DeviceArray(H.lib.ophip_wsq_bytes(d, mode), np.uint8)
+            self.keep += arrs
+            self.states.append(st)
+            w0 = H.lib.ophip_wsq_w(st.h, d, mode, 0)
+            H.check(H.lib.ophip_sek_rs(cfg.nx, cfg.nr, arrs[0].h, arrs[1].h, arrs[2].h, cfg.q_off,
+                                       math.ldexp(cfg.DeltaT, -cfg.nsq), 2, w0))
+            sh, ds = (C.c_uint * 1)(st.h), (C.c_int * 1)(d)
+            plan = H.lib.ophip_wsq_plan(1, sh, ds, None, mode, 0, 0)
+            H.check(H.lib.ophip_wsq_exec(plan, cfg.nsq))
+            H.sync()
+            H.lib.ophip_wsq_plan_free(plan)
+        cfg = cfgs[0]
         self.buf = cfg.nsq & 1
-        H.check(H.lib.ophip_wsq_bwd_layout(self.state.h, d, mode, self.buf))
+        nkt = d // 32
+        self.nz = 0  # nonzero 16 x 32 fragments over all lattices
+        for st in self.states:
+            H.check(H.lib.ophip_wsq_bwd_layout(st.h, d, mode, self.buf))
+            tl = np.empty((d // 16) * (nkt + 1), dtype=np.int32)
+            H.check(H.lib.opcuda_memcpy_d2h(H.lib.ophip_wsq_array(st.h, d, mode, 0, 7),
+                                            tl.ctypes.data, tl.nbytes))
+            self.nz += int(tl.reshape(d // 16, nkt + 1)[:, 0].sum())
         H.sync()
-        H.lib.ophip_wsq_plan_free(plan)
         xs = np.tile(x, cfg.nr)
         pay = np.zeros((d, ncp), np.float32)
         for k in range(cfg.nstrikes):
             pay[:, k] = np.maximum(xs - cfg.strikes[k], 0.0)
-        self.V = [H.DeviceArray.from_numpy(pay, order="C"), H.DeviceArray.empty(d * ncp)]
-        self.VT = [H.DeviceArray.empty(2 * d * ncp, np.uint16) for _ in range(2)]
-        H.check(H.lib.ophip_wsq_bwd_prep(d, ncp, mode, self.V[0].h, self.VT[0].h))
+        self.V = [[H.DeviceArray.from_numpy(pay, order="C"), H.DeviceArray.empty(d * ncp)]
+                  for _ in cfgs]
+        self.VT = [[H.DeviceArray.empty(2 * d * ncp, np.uint16) for _ in range(2)] for _ in cfgs]
+        for i in range(self.n):
+            H.check(H.lib.ophip_wsq_bwd_prep(d, ncp, mode, self.V[i][0].h, self.VT[i][0].h))
+        arr = lambda ty, v: (ty * self.n)(*v)  # noqa: E731
+        self.st = arr(C.c_uint, [s.h for s in self.states])
+        self.df = arr(C.c_float, [0.9999] * self.n)
+        self.X = [arr(C.c_uint, [v[p].h for v in self.V]) for p in (0, 1)]
+        self.XT = [arr(C.c_uint, [v[p].h for v in self.VT]) for p in (0, 1)]
         self.t = 0
-        # algorithmic bytes per step: the matrix (hi + lo = 4 d^2) and the
-        # 41 live value columns read and written
-        self.bytes = 4.0 * d * d + 4.0 * d * cfg.nstrikes * 2
+        # bytes per launch: dense algorithmic (the hi/lo copies of every W,
+        # 4 d^2 each, and the live value columns read and written) and the
+        # part the band-sparse kernel must read (the nonzero fragments, 2 KB
+        # each with hi and lo)
+        self.bytes_dense = self.n * (4.0 * d * d + 4.0 * d * cfg.nstrikes * 2)
+        self.bytes = 2048.0 * self.nz + self.n * 4.0 * d * cfg.nstrikes * 2
+        self.nz_frac = self.nz / (self.n * (d // 16) * nkt)
 
     def step(self):
         a, b = self.t & 1, (self.t + 1) & 1
-        self.H.lib.ophip_wsq_bwd_step(self.state.h, self.d, 1, self.buf, self.ncp, np.float32(0.9999),
-                                      self.V[a].h, self.VT[a].h, self.V[b].h, self.VT[b].h)
+        self.H.lib.ophip_wsq_bwd_step_batch(self.n, self.st, self.d, 1, self.buf, self.ncp, self.df,
+                                            self.X[a], self.XT[a], self.X[b], self.XT[b])
         self.t += 1
 
 
@@ -355,6 +423,31 @@ def bench_method(H, M, prec, steps, warmup, gold, make=None):
     s.close()
     return {"scenarios_per_s": 1.0 / t, "ms": t * 1e3, "max_rel_err": float(e.max()),
             "rms_rel_err": float(np.sqrt(np.mean(e ** 2))), "meets_1e-4": bool(e.max() <= 1e-4)}
+
+
+def bench_group(H, M, nb, steps, warmup):
+    """nb scenarios of the 2048-state lattice (the workload, then seeded vol
+    levels, regime rates and rates) priced together in FP16 x3: one
+    squaring launch per round and one backward-step launch per step for
+    all of them, the whole group one graph launch"""
+    gold = np.load(os.path.join(ROOT, "tests", "golden", "prices_rs2048_scenarios.npz"))["prices"]
+    nb = min(nb, len(gold))
+    g = M.Group(M.config_rs2048_scenarios(nb))
+    g.solve()
+    g.capture()
+    for _ in range(warmup):
+        g.launch()
+    H.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.launch()
+    H.sync()
+    t = (time.perf_counter() - t0) / steps
+    e = rel_err(g.prices(), gold[:nb])
+    g.close()
+    return {"scenarios_per_s": nb / t, "ms": t * 1e3 / nb, "ms_per_group": t * 1e3, "group": nb,
+            "max_rel_err": float(e.max()), "rms_rel_err": float(np.sqrt(np.mean(e ** 2))),
+            "meets_1e-4": bool(e.max() <= 1e-4)}
 
 
 def bench_sweep(H, M, reps):

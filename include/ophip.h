@@ -121,6 +121,11 @@ int ophip_dgen_rs(int nx, int nr, unsigned xval, unsigned drift, unsigned vol, d
  * node-major state order y = nr*x + r (one band; used by the W form) */
 int ophip_sek_rs(int nx, int nr, unsigned xval, unsigned drift, unsigned vol, double q_off,
                  double dt, int out_kind, unsigned out);
+/* ophip_sek_rs for n lattices in one launch per 32 (lattice k takes the
+ * k-th entry of every array, one out_kind for all); the same values */
+int ophip_sek_rs_batch(int n, const int *nx, const int *nr, const unsigned *xval,
+                       const unsigned *drift, const unsigned *vol, const double *q_off,
+                       const double *dt, int out_kind, const unsigned *out);
 /* cumulative tables stored row-contiguous: crow[m*d*d + row*d + y]; the
  * same values as opcuda_sck (sck.cu:43-84) */
 int ophip_sck_rows(int d, int nm, unsigned tpk, unsigned crow);
@@ -162,7 +167,10 @@ unsigned long ophip_wsq_bytes(int d, int mode);
 unsigned      ophip_wsq_w(unsigned state, int d, int mode, int buf);
 /* any array of a state: which = 0 W, 1 Hr, 2 Hc, 3 Lr, 4 Lc, 5 partial
  * row sums, 6 cell flags (d/64 x d/64 bytes; 0 = every 16-bit copy value of
- * that 64 x 64 cell is +-0) (0 when the mode has no such array) */
+ * that 64 x 64 cell is +-0), 7 backward-step tile lists (d/16 + d/64 +
+ * d/32 lists of d/32 + 1 ints, per 16-row tile, 64-row block and 32-row
+ * block: count, then the k tiles where a fragment of those rows holds a
+ * nonzero hi or lo value; buf ignored) (0 when the mode has no such array) */
 unsigned      ophip_wsq_array(unsigned state, int d, int mode, int buf, int which);
 /* Zero-cell skipping: the squaring visits only the K cells whose A and B
  * cells are flagged nonzero (the skipped products are exact zeros: same
@@ -201,6 +209,21 @@ int ophip_wsq_bwd_layout_perm(unsigned state, int d, int mode, int buf, unsigned
 int ophip_wsq_bwd_prep(int d, int ncp, int mode, unsigned X, unsigned XT);
 int ophip_wsq_bwd_step(unsigned state, int d, int mode, int buf, int ncp, float df, unsigned X,
                        unsigned XT, unsigned Y, unsigned YT);
+/* The backward step of nb lattices of one d in one launch (per 32): Y[i] =
+ * df[i] (X[i] + W_i X[i]), W_i from states[i] laid out as above.  Both
+ * step entry points visit only the 16 x 32 fragments that hold a nonzero
+ * hi or lo value (lists built by ophip_wsq_bwd_layout*): outside the band
+ * of a squared generator both copies flush to zero, and skipped products
+ * are exact zeros.  ophip_wsq_set_bwd_kernel(k) selects the kernel: 0 the
+ * dense one (every fragment), 1 automatic (default), 2 per 16-row tile, 3
+ * per 64-row block, 4 per 32-row block (3 and 4: ncp 48); it returns the
+ * previous setting and applies to kernels launched afterwards.  No
+ * reference counterpart: the reference steps one model at a time
+ * (sgemv.cu:44-51). */
+int ophip_wsq_bwd_step_batch(int nb, const unsigned *states, int d, int mode, int buf, int ncp,
+                             const float *df, const unsigned *X, const unsigned *XT,
+                             const unsigned *Y, const unsigned *YT);
+int ophip_wsq_set_bwd_kernel(int k);
 /* the same plan with a tile per lattice (64 or 128 each): one persistent
  * launch per tile class and round */
 unsigned long ophip_wsq_plan_tiles(int n, const unsigned *states, const int *ds, const int *nsqs,
@@ -223,6 +246,10 @@ unsigned long ophip_fwd_plan(int n, const int *d, const unsigned *K, const int *
                              const float *rate, float dT, int nmat, const float *strikes,
                              int nstrikes);
 int  ophip_fwd_exec(unsigned long plan);
+/* per lattice the 64 x 64 cell flags of its kernel (handles of (d/64)^2
+ * bytes as ophip_wsq_array which = 6 gives; 0 = none): the forward steps
+ * then skip zero cells (exact: their entries are 0).  d % 64 == 0. */
+int  ophip_fwd_set_cells(unsigned long plan, const unsigned *flags);
 void ophip_fwd_plan_free(unsigned long plan);
 /* prices at up to 16 spot rows of one kernel (U, or W with plus_identity):
  * out[r*nstrikes + c] = df (plus_identity pay(y0_r, c) + sum_j U[y0_r][j]

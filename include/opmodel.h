@@ -86,6 +86,20 @@ int opm_kernel_raw(opm_solver *s, float *K, int *is_w);
 unsigned opm_kernel_handle(opm_solver *s);
 int opm_dims(opm_solver *s, int *d, int *n_mat);
 
+/* ---- a group of scenarios of one lattice size, priced together ----
+ * Solvers (opm_create) of one d, W-form precision (F16, F16X3, BF16X3),
+ * backward pricing, squaring count and step count: their squarings run in
+ * one persistent launch per round and, in F16X3, every backward step of
+ * all of them in one launch.  Each solver keeps its buffers; opm_prices on
+ * a member reads its prices after opm_group_solve / opm_group_launch.  A
+ * solver belongs to at most one group at a time; destroy the group first. */
+typedef struct opm_group opm_group;
+opm_group *opm_group_create(opm_solver *const *solvers, int n);
+void opm_group_destroy(opm_group *g);
+int opm_group_solve(opm_group *g);
+int opm_group_capture(opm_group *g);
+int opm_group_launch(opm_group *g);
+
 /* ---- calibration sweep: many one-regime lattices of different size ----
  * Every configuration shares precision, nsq, DeltaT, nsteps (maturities
  * k*DeltaT, k = 1..nsteps) and the strikes; nx, the vol parameters and

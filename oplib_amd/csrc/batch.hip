@@ -21,6 +21,7 @@ struct FwdScen {
     const float *x;   // grid values (FP32), d
     float *out;       // nmat x nstrikes prices
     float rate, dT;
+    const uint8_t *fl;  // optional 64 x 64 cell flags of K ([d/64][d/64], 0 = all zero)
 };
 
 __global__ __launch_bounds__(256) void k_fwd_step(const FwdScen *__restrict__ sc, int k) {
@@ -31,7 +32,17 @@ __global__ __launch_bounds__(256) void k_fwd_step(const FwdScen *__restrict__ sc
     const float *p = s.P + (size_t)(k - 1) * s.d;
     const float *col = s.K + (size_t)s.d * j;
     float acc = 0.0f;
-    for (int i = lane; i < s.d; i += 64) acc += p[i] * col[i];
+    if (s.fl) {
+        // only the 64-row chunks of this column whose cell may be nonzero
+        // (a zero cell is exactly 0: the skipped terms add nothing); d % 64
+        // == 0 and d <= 4096
+        const int nc = s.d / 64;
+        const uint64_t m = __ballot(lane < nc && s.fl[(size_t)lane * nc + j / 64] != 0);
+        for (int c = 0; c < nc; c++)
+            if ((m >> c) & 1) acc += p[64 * c + lane] * col[64 * c + lane];
+    } else {
+        for (int i = lane; i < s.d; i += 64) acc += p[i] * col[i];
+    }
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
     if (lane == 0) s.P[(size_t)k * s.d + j] = (s.plus_identity ? p[j] : 0.0f) + acc;
 }
@@ -173,6 +184,25 @@ OPHIP_EXPORT unsigned long ophip_fwd_plan(int n, const int *d, const unsigned *K
     (void)D;
     set_status(kOk);
     return (unsigned long)(uintptr_t)pl;
+}
+
+// per lattice the cell flags of its kernel (handles of (d/64)^2 bytes,
+// [row cell][column cell], 0 = every entry of the cell is 0; a 0 handle
+// = none): the forward steps skip the zero cells.  d % 64 == 0, d <= 4096.
+OPHIP_EXPORT int ophip_fwd_set_cells(unsigned long plan, const unsigned *flags) {
+    FwdPlan *pl = (FwdPlan *)(uintptr_t)plan;
+    if (!pl || !flags) return (int)set_status(kErrBadArg);
+    for (int i = 0; i < pl->n; i++) {
+        const int d = pl->host[i].d;
+        if (flags[i] && (d % 64 || d > 4096 || !handle_range_ok(flags[i], (size_t)(d / 64) * (d / 64))))
+            return (int)set_status(kErrBadArg);
+    }
+    for (int i = 0; i < pl->n; i++) pl->host[i].fl = flags[i] ? P<const uint8_t>(flags[i]) : nullptr;
+    hipError_t e = hipStreamSynchronize(dev().stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(pl->sc, pl->host, sizeof(FwdScen) * pl->n, hipMemcpyHostToDevice, dev().stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(dev().stream);
+    return (int)set_status(hip_status(e));
 }
 
 // nmat forward steps then the prices (kernel launches only)

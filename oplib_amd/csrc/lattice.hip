@@ -156,32 +156,69 @@ __device__ inline void rs_rates(const RSLattice &L, int y, double &am, double &a
 // the 8 regimes of a node are neighbours, so the matrix is one band and
 // the W form's zero-cell skipping applies); drift/vol stay indexed by the
 // canonical state x + nx*r.
+template <int MODE, bool NM>
+__device__ inline double rs_elem(const RSLattice &L, double dt, int i, int j) {
+    const int nx = L.nx, nr = L.nr;
+    const int xi = NM ? i / nr : i % nx, ri = NM ? i % nr : i / nx;
+    const int xj = NM ? j / nr : j % nx, rj = NM ? j % nr : j / nx;
+    double g = 0.0;
+    if (ri == rj) {
+        if (xj == xi - 1 || xj == xi + 1 || xj == xi) {
+            double am, ap;
+            rs_rates(L, xi + nx * ri, am, ap);
+            if (xj == xi - 1) g = am;
+            else if (xj == xi + 1) g = ap;
+            else g = -(am + ap + (double)(L.nr - 1) * L.q_off);
+        }
+    } else if (xi == xj) {
+        g = L.q_off;
+    }
+    if (MODE == 0) return g;
+    if (MODE == 1) return (i == j ? 1.0 : 0.0) + dt * g;
+    return dt * g;
+}
+
 template <int MODE, typename OUT, bool NM>
 __global__ void k_rs_assemble(RSLattice L, double dt, OUT *__restrict__ out) {
-    const int nx = L.nx, nr = L.nr, d = L.nx * L.nr;
+    const int d = L.nx * L.nr;
     size_t n = (size_t)d * d;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
          e += (size_t)gridDim.x * blockDim.x) {
-        int i = (int)(e % d), j = (int)(e / d);  // row i (from), column j (to)
-        int xi = NM ? i / nr : i % nx, ri = NM ? i % nr : i / nx;
-        int xj = NM ? j / nr : j % nx, rj = NM ? j % nr : j / nx;
-        double g = 0.0;
-        if (ri == rj) {
-            if (xj == xi - 1 || xj == xi + 1 || xj == xi) {
-                double am, ap;
-                rs_rates(L, xi + nx * ri, am, ap);
-                if (xj == xi - 1) g = am;
-                else if (xj == xi + 1) g = ap;
-                else g = -(am + ap + (double)(L.nr - 1) * L.q_off);
-            }
-        } else if (xi == xj) {
-            g = L.q_off;
+        const int i = (int)(e % d), j = (int)(e / d);  // row i (from), column j (to)
+        out[e] = (OUT)rs_elem<MODE, NM>(L, dt, i, j);
+    }
+}
+
+// Many lattices in one launch (blockIdx.y = lattice): each thread writes
+// four consecutive rows of one column (d % 4 == 0), 32-bit index math
+// (d * d / 4 < 2^32).  The values are those of k_rs_assemble.
+struct RSEntry {
+    RSLattice L;
+    double dt;
+    void *out;
+};
+constexpr int kRSBatch = 32;
+struct RSBatch {
+    RSEntry e[kRSBatch];
+};
+
+template <int MODE, typename OUT, bool NM>
+__global__ __launch_bounds__(256) void k_rs_assemble_batch(const RSBatch b) {
+    const RSEntry &E = b.e[blockIdx.y];
+    const unsigned d = (unsigned)(E.L.nx * E.L.nr), d4 = d / 4, n4 = d * d4;
+    OUT *out = reinterpret_cast<OUT *>(E.out);
+    for (unsigned q = blockIdx.x * 256u + threadIdx.x; q < n4; q += gridDim.x * 256u) {
+        const unsigned j = q / d4, i = 4 * (q - j * d4);
+        OUT v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = (OUT)rs_elem<MODE, NM>(E.L, E.dt, (int)i + u, (int)j);
+        OUT *o = out + (size_t)j * d + i;
+        if (sizeof(OUT) == 4) {
+            *reinterpret_cast<float4 *>(o) = make_float4(v[0], v[1], v[2], v[3]);
+        } else {
+            reinterpret_cast<double2 *>(o)[0] = make_double2(v[0], v[1]);
+            reinterpret_cast<double2 *>(o)[1] = make_double2(v[2], v[3]);
         }
-        double v;
-        if (MODE == 0) v = g;
-        else if (MODE == 1) v = (i == j ? 1.0 : 0.0) + dt * g;
-        else v = dt * g;
-        out[e] = (OUT)v;
     }
 }
 
@@ -232,6 +269,62 @@ OPHIP_EXPORT int ophip_sek_rs(int nx, int nr, unsigned xval, unsigned drift, uns
     default: return (int)set_status(kErrBadArg);
     }
     return (int)check_launch();
+}
+
+// ophip_sek_rs for n lattices in one launch per 32: lattice k takes the
+// k-th entry of every array; all share out_kind.  Each output holds the
+// values ophip_sek_rs writes.  Every d = nx[k]*nr[k] must be a multiple of
+// 4 (else each lattice is assembled by its own ophip_sek_rs launch).
+OPHIP_EXPORT int ophip_sek_rs_batch(int n, const int *nx, const int *nr, const unsigned *xval,
+                                    const unsigned *drift, const unsigned *vol, const double *q_off,
+                                    const double *dt, int out_kind, const unsigned *out) {
+    if (n <= 0 || !nx || !nr || !xval || !drift || !vol || !q_off || !dt || !out || out_kind < 0 ||
+        out_kind > 7)
+        return (int)set_status(kErrBadArg);
+    bool vec = true;
+    for (int k = 0; k < n; k++) {
+        RSLattice L;
+        if (!rs_args(nx[k], nr[k], xval[k], drift[k], vol[k], L) || !ptr(out[k]))
+            return (int)set_status(kErrBadArg);
+        vec = vec && (nx[k] * nr[k]) % 4 == 0;
+    }
+    if (!vec) {
+        for (int k = 0; k < n; k++) {
+            const int e = ophip_sek_rs(nx[k], nr[k], xval[k], drift[k], vol[k], q_off[k], dt[k], out_kind, out[k]);
+            if (e) return e;
+        }
+        return 0;
+    }
+    Device &D = dev();
+    for (int k0 = 0; k0 < n; k0 += kRSBatch) {
+        const int m = n - k0 < kRSBatch ? n - k0 : kRSBatch;
+        RSBatch b{};
+        size_t most = 0;
+        for (int k = 0; k < m; k++) {
+            RSEntry &E = b.e[k];
+            rs_args(nx[k0 + k], nr[k0 + k], xval[k0 + k], drift[k0 + k], vol[k0 + k], E.L);
+            E.L.q_off = q_off[k0 + k];
+            E.dt = dt[k0 + k];
+            E.out = ptr(out[k0 + k]);
+            const size_t d = (size_t)nx[k0 + k] * nr[k0 + k];
+            most = d * d / 4 > most ? d * d / 4 : most;
+        }
+        const dim3 g((unsigned)grid1d(most, 256), (unsigned)m), bl(256);
+        hipStream_t st = D.stream;
+        switch (out_kind) {
+        case 0: hipLaunchKernelGGL((k_rs_assemble_batch<1, float, false>), g, bl, 0, st, b); break;
+        case 1: hipLaunchKernelGGL((k_rs_assemble_batch<1, double, false>), g, bl, 0, st, b); break;
+        case 2: hipLaunchKernelGGL((k_rs_assemble_batch<2, float, false>), g, bl, 0, st, b); break;
+        case 3: hipLaunchKernelGGL((k_rs_assemble_batch<2, double, false>), g, bl, 0, st, b); break;
+        case 4: hipLaunchKernelGGL((k_rs_assemble_batch<1, float, true>), g, bl, 0, st, b); break;
+        case 5: hipLaunchKernelGGL((k_rs_assemble_batch<1, double, true>), g, bl, 0, st, b); break;
+        case 6: hipLaunchKernelGGL((k_rs_assemble_batch<2, float, true>), g, bl, 0, st, b); break;
+        default: hipLaunchKernelGGL((k_rs_assemble_batch<2, double, true>), g, bl, 0, st, b); break;
+        }
+        const unsigned e = check_launch();
+        if (e) return (int)e;
+    }
+    return 0;
 }
 
 // ---------------------------------------------------------------------

@@ -265,6 +265,10 @@ opm_solver *opm_create(const opm_config *cfg) {
                                       oh.data(), rate.data(), (float)c->DeltaT, c->nsteps, ks.data(),
                                       c->nstrikes);
             ok = s->fplan != 0;
+            if (ok && s->wmode != 3) {  // the forward steps skip W's zero cells
+                std::vector<unsigned> fl(nr, ophip_wsq_array(s->wstate.h, d, s->wmode, c->nsq & 1, 6));
+                ok = ophip_fwd_set_cells(s->fplan, fl.data()) == 0;
+            }
         }
     } else if (ok) {
         s->n_mat = c->nsteps;
@@ -510,6 +514,157 @@ int opm_dims(opm_solver *s, int *d, int *n_mat) {
 }  // extern "C"
 
 // ---------------------------------------------------------------------
+// A group of solvers priced together: many scenarios of one lattice size,
+// each a whole backward-induction solve in the W form.  The squarings of
+// all of them run in one persistent launch per round (one plan over every
+// state) and, in FP16 x3, each backward step of all of them in one launch
+// (ophip_wsq_bwd_step_batch).  Every solver keeps its own buffers and
+// prices; the reference solves one model at a time (CModel Device.cs:
+// 186-295), which is what opm_solve on each member does.
+// ---------------------------------------------------------------------
+struct opm_group {
+    std::vector<opm_solver *> s;
+    unsigned long wplan = 0;
+    unsigned long graph = 0;
+};
+
+extern "C" {
+
+opm_group *opm_group_create(opm_solver *const *solvers, int n) {
+    if (!solvers || n <= 0) return nullptr;
+    const opm_solver *a = solvers[0];
+    if (!a) return nullptr;
+    for (int i = 0; i < n; i++) {
+        const opm_solver *b = solvers[i];
+        if (!b || b->wmode < 0 || b->wmode > 2 || b->cfg.price_kind != OPM_PRICE_BACKWARD ||
+            b->d != a->d || b->wmode != a->wmode || b->cfg.nsq != a->cfg.nsq ||
+            b->cfg.nsteps != a->cfg.nsteps || b->ncp != a->ncp || b->v_ref != a->v_ref || b->nm != a->nm)
+            return nullptr;
+        for (int k = 0; k < i; k++)
+            if (solvers[k] == b) return nullptr;  // each solver once
+    }
+    opm_group *g = new opm_group();
+    g->s.assign(solvers, solvers + n);
+    std::vector<unsigned> st(n);
+    std::vector<int> ds(n, a->d);
+    for (int i = 0; i < n; i++) st[i] = solvers[i]->wstate.h;
+    g->wplan = ophip_wsq_plan(n, st.data(), ds.data(), nullptr, a->wmode, 0, 0);
+    if (!g->wplan) {
+        delete g;
+        return nullptr;
+    }
+    return g;
+}
+
+void opm_group_destroy(opm_group *g) {
+    if (!g) return;
+    if (g->graph) ophip_graph_destroy(g->graph);
+    if (g->wplan) ophip_wsq_plan_free(g->wplan);
+    delete g;
+}
+
+}  // extern "C"
+
+// the group's solve body: only kernel launches on the library stream
+static int group_body(opm_group *g) {
+    const int n = (int)g->s.size();
+    opm_solver *a = g->s[0];
+    const int d = a->d, wmode = a->wmode, nsq = a->cfg.nsq, nsteps = a->cfg.nsteps;
+    int st;
+    {  // every scenario's FP64 assembly and first step, one launch per 32
+        std::vector<int> nx(n), nr(n);
+        std::vector<unsigned> xv(n), dr(n), vo(n), dst(n);
+        std::vector<double> q(n), dt(n);
+        for (int i = 0; i < n; i++) {
+            const opm_solver *s = g->s[i];
+            nx[i] = s->cfg.nx;
+            nr[i] = s->cfg.nr;
+            xv[i] = s->x.h;
+            dr[i] = s->drift.h;
+            vo[i] = s->vol.h;
+            q[i] = s->cfg.q_off;
+            dt[i] = s->dt0;
+            dst[i] = ophip_wsq_w(s->wstate.h, d, wmode, 0);
+        }
+        st = ophip_sek_rs_batch(n, nx.data(), nr.data(), xv.data(), dr.data(), vo.data(), q.data(), dt.data(),
+                                a->nm ? 6 : 2, dst.data());
+        if (st) return st;
+    }
+    st = ophip_wsq_exec(g->wplan, nsq);
+    if (st) return st;
+    const int buf = nsq & 1;
+    std::vector<unsigned> states(n), X[2], XT[2];
+    std::vector<float> df(n);
+    for (int i = 0; i < n; i++) {
+        opm_solver *s = g->s[i];
+        s->u_final = buf;
+        s->kernel_h = ophip_wsq_w(s->wstate.h, d, wmode, buf);
+        s->v_final = nsteps & 1;
+        st = opcuda_scopy1(s->V[0].h, s->pay0.h, (unsigned)((size_t)d * s->ncp));
+        if (st) return st;
+        states[i] = s->wstate.h;
+        df[i] = (float)std::exp(-s->cfg.rate * s->cfg.DeltaT);
+        for (int p = 0; p < 2; p++) {
+            X[p].push_back(s->V[p].h);
+            XT[p].push_back(s->VT[p].h);
+        }
+    }
+    if (wmode == 1) {
+        // FP16 x3 on the matrix cores: every lattice's step in one launch
+        for (opm_solver *s : g->s) {
+            st = s->v_ref ? ophip_wsq_bwd_layout_perm(s->wstate.h, d, wmode, buf, s->perm.h)
+                          : ophip_wsq_bwd_layout(s->wstate.h, d, wmode, buf);
+            if (st) return st;
+            st = ophip_wsq_bwd_prep(d, s->ncp, wmode, s->V[0].h, s->VT[0].h);
+            if (st) return st;
+        }
+        for (int t = 0; t < nsteps; t++) {
+            const int p = t & 1, q = p ^ 1;
+            st = ophip_wsq_bwd_step_batch(n, states.data(), d, wmode, buf, a->ncp, df.data(), X[p].data(),
+                                          XT[p].data(), X[q].data(), XT[q].data());
+            if (st) return st;
+        }
+    } else {
+        for (int t = 0; t < nsteps; t++)
+            for (int i = 0; i < n; i++) {
+                opm_solver *s = g->s[i];
+                st = ophip_sgemm_skinny_ex(d, d, s->ncp, df[i], s->kernel_h, d, s->V[t & 1].h, s->ncp,
+                                           s->V[(t + 1) & 1].h, s->ncp, 1);
+                if (st) return st;
+            }
+    }
+    return 0;
+}
+
+extern "C" {
+
+int opm_group_solve(opm_group *g) { return g ? group_body(g) : 1; }
+
+int opm_group_capture(opm_group *g) {
+    if (!g) return 1;
+    if (g->graph) {
+        ophip_graph_destroy(g->graph);
+        g->graph = 0;
+    }
+    ophip_reserve_workspace((unsigned long)64 << 20);
+    int st = ophip_capture_begin();
+    if (st) return st;
+    int body = group_body(g);
+    unsigned long gr = ophip_capture_end();
+    if (body) {
+        if (gr) ophip_graph_destroy(gr);
+        return body;
+    }
+    if (!gr) return (int)opcuda_get_status();
+    g->graph = gr;
+    return 0;
+}
+
+int opm_group_launch(opm_group *g) { return (g && g->graph) ? ophip_graph_launch(g->graph) : 1; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------
 // calibration sweep (see opmodel.h)
 // ---------------------------------------------------------------------
 
@@ -651,6 +806,11 @@ opm_batch *opm_batch_create(const opm_config *cfgs, int n, int tile) {
                                   oh.data(), rate.data(), (float)c0.DeltaT, b->nmat, ks.data(),
                                   b->nstrikes);
         ok = b->fplan != 0;
+        if (ok && b->wmode >= 0) {  // the forward steps skip W's zero cells
+            std::vector<unsigned> fl(n);
+            for (int i = 0; i < n; i++) fl[i] = ophip_wsq_array(b->state[i].h, b->d[i], b->wmode, b->nsq[i] & 1, 6);
+            ok = ophip_fwd_set_cells(b->fplan, fl.data()) == 0;
+        }
     }
     if (!ok) {
         opm_batch_destroy(b);
@@ -661,12 +821,20 @@ opm_batch *opm_batch_create(const opm_config *cfgs, int n, int tile) {
 
 static int batch_body(opm_batch *b) {
     int st;
-    for (int i = 0; i < b->n; i++) {
-        const opm_config *c = &b->cfg[i];
-        const unsigned dst = b->wmode >= 0 ? ophip_wsq_w(b->state[i].h, b->d[i], b->wmode, 0)
-                                           : b->state[i].h;
-        st = ophip_sek_rs(c->nx, 1, b->x[i].h, b->drift[i].h, b->vol[i].h, 0.0, b->dt0[i],
-                          b->wmode >= 0 ? 2 : 0, dst);
+    {  // every lattice's FP64 assembly and first step, one launch per 32
+        const int n = b->n;
+        std::vector<int> nx(n), nr(n, 1);
+        std::vector<unsigned> xv(n), dr(n), vo(n), dst(n);
+        std::vector<double> q(n, 0.0);
+        for (int i = 0; i < n; i++) {
+            nx[i] = b->cfg[i].nx;
+            xv[i] = b->x[i].h;
+            dr[i] = b->drift[i].h;
+            vo[i] = b->vol[i].h;
+            dst[i] = b->wmode >= 0 ? ophip_wsq_w(b->state[i].h, b->d[i], b->wmode, 0) : b->state[i].h;
+        }
+        st = ophip_sek_rs_batch(n, nx.data(), nr.data(), xv.data(), dr.data(), vo.data(), q.data(),
+                                b->dt0.data(), b->wmode >= 0 ? 2 : 0, dst.data());
         if (st) return st;
     }
     if (b->wmode >= 0) {

@@ -72,6 +72,9 @@ SIGNATURES = {
     "ophip_stream": (ULONG, []),
     "ophip_dgen_rs": (I32, [I32, I32, U32, U32, U32, F64, U32]),
     "ophip_sek_rs": (I32, [I32, I32, U32, U32, U32, F64, F64, I32, U32]),
+    "ophip_sek_rs_batch": (I32, [I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(U32),
+                                 C.POINTER(U32), C.POINTER(U32), C.POINTER(F64), C.POINTER(F64),
+                                 I32, C.POINTER(U32)]),
     "ophip_sck_rows": (I32, [I32, I32, U32, U32]),
     "ophip_sgemm_strided_batched": (I32, [I32, I32, I32, F32, U32, I32, LL, U32, I32, LL, F32,
                                           U32, I32, LL, I32]),
@@ -95,6 +98,10 @@ SIGNATURES = {
     "ophip_wsq_bwd_layout_perm": (I32, [U32, I32, I32, I32, U32]),
     "ophip_wsq_bwd_prep": (I32, [I32, I32, I32, U32, U32]),
     "ophip_wsq_bwd_step": (I32, [U32, I32, I32, I32, I32, F32, U32, U32, U32, U32]),
+    "ophip_wsq_bwd_step_batch": (I32, [I32, C.POINTER(U32), I32, I32, I32, I32, C.POINTER(F32),
+                                       C.POINTER(U32), C.POINTER(U32), C.POINTER(U32),
+                                       C.POINTER(U32)]),
+    "ophip_wsq_set_bwd_kernel": (I32, [I32]),
     "ophip_wsq_plan_tiles": (ULONG, [I32, C.POINTER(U32), C.POINTER(I32), C.POINTER(I32), I32,
                                      C.POINTER(I32), I32]),
     "ophip_wsq_autotune": (I32, [I32, C.POINTER(U32), C.POINTER(I32), C.POINTER(I32), I32,
@@ -103,6 +110,7 @@ SIGNATURES = {
                                C.POINTER(U32), C.POINTER(U32), C.POINTER(U32), C.POINTER(F32), F32,
                                I32, C.POINTER(F32), I32]),
     "ophip_fwd_exec": (I32, [ULONG]),
+    "ophip_fwd_set_cells": (I32, [ULONG, C.POINTER(U32)]),
     "ophip_fwd_plan_free": (V, [ULONG]),
     "ophip_spot_prices": (I32, [I32, U32, I32, C.POINTER(I32), I32, U32, I32, F32, U32]),
     "ophip_fwd_set_rates": (I32, [ULONG, C.POINTER(F32)]),

@@ -366,3 +366,80 @@ def sweep_flops(cfgs):
         d = c.d
         f += c.nsq * 2.0 * d ** 3 + c.nsteps * 2.0 * d * d + c.nsteps * c.nstrikes * 2.0 * d
     return f
+
+
+# ---------------------------------------------------------------------------
+# groups of scenarios of one lattice size (regime-switching family)
+# ---------------------------------------------------------------------------
+
+def config_rs2048_scenarios(n=8, seed=5489, precision=OPM_PREC_F16X3, nsteps=252):
+    """n scenarios of the 2048-state regime-switching lattice: scenario 0 is
+    the workload itself (sigma_k = 0.1..0.8, regime rate 0.5/yr, r = 0.02);
+    scenario i >= 1 draws three MT19937(seed) uniforms in this order (after
+    the 3(i-1) before it): a vol level s = 0.8 + 0.4u scaling every sigma_k,
+    a regime rate 0.25 + 0.75u and r = 0.05u.  Same grid, dt0, 16 squarings
+    to U(1/252), 252 backward steps and 41 strikes."""
+    u = _mt_uniforms(seed, 3 * max(n - 1, 0)).reshape(-1, 3)
+    cfgs = []
+    for i in range(n):
+        c = config_rs2048(nsteps)
+        if i > 0:
+            lv, q, r = u[i - 1]
+            for k in range(8):
+                c.sigma_r[k] = (0.8 + 0.4 * lv) * 0.1 * (k + 1)
+            c.q_off = 0.25 + 0.75 * q
+            c.rate = 0.05 * r
+        c.precision = precision
+        cfgs.append(c)
+    return cfgs
+
+
+lib.opm_group_create.argtypes = [C.POINTER(C.c_void_p), C.c_int]
+lib.opm_group_create.restype = C.c_void_p
+lib.opm_group_destroy.argtypes = [C.c_void_p]
+lib.opm_group_destroy.restype = None
+for _n in ("opm_group_solve", "opm_group_capture", "opm_group_launch"):
+    getattr(lib, _n).argtypes = [C.c_void_p]
+    getattr(lib, _n).restype = C.c_int
+
+
+class Group:
+    """Scenarios of one lattice size priced together (opm_group): one
+    squaring launch per round for all, one backward-step launch per step
+    for all (FP16 x3).  prices() -> [n][nr][1][nstrikes]."""
+
+    def __init__(self, cfgs):
+        ophip.init()
+        self.solvers = [Solver(c) for c in cfgs]
+        arr = (C.c_void_p * len(self.solvers))(*[s.p for s in self.solvers])
+        self.p = lib.opm_group_create(arr, len(self.solvers))
+        if not self.p:
+            for s in self.solvers:
+                s.close()
+            raise ophip.OphipError(
+                f"opm_group_create failed: 0x{ophip.lib.opcuda_get_status():x}")
+
+    def solve(self):
+        ophip.check(lib.opm_group_solve(self.p), "opm_group_solve")
+
+    def capture(self):
+        ophip.check(lib.opm_group_capture(self.p), "opm_group_capture")
+
+    def launch(self):
+        ophip.check(lib.opm_group_launch(self.p), "opm_group_launch")
+
+    def prices(self):
+        return np.stack([s.prices() for s in self.solvers])
+
+    def close(self):
+        if self.p:
+            lib.opm_group_destroy(self.p)
+            self.p = None
+        for s in self.solvers:
+            s.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,6 +1,6 @@
 """Writes the committed golden fixtures under tests/golden/.
 
-    python tests/golden/make_golden.py [surrogate]
+    python tests/golden/make_golden.py [surrogate | rs_scenarios]
 
 * prices_*.npz   FP64 prices of the named workloads from the CPU oracle
                  (the reference's FP64 host method restated: generator,
@@ -54,9 +54,24 @@ def write_surrogate():
     print("surrogate", ps.shape, float(ps.min()), float(ps.max()))
 
 
+def write_rs_scenarios(n=32):
+    """FP64 prices of the 2048-state regime-switching scenario set
+    (scenario 0 = the workload, then seeded vol levels / regime rates /
+    rates), backward induction over 252 daily steps, 41 strikes"""
+    cfgs = M.config_rs2048_scenarios(n)
+    ps = np.stack([O.solve_fp64(c) for c in cfgs])  # [n][8][1][41]
+    np.savez(os.path.join(HERE, "prices_rs2048_scenarios.npz"), prices=ps,
+             sigma=np.array([c.sigma_list() for c in cfgs]), q_off=np.array([c.q_off for c in cfgs]),
+             rate=np.array([c.rate for c in cfgs]))
+    print("rs2048 scenarios", ps.shape, float(ps.min()), float(ps.max()))
+
+
 def main():
     if sys.argv[1:] == ["surrogate"]:
         write_surrogate()
+        return
+    if sys.argv[1:] == ["rs_scenarios"]:
+        write_rs_scenarios()
         return
     # MT19937 known answers (C++ [rand.predef]: 10000th output of the
     # default-seeded mt19937 is 4123659995)
@@ -85,6 +100,7 @@ def main():
     write_prices("small_prop", M.config_small(nx=64, nr=2, price_kind=M.OPM_PRICE_PROPAGATE,
                                               nsteps=6))
     write_surrogate()
+    write_rs_scenarios()
 
 
 if __name__ == "__main__":

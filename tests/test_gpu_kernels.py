@@ -157,6 +157,40 @@ def test_fp64_assembly_and_first_step(H):
     assert np.max(np.abs(got.astype(np.float64) - want.astype(np.float64))) <= 6e-8
 
 
+@pytest.mark.parametrize("out_kind", [0, 3, 6])
+def test_sek_rs_batch_equals_single(H, out_kind):
+    """ophip_sek_rs_batch: three lattices of different size and parameters
+    in one launch, every output the bits of its own ophip_sek_rs"""
+    banner(f"sek_rs_batch out_kind={out_kind}")
+    import ctypes as C
+    lats = [(48, 3, [0.2, 0.35, 0.5], 0.5), (64, 2, [0.3, 0.1], 0.8), (100, 1, [0.25], 0.0)]
+    dt, dbl = 2.0**-18, out_kind in (1, 3, 5, 7)
+    ty = np.float64 if dbl else np.float32
+    keep, args = [], []
+    for nx, nr, sig, q in lats:
+        x, _ = O.xgrid(nx, 0.0, 400.0, 100.0)
+        drift = np.tile(0.02 * x, nr)
+        vol = np.concatenate([s * x for s in sig])
+        xd, dd, vd = dev(H, x), dev(H, drift), dev(H, vol)
+        d = nx * nr
+        a, b = H.DeviceArray.empty((d, d), dtype=ty), H.DeviceArray.empty((d, d), dtype=ty)
+        keep += [xd, dd, vd, a, b]
+        args.append((nx, nr, xd.h, dd.h, vd.h, q, a, b))
+    for nx, nr, xh, dh, vh, q, a, _ in args:
+        H.check(H.lib.ophip_sek_rs(nx, nr, xh, dh, vh, q, dt, out_kind, a.h))
+    n = len(args)
+
+    def arr(t, i):
+        return (t * n)(*[r[i] for r in args])
+    H.check(H.lib.ophip_sek_rs_batch(n, arr(C.c_int, 0), arr(C.c_int, 1), arr(C.c_uint, 2),
+                                     arr(C.c_uint, 3), arr(C.c_uint, 4), arr(C.c_double, 5),
+                                     (C.c_double * n)(*[dt] * n), out_kind,
+                                     (C.c_uint * n)(*[r[7].h for r in args])))
+    H.sync()
+    for r in args:
+        assert np.array_equal(r[6].download(), r[7].download())
+
+
 # ---- GEMM ---------------------------------------------------------------------
 
 @pytest.mark.parametrize("m,n,k,lda,ldb,ldc,alpha,beta", [

@@ -199,3 +199,69 @@ def test_sweep_batch_equals_single(M):
         s.solve()
         assert np.array_equal(s.prices()[0], pb[i]), i
         s.close()
+
+
+@pytest.mark.parametrize("prec", [1, 2, 3])
+def test_group_equals_single_small(M, prec):
+    """opm_group: three scenarios of one small regime lattice (different
+    vols, regime rates, rates) squared in one launch per round and stepped
+    in one launch per step give each member the bits of its own opm_solve;
+    within the method's bound of the FP64 oracle"""
+    banner(f"group small prec={prec}")
+    cfgs = []
+    for i, (lv, q, r) in enumerate([(1.0, 0.5, 0.03), (0.8, 1.0, 0.0), (1.3, 0.2, 0.05)]):
+        c = M.config_small(nx=64, nr=2, nsteps=12)
+        for k in range(c.nr):
+            c.sigma_r[k] *= lv
+        c.q_off, c.rate, c.precision = q, r, prec
+        cfgs.append(c)
+    g = M.Group(cfgs)
+    g.solve()
+    pg = g.prices()
+    for i, c in enumerate(cfgs):
+        s = M.Solver(c)
+        s.solve()
+        assert np.array_equal(s.prices(), pg[i]), i
+        s.close()
+        e = report(f"group small prec={prec} member {i}", pg[i], O.solve_fp64(c))
+        assert e.max() < (1e-3 if prec == 1 else 3e-5)
+    g.close()
+
+
+def test_group_rs2048_vs_fp64(M):
+    """four scenarios of the 2048-state regime lattice (the workload, then
+    seeded vol levels, regime rates and rates) priced together in FP16 x3:
+    each within 1e-5 of the FP64 oracle (fixture); the first one the bits of
+    the single-scenario solve when both use the same backward-step kernel
+    (64-row blocks), and within FP32 accumulation-order rounding of the
+    single solve's own default kernel; a captured group replays to the same
+    bits"""
+    banner("group rs2048 x4 FP16x3")
+    from oplib_amd import ophip as H
+    gold = np.load(os.path.join(GOLD, "prices_rs2048_scenarios.npz"))["prices"]
+    cfgs = M.config_rs2048_scenarios(4)
+    g = M.Group(cfgs)
+    g.solve()
+    pg = g.prices()
+    e = report("group rs2048 x4", pg, gold[:4])
+    assert np.all(np.isfinite(pg)) and e.max() < 1e-5
+    s = M.Solver(cfgs[0])
+    s.solve()
+    p1 = s.prices()
+    assert np.abs(p1 - pg[0]).max() <= 1e-5 * np.abs(p1).max()
+    old = H.lib.ophip_wsq_set_bwd_kernel(3)
+    try:
+        s.solve()
+        g.solve()
+        assert np.array_equal(s.prices(), g.prices()[0])
+    finally:
+        H.lib.ophip_wsq_set_bwd_kernel(old)
+    s.close()
+    g.solve()
+    g.capture()
+    for rep in range(2):
+        g.launch()
+        assert np.array_equal(g.prices(), pg), rep
+        g.solve()
+        assert np.array_equal(g.prices(), pg)
+    g.close()

@@ -9,6 +9,7 @@ largest entry); the diagonal fix leaves FP64 row sums of 0 up to FP32
 rounding of the diagonal; a batch gives the same bits as the lattices
 squared alone.
 """
+import functools
 import math
 import sys
 
@@ -278,14 +279,27 @@ def unfrag(a, nrows, K):
     return a.transpose(0, 3, 1, 2, 4).reshape(nrows, K)
 
 
+BWD_KERNELS = {0: "dense", 2: "16-row tiles", 3: "64-row blocks", 4: "32-row blocks"}
+
+
+@pytest.mark.parametrize("kernel", sorted(BWD_KERNELS))
 @pytest.mark.parametrize("mode", [1, 2])
-def test_wform_backward_step(hip, mode):
+def test_wform_backward_step(hip, mode, kernel):
     """Y = df (X + W X) from the state's hi/lo copies (fragment-major): against
     the same arithmetic in numpy (FP64 accumulation, 1e-6 of max|Y|),
     against df (X + W X) with the FP32 W (2e-5), and X's / Y's 16-bit
-    copies bit-exact"""
-    banner(f"wsq backward step mode={mode}")
+    copies bit-exact; every step kernel (the dense one and the band-sparse
+    ones visiting the listed fragments only)"""
+    banner(f"wsq backward step mode={mode} kernel={BWD_KERNELS[kernel]}")
     H = hip
+    old = H.lib.ophip_wsq_set_bwd_kernel(kernel)
+    try:
+        _backward_step_case(H, mode)
+    finally:
+        H.lib.ophip_wsq_set_bwd_kernel(old)
+
+
+def _backward_step_case(H, mode):
     L = Lattice(H, 128, 2, mode)  # d = 256
     p = plan(H, [L], 128)
     L.first_step()
@@ -327,6 +341,119 @@ def test_wform_backward_step(hip, mode):
     bad = np.argwhere(yt[1] != yl)
     assert len(bad) == 0, (len(bad), [(float(Y.T[tuple(b)]), int(yh[tuple(b)]), int(yt[1][tuple(b)]),
                                        int(yl[tuple(b)])) for b in bad[:4]])
+
+
+def tile_lists(hf, lf, d):
+    """numpy: per 16-row tile, the k tiles (32 wide) whose H or L fragment
+    holds a value other than +-0"""
+    nz = ((unfrag(hf, d, d) & 0x7FFF) != 0) | ((unfrag(lf, d, d) & 0x7FFF) != 0)
+    t = nz.reshape(d // 16, 16, d // 32, 32).any(axis=(1, 3))
+    return [np.nonzero(t[i])[0] for i in range(d // 16)]
+
+
+def backward_lattice(H, nsq, mode=1, nx=128, nr=2):
+    L = Lattice(H, nx, nr, mode)
+    p = plan(H, [L], 128)
+    L.first_step()
+    H.check(H.lib.ophip_wsq_exec(p, nsq))
+    H.sync()
+    H.lib.ophip_wsq_plan_free(p)
+    return L
+
+
+@pytest.mark.parametrize("nsq", [2, 9])
+def test_wform_backward_tile_lists(hip, nsq):
+    """the tile lists built with the fragment layout are the nonzero 16 x 32
+    fragments of W's hi/lo copies (numpy on the downloaded fragments); a
+    banded W after few squarings skips most of them"""
+    banner(f"wsq backward tile lists nsq={nsq}")
+    H, mode = hip, 1
+    L = backward_lattice(H, nsq, mode)
+    d, buf = L.d, nsq & 1
+    H.check(H.lib.ophip_wsq_bwd_layout(L.state.h, d, mode, buf))
+    H.sync()
+    hf = L.arr(1 - buf, 2, np.uint16, "F").reshape(-1, order="F")
+    lf = L.arr(1 - buf, 4, np.uint16, "F").reshape(-1, order="F")
+    ref = tile_lists(hf, lf, d)
+    h = H.lib.ophip_wsq_array(L.state.h, d, mode, 0, 7)
+    nkt = d // 32
+    nl = d // 16 + d // 64 + d // 32  # lists per 16-row tile, 64-row block, 32-row block
+    tl = np.empty(nl * (nkt + 1), dtype=np.int32)
+    H.check(H.lib.opcuda_memcpy_d2h(h, tl.ctypes.data, tl.nbytes))
+    tl = tl.reshape(nl, nkt + 1)
+    blocks = [[i] for i in range(d // 16)]
+    blocks += [list(range(4 * b, 4 * b + 4)) for b in range(d // 64)]
+    blocks += [[2 * b, 2 * b + 1] for b in range(d // 32)]
+    for i, rows in enumerate(blocks):
+        want = functools.reduce(np.union1d, [ref[k] for k in rows])
+        assert tl[i, 0] == len(want), (i, tl[i, 0], len(want))
+        assert np.array_equal(tl[i, 1:1 + tl[i, 0]], want), i
+    frac = sum(len(r) for r in ref) / ((d // 16) * nkt)
+    print(f"nonzero fragment fraction {frac:.3f}")
+    if nsq == 2:
+        assert frac < 0.5
+
+
+@pytest.mark.parametrize("kernel", [2, 3, 4])
+def test_wform_backward_batch_equals_single(hip, kernel):
+    """one batched launch over three lattices (different squaring counts,
+    values and discounts) gives each lattice the bits of its own single
+    step, and stays within FP32 accumulation rounding of the dense kernel"""
+    banner(f"wsq backward step batch vs single vs dense, kernel={BWD_KERNELS[kernel]}")
+    H = hip
+    old = H.lib.ophip_wsq_set_bwd_kernel(kernel)
+    try:
+        _batch_case(H)
+    finally:
+        H.lib.ophip_wsq_set_bwd_kernel(old)
+
+
+def _batch_case(H):
+    import ctypes as C
+    mode, ncp, buf = 1, 48, 1
+    nsqs, dfs = (3, 5, 9), (0.9999, 0.999, 0.99)
+    lats = [backward_lattice(H, n, mode) for n in nsqs]  # odd counts end in buffer 1
+    d, n = lats[0].d, len(lats)
+    rs = np.random.RandomState(11)
+    Xd, XT, Yd, YT = [], [], [], []
+    for L in lats:
+        H.check(H.lib.ophip_wsq_bwd_layout(L.state.h, d, mode, buf))
+        Xd.append(H.DeviceArray.from_numpy(rs.uniform(0.0, 300.0, size=(d, ncp)).astype(np.float32),
+                                           order="C"))
+        XT.append(H.DeviceArray.empty(2 * ncp * d, dtype=np.uint16))
+        Yd.append(H.DeviceArray.empty(d * ncp))
+        YT.append(H.DeviceArray.empty(2 * ncp * d, dtype=np.uint16))
+        H.check(H.lib.ophip_wsq_bwd_prep(d, ncp, mode, Xd[-1].h, XT[-1].h))
+    single = []
+    for i, L in enumerate(lats):
+        H.check(H.lib.ophip_wsq_bwd_step(L.state.h, d, mode, buf, ncp, np.float32(dfs[i]), Xd[i].h,
+                                         XT[i].h, Yd[i].h, YT[i].h))
+        H.sync()
+        single.append((Yd[i].download().copy(), YT[i].download().copy()))
+        H.lib.opcuda_ssetall(Yd[i].h, d * ncp, np.float32(-1.0), 1)
+    H.check()
+
+    def arr(ty, vals):
+        return (ty * n)(*vals)
+    H.check(H.lib.ophip_wsq_bwd_step_batch(
+        n, arr(C.c_uint, [L.state.h for L in lats]), d, mode, buf, ncp, arr(C.c_float, dfs),
+        arr(C.c_uint, [a.h for a in Xd]), arr(C.c_uint, [a.h for a in XT]),
+        arr(C.c_uint, [a.h for a in Yd]), arr(C.c_uint, [a.h for a in YT])))
+    H.sync()
+    for i in range(n):
+        y1, yt1 = single[i]
+        assert np.array_equal(Yd[i].download(), y1), i
+        assert np.array_equal(YT[i].download(), yt1), i
+    kern = H.lib.ophip_wsq_set_bwd_kernel(0)
+    try:
+        for i, L in enumerate(lats):
+            y1 = single[i][0]
+            H.check(H.lib.ophip_wsq_bwd_step(L.state.h, d, mode, buf, ncp, np.float32(dfs[i]),
+                                             Xd[i].h, XT[i].h, Yd[i].h, YT[i].h))
+            H.sync()
+            assert np.abs(Yd[i].download() - y1).max() <= 1e-6 * np.abs(y1).max(), i
+    finally:
+        H.lib.ophip_wsq_set_bwd_kernel(kern)
 
 
 def test_wform_backward_layout_perm(hip):
@@ -496,3 +623,41 @@ def test_zero_cell_skip_same_bits(hip, mode, tile):
     finally:
         H.lib.ophip_wsq_set_skip(old)
         H.lib.ophip_wsq_plan_free(p)
+
+
+def test_forward_steps_skip_zero_cells(hip):
+    """the forward steps p_k = p_{k-1} (I + W) visiting only W's flagged
+    cells (ophip_fwd_set_cells) give the bits of the dense steps, on a
+    banded W with zero cells"""
+    banner("forward steps with zero-cell skipping")
+    import ctypes as C
+    H, mode, nsq = hip, 1, 2
+    L = backward_lattice(H, nsq, mode, nx=512, nr=1)  # one regime: a band, zero cells off it
+    d, buf = L.d, nsq & 1
+    K = H.lib.ophip_wsq_w(L.state.h, d, mode, buf)
+    fl = H.lib.ophip_wsq_array(L.state.h, d, mode, buf, 6)
+    flags = np.empty((d // 64) ** 2, dtype=np.uint8)
+    H.check(H.lib.opcuda_memcpy_d2h(fl, flags.ctypes.data, flags.nbytes))
+    assert (flags == 0).any() and (flags != 0).any()
+    nmat, ks = 5, np.array([80.0, 100.0, 120.0], dtype=np.float32)
+    x = np.linspace(0.0, 400.0, d).astype(np.float32)
+    xd = H.DeviceArray.from_numpy(x)
+    outs = []
+    for cells in (False, True):
+        P0 = np.zeros((nmat + 1) * d, dtype=np.float32)
+        P0[300] = 1.0  # x = 235, in the money for every strike
+        Pd, od = H.DeviceArray.from_numpy(P0), H.DeviceArray.empty(nmat * len(ks))
+        plan = H.lib.ophip_fwd_plan(1, (C.c_int * 1)(d), (C.c_uint * 1)(K), (C.c_int * 1)(1),
+                                    (C.c_uint * 1)(Pd.h), (C.c_uint * 1)(xd.h), (C.c_uint * 1)(od.h),
+                                    (C.c_float * 1)(0.02), np.float32(0.1), nmat,
+                                    (C.c_float * len(ks))(*ks), len(ks))
+        assert plan
+        if cells:
+            H.check(H.lib.ophip_fwd_set_cells(plan, (C.c_uint * 1)(fl)))
+        H.check(H.lib.ophip_fwd_exec(plan))
+        H.sync()
+        outs.append((Pd.download().copy(), od.download().copy()))
+        H.lib.ophip_fwd_plan_free(plan)
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    assert np.all(np.isfinite(outs[0][1])) and outs[0][1].max() > 0
