@@ -72,7 +72,7 @@ def roofline_backward(best, methods, wbk, bi_gbs, bi_us, bi_bytes):
     if best.startswith("wform_fp16x3"):
         nb = methods[best].get("group", 1)
         w = wbk.get(nb) or wbk[1]
-        key = f"k_wbwd_sp/{nb}"
+        key = f"{w['kernel'].split('<')[0]}/{nb}"
         return {"bound": "hbm", "kernel": w["kernel"], "achieved": w["GBps"], "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": w["GBps"] / HBM_PEAK_GBS,
                 "traffic": traffic_from_profiles(key),
@@ -105,7 +105,7 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="skip the calibration-sweep leg")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate leg")
     ap.add_argument("--no-fp8", action="store_true", help="skip the FP8 W-form method")
-    ap.add_argument("--groups", type=int, nargs="*", default=[8, 16],
+    ap.add_argument("--groups", type=int, nargs="*", default=[16],
                     help="group sizes of the FP16 x3 scenario-group method")
     ap.add_argument("--probe-group", type=int, default=16,
                     help="lattices per backward-step launch in --probe")
@@ -137,7 +137,9 @@ def main():
         for _ in range(20):
             H.check(H.lib.opcuda_scopy1(B.h, A.h, n), "scopy1")
         H.sync()
-        print(json.dumps({"probe": f"k_wbwd_sp ({nb} lattices per launch) x20, k_copy4 (64 MiB) x20",
+        print(json.dumps({"probe": f"FP16 x3 backward step ({nb} lattices per launch, kernel "
+                                   f"{H.lib.ophip_wsq_bwd_kernel_for(nb, 2048, 48)}) x20, "
+                                   "k_copy4 (64 MiB) x20",
                           "step_bytes_nonzero": wb.bytes, "step_bytes_dense": wb.bytes_dense,
                           "nonzero_fragment_fraction": wb.nz_frac,
                           "copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
@@ -223,8 +225,12 @@ def main():
             wb.step()
         us = timed_events(H, wb.step, 100) * 1e3
         H.check()
-        wbk[nb] = {"kernel": f"k_wbwd_sp<FP16, 16 waves, 3> ({nb} lattice{'s' if nb > 1 else ''} "
-                             "per launch)",
+        kid = H.lib.ophip_wsq_bwd_kernel_for(nb, d, 48)
+        kname = {0: "k_wbwd1<FP16, 3, 8> (dense)", 2: "k_wbwd_sp<FP16, 16, 3> (16-row tiles)",
+                 3: "k_wbwd_blk<FP16, 4, 4, 2> (64-row blocks)",
+                 4: "k_wbwd_blk<FP16, 2, 8, 2> (32-row blocks)"}[kid]
+        wbk[nb] = {"kernel": f"{kname}, {nb} lattice{'s' if nb > 1 else ''} per launch",
+                   "kernel_id": kid,
                    "avg_launch_us": us, "bytes_per_launch": wb.bytes,
                    "dense_bytes_per_launch": wb.bytes_dense,
                    "nonzero_fragment_fraction": wb.nz_frac,

@@ -224,6 +224,9 @@ int ophip_wsq_bwd_step_batch(int nb, const unsigned *states, int d, int mode, in
                              const float *df, const unsigned *X, const unsigned *XT,
                              const unsigned *Y, const unsigned *YT);
 int ophip_wsq_set_bwd_kernel(int k);
+/* the kernel (0, 2, 3, 4 as above) the current setting gives a step of nb
+ * lattices of size d with ncp value columns */
+int ophip_wsq_bwd_kernel_for(int nb, int d, int ncp);
 /* the same plan with a tile per lattice (64 or 128 each): one persistent
  * launch per tile class and round */
 unsigned long ophip_wsq_plan_tiles(int n, const unsigned *states, const int *ds, const int *nsqs,

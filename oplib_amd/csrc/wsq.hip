@@ -1664,6 +1664,21 @@ int g_bwd_kernel = 1;
 OPHIP_EXPORT int ophip_wsq_bwd_step(unsigned state, int d, int mode, int buf, int ncp, float df,
                                     unsigned X, unsigned XT, unsigned Y, unsigned YT);
 
+namespace {
+// the kernel a batched step of nb lattices uses (2, 3 or 4; 0 = dense)
+int bwd_kernel_for(int nb, int d, int ncp) {
+    if (g_bwd_kernel == 0) return 0;
+    if (ncp != 48) return 2;
+    if (g_bwd_kernel != 1) return g_bwd_kernel;
+    return std::min(nb, kBwdBatch) * (d / 32) >= 256 ? 4 : 2;
+}
+}  // namespace
+
+// the backward-step kernel the current setting gives nb lattices of size d
+// with ncp value columns: 0 dense, 2 16-row tiles, 3 64-row blocks, 4
+// 32-row blocks
+OPHIP_EXPORT int ophip_wsq_bwd_kernel_for(int nb, int d, int ncp) { return bwd_kernel_for(nb, d, ncp); }
+
 OPHIP_EXPORT int ophip_wsq_set_bwd_kernel(int k) {
     const int old = g_bwd_kernel;
     if (k >= 0 && k <= 4) g_bwd_kernel = k;
@@ -1700,9 +1715,7 @@ OPHIP_EXPORT int ophip_wsq_bwd_step_batch(int nb, const unsigned *states, int d,
     // or more (measured at d = 2048: 1.95 us per lattice and step for 16
     // lattices, against 2.35 us with 64-row blocks and 5.1 us with 16-row
     // tiles), 16-row tiles (more, smaller workgroups) below that
-    int kern = g_bwd_kernel;
-    if (kern == 1) kern = (ncp == 48 && std::min(nb, kBwdBatch) * (d / 32) >= 256) ? 4 : 2;
-    if (ncp != 48) kern = 2;
+    const int kern = bwd_kernel_for(nb, d, ncp);
     for (int i0 = 0; i0 < nb; i0 += kBwdBatch) {
         const int m = std::min(kBwdBatch, nb - i0);
         BwdBatch bt{};
