@@ -109,8 +109,8 @@ def main():
                     help="group sizes of the FP16 x3 scenario-group method")
     ap.add_argument("--probe-group", type=int, default=16,
                     help="lattices per backward-step launch in --probe")
-    ap.add_argument("--surr-train", type=int, default=131072, help="surrogate training draws")
-    ap.add_argument("--surr-steps", type=int, default=80000, help="surrogate Adam steps")
+    ap.add_argument("--surr-train", type=int, default=262144, help="surrogate training draws")
+    ap.add_argument("--surr-steps", type=int, default=120000, help="surrogate Adam steps")
     ap.add_argument("--probe", action="store_true",
                     help="only launch the dominant kernel (W-form backward step) 20 times and a "
                          "64 MiB copy kernel 20 times, for counter collection")

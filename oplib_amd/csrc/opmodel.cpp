@@ -696,6 +696,41 @@ void opm_batch_destroy(opm_batch *b) {
     delete b;
 }
 
+static int batch_body(opm_batch *b);
+
+// The squaring plan of a batch, picked on the batch itself: the
+// autotuner's plan (timed per size bucket with the dense K loop) against
+// uniform 128- and 64-tile plans, each timed over two whole solves with
+// zero-cell skipping; the fastest is kept.  Runs the solve (outputs are
+// overwritten by the caller's next solve).
+static bool batch_pick_plan(opm_batch *b) {
+    const int n = b->n;
+    std::vector<unsigned> st(n);
+    for (int i = 0; i < n; i++) st[i] = b->state[i].h;
+    std::vector<unsigned long> cand = {b->wplan};
+    for (int t : {128, 64}) {
+        const unsigned long p = ophip_wsq_plan(n, st.data(), b->d.data(), b->nsq.data(), b->wmode, t, 0);
+        if (p) cand.push_back(p);
+    }
+    double best = -1.0;
+    size_t bi = 0;
+    for (size_t c = 0; c < cand.size(); c++) {
+        b->wplan = cand[c];
+        if (batch_body(b)) return false;  // warm
+        if (ophip_event_record(14) || batch_body(b) || batch_body(b) || ophip_event_record(15)) return false;
+        const double ms = ophip_event_elapsed_ms(14, 15);
+        if (ms < 0.0) return false;
+        if (best < 0.0 || ms < best) {
+            best = ms;
+            bi = c;
+        }
+    }
+    for (size_t c = 0; c < cand.size(); c++)
+        if (c != bi) ophip_wsq_plan_free(cand[c]);
+    b->wplan = cand[bi];
+    return true;
+}
+
 opm_batch *opm_batch_create(const opm_config *cfgs, int n, int tile) {
     if (!cfgs || n <= 0) return nullptr;
     if (opcuda_cublas_init() != 0) return nullptr;
@@ -812,6 +847,7 @@ opm_batch *opm_batch_create(const opm_config *cfgs, int n, int tile) {
             ok = ophip_fwd_set_cells(b->fplan, fl.data()) == 0;
         }
     }
+    if (ok && b->wmode >= 0 && tile == 0) ok = batch_pick_plan(b);
     if (!ok) {
         opm_batch_destroy(b);
         return nullptr;

@@ -500,35 +500,52 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
 // the xcd_order placement); the list order within an XCD does not change
 // any result.  Requires d <= 4096 (64 cells).
 template <int TPC>
-__global__ __launch_bounds__(256) void k_wlive(char *__restrict__ base, const WScen *__restrict__ scen,
-                                               const WItem *__restrict__ items, int nitems,
-                                               WLive *__restrict__ live, int cap, int *__restrict__ counts,
-                                               int wskip) {
-    const int lane = threadIdx.x & 63;
-    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (j >= nitems) return;  // wave-uniform
-    const WItem it = items[j];
-    const WScen sc = scen[it.s];
-    const int nc = sc.d / 64;
-    const uint8_t *fl = at<const uint8_t>(base, sc.fl);
-    int fa = 0, fb = 0;
-    if (lane < nc) {
+__global__ __launch_bounds__(1024) void k_wlive(char *__restrict__ base, const WScen *__restrict__ scen,
+                                                const WItem *__restrict__ items, int nitems,
+                                                WLive *__restrict__ live, int cap, int *__restrict__ counts,
+                                                int wskip) {
+    // block (x, chunk): the 16 items j = x + 8 (16 chunk + w) of XCD x, one
+    // per wave w; one atomic per block reserves their slots in list x
+    __shared__ int slive[16];
+    __shared__ int sbase;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int x = blockIdx.x & 7;
+    const int j = x + 8 * (16 * (int)(blockIdx.x >> 3) + w);
+    bool ok = false;
+    uint64_t cells = 0;
+    WItem it{};
+    if (j < nitems) {  // wave-uniform
+        it = items[j];
+        const WScen sc = scen[it.s];
+        const int nc = sc.d / 64;
+        const uint8_t *fl = at<const uint8_t>(base, sc.fl);
+        int fa = 0, fb = 0;
+        if (lane < nc) {
 #pragma unroll
-        for (int q = 0; q < TPC; q++) {
-            fa |= fl[(size_t)(it.tm * TPC + q) * nc + lane];
-            fb |= fl[(size_t)lane * nc + it.tn * TPC + q];
+            for (int q = 0; q < TPC; q++) {
+                fa |= fl[(size_t)(it.tm * TPC + q) * nc + lane];
+                fb |= fl[(size_t)lane * nc + it.tn * TPC + q];
+            }
         }
+        cells = __ballot(fa != 0 && fb != 0);
+        bool need = (lane >= it.tn * TPC && lane < it.tn * TPC + TPC && fa != 0) || !wskip;
+        if (lane < TPC * TPC)
+            need = need || at<const uint8_t>(base, sc.fln)[(size_t)(it.tm * TPC + lane / TPC) * nc +
+                                                          it.tn * TPC + lane % TPC] != 0;
+        ok = cells != 0 || __ballot(need) != 0;
     }
-    const uint64_t cells = __ballot(fa != 0 && fb != 0);
-    bool need = (lane >= it.tn * TPC && lane < it.tn * TPC + TPC && fa != 0) || !wskip;
-    if (lane < TPC * TPC)
-        need = need || at<const uint8_t>(base, sc.fln)[(size_t)(it.tm * TPC + lane / TPC) * nc + it.tn * TPC +
-                                                       lane % TPC] != 0;
-    if (cells == 0 && __ballot(need) == 0) return;
-    if (lane == 0) {
-        const int x = j & 7;
-        const int p = atomicAdd(&counts[x], 1);
-        live[(size_t)x * cap + p] = WLive{it.s, it.tm, it.tn, 0, cells, 0};
+    if (lane == 0) slive[w] = ok ? 1 : 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int n = 0;
+        for (int q = 0; q < 16; q++) n += slive[q];
+        sbase = n ? atomicAdd(&counts[x], n) : 0;
+    }
+    __syncthreads();
+    if (ok && lane == 0) {
+        int rank = 0;
+        for (int q = 0; q < w; q++) rank += slive[q];
+        live[(size_t)x * cap + sbase + rank] = WLive{it.s, it.tm, it.tn, 0, cells, 0};
     }
 }
 
@@ -553,27 +570,26 @@ __global__ void k_wfix(char *__restrict__ base, const WScen *__restrict__ scen, 
     // slots of zero cells (flag 0) hold no partial sum of this round (the
     // squaring may have skipped them: stale values, selected away) and add 0
     const uint8_t *fl = at<const uint8_t>(base, sc.fln) + (size_t)(i >> 6) * nslots;
-    double rs = 0.0;
-    // loads issued eight at a time, added in slot order
-    int q = 0;
-    for (; q + 8 <= nslots; q += 8) {
-        double v[8];
-        uint8_t f[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            v[u] = part[(size_t)(q + u) * d + i];
-            f[u] = fl[q + u];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; u++) rs += f[u] ? v[u] : 0.0;
-    }
-    for (; q < nslots; q++) {
-        const double v = part[(size_t)q * d + i];
-        rs += fl[q] ? v : 0.0;
-    }
     const size_t e = (size_t)i + (size_t)d * i;
     float *Wn = at<float>(base, sc.Wn);
-    const float w = (float)((double)Wn[e] - rs);
+    const float w0 = Wn[e];  // issued with the partial sums' loads
+    double rs = 0.0;
+    // loads issued sixteen at a time (every slot of d <= 1024 in one
+    // round trip), added in slot order; nslots is even
+    for (int q = 0; q < nslots; q += 16) {
+        double v[16];
+        uint8_t f[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int k = q + u < nslots ? q + u : nslots - 1;  // clamped: never added
+            v[u] = part[(size_t)k * d + i];
+            f[u] = fl[k];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+            if (q + u < nslots) rs += f[u] ? v[u] : 0.0;
+    }
+    const float w = (float)((double)w0 - rs);
     Wn[e] = w;
     const float ws = w * N::S;
     const uint16_t h = N::enc(ws);
@@ -778,60 +794,67 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd1(int d, float df, const uint16
 // Band-sparse backward step.  Outside a band around the diagonal of each
 // regime block the squared W is below 2^-51, where its FP16 hi and lo
 // copies are both zero: at the 2048-state lattice (d = 2048, U(1/252))
-// only 42% of the 16 x 32 fragments hold a nonzero value.  k_wbwd_tiles
-// lists, per 16-row tile, the k tiles whose H or L fragment is nonzero;
-// k_wbwd_sp visits only those (a skipped fragment's products are exact
-// zeros).  List of row tile it: tl[it (nkt + 1)] = count, then the k tiles
-// in increasing order.
+// only 42% of the 16 x 32 fragments hold a nonzero value.  k_wbwd_flags
+// marks the fragments whose H or L part is nonzero and k_wbwd_lists lists
+// their k tiles; the step kernels visit only those (a skipped fragment's
+// products are exact zeros).
 // ---------------------------------------------------------------------
 // Lists: per 16-row tile (d/16 lists), then per 64-row block (d/64: the
 // union of its four tiles' lists) and per 32-row block (d/32: two tiles),
-// each list nkt + 1 ints.  One workgroup per 64-row block, wave w scanning
-// row tile 4 rb + w.
+// each nkt + 1 ints: the count, then the k tiles in increasing order.
 __device__ inline size_t tl_list4(int d) { return (size_t)(d / 16) * (d / 32 + 1); }
 __device__ inline size_t tl_list2(int d) { return tl_list4(d) + (size_t)(d / 64) * (d / 32 + 1); }
 
-__global__ __launch_bounds__(256) void k_wbwd_tiles(int d, const uint16_t *__restrict__ Hf,
+// k_wbwd_flags: byte flags[it][kt] (d/16 x d/32) = 1 where the hi or lo
+// fragment (it, kt) holds a value other than +-0; a wave per (it, four k
+// tiles), all eight 16-byte loads in flight.
+__global__ __launch_bounds__(256) void k_wbwd_flags(int d, const uint16_t *__restrict__ Hf,
                                                     const uint16_t *__restrict__ Lf,
+                                                    uint8_t *__restrict__ flags) {
+    const int nkt = d / 32, lane = threadIdx.x & 63;
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);  // (it, k0 / 4)
+    if (w >= d / 16 * (nkt / 4)) return;  // wave-uniform
+    const int it = w / (nkt / 4), k0 = 4 * (w % (nkt / 4));
+    uint4 h[4], l[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const size_t o = frag_off(it, k0 + u, lane, nkt);
+        h[u] = *(const uint4 *)(Hf + o);
+        l[u] = *(const uint4 *)(Lf + o);
+    }
+    unsigned word = 0;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        // +-0 counts as zero: the sign bits of both 16-bit halves dropped
+        const unsigned v =
+            (h[u].x | h[u].y | h[u].z | h[u].w | l[u].x | l[u].y | l[u].z | l[u].w) & 0x7fff7fffu;
+        word |= (__any(v != 0u) ? 1u : 0u) << (8 * u);
+    }
+    if (lane == 0) *(unsigned *)(flags + (size_t)it * nkt + k0) = word;
+}
+
+// k_wbwd_lists: one thread per list (d/16 row tiles, then d/64 64-row
+// blocks, then d/32 32-row blocks), the union of its row tiles' flags in
+// k order
+__global__ __launch_bounds__(256) void k_wbwd_lists(int d, const uint8_t *__restrict__ flags,
                                                     int *__restrict__ tl) {
-    __shared__ int flag[4][128];  // nkt <= 128 (d <= 4096)
-    const int rb = blockIdx.x, nkt = d / 32, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int it = 4 * rb + wave;
-    for (int k0 = 0; k0 < nkt; k0 += 4) {  // nkt % 4 == 0; four fragments' loads in flight
-        uint4 h[4], l[4];
+    const int nkt = d / 32, n16 = d / 16, n64 = d / 64, n32 = d / 32;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n16 + n64 + n32) return;
+    int r0, nr;
+    if (t < n16) { r0 = t; nr = 1; }
+    else if (t < n16 + n64) { r0 = 4 * (t - n16); nr = 4; }
+    else { r0 = 2 * (t - n16 - n64); nr = 2; }
+    int *out = tl + (size_t)t * (nkt + 1);
+    int n = 0;
+    for (int k0 = 0; k0 < nkt; k0 += 4) {  // nkt % 4 == 0
+        unsigned f = 0;
+        for (int q = 0; q < nr; q++) f |= *(const unsigned *)(flags + (size_t)(r0 + q) * nkt + k0);
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const size_t o = frag_off(it, k0 + u, lane, nkt);
-            h[u] = *(const uint4 *)(Hf + o);
-            l[u] = *(const uint4 *)(Lf + o);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            // +-0 counts as zero: the sign bits of both 16-bit halves dropped
-            const unsigned v = (h[u].x | h[u].y | h[u].z | h[u].w | l[u].x | l[u].y | l[u].z | l[u].w) &
-                               0x7fff7fffu;
-            const int nz = __any(v != 0u);
-            if (lane == 0) flag[wave][k0 + u] = nz;
-        }
+        for (int u = 0; u < 4; u++)
+            if ((f >> (8 * u)) & 0xffu) out[1 + n++] = k0 + u;
     }
-    __syncthreads();
-    const int t = threadIdx.x;
-    if (t < 7) {
-        // t < 4: row tile 4 rb + t; t == 4: the 64-row block; t = 5, 6: the
-        // 32-row blocks 2 rb, 2 rb + 1
-        int *out = t < 4 ? tl + (size_t)(4 * rb + t) * (nkt + 1)
-                 : t == 4 ? tl + tl_list4(d) + (size_t)rb * (nkt + 1)
-                          : tl + tl_list2(d) + (size_t)(2 * rb + t - 5) * (nkt + 1);
-        const int w0 = t < 4 ? t : t == 4 ? 0 : 2 * (t - 5);
-        const int w1 = t < 4 ? t : t == 4 ? 3 : w0 + 1;
-        int n = 0;
-        for (int kt = 0; kt < nkt; kt++) {
-            int f = 0;
-            for (int w = w0; w <= w1; w++) f |= flag[w][kt];
-            if (f) out[1 + n++] = kt;
-        }
-        out[0] = n;
-    }
+    out[0] = n;
 }
 
 // one lattice of a batched backward step (arena offsets)
@@ -962,6 +985,19 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd_blk(char *__restrict__ base, c
     const uint16_t *la = at<const uint16_t>(base, S.Lf) + frag_off(RB * rb, 0, lane, nkt);
     const uint16_t *xh = at<const uint16_t>(base, S.XT) + frag_off(0, 0, lane, nkt);
     const uint16_t *xl = xh + n;
+    // the epilogue's X values (tiles wave, wave + NW, ..) loaded up front
+    constexpr int TPW = (NTILE + NW - 1) / NW;
+    const float *X = at<const float>(base, S.X);
+    float xv[TPW][4];
+#pragma unroll
+    for (int q = 0; q < TPW; q++) {
+        const int tile = wave + q * NW;
+        if (tile < NTILE) {  // uniform
+            const int i = tile / NT, j = tile % NT;
+#pragma unroll
+            for (int e = 0; e < 4; e++) xv[q][e] = X[(size_t)(row0 + 16 * i + 4 * g + e) * NC + 16 * j + r];
+        }
+    }
     f4 acc[RB][NT], cor[RB][NT];
 #pragma unroll
     for (int i = 0; i < RB; i++)
@@ -1006,9 +1042,11 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd_blk(char *__restrict__ base, c
 #pragma unroll
         for (int j = 0; j < NT; j++) red[wave][i * NT + j][lane] = (acc[i][j] + cor[i][j] * N::IG) * N::IS;
     __syncthreads();
-    const float *X = at<const float>(base, S.X);
     float *Y = at<float>(base, S.Y);
-    for (int tile = wave; tile < NTILE; tile += NW) {  // uniform
+#pragma unroll
+    for (int q = 0; q < TPW; q++) {
+        const int tile = wave + q * NW;
+        if (tile >= NTILE) break;  // uniform
         const int i = tile / NT, j = tile % NT;
         f4 sum = red[0][tile][lane];
 #pragma unroll
@@ -1018,7 +1056,7 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd_blk(char *__restrict__ base, c
         for (int e = 0; e < 4; e++) {
             const int row = 16 * i + 4 * g + e;
             const size_t o = (size_t)(row0 + row) * NC + col;
-            float y = S.df * (X[o] + sum[e]);
+            float y = S.df * (xv[q][e] + sum[e]);
             asm volatile("" : "+v"(y));  // the copies split the rounded y
             Y[o] = y;
             const uint16_t h = N::enc(y);
@@ -1052,7 +1090,7 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd_blk(char *__restrict__ base, c
 // mode: 0 = FP16 one product, 1 = FP16 three products (hi/lo),
 //       2 = BF16 three products
 struct WsqLayout {
-    size_t W0, W1, H[2][4], part, fl[2], tl, total;  // byte offsets; H[buf][Hr,Hc,Lr,Lc]
+    size_t W0, W1, H[2][4], part, fl[2], tl, tf, total;  // byte offsets; H[buf][Hr,Hc,Lr,Lc]
 };
 
 static WsqLayout wsq_layout(int d, int mode) {
@@ -1072,6 +1110,7 @@ static WsqLayout wsq_layout(int d, int mode) {
     for (int b = 0; b < 2; b++) L.fl[b] = take((size_t)(d / 64) * (d / 64));  // cell flags
     // backward-step tile lists: per 16-row tile, 64-row block, 32-row block
     L.tl = take(4 * (size_t)(d / 16 + d / 64 + d / 32) * (d / 32 + 1));
+    L.tf = take((size_t)(d / 16) * (d / 32));  // their fragment flags
     L.total = off;
     return L;
 }
@@ -1399,11 +1438,11 @@ OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
                     lv = P->live[c];
                     cn = P->counts + 8 * c;
                     if (c == 1)
-                        hipLaunchKernelGGL((k_wlive<2>), dim3(cdiv(nit, 4)), dim3(256), 0, D.stream, D.base,
+                        hipLaunchKernelGGL((k_wlive<2>), dim3(8 * cdiv(cdiv(nit, 8), 16)), dim3(1024), 0, D.stream, D.base,
                                            P->sc, its, nit, P->live[c], P->cap[c], P->counts + 8 * c,
                                            rr > 0 ? 1 : 0);
                     else
-                        hipLaunchKernelGGL((k_wlive<1>), dim3(cdiv(nit, 4)), dim3(256), 0, D.stream, D.base,
+                        hipLaunchKernelGGL((k_wlive<1>), dim3(8 * cdiv(cdiv(nit, 8), 16)), dim3(1024), 0, D.stream, D.base,
                                            P->sc, its, nit, P->live[c], P->cap[c], P->counts + 8 * c,
                                            rr > 0 ? 1 : 0);
                 }
@@ -1612,6 +1651,18 @@ OPHIP_EXPORT int ophip_wsq_bwd_prep(int d, int ncp, int mode, unsigned X, unsign
     });
 }
 
+namespace {
+// the tile lists of fragment copies Hf/Lf (after their layout)
+void bwd_tile_lists(Device &D, const WsqLayout &L, unsigned state, int d, const uint16_t *Hf,
+                    const uint16_t *Lf) {
+    uint8_t *flags = P<uint8_t>(state + (unsigned)L.tf);
+    const int waves = d / 16 * (d / 128);
+    hipLaunchKernelGGL(k_wbwd_flags, dim3(cdiv(waves, 4)), dim3(256), 0, D.stream, d, Hf, Lf, flags);
+    hipLaunchKernelGGL(k_wbwd_lists, dim3(cdiv(d / 16 + d / 64 + d / 32, 256)), dim3(256), 0, D.stream, d,
+                       (const uint8_t *)flags, P<int>(state + (unsigned)L.tl));
+}
+}  // namespace
+
 // W's hi/lo copies of buffer `buf`, fragment-major, into the column-major
 // slots of the other buffer (free once the squarings are done); once per
 // solve, before the steps
@@ -1627,8 +1678,7 @@ OPHIP_EXPORT int ophip_wsq_bwd_layout(unsigned state, int d, int mode, int buf) 
     const size_t nf = (size_t)d * d / 8;
     hipLaunchKernelGGL((k_wfrag<true>), dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, D.stream, d, Hr,
                        Lr, Hf, Lf);
-    hipLaunchKernelGGL(k_wbwd_tiles, dim3(d / 64), dim3(256), 0, D.stream, d, Hf, Lf,
-                       P<int>(state + (unsigned)L.tl));
+    bwd_tile_lists(D, L, state, d, Hf, Lf);
     return (int)check_launch();
 }
 
@@ -1650,8 +1700,7 @@ OPHIP_EXPORT int ophip_wsq_bwd_layout_perm(unsigned state, int d, int mode, int 
     const size_t nf = (size_t)d * d / 8;
     hipLaunchKernelGGL(k_wfragp, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, D.stream, d, Hr, Lr,
                        P<const int>(perm), Hf, Lf);
-    hipLaunchKernelGGL(k_wbwd_tiles, dim3(d / 64), dim3(256), 0, D.stream, d, Hf, Lf,
-                       P<int>(state + (unsigned)L.tl));
+    bwd_tile_lists(D, L, state, d, Hf, Lf);
     return (int)check_launch();
 }
 
@@ -1712,9 +1761,10 @@ OPHIP_EXPORT int ophip_wsq_bwd_step_batch(int nb, const unsigned *states, int d,
     Device &D = dev();
     const int nct = ncp / 16;
     // automatic: 32-row blocks once the lattices give a workgroup per CU
-    // or more (measured at d = 2048: 1.95 us per lattice and step for 16
-    // lattices, against 2.35 us with 64-row blocks and 5.1 us with 16-row
-    // tiles), 16-row tiles (more, smaller workgroups) below that
+    // or more (measured at d = 2048: 1.89 us per lattice and step for 16
+    // lattices, against 2.30 us with 64-row blocks and 5.1 us with 16-row
+    // tiles; 3 tiles in flight per wave 2.27 us, 4 waves with 4 tiles in
+    // flight 1.99 us), 16-row tiles (more, smaller workgroups) below that
     const int kern = bwd_kernel_for(nb, d, ncp);
     for (int i0 = 0; i0 < nb; i0 += kBwdBatch) {
         const int m = std::min(kBwdBatch, nb - i0);
