@@ -206,6 +206,10 @@ int ophip_wsq_bwd_layout(unsigned state, int d, int mode, int buf);
  * squared in node-major order stepped in the reference's order (the step's
  * FP32 accumulation depends on the order; its algebra does not) */
 int ophip_wsq_bwd_layout_perm(unsigned state, int d, int mode, int buf, unsigned perm);
+/* the same for the regime permutation perm[x + nx*r] = nr*x + r (node-major
+ * squaring, reference-order steps) without a permutation array, through
+ * LDS tiles: nr in {1, 2, 4, 8}, nx % 8 == 0, nx*nr = d */
+int ophip_wsq_bwd_layout_regime(unsigned state, int d, int mode, int buf, int nx, int nr);
 int ophip_wsq_bwd_prep(int d, int ncp, int mode, unsigned X, unsigned XT);
 int ophip_wsq_bwd_step(unsigned state, int d, int mode, int buf, int ncp, float df, unsigned X,
                        unsigned XT, unsigned Y, unsigned YT);

@@ -305,6 +305,17 @@ void opm_destroy(opm_solver *s) {
     delete s;
 }
 
+// W's fragment layout for the FP16 x3 backward steps: in the reference's
+// state order when the squarings ran node-major (the regime permutation,
+// through LDS tiles when its shape allows, else gathered through perm)
+static int bwd_layout(opm_solver *s, int d, int buf) {
+    if (!s->v_ref) return ophip_wsq_bwd_layout(s->wstate.h, d, s->wmode, buf);
+    const int nx = s->cfg.nx, nr = s->cfg.nr;
+    if (nx % 8 == 0 && (nr == 1 || nr == 2 || nr == 4 || nr == 8))
+        return ophip_wsq_bwd_layout_regime(s->wstate.h, d, s->wmode, buf, nx, nr);
+    return ophip_wsq_bwd_layout_perm(s->wstate.h, d, s->wmode, buf, s->perm.h);
+}
+
 // the solve body: only kernel launches on the library stream (capturable)
 static int solve_body(opm_solver *s) {
     const opm_config *c = &s->cfg;
@@ -345,8 +356,7 @@ static int solve_body(opm_solver *s) {
             // V <- df*(V + W*V) on the FP16 matrix cores from W's hi/lo
             // copies (a BF16 hi/lo pair of V keeps 16 bits, too few over 252
             // steps: the BF16 method steps in FP32)
-            st = s->v_ref ? ophip_wsq_bwd_layout_perm(s->wstate.h, d, s->wmode, s->u_final, s->perm.h)
-                          : ophip_wsq_bwd_layout(s->wstate.h, d, s->wmode, s->u_final);
+            st = bwd_layout(s, d, s->u_final);
             if (st) return st;
             st = ophip_wsq_bwd_prep(d, s->ncp, s->wmode, s->V[0].h, s->VT[0].h);
             if (st) return st;
@@ -528,6 +538,8 @@ struct opm_group {
     unsigned long graph = 0;
 };
 
+static int group_body(opm_group *g);
+
 extern "C" {
 
 opm_group *opm_group_create(opm_solver *const *solvers, int n) {
@@ -548,7 +560,33 @@ opm_group *opm_group_create(opm_solver *const *solvers, int n) {
     std::vector<unsigned> st(n);
     std::vector<int> ds(n, a->d);
     for (int i = 0; i < n; i++) st[i] = solvers[i]->wstate.h;
-    g->wplan = ophip_wsq_plan(n, st.data(), ds.data(), nullptr, a->wmode, 0, 0);
+    // the squaring plan, picked on the group itself: the tile a single
+    // solve of this d uses (ophip_wsq_plan's rule: the same bits) with 1, 2
+    // or 4 persistent workgroups per CU, each timed over two whole group
+    // solves (zero-cell skipping makes the cost depend on the lattices'
+    // bands); the fastest is kept
+    const int ncu = opcuda_multi_processor_count(0) > 0 ? opcuda_multi_processor_count(0) : 256;
+    const int tile = (long)(a->d / 128) * (a->d / 128) >= ncu ? 128 : 64;
+    const int cand[3][2] = {{tile, 2 * ncu}, {tile, ncu}, {tile, 4 * ncu}};
+    double best = -1.0;
+    unsigned long keep = 0;
+    for (const auto &c : cand) {
+        g->wplan = ophip_wsq_plan(n, st.data(), ds.data(), nullptr, a->wmode, c[0], c[1]);
+        if (!g->wplan) continue;
+        double ms = -1.0;
+        if (!group_body(g) && !ophip_event_record(14) && !group_body(g) && !group_body(g) &&
+            !ophip_event_record(15))
+            ms = ophip_event_elapsed_ms(14, 15);
+        if (ms >= 0.0 && (best < 0.0 || ms < best)) {
+            if (keep) ophip_wsq_plan_free(keep);
+            keep = g->wplan;
+            best = ms;
+        } else {
+            ophip_wsq_plan_free(g->wplan);
+        }
+        g->wplan = 0;
+    }
+    g->wplan = keep;
     if (!g->wplan) {
         delete g;
         return nullptr;
@@ -612,8 +650,7 @@ static int group_body(opm_group *g) {
     if (wmode == 1) {
         // FP16 x3 on the matrix cores: every lattice's step in one launch
         for (opm_solver *s : g->s) {
-            st = s->v_ref ? ophip_wsq_bwd_layout_perm(s->wstate.h, d, wmode, buf, s->perm.h)
-                          : ophip_wsq_bwd_layout(s->wstate.h, d, wmode, buf);
+            st = bwd_layout(s, d, buf);
             if (st) return st;
             st = ophip_wsq_bwd_prep(d, s->ncp, wmode, s->V[0].h, s->VT[0].h);
             if (st) return st;

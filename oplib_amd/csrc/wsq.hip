@@ -664,6 +664,45 @@ __global__ __launch_bounds__(256) void k_wfragp(int d, const uint16_t *__restric
     *(uint4 *)(Lf + f * 8) = *(const uint4 *)lo;
 }
 
+// The same for the regime permutation of a lattice squared node-major
+// (copies' state nr x + r) stepped in the reference order (fragment state
+// x + nx r), through LDS: a workgroup reads one 64 x 64 block of the
+// row-major copies (64/nr nodes x nr regimes per side) with 16-byte loads
+// and writes every 8-column run of its fragment rows as one 16-byte store.
+// nr in {1, 2, 4, 8}, nx % 8 == 0, d = nx nr, d % 64 == 0.
+__global__ __launch_bounds__(256) void k_wfragk(int d, int nx, int nr, const uint16_t *__restrict__ Hr,
+                                                const uint16_t *__restrict__ Lr, uint16_t *__restrict__ Hf,
+                                                uint16_t *__restrict__ Lf) {
+    __shared__ __attribute__((aligned(16))) uint16_t sh[2][64][72];
+    const int nb = d / 64, bi = blockIdx.x / nb, bj = blockIdx.x % nb, t = threadIdx.x;
+    const int nkt = d / 32, NB = 64 / nr;
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        const int q = t + 256 * u, row = q >> 3, ch = q & 7;  // 512 16-byte chunks per array
+        const size_t src = (size_t)(64 * bi + row) * d + 64 * bj + 8 * ch;
+        *(uint4 *)&sh[0][row][8 * ch] = *(const uint4 *)(Hr + src);
+        *(uint4 *)&sh[1][row][8 * ch] = *(const uint4 *)(Lr + src);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        const int q = t + 256 * u;
+        const int xl = q % NB, r = (q / NB) % nr, rest = q / (NB * nr);  // rest < 8
+        const int rp = rest / (NB / 8), c = rest % (NB / 8);
+        const int rho = nr * xl + r;
+        const int orow = NB * bi + xl + nx * r, ocol = NB * bj + 8 * c + nx * rp;
+        const size_t o = frag_off(orow / 16, ocol / 32, 16 * ((ocol % 32) / 8) + orow % 16, nkt);
+        uint16_t h[8], l[8];
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            h[b] = sh[0][rho][nr * (8 * c + b) + rp];
+            l[b] = sh[1][rho][nr * (8 * c + b) + rp];
+        }
+        *(uint4 *)(Hf + o) = *(const uint4 *)h;
+        *(uint4 *)(Lf + o) = *(const uint4 *)l;
+    }
+}
+
 // X (d x 16NT row-major FP32) -> its hi/lo copies, fragment-major with the
 // columns as rows: thread per (column, 8 consecutive k)
 template <bool BF, int NT>
@@ -1807,6 +1846,26 @@ OPHIP_EXPORT int ophip_wsq_bwd_step_batch(int nb, const unsigned *states, int d,
         if (e) return (int)e;
     }
     return 0;
+}
+
+// ophip_wsq_bwd_layout_perm for the regime permutation perm[x + nx r] =
+// nr x + r (a lattice squared node-major, stepped in the reference order)
+// without a permutation array: nr in {1, 2, 4, 8}, nx % 8 == 0, nx nr = d.
+OPHIP_EXPORT int ophip_wsq_bwd_layout_regime(unsigned state, int d, int mode, int buf, int nx, int nr) {
+    if (!bwd_args_ok(d, 16, mode) || buf < 0 || buf > 1 || nx <= 0 || nx % 8 || nx * nr != d ||
+        (nr != 1 && nr != 2 && nr != 4 && nr != 8))
+        return (int)set_status(kErrBadArg);
+    const WsqLayout L = wsq_layout(d, mode);
+    if (!handle_range_ok(state, L.total)) return (int)set_status(kErrBadHandle);
+    Device &D = dev();
+    const uint16_t *Hr = P<const uint16_t>(state + (unsigned)L.H[buf][0]);
+    const uint16_t *Lr = P<const uint16_t>(state + (unsigned)L.H[buf][2]);
+    uint16_t *Hf = P<uint16_t>(state + (unsigned)L.H[1 - buf][1]);
+    uint16_t *Lf = P<uint16_t>(state + (unsigned)L.H[1 - buf][3]);
+    hipLaunchKernelGGL(k_wfragk, dim3((unsigned)((d / 64) * (d / 64))), dim3(256), 0, D.stream, d, nx, nr, Hr,
+                       Lr, Hf, Lf);
+    bwd_tile_lists(D, L, state, d, Hf, Lf);
+    return (int)check_launch();
 }
 
 // Y = df (X + W X) with W laid out by ophip_wsq_bwd_layout(state, .., buf),

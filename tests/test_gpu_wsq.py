@@ -661,3 +661,31 @@ def test_forward_steps_skip_zero_cells(hip):
     assert np.array_equal(outs[0][0], outs[1][0])
     assert np.array_equal(outs[0][1], outs[1][1])
     assert np.all(np.isfinite(outs[0][1])) and outs[0][1].max() > 0
+
+
+@pytest.mark.parametrize("nx,nr", [(64, 2), (16, 8)])
+def test_wform_backward_layout_regime(hip, nx, nr):
+    """the regime layout (ophip_wsq_bwd_layout_regime, LDS tiles) is the
+    permuted layout with perm[x + nx r] = nr x + r, bit for bit, and gives
+    the same tile lists"""
+    banner(f"wsq backward layout, regime permutation nx={nx} nr={nr}")
+    H, mode, buf = hip, 1, 1
+    L = backward_lattice(H, 3, mode, nx=nx, nr=nr)
+    d = L.d
+    perm = np.array([(i % nx) * nr + i // nx for i in range(d)], dtype=np.int32)
+    pd = H.DeviceArray.from_numpy(perm)
+    nkt = d // 32
+    nl = d // 16 + d // 64 + d // 32
+    out = []
+    for regime in (False, True):
+        if regime:
+            H.check(H.lib.ophip_wsq_bwd_layout_regime(L.state.h, d, mode, buf, nx, nr))
+        else:
+            H.check(H.lib.ophip_wsq_bwd_layout_perm(L.state.h, d, mode, buf, pd.h))
+        H.sync()
+        tl = np.empty(nl * (nkt + 1), dtype=np.int32)
+        H.check(H.lib.opcuda_memcpy_d2h(H.lib.ophip_wsq_array(L.state.h, d, mode, 0, 7), tl.ctypes.data,
+                                        tl.nbytes))
+        out.append((L.arr(1 - buf, 2, np.uint16, "F").copy(), L.arr(1 - buf, 4, np.uint16, "F").copy(), tl))
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
