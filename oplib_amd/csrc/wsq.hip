@@ -872,28 +872,31 @@ __global__ __launch_bounds__(256) void k_wbwd_flags(int d, const uint16_t *__res
     if (lane == 0) *(unsigned *)(flags + (size_t)it * nkt + k0) = word;
 }
 
-// k_wbwd_lists: one thread per list (d/16 row tiles, then d/64 64-row
-// blocks, then d/32 32-row blocks), the union of its row tiles' flags in
-// k order
+// k_wbwd_lists: one wave per list (d/16 row tiles, then d/64 64-row
+// blocks, then d/32 32-row blocks), lanes over k tiles: the union of the
+// list's row tiles' flags, compacted in k order with a ballot prefix
 __global__ __launch_bounds__(256) void k_wbwd_lists(int d, const uint8_t *__restrict__ flags,
                                                     int *__restrict__ tl) {
     const int nkt = d / 32, n16 = d / 16, n64 = d / 64, n32 = d / 32;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n16 + n64 + n32) return;
+    const int lane = threadIdx.x & 63;
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= n16 + n64 + n32) return;  // wave-uniform
     int r0, nr;
     if (t < n16) { r0 = t; nr = 1; }
     else if (t < n16 + n64) { r0 = 4 * (t - n16); nr = 4; }
     else { r0 = 2 * (t - n16 - n64); nr = 2; }
     int *out = tl + (size_t)t * (nkt + 1);
     int n = 0;
-    for (int k0 = 0; k0 < nkt; k0 += 4) {  // nkt % 4 == 0
-        unsigned f = 0;
-        for (int q = 0; q < nr; q++) f |= *(const unsigned *)(flags + (size_t)(r0 + q) * nkt + k0);
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-            if ((f >> (8 * u)) & 0xffu) out[1 + n++] = k0 + u;
+    for (int k0 = 0; k0 < nkt; k0 += 64) {  // nkt <= 128
+        const int kt = k0 + lane;
+        int f = 0;
+        if (kt < nkt)
+            for (int q = 0; q < nr; q++) f |= flags[(size_t)(r0 + q) * nkt + kt];
+        const uint64_t m = __ballot(f != 0);
+        if (f) out[1 + n + __popcll(m & ((1ull << lane) - 1))] = kt;
+        n += __popcll(m);
     }
-    out[0] = n;
+    if (lane == 0) out[0] = n;
 }
 
 // one lattice of a batched backward step (arena offsets)
@@ -1697,7 +1700,7 @@ void bwd_tile_lists(Device &D, const WsqLayout &L, unsigned state, int d, const 
     uint8_t *flags = P<uint8_t>(state + (unsigned)L.tf);
     const int waves = d / 16 * (d / 128);
     hipLaunchKernelGGL(k_wbwd_flags, dim3(cdiv(waves, 4)), dim3(256), 0, D.stream, d, Hf, Lf, flags);
-    hipLaunchKernelGGL(k_wbwd_lists, dim3(cdiv(d / 16 + d / 64 + d / 32, 256)), dim3(256), 0, D.stream, d,
+    hipLaunchKernelGGL(k_wbwd_lists, dim3(cdiv(d / 16 + d / 64 + d / 32, 4)), dim3(256), 0, D.stream, d,
                        (const uint8_t *)flags, P<int>(state + (unsigned)L.tl));
 }
 }  // namespace
