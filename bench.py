@@ -32,6 +32,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -262,7 +263,16 @@ def main():
     # ---- Monte Carlo on the daily table ----
     mc = None
     if not args.no_mc:
-        mc = bench_mc(H, U, d, cfg)
+        # the daily kernel of the FP16 x3 method (within 3e-6 of FP64), so
+        # that the Monte Carlo prices can be held against the FP64 oracle
+        cw = M.config_rs2048()
+        cw.precision = M.OPM_PREC_F16X3
+        sw = M.Solver(cw)
+        sw.solve()
+        Uw = H.DeviceArray.empty(d * d)
+        Uw.upload(np.ascontiguousarray(sw.kernel().reshape(-1, order="F")))
+        sw.close()
+        mc = bench_mc(H, Uw, d, cfg, gold[1, 0, :])
 
     # ---- learned surrogate with fallback ----
     surrogate = None
@@ -490,7 +500,7 @@ def bench_sweep(H, M, reps):
     return out
 
 
-def bench_mc(H, U, d, cfg):
+def bench_mc(H, U, d, cfg, op_prices):
     # cumulative table of the daily kernel, row-contiguous
     ct = H.DeviceArray.empty(d * d)
     H.check(H.lib.ophip_sck_rows(d, 1, U.h, ct.h))
@@ -528,10 +538,38 @@ def bench_mc(H, U, d, cfg):
     ms = timed_events(H, run, 3)
     trans = float(nscen) * nk
     log(f"[bench] MC {ms:.2f} ms per 2^20 x 252, {trans / ms / 1e6:.2f} G transitions/s")
+    # the paths' call payoffs at T = 1 yr evaluated on the device, against the
+    # backward-induction prices of the same FP32 kernel (regime 1's spot)
+    import ctypes as C
+    from oplib_amd import opmodel as M
+    x, _ = M.grid(cfg)
+    xs = H.DeviceArray.from_numpy(np.tile(x, cfg.nr).astype(np.float32))
+    ks = np.array(cfg.strike_list(), dtype=np.float32)
+    dfs = np.array([math.exp(-cfg.rate * cfg.DeltaT * nk)])
+    steps = np.array([nk - 1], dtype=np.int32)
+    out = H.DeviceArray.empty(2 * len(ks), dtype=np.float64)
+
+    def pay():
+        H.lib.ophip_mc_payoff(ysk.h, nscen, nk, 1, steps.ctypes.data_as(C.POINTER(C.c_int)),
+                              dfs.ctypes.data_as(C.POINTER(C.c_double)), xs.h, d,
+                              ks.ctypes.data_as(C.POINTER(C.c_float)), len(ks), out.h)
+    pay()
+    H.check()
+    pay_us = timed_events(H, pay, 10) * 1e3
+    m = out.download().reshape(2, len(ks))
+    se = np.sqrt(np.maximum(m[1] - m[0] ** 2, 0.0) / nscen)
+    z = np.abs(m[0] - op_prices) / np.maximum(se, 1e-300)
+    log(f"[bench] MC payoffs on the device {pay_us:.1f} us; max |MC - FP64 operator| = {z.max():.2f} SE")
     ysk.free()
     log(f"[bench] MC full bisection {ms_bisect:.2f} ms")
     return {"kernel": "k_mc1f_guided (MT19937 per wavefront + guide table + bisection)",
             "paths": nscen, "ms_full_bisection": ms_bisect, "guide_table_us": guide_us,
+            "payoff_on_device_us": pay_us,
+            "price_vs_operator": {"strikes": len(ks), "max_abs_diff_in_standard_errors": float(z.max()),
+                                  "max_rel_diff": float((np.abs(m[0] - op_prices) /
+                                                         np.maximum(np.abs(op_prices), 1e-3)).max()),
+                                  "operator": "FP64 oracle backward induction, regime 1 spot (the chains "
+                                              "sample the FP16 x3 method's U(1/252))"},
             "cumulative_table_us": sck_ms * 1e3, "cumulative_table_GBps": sck_gbs,
             "steps": nk, "ms": ms, "transitions_per_s": trans / (ms * 1e-3),
             "streams": nrng, "d": d}

@@ -312,6 +312,16 @@ int ophip_mc1f_rows(unsigned mt, int y0, unsigned y_sk, int nscen, int nk, int d
  * min{j : b/256 < C[j]}; the guided sampler returns the same states as
  * ophip_mc1f_rows, bit for bit, with fewer dependent table probes */
 int ophip_mc_guide(int d, int nm, unsigned crow, unsigned guide);
+/* Monte-Carlo payoffs evaluated on the device instead of after a copy of
+ * every batch of states to the host (smc.cu:653, CModel Device.cs:352-363;
+ * VanillaPayoffEvaluator OPPricer.cs:22-53, discounting OPPricer.cs:143-146):
+ * for the listed steps (each < nk; y_sk holds nk steps of nscen paths,
+ * time-major int16) and strikes, out[i*nstrikes + c] = df[i] * mean over
+ * paths of max(xval[y] - K_c, 0) and out[(nsteps + i)*nstrikes + c] =
+ * df[i]^2 * the mean of its square (FP64, fixed summation order).  xval: d
+ * FP32 state values; nsteps, nstrikes <= 64; out: 2*nsteps*nstrikes doubles */
+int ophip_mc_payoff(unsigned y_sk, int nscen, int nk, int nsteps, const int *steps, const double *df,
+                    unsigned xval, int d, const float *strikes, int nstrikes, unsigned out);
 int ophip_mc1f_guided(unsigned mt, int y0, unsigned y_sk, int nscen, int nk, int d, unsigned crow,
                       unsigned guide, unsigned m_k);
 

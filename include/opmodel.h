@@ -119,6 +119,32 @@ int opm_batch_prices(opm_batch *b, double *prices);
  * maturities and strikes: plans and the captured graph stay valid */
 int opm_batch_update(opm_batch *b, const opm_config *cfgs);
 
+/* ---- the reference's step plan (CModel Common.cs:163-644), in days ----
+ * today = day 0; step dates t_k[nk] and generator dates t_i[ni] strictly
+ * increasing and > 0 (t_k[nk-1] <= t_i[ni-1]); generator i covers
+ * (t_{i-1}, t_i]; dt0_days caps the first step; maxdiag[i] = max_y |L_yy| of
+ * generator i (per year).  Kernels j: distinct (generator, interval length)
+ * pairs of the merged dates, each I + dt_j L squared niter_j times; step
+ * kernels m: the product of a step's interval kernels (chain), shared by
+ * later steps of the same length inside one generator period; m_k[k] maps
+ * step k to its step kernel.  Returns 0 on invalid dates. */
+typedef struct opm_mcplan opm_mcplan;
+opm_mcplan *opm_mcplan_create(int nk, const int *t_k, int ni, const int *t_i, int dt0_days,
+                              const double *maxdiag);
+void opm_mcplan_destroy(opm_mcplan *p);
+/* nr = the most squarings of any kernel, ns = the longest chain */
+int opm_mcplan_sizes(const opm_mcplan *p, int *nq, int *nj, int *nm, int *nr, int *ns);
+/* any pointer may be 0: t_q/i_q/j_q[nq], i_j/niter_j/dt_j[nj] (dt_j in
+ * years), m_k[nk], chain[nm*ns] (-1 padded), chain_len[nm] */
+int opm_mcplan_arrays(const opm_mcplan *p, int *t_q, int *i_q, int *j_q, int *i_j, int *niter_j,
+                      double *dt_j, int *m_k, int *chain, int *chain_len);
+/* the plan on the device in FP32 through opcuda_sek / opcuda_ssqmm pointer
+ * tables / opcuda_scopy1 (CModel Device.cs:186-295): gen = ni generators
+ * (d*d column-major each, per year), xtpk = nj*d*d, xbuf = max(nj,nm)*d*d,
+ * tpk = nm*d*d floats; leaves tpk_m = the step kernel of m (each chain
+ * factor once, in date order) */
+int opm_mcplan_exec(opm_mcplan *p, int d, unsigned gen, unsigned xtpk, unsigned xbuf, unsigned tpk);
+
 #ifdef __cplusplus
 }
 #endif

@@ -370,3 +370,121 @@ OPHIP_EXPORT int ophip_mc1f_guided(unsigned mtptr, int y0, unsigned y_skp, int n
                        ysk, (size_t)nscen);
     return (int)check_launch();
 }
+
+// ---------------------------------------------------------------------
+// Monte-Carlo payoff evaluation on the device.  The reference copies every
+// batch of states to the host (smc.cu:653, CModel Device.cs:352-363) and
+// evaluates there: VanillaPayoffEvaluator (OPPricer.cs:22-53) takes
+// max(x[y_T] - K, 0) per path at the last step, the mean over paths, and
+// discounts it by df(T) (OPPricer.cs:143-146).  Here the states stay in
+// HBM: for each listed step i and strike c,
+//   out[i][c]         = df_i   mean_s max(x[y_sk[step_i][s]] - K_c, 0)
+//   out[n + i][c]     = df_i^2 mean_s max(...)^2   (n = nsteps: the second
+//                                                    moment, for the
+//                                                    standard error)
+// in FP64, summed in a fixed order (chunks of kPayChunk paths: per chunk
+// four threads per strike over interleaved quarters, then the quarters,
+// then the chunks, in index order), so results do not depend on timing.
+// ---------------------------------------------------------------------
+namespace {
+
+constexpr int kPayMax = 64;      // listed steps and strikes per call
+constexpr int kPayChunk = 4096;  // paths per workgroup
+
+struct PayArgs {
+    int nsteps, nstrikes;
+    int steps[kPayMax];
+    float strikes[kPayMax];
+    double df[kPayMax];
+};
+
+__global__ __launch_bounds__(256) void k_mc_pay_part(const int16_t *__restrict__ ysk, size_t nscen,
+                                                     const float *__restrict__ xval, int d, const PayArgs a,
+                                                     int nchunk, double *__restrict__ part) {
+    __shared__ float xs[kPayChunk];
+    __shared__ double red[256][2];
+    const int i = blockIdx.y, chunk = blockIdx.x, t = threadIdx.x;
+    const int16_t *y = ysk + (size_t)a.steps[i] * nscen;
+    const size_t s0 = (size_t)chunk * kPayChunk;
+    const int n = (int)((nscen - s0) < (size_t)kPayChunk ? nscen - s0 : (size_t)kPayChunk);
+    for (int p = t; p < n; p += 256) {
+        int v = y[s0 + p];
+        v = v < 0 ? 0 : (v >= d ? d - 1 : v);  // states outside the lattice read its edge
+        xs[p] = xval[v];
+    }
+    __syncthreads();
+    // thread t: strike c = t / 4 (when c < nstrikes), paths q, q + 4, ... with q = t % 4
+    const int c = t >> 2, q = t & 3;
+    double s1 = 0.0, s2 = 0.0;
+    if (c < a.nstrikes) {
+        const float K = a.strikes[c];
+        for (int p = q; p < n; p += 4) {
+            const double v = (double)fmaxf(xs[p] - K, 0.0f);
+            s1 += v;
+            s2 += v * v;
+        }
+    }
+    red[t][0] = s1;
+    red[t][1] = s2;
+    __syncthreads();
+    if (q == 0 && c < a.nstrikes) {
+        const double r1 = ((red[t][0] + red[t + 1][0]) + red[t + 2][0]) + red[t + 3][0];
+        const double r2 = ((red[t][1] + red[t + 1][1]) + red[t + 2][1]) + red[t + 3][1];
+        double *o = part + ((size_t)i * nchunk + chunk) * 2 * kPayMax;
+        o[c] = r1;
+        o[kPayMax + c] = r2;
+    }
+}
+
+__global__ void k_mc_pay_sum(size_t nscen, const PayArgs a, int nchunk, const double *__restrict__ part,
+                             double *__restrict__ out) {
+    const int i = blockIdx.x, c = threadIdx.x;
+    if (c >= a.nstrikes) return;
+    double r1 = 0.0, r2 = 0.0;
+    for (int k = 0; k < nchunk; k++) {
+        const double *o = part + ((size_t)i * nchunk + k) * 2 * kPayMax;
+        r1 += o[c];
+        r2 += o[kPayMax + c];
+    }
+    const double inv = 1.0 / (double)nscen, df = a.df[i];
+    out[(size_t)i * a.nstrikes + c] = df * r1 * inv;
+    out[(size_t)(a.nsteps + i) * a.nstrikes + c] = df * df * r2 * inv;
+}
+
+}  // namespace
+
+// European call payoffs of nscen sampled paths (y_sk: time-major int16
+// states, as opcuda_mc1f / ophip_mc1f_rows / ophip_mc1f_guided write them)
+// at nsteps listed steps (each < the steps y_sk holds) for nstrikes
+// strikes; xval: handle of d FP32 state values.  out: handle of 2 * nsteps
+// * nstrikes doubles (discounted means, then discounted second moments).
+// nsteps, nstrikes <= 64.  Kernel launches only (partials in the library
+// workspace): graph-capturable.
+OPHIP_EXPORT int ophip_mc_payoff(unsigned y_skp, int nscen, int nk, int nsteps, const int *steps,
+                                 const double *df, unsigned xvalp, int d, const float *strikes, int nstrikes,
+                                 unsigned outp) {
+    if (nscen <= 0 || nk <= 0 || nsteps <= 0 || nsteps > kPayMax || nstrikes <= 0 || nstrikes > kPayMax ||
+        d <= 0 || d > 32767 || !steps || !df || !strikes)
+        return (int)set_status(kErrBadArg);
+    PayArgs a{};
+    a.nsteps = nsteps;
+    a.nstrikes = nstrikes;
+    for (int i = 0; i < nsteps; i++) {
+        if (steps[i] < 0 || steps[i] >= nk) return (int)set_status(kErrBadArg);
+        a.steps[i] = steps[i];
+        a.df[i] = df[i];
+    }
+    for (int c = 0; c < nstrikes; c++) a.strikes[c] = strikes[c];
+    if (!handle_range_ok(y_skp, (size_t)nscen * nk * 2) || !handle_range_ok(xvalp, (size_t)d * 4) ||
+        !handle_range_ok(outp, (size_t)2 * nsteps * nstrikes * 8))
+        return (int)set_status(kErrBadHandle);
+    const int nchunk = (int)(((size_t)nscen + kPayChunk - 1) / kPayChunk);
+    double *part = (double *)workspace(sizeof(double) * 2 * kPayMax * (size_t)nchunk * nsteps);
+    if (!part) return (int)kErrOutOfArena;
+    Device &D = dev();
+    hipLaunchKernelGGL(k_mc_pay_part, dim3((unsigned)nchunk, (unsigned)nsteps), dim3(256), 0, D.stream,
+                       P<const int16_t>(y_skp), (size_t)nscen, P<const float>(xvalp), d, a, nchunk, part);
+    hipLaunchKernelGGL(k_mc_pay_sum, dim3((unsigned)nsteps), dim3(64), 0, D.stream, (size_t)nscen, a, nchunk,
+                       (const double *)part, P<double>(outp));
+    return (int)check_launch();
+}

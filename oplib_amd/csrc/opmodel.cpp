@@ -996,3 +996,224 @@ int opm_batch_prices(opm_batch *b, double *prices) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------
+// The step plan of CModel.make_mc_plan (CModel Common.cs:163-644), in days
+// as the reference keeps dates (TimeSpan.Days / 365.25), and its device
+// execution (device_mc_sfex / sminit / smk / smcopy, CModel Device.cs:
+// 186-295) through the drop-in pointer-table ABI (opcuda_sek,
+// opcuda_ssqmm, opcuda_scopy1).
+//
+// Dates: today = day 0; step dates t_k and generator dates t_i strictly
+// increasing and > 0; generator i covers (t_{i-1}, t_i].  Merged dates t_q
+// (every step and generator date up to the last step) cut time into
+// intervals; interval q uses generator i_q.  A kernel j is one (generator,
+// interval length) pair, first occurrence first (the reference's todo_q
+// dedup, Common.cs:214-264); it starts from I + dt_j L_{i_j} with dt_j =
+// DeltaT_j 2^-niter_j, niter_j = ceil(log2(DeltaT_j / dt)), dt = min(0.5 /
+// max|L_yy|, dt0) (Common.cs:290-308), and is squared niter_j times.  A step
+// kernel m is one step k's product of its intervals' kernels (its chain),
+// shared by later steps of the same length inside one generator period
+// (todo_k, Common.cs:318-415).
+//
+// Chain products.  The reference copies the chain's first kernel, then
+// multiplies in jfactor_ms[m][s] for s = 0 .. ns-1 (Device.cs:269-279,
+// Host Double.cs:227-283): the first factor enters twice and, for the
+// longest chain, the last one never does (and shorter chains get extra
+// factors).  Here step kernel m is the product of its chain's kernels, each
+// once, in date order -- the discretised transition over the step.
+// ---------------------------------------------------------------------
+struct opm_mcplan {
+    int nk = 0, ni = 0, nq = 0, nj = 0, nm = 0, nr = 0, ns = 0;
+    std::vector<int> t_q, i_q, j_q, i_j, niter_j, m_k, chain_len;
+    std::vector<double> dt_j;
+    std::vector<std::vector<int>> chain;  // per m: kernels j in date order
+};
+
+extern "C" {
+
+opm_mcplan *opm_mcplan_create(int nk, const int *t_k, int ni, const int *t_i, int dt0_days,
+                              const double *maxdiag) {
+    if (nk <= 0 || ni <= 0 || !t_k || !t_i || !maxdiag || dt0_days <= 0) return nullptr;
+    for (int k = 0; k < nk; k++)
+        if (t_k[k] <= 0 || (k > 0 && t_k[k] <= t_k[k - 1])) return nullptr;
+    for (int i = 0; i < ni; i++)
+        if (t_i[i] <= 0 || (i > 0 && t_i[i] <= t_i[i - 1]) || !(maxdiag[i] >= 0.0)) return nullptr;
+    // the last step must lie inside the generators' span (the reference
+    // would read a generator past its table)
+    if (t_k[nk - 1] > t_i[ni - 1]) return nullptr;
+    opm_mcplan *p = new opm_mcplan();
+    p->nk = nk;
+    p->ni = ni;
+    // merged dates up to the last step
+    std::vector<int> all(t_k, t_k + nk);
+    all.insert(all.end(), t_i, t_i + ni);
+    std::sort(all.begin(), all.end());
+    for (int v : all)
+        if ((p->t_q.empty() || v > p->t_q.back()) && v <= t_k[nk - 1]) p->t_q.push_back(v);
+    p->nq = (int)p->t_q.size();
+    // generator of each interval: the first i with t_q <= t_i[i]
+    p->i_q.resize(p->nq);
+    for (int q = 0, i = 0; q < p->nq; q++) {
+        while (p->t_q[q] > t_i[i]) i++;  // t_q[q] <= t_i[ni-1]: stops inside
+        p->i_q[q] = i;
+    }
+    // kernels: distinct (generator, interval length) pairs, first occurrence first
+    auto len_q = [&](int q) { return q == 0 ? p->t_q[0] : p->t_q[q] - p->t_q[q - 1]; };
+    p->j_q.assign(p->nq, -1);
+    for (int q = 0; q < p->nq; q++) {
+        for (int q1 = 0; q1 < q; q1++)
+            if (p->i_q[q1] == p->i_q[q] && len_q(q1) == len_q(q)) {
+                p->j_q[q] = p->j_q[q1];
+                break;
+            }
+        if (p->j_q[q] >= 0) continue;
+        p->j_q[q] = p->nj++;
+        const int i = p->i_q[q];
+        const double DT = len_q(q) / 365.25;
+        double dt = maxdiag[i] > 0.0 ? 0.5 / maxdiag[i] : dt0_days / 365.25;
+        if (dt > dt0_days / 365.25) dt = dt0_days / 365.25;
+        const int niter = (int)std::ceil(std::log(DT / dt) / std::log(2.0));
+        p->i_j.push_back(i);
+        p->niter_j.push_back(niter);
+        p->dt_j.push_back(DT * std::pow(2.0, -niter));
+        p->nr = std::max(p->nr, niter);
+    }
+    // steps: the generator of a step's end and of its start (the first i
+    // with t_i[i] >= the date; a step inside one period has both equal)
+    auto gen_at = [&](int t) {
+        int i = 0;
+        while (i < ni && t_i[i] < t) i++;
+        return i;
+    };
+    p->m_k.assign(nk, -1);
+    for (int k = 0; k < nk; k++) {
+        const int len = k == 0 ? t_k[0] : t_k[k] - t_k[k - 1];
+        const int ie = gen_at(t_k[k]), is = k == 0 ? 0 : gen_at(t_k[k - 1]);
+        if (ie == is)
+            for (int k1 = 0; k1 < k; k1++) {
+                const int len1 = k1 == 0 ? t_k[0] : t_k[k1] - t_k[k1 - 1];
+                const int ie1 = gen_at(t_k[k1]), is1 = k1 == 0 ? 0 : gen_at(t_k[k1 - 1]);
+                if (ie1 == is1 && ie1 == ie && len1 == len) {
+                    p->m_k[k] = p->m_k[k1];
+                    break;
+                }
+            }
+        if (p->m_k[k] >= 0) continue;
+        p->m_k[k] = p->nm++;
+        // chain: the intervals inside (t_{k-1}, t_k], in date order
+        const int t0 = k == 0 ? 0 : t_k[k - 1], t1 = t_k[k];
+        std::vector<int> ch;
+        for (int q = 0; q < p->nq; q++) {
+            const int a = q == 0 ? 0 : p->t_q[q - 1], b = p->t_q[q];
+            if (a >= t0 && b <= t1) ch.push_back(p->j_q[q]);
+        }
+        p->chain.push_back(ch);
+        p->chain_len.push_back((int)ch.size());
+        p->ns = std::max(p->ns, (int)ch.size());
+    }
+    return p;
+}
+
+void opm_mcplan_destroy(opm_mcplan *p) { delete p; }
+
+int opm_mcplan_sizes(const opm_mcplan *p, int *nq, int *nj, int *nm, int *nr, int *ns) {
+    if (!p) return 1;
+    if (nq) *nq = p->nq;
+    if (nj) *nj = p->nj;
+    if (nm) *nm = p->nm;
+    if (nr) *nr = p->nr;
+    if (ns) *ns = p->ns;
+    return 0;
+}
+
+int opm_mcplan_arrays(const opm_mcplan *p, int *t_q, int *i_q, int *j_q, int *i_j, int *niter_j, double *dt_j,
+                      int *m_k, int *chain, int *chain_len) {
+    if (!p) return 1;
+    for (int q = 0; q < p->nq; q++) {
+        if (t_q) t_q[q] = p->t_q[q];
+        if (i_q) i_q[q] = p->i_q[q];
+        if (j_q) j_q[q] = p->j_q[q];
+    }
+    for (int j = 0; j < p->nj; j++) {
+        if (i_j) i_j[j] = p->i_j[j];
+        if (niter_j) niter_j[j] = p->niter_j[j];
+        if (dt_j) dt_j[j] = p->dt_j[j];
+    }
+    if (m_k) std::copy(p->m_k.begin(), p->m_k.end(), m_k);
+    for (int m = 0; m < p->nm; m++) {
+        if (chain_len) chain_len[m] = p->chain_len[m];
+        if (chain)
+            for (int s = 0; s < p->ns; s++) chain[(size_t)m * p->ns + s] = s < p->chain_len[m] ? p->chain[m][s] : -1;
+    }
+    return 0;
+}
+
+int opm_mcplan_exec(opm_mcplan *p, int d, unsigned gen, unsigned xtpk, unsigned xbuf, unsigned tpk) {
+    if (!p || d <= 0 || !gen || !xtpk || !xbuf || !tpk) return 0x10003;
+    const unsigned dd = (unsigned)(4 * (size_t)d * d);
+    const int nj = p->nj, nm = p->nm;
+    // first steps U_j = I + dt_j L_{i_j} (device_mc_sek, Device.cs:186-196)
+    for (int j = 0; j < nj; j++) {
+        opcuda_sek(d, gen + (unsigned)p->i_j[j] * dd, (float)p->dt_j[j], xtpk + (unsigned)j * dd);
+        const int st = (int)opcuda_get_status();
+        if (st) return st;
+    }
+    // device pointer tables: per squaring round the kernels still squaring
+    // (ping-pong xtpk_j <-> xbuf_j), per chain position the products
+    const int cap = std::max(nj, nm);
+    const int rounds = std::max(p->nr, 1), spos = std::max(p->ns - 1, 1);
+    const unsigned tab_bytes = (unsigned)(4 * (size_t)cap * (2 * rounds + 3 * spos));
+    const unsigned tab = opcuda_mem_alloc(tab_bytes);
+    if (!tab) return (int)opcuda_get_status();
+    std::vector<unsigned> host(tab_bytes / 4, 0);
+    std::vector<int> count_r(rounds, 0), count_s(spos, 0);
+    auto A_r = [&](int r) { return (size_t)cap * 2 * r; };
+    auto C_r = [&](int r) { return (size_t)cap * (2 * r + 1); };
+    auto T_s = [&](int s, int w) { return (size_t)cap * (2 * rounds + 3 * s + w); };  // w: 0 A, 1 B, 2 C
+    for (int r = 0; r < p->nr; r++)
+        for (int j = 0; j < nj; j++)
+            if (r < p->niter_j[j]) {
+                const unsigned a = (r & 1) ? xbuf : xtpk, c = (r & 1) ? xtpk : xbuf;
+                host[A_r(r) + count_r[r]] = a + (unsigned)j * dd;
+                host[C_r(r) + count_r[r]] = c + (unsigned)j * dd;
+                count_r[r]++;
+            }
+    // chain products: position s = 1 .. len-1; odd s: xbuf_m = tpk_m U_j,
+    // even s: tpk_m = xbuf_m U_j
+    for (int s = 1; s < p->ns; s++)
+        for (int m = 0; m < nm; m++)
+            if (s < p->chain_len[m]) {
+                const unsigned a = (s & 1) ? tpk : xbuf, c = (s & 1) ? xbuf : tpk;
+                host[T_s(s - 1, 0) + count_s[s - 1]] = a + (unsigned)m * dd;
+                host[T_s(s - 1, 1) + count_s[s - 1]] = xtpk + (unsigned)p->chain[m][s] * dd;
+                host[T_s(s - 1, 2) + count_s[s - 1]] = c + (unsigned)m * dd;
+                count_s[s - 1]++;
+            }
+    int st = opcuda_memcpy_h2d(tab, (unsigned long)(uintptr_t)host.data(), tab_bytes);
+    // squaring rounds (device_mc_sfex, Device.cs:216-228: A passed twice)
+    for (int r = 0; r < p->nr && !st; r++) {
+        opcuda_ssqmm(d, count_r[r], tab + 4u * (unsigned)A_r(r), tab + 4u * (unsigned)A_r(r),
+                     tab + 4u * (unsigned)C_r(r));
+        st = (int)opcuda_get_status();
+    }
+    // kernels squared an odd number of times end in xbuf (device_mc_scopy)
+    for (int j = 0; j < nj && !st; j++)
+        if (p->niter_j[j] & 1) st = opcuda_scopy1(xtpk + (unsigned)j * dd, xbuf + (unsigned)j * dd, (unsigned)(d * d));
+    // chain heads, products, and the chains of even length end in xbuf
+    for (int m = 0; m < nm && !st; m++)
+        st = opcuda_scopy1(tpk + (unsigned)m * dd, xtpk + (unsigned)p->chain[m][0] * dd, (unsigned)(d * d));
+    for (int s = 0; s + 1 < p->ns && !st; s++) {
+        opcuda_ssqmm(d, count_s[s], tab + 4u * (unsigned)T_s(s, 0), tab + 4u * (unsigned)T_s(s, 1),
+                     tab + 4u * (unsigned)T_s(s, 2));
+        st = (int)opcuda_get_status();
+    }
+    for (int m = 0; m < nm && !st; m++)
+        if (p->chain_len[m] % 2 == 0)
+            st = opcuda_scopy1(tpk + (unsigned)m * dd, xbuf + (unsigned)m * dd, (unsigned)(d * d));
+    opcuda_thread_synchronize();
+    opcuda_mem_free_device(tab);
+    return st;
+}
+
+}  // extern "C"

@@ -443,3 +443,60 @@ class Group:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------
+# the reference's step plan (opm_mcplan_*)
+# ---------------------------------------------------------------------------
+
+_IP, _DP = C.POINTER(C.c_int), C.POINTER(C.c_double)
+lib.opm_mcplan_create.argtypes = [C.c_int, _IP, C.c_int, _IP, C.c_int, _DP]
+lib.opm_mcplan_create.restype = C.c_void_p
+lib.opm_mcplan_destroy.argtypes = [C.c_void_p]
+lib.opm_mcplan_destroy.restype = None
+lib.opm_mcplan_sizes.argtypes = [C.c_void_p] + [_IP] * 5
+lib.opm_mcplan_sizes.restype = C.c_int
+lib.opm_mcplan_arrays.argtypes = [C.c_void_p, _IP, _IP, _IP, _IP, _IP, _DP, _IP, _IP, _IP]
+lib.opm_mcplan_arrays.restype = C.c_int
+lib.opm_mcplan_exec.argtypes = [C.c_void_p, C.c_int, C.c_uint, C.c_uint, C.c_uint, C.c_uint]
+lib.opm_mcplan_exec.restype = C.c_int
+
+
+class McPlan:
+    """CModel.make_mc_plan in days (host logic only; exec needs the GPU)."""
+
+    def __init__(self, t_k, t_i, dt0_days, maxdiag):
+        tk = np.ascontiguousarray(t_k, dtype=np.int32)
+        ti = np.ascontiguousarray(t_i, dtype=np.int32)
+        md = np.ascontiguousarray(maxdiag, dtype=np.float64)
+        self.p = lib.opm_mcplan_create(len(tk), tk.ctypes.data_as(_IP), len(ti), ti.ctypes.data_as(_IP),
+                                       int(dt0_days), md.ctypes.data_as(_DP))
+        if not self.p:
+            raise ValueError("invalid step or generator dates")
+        v = [C.c_int() for _ in range(5)]
+        lib.opm_mcplan_sizes(self.p, *[C.byref(x) for x in v])
+        self.nq, self.nj, self.nm, self.nr, self.ns = [x.value for x in v]
+        a = {k: np.zeros(self.nq, np.int32) for k in ("t_q", "i_q", "j_q")}
+        a.update({k: np.zeros(self.nj, np.int32) for k in ("i_j", "niter_j")})
+        a["dt_j"] = np.zeros(self.nj)
+        a["m_k"] = np.zeros(len(tk), np.int32)
+        a["chain"] = np.zeros(max(self.nm * self.ns, 1), np.int32)
+        a["chain_len"] = np.zeros(self.nm, np.int32)
+        order = ("t_q", "i_q", "j_q", "i_j", "niter_j", "dt_j", "m_k", "chain", "chain_len")
+        lib.opm_mcplan_arrays(self.p, *[a[k].ctypes.data_as(_DP if k == "dt_j" else _IP) for k in order])
+        self.__dict__.update(a)
+        self.chains = [list(a["chain"][m * self.ns:m * self.ns + a["chain_len"][m]]) for m in range(self.nm)]
+
+    def exec(self, d, gen, xtpk, xbuf, tpk):
+        ophip.check(lib.opm_mcplan_exec(self.p, d, gen, xtpk, xbuf, tpk), "opm_mcplan_exec")
+
+    def close(self):
+        if self.p:
+            lib.opm_mcplan_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -477,6 +477,105 @@ def test_mc_chain_bit_exact(H, d, nrng, groups, nk):
         H.lib.ophip_mc_set_nrng(4096)
 
 
+def payoff_moments(ysk, steps, xval, strikes, df):
+    """numpy: discounted mean and second moment of max(x[y] - K, 0) per
+    listed step and strike, the payoff in FP32 as the kernel forms it,
+    sums in FP64"""
+    n = ysk.shape[1]
+    m1 = np.zeros((len(steps), len(strikes)))
+    m2 = np.zeros_like(m1)
+    for i, k in enumerate(steps):
+        x = xval[np.clip(ysk[k], 0, len(xval) - 1)].astype(np.float32)
+        for c, K in enumerate(strikes):
+            v = np.maximum(x - np.float32(K), np.float32(0)).astype(np.float64)
+            m1[i, c] = df[i] * v.sum() / n
+            m2[i, c] = df[i] ** 2 * (v * v).sum() / n
+    return m1, m2
+
+
+def test_mc_payoff_device(H):
+    """ophip_mc_payoff against numpy on random states: discounted means and
+    second moments of the call payoffs per listed step and strike (FP64
+    sums in another order: 1e-12 relative), out-of-lattice states read the
+    lattice edge"""
+    banner("mc payoff reduction")
+    import ctypes as C
+    rng = np.random.default_rng(5)
+    d, nk, nscen = 300, 6, 3 * 4096 + 77
+    ysk = rng.integers(0, d, size=(nk, nscen)).astype(np.int16)
+    ysk[2, :5] = [-3, d, d + 9, 0, d - 1]
+    xval = np.linspace(0.0, 400.0, d).astype(np.float32)
+    steps = np.array([5, 0, 2], dtype=np.int32)
+    strikes = np.array([80.0 + 2.5 * c for c in range(17)], dtype=np.float32)
+    df = np.array([0.97, 1.0, 0.99])
+    yd = H.DeviceArray.from_numpy(ysk.reshape(-1))
+    xd = H.DeviceArray.from_numpy(xval)
+    out = H.DeviceArray.empty(2 * len(steps) * len(strikes), dtype=np.float64)
+    H.check(H.lib.ophip_mc_payoff(yd.h, nscen, nk, len(steps), steps.ctypes.data_as(C.POINTER(C.c_int)),
+                                  df.ctypes.data_as(C.POINTER(C.c_double)), xd.h, d,
+                                  strikes.ctypes.data_as(C.POINTER(C.c_float)), len(strikes), out.h))
+    got = out.download().reshape(2, len(steps), len(strikes))
+    m1, m2 = payoff_moments(ysk, steps, xval, strikes, df)
+    assert np.allclose(got[0], m1, rtol=1e-12, atol=0)
+    assert np.allclose(got[1], m2, rtol=1e-12, atol=0)
+    # a step beyond the states held is refused
+    bad = np.array([nk], dtype=np.int32)
+    assert H.lib.ophip_mc_payoff(yd.h, nscen, nk, 1, bad.ctypes.data_as(C.POINTER(C.c_int)),
+                                 df.ctypes.data_as(C.POINTER(C.c_double)), xd.h, d,
+                                 strikes.ctypes.data_as(C.POINTER(C.c_float)), len(strikes), out.h) != 0
+
+
+def test_mc_prices_match_operator(H):
+    """Monte Carlo on the operator's own kernel: 2^18 chains on the
+    cumulative table of the plain FP32 U(dt) of a small regime lattice,
+    payoffs evaluated on the device, against the backward-induction prices
+    of the same kernel (the expectation the sampler estimates), within five
+    standard errors"""
+    banner("mc vs operator")
+    import ctypes as C
+    from oplib_amd import opmodel as M
+    cfg = M.config_small(nx=64, nr=2, nsteps=8)
+    s = M.Solver(cfg)
+    s.solve()
+    p_bi = s.prices()  # [nr][1][nstrikes]
+    U = s.kernel()     # reference order, column-major [y0, y]
+    s.close()
+    d, nk = cfg.d, cfg.nsteps
+    x, x0 = M.grid(cfg)
+    Ud = H.DeviceArray.from_numpy(np.ascontiguousarray(U.reshape(-1, order="F")))
+    ct = H.DeviceArray.empty(d * d)
+    H.check(H.lib.ophip_sck_rows(d, 1, Ud.h, ct.h))
+    nrng, G = 64, 64
+    nscen = 64 * nrng * G
+    H.check(H.lib.ophip_mc_set_nrng(nrng))
+    try:
+        mt = H.DeviceArray.empty(625 * nrng, dtype=np.uint32)
+        seeds = (5489 + np.arange(nrng)).astype(np.uint32)
+        H.lib.opcuda_mc_setseed(seeds.ctypes.data, mt.h)
+        H.check()
+        mk = H.DeviceArray.from_numpy(np.zeros(nk, dtype=np.int32))
+        ysk = H.DeviceArray.empty(nscen * nk, dtype=np.int16)
+        xs = H.DeviceArray.from_numpy(np.tile(x, cfg.nr).astype(np.float32))
+        ks = np.array(cfg.strike_list(), dtype=np.float32)
+        df = np.array([M.df(cfg, cfg.DeltaT * nk)])
+        steps = np.array([nk - 1], dtype=np.int32)
+        out = H.DeviceArray.empty(2 * len(ks), dtype=np.float64)
+        for r in range(cfg.nr):
+            y0 = x0 + cfg.nx * r
+            H.check(H.lib.ophip_mc1f_rows(mt.h, y0, ysk.h, nscen, nk, d, ct.h, mk.h, 0))
+            H.check(H.lib.ophip_mc_payoff(ysk.h, nscen, nk, 1, steps.ctypes.data_as(C.POINTER(C.c_int)),
+                                          df.ctypes.data_as(C.POINTER(C.c_double)), xs.h, d,
+                                          ks.ctypes.data_as(C.POINTER(C.c_float)), len(ks), out.h))
+            m = out.download().reshape(2, len(ks))
+            se = np.sqrt(np.maximum(m[1] - m[0] ** 2, 0.0) / nscen)
+            dev_ = np.abs(m[0] - p_bi[r, 0])
+            print(f"[mc] regime {r}: max |mc - operator| / se = {(dev_ / np.maximum(se, 1e-12)).max():.2f}",
+                  flush=True)
+            assert np.all(dev_ <= 5.0 * se + 1e-6 * np.abs(p_bi[r, 0]).max())
+    finally:
+        H.lib.ophip_mc_set_nrng(4096)
+
+
 def test_mc1f_rejects_bad_shapes(H):
     banner("mc1f args")
     H.check(H.lib.ophip_mc_set_nrng(4))

@@ -265,3 +265,36 @@ def test_group_rs2048_vs_fp64(M):
         g.solve()
         assert np.array_equal(g.prices(), pg)
     g.close()
+
+
+def test_mcplan_exec_vs_fp64(M):
+    """the reference's step plan executed on the device (first steps,
+    squaring rounds and chain products through opcuda_ssqmm pointer tables)
+    against the FP64 oracle's first steps, squarings and products of the
+    same plan: weekly steps over three local-vol generators"""
+    banner("mc plan exec")
+    from oplib_amd import ophip as H
+    nx = 64
+    x, _ = O.xgrid(nx, 0.0, 400.0, 100.0)
+    gens = []
+    for scale in (0.8, 1.0, 1.3):
+        dr, vo = O.lv_params_c1(x, 0.02)
+        gens.append(O.lv_generator(x, dr, scale * vo))
+    maxdiag = [np.abs(np.diag(g)).max() for g in gens]
+    p = M.McPlan([7, 14, 21, 28, 35, 42, 49, 56, 63], [30, 60, 90], 1, maxdiag)
+    d = nx
+    gd = H.DeviceArray.from_numpy(np.concatenate([g.astype(np.float32).reshape(-1, order="F") for g in gens]))
+    xt = H.DeviceArray.empty(p.nj * d * d)
+    xb = H.DeviceArray.empty(max(p.nj, p.nm) * d * d)
+    tp = H.DeviceArray.empty(p.nm * d * d)
+    p.exec(d, gd.h, xt.h, xb.h, tp.h)
+    got = tp.download().reshape(p.nm, d, d).transpose(0, 2, 1)  # column-major blocks
+    Uj = [O.square_chain(O.first_step(gens[p.i_j[j]], p.dt_j[j]), int(p.niter_j[j])) for j in range(p.nj)]
+    for m, ch in enumerate(p.chains):
+        want = Uj[ch[0]]
+        for j in ch[1:]:
+            want = want @ Uj[j]
+        err = np.abs(got[m] - want).max()
+        print(f"[plan] m={m} chain={list(map(int, ch))} max |U_m - FP64| = {err:.2e}", flush=True)
+        assert err < 2e-5, (m, err)
+    p.close()
