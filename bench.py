@@ -274,6 +274,9 @@ def main():
         sw.close()
         mc = bench_mc(H, Uw, d, cfg, gold[1, 0, :])
 
+    # ---- stochastic-volatility generator assembly (FP64) at 256 x 8 states ----
+    sv = bench_sv(H, cfg)
+
     # ---- learned surrogate with fallback ----
     surrogate = None
     if not args.no_surrogate:
@@ -341,6 +344,7 @@ def main():
         "precision_study": precision,
         "sweep": sweep,
         "mc": mc,
+        "sv_generator": sv,
         "surrogate": surrogate,
         "cpu_baseline": cpu,
     }
@@ -573,6 +577,32 @@ def bench_mc(H, U, d, cfg, op_prices):
             "cumulative_table_us": sck_ms * 1e3, "cumulative_table_GBps": sck_gbs,
             "steps": nk, "ms": ms, "transitions_per_s": trans / (ms * 1e-3),
             "streams": nrng, "d": d}
+
+
+def bench_sv(H, cfg):
+    """FP64 stochastic-volatility generator (ophip_dgen_sv) on the 256 x 8
+    lattice: prices on [0, 400], regimes 0.1..0.8, mean-reverting regime
+    drift, vol of vol 0.5, jumps of 10% / 5% of the price; bytes = the d x d
+    FP64 output"""
+    nx, nr = cfg.nx, cfg.nr
+    from oplib_amd import opmodel as M
+    x, _ = M.grid(cfg)
+    d = nx * nr
+    xv = np.tile(x, nr)
+    rv = np.repeat(np.linspace(0.1, 0.8, nr), nx)
+    arrs = [xv, rv, cfg.rate * xv, rv * xv, 0.5 * np.ones(d), 2.0 * (0.45 - rv), 0.1 * np.ones(d),
+            0.05 * np.ones(d)]
+    hd = [H.DeviceArray.from_numpy(np.ascontiguousarray(a, dtype=np.float64)) for a in arrs]
+    out = H.DeviceArray.empty(d * d, dtype=np.float64)
+
+    def run():
+        H.lib.ophip_dgen_sv(nx, nr, *[a.h for a in hd], 0.0, 4, out.h)
+    run()
+    H.check()
+    us = timed_events(H, run, 10) * 1e3
+    gbs = 8.0 * d * d / (us * 1e-6) / 1e9
+    log(f"[bench] SV generator d={d}: {us:.1f} us, {gbs:.0f} GB/s written")
+    return {"kernel": "k_sv_rows + k_sv_fill<0, double>", "d": d, "us": us, "GBps_written": gbs}
 
 
 def bench_surrogate(H, M, ntrain, steps, log_fn):

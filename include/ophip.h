@@ -121,6 +121,19 @@ int ophip_dgen_rs(int nx, int nr, unsigned xval, unsigned drift, unsigned vol, d
  * node-major state order y = nr*x + r (one band; used by the W form) */
 int ophip_sek_rs(int nx, int nr, unsigned xval, unsigned drift, unsigned vol, double q_off,
                  double dt, int out_kind, unsigned out);
+/* FP64 stochastic-volatility generator (CSVModel.host_d_gen, CSVModel.cs:
+ * 187-379; the reference's FP32 device version opcuda_sgsvg, sgsvg.cu:215):
+ * per-state handles of d = nx*nr doubles (state y = x + nx*r): grid values
+ * xval, regime values rval, the row drift sdrift (r x), x vol svol (price
+ * units), volvol, voldrift, relative jump sizes jm, jp.  out_kind 0 FP32
+ * I+dt*L, 1 FP64 I+dt*L, 2 FP32 dt*L, 3 FP64 dt*L, 4 FP64 L.  nr >= 2.
+ * The jump weights use the scaled sizes in normalisation and assignment,
+ * and the down-neighbour drift correction has the sign that removes the
+ * excess (both differ from the reference, which mixes the sizes and
+ * doubles the excess). */
+int ophip_dgen_sv(int nx, int nr, unsigned xval, unsigned rval, unsigned sdrift, unsigned svol,
+                  unsigned volvol, unsigned voldrift, unsigned jm, unsigned jp, double dt, int out_kind,
+                  unsigned out);
 /* ophip_sek_rs for n lattices in one launch per 32 (lattice k takes the
  * k-th entry of every array, one out_kind for all); the same values */
 int ophip_sek_rs_batch(int n, const int *nx, const int *nr, const unsigned *xval,

@@ -446,3 +446,189 @@ OPHIP_EXPORT int ophip_sck_rows(int d, int nm, unsigned tpk, unsigned crow) {
 OPHIP_EXPORT int ophip_sck_rows_ex(int d, int nm, unsigned tpk, unsigned crow, int plus_identity) {
     return sck_rows(d, nm, tpk, crow, plus_identity);
 }
+
+// ---------------------------------------------------------------------
+// Stochastic-volatility lattice generator in FP64 (CSVModel.host_d_gen,
+// CSVModel.cs:187-379; the reference's FP32 device version global_sgsvg,
+// sgsvg.cu:40-209), with the 4x4 stencil of S2DGrid.pregen (S2DGrid.cs:
+// 123-245) solved in closed form: the system is block diagonal, a 2x2 in
+// x (zero drift, variance svol^2) and a 2x2 in r (drift voldrift,
+// variance volvol^2; one-sided voldrift/delta at r = 0, nr-1), no x moves
+// at x = 0, nx-1.  A positive rate up to regime r+1 is mirrored by jumps
+// into regime r+1 (x1 < x0-1 weighted exp((x1-x0)/(jm x0)), x1 > x0
+// weighted exp(-(x1-x0)/(jp x0)), normalised to that rate).  Negative
+// rates are clipped; one neighbour sets the row drift sum L (x1-x0) to
+// sdrift; the diagonal closes the row.  Two fixes against the reference:
+// the jump weights use the scaled sizes jm x0, jp x0 in both the
+// normalisation and the assignment (CSVModel.cs:296-297 vs 315, 321;
+// sgsvg.cu:111-112 takes the minus size for both sides), and the drift
+// correction on the down neighbour is (drift - sdrift)/(x0 - x_{-1}) > 0
+// (CSVModel.cs:356-357 and sgsvg.cu:186 divide by x_{-1} - x0, which
+// doubles the excess).  Sums run in state order, as the reference's.
+//
+// k_sv_rows: one thread per row, its few distinct values (stencil rates,
+// jump normalisation, diagonal) into a row table; k_sv_fill: one thread
+// per element, column-major (coalesced), the value from the row table.
+// ---------------------------------------------------------------------
+namespace {
+
+struct SVLattice {
+    int nx, nr;
+    const double *xval, *rval, *sdrift, *svol, *volvol, *voldrift, *jm, *jp;
+};
+
+// row table: am, ap, bm, bp (after clipping and the drift fix), jump ratio,
+// scaled jump sizes sm, sp (0: no jumps on that side), diagonal
+constexpr int kSVRow = 8;
+
+__global__ void k_sv_rows(SVLattice L, double *__restrict__ rt) {
+    const int nx = L.nx, nr = L.nr, d = nx * nr;
+    const int y0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y0 >= d) return;
+    const int x0 = y0 % nx, r0 = y0 / nx;
+    const double X0 = L.xval[y0];
+    double am = 0.0, ap = 0.0, bm = 0.0, bp = 0.0, ratio = 0.0, sm = 0.0, sp = 0.0;
+    if (x0 > 0 && x0 < nx - 1) {
+        const double dm = L.xval[y0 - 1] - X0, dp = L.xval[y0 + 1] - X0;
+        const double s2 = L.svol[y0] * L.svol[y0];
+        const double det = dm * dp * dp - dp * dm * dm;
+        am = (-dp * s2) / det;
+        ap = (dm * s2) / det;
+    }
+    if (nr > 1) {
+        if (r0 > 0 && r0 < nr - 1) {
+            const double em = L.rval[y0 - nx] - L.rval[y0], ep = L.rval[y0 + nx] - L.rval[y0];
+            const double v2 = L.volvol[y0] * L.volvol[y0], mu = L.voldrift[y0];
+            const double det = em * ep * ep - ep * em * em;
+            bm = (ep * ep * mu - ep * v2) / det;
+            bp = (-em * em * mu + em * v2) / det;
+        } else if (r0 == 0) {
+            bp = L.voldrift[y0] / (L.rval[y0 + nx] - L.rval[y0]);
+        } else {
+            bm = L.voldrift[y0] / (L.rval[y0 - nx] - L.rval[y0]);
+        }
+        if (r0 < nr - 1 && bp > 0.0) {
+            const double jsm = L.jm[y0] * X0, jsp = L.jp[y0] * X0;
+            const double *xr = L.xval + nx * (r0 + 1);
+            double s = 0.0;
+            if (jsm > 0.0)
+                for (int x1 = 0; x1 < x0 - 1; x1++) s += exp((xr[x1] - X0) / jsm);
+            if (jsp > 0.0)
+                for (int x1 = x0 + 1; x1 < nx; x1++) s += exp(-(xr[x1] - X0) / jsp);
+            if (s > 0.0) {
+                ratio = bp / s;
+                sm = jsm > 0.0 ? jsm : 0.0;
+                sp = jsp > 0.0 ? jsp : 0.0;
+            }
+        }
+    }
+    if (r0 == 0) bm = 0.0;       // no such neighbour
+    if (r0 == nr - 1) bp = 0.0;
+    if (x0 == 0) am = 0.0;
+    if (x0 == nx - 1) ap = 0.0;
+    // clip, then drift and mass over the row's entries in state order:
+    // regime r0-1 (x0), regime r0 (x0-1, x0+1), regime r0+1 (jumps below
+    // x0-1, x0, jumps above x0); zero entries add exact zeros
+    am = am < 0.0 ? 0.0 : am;
+    ap = ap < 0.0 ? 0.0 : ap;
+    bm = bm < 0.0 ? 0.0 : bm;
+    const double bp_kept = bp < 0.0 ? 0.0 : bp;
+    double drift = 0.0, sum0 = 0.0;
+    auto add = [&](double v, double x1) {
+        drift += v * (x1 - X0);
+        sum0 += v;
+    };
+    if (r0 > 0) add(bm, L.xval[y0 - nx]);
+    if (x0 > 0) add(am, L.xval[y0 - 1]);
+    if (x0 < nx - 1) add(ap, L.xval[y0 + 1]);
+    if (r0 < nr - 1) {
+        const double *xr = L.xval + nx * (r0 + 1);
+        if (sm > 0.0)
+            for (int x1 = 0; x1 < x0 - 1; x1++) add(ratio * exp((xr[x1] - X0) / sm), xr[x1]);
+        add(bp_kept, xr[x0]);
+        if (sp > 0.0)
+            for (int x1 = x0 + 1; x1 < nx; x1++) add(ratio * exp(-(xr[x1] - X0) / sp), xr[x1]);
+    }
+    const double drift0 = L.sdrift[y0];
+    if (drift > drift0) {
+        if (x0 > 0) {
+            const double c = (drift - drift0) / (X0 - L.xval[y0 - 1]);
+            am += c;
+            sum0 += c;
+        }
+    } else if (x0 < nx - 1) {
+        const double c = (drift0 - drift) / (L.xval[y0 + 1] - X0);
+        ap += c;
+        sum0 += c;
+    }
+    double *o = rt + (size_t)kSVRow * y0;
+    o[0] = am;
+    o[1] = ap;
+    o[2] = bm;
+    o[3] = bp_kept;
+    o[4] = ratio;
+    o[5] = sm;
+    o[6] = sp;
+    o[7] = -sum0;
+}
+
+template <int MODE, typename OUT>
+__global__ __launch_bounds__(256) void k_sv_fill(SVLattice L, const double *__restrict__ rt, double dt,
+                                                 OUT *__restrict__ out) {
+    // 32-bit index math: d <= 32767, so d * d < 2^31
+    const unsigned nx = (unsigned)L.nx, d = nx * (unsigned)L.nr, n = d * d;
+    for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const unsigned y1 = e / d, y0 = e - y1 * d;
+        const unsigned r0 = y0 / nx, x0 = y0 - r0 * nx, r1 = y1 / nx, x1 = y1 - r1 * nx;
+        const double *o = rt + (size_t)kSVRow * y0;
+        double g = 0.0;
+        if (y1 == y0) g = o[7];
+        else if (r1 == r0) g = x1 + 1 == x0 ? o[0] : (x1 == x0 + 1 ? o[1] : 0.0);
+        else if (r1 + 1 == r0) g = x1 == x0 ? o[2] : 0.0;
+        else if (r1 == r0 + 1) {
+            if (x1 == x0) g = o[3];
+            else if (x1 + 1 < x0 && o[5] > 0.0) g = o[4] * exp((L.xval[y1] - L.xval[y0]) / o[5]);
+            else if (x1 > x0 && o[6] > 0.0) g = o[4] * exp(-(L.xval[y1] - L.xval[y0]) / o[6]);
+        }
+        double v;
+        if (MODE == 0) v = g;
+        else if (MODE == 1) v = (y0 == y1 ? 1.0 : 0.0) + dt * g;
+        else v = dt * g;
+        out[e] = (OUT)v;
+    }
+}
+
+}  // namespace
+
+// out_kind: 0 FP32 I + dt L, 1 FP64 I + dt L, 2 FP32 dt L, 3 FP64 dt L
+// (as ophip_sek_rs), 4 the FP64 generator L.  Every array is a handle of
+// d = nx*nr doubles indexed by state y = x + nx*r; nr >= 2 (the reference's
+// 4x4 system is singular with one regime).
+OPHIP_EXPORT int ophip_dgen_sv(int nx, int nr, unsigned xval, unsigned rval, unsigned sdrift, unsigned svol,
+                               unsigned volvol, unsigned voldrift, unsigned jm, unsigned jp, double dt,
+                               int out_kind, unsigned out) {
+    if (nx < 3 || nr < 2 || (long)nx * nr > 32767 || out_kind < 0 || out_kind > 4)
+        return (int)set_status(kErrBadArg);
+    const size_t d = (size_t)nx * nr;
+    const unsigned hs[8] = {xval, rval, sdrift, svol, volvol, voldrift, jm, jp};
+    for (unsigned h : hs)
+        if (!handle_range_ok(h, 8 * d)) return (int)set_status(kErrBadHandle);
+    const size_t eb = (out_kind == 1 || out_kind >= 3) ? 8 : 4;
+    if (!handle_range_ok(out, eb * d * d)) return (int)set_status(kErrBadHandle);
+    SVLattice L{nx, nr, P<const double>(xval), P<const double>(rval), P<const double>(sdrift),
+                P<const double>(svol), P<const double>(volvol), P<const double>(voldrift),
+                P<const double>(jm), P<const double>(jp)};
+    double *rt = (double *)workspace(sizeof(double) * kSVRow * d);
+    if (!rt) return (int)kErrOutOfArena;
+    Device &D = dev();
+    hipLaunchKernelGGL(k_sv_rows, dim3(cdiv((int)d, 64)), dim3(64), 0, D.stream, L, rt);
+    const dim3 g(grid1d(d * d, 256)), b(256);
+    switch (out_kind) {
+    case 0: hipLaunchKernelGGL((k_sv_fill<1, float>), g, b, 0, D.stream, L, (const double *)rt, dt, P<float>(out)); break;
+    case 1: hipLaunchKernelGGL((k_sv_fill<1, double>), g, b, 0, D.stream, L, (const double *)rt, dt, P<double>(out)); break;
+    case 2: hipLaunchKernelGGL((k_sv_fill<2, float>), g, b, 0, D.stream, L, (const double *)rt, dt, P<float>(out)); break;
+    case 3: hipLaunchKernelGGL((k_sv_fill<2, double>), g, b, 0, D.stream, L, (const double *)rt, dt, P<double>(out)); break;
+    default: hipLaunchKernelGGL((k_sv_fill<0, double>), g, b, 0, D.stream, L, (const double *)rt, dt, P<double>(out)); break;
+    }
+    return (int)check_launch();
+}

@@ -72,6 +72,7 @@ SIGNATURES = {
     "ophip_stream": (ULONG, []),
     "ophip_dgen_rs": (I32, [I32, I32, U32, U32, U32, F64, U32]),
     "ophip_sek_rs": (I32, [I32, I32, U32, U32, U32, F64, F64, I32, U32]),
+    "ophip_dgen_sv": (I32, [I32, I32] + [U32] * 8 + [F64, I32, U32]),
     "ophip_sek_rs_batch": (I32, [I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(U32),
                                  C.POINTER(U32), C.POINTER(U32), C.POINTER(F64), C.POINTER(F64),
                                  I32, C.POINTER(U32)]),

@@ -697,3 +697,111 @@ void or_adam_step(int n, double *p, const double *g, double *m, double *v,
         p[i] -= lr * mh / (sqrt(vh) + eps);
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* Stochastic-volatility lattice generator (CSVModel.host_d_gen,        */
+/* CSVModel.cs:187-379, with the 4x4 stencil of S2DGrid.pregen,         */
+/* S2DGrid.cs:123-245).  States y = x + nx*r; per state the x-variance  */
+/* svol^2, regime drift voldrift and regime variance volvol^2 set the   */
+/* four neighbour rates (the 4x4 system is block diagonal: a 2x2 in x   */
+/* with zero drift, a 2x2 in r; at r = 0 / nr-1 the one-sided rate      */
+/* voldrift/delta, at x = 0 / nx-1 no x moves).  A positive rate up to  */
+/* regime r+1 is mirrored by jumps into regime r+1 (x1 < x0-1 with      */
+/* weight exp((x1-x0)/(jm x0)), x1 > x0 with exp(-(x1-x0)/(jp x0)),     */
+/* normalised to the same total).  Negative rates are clipped, the row  */
+/* drift sum(L (x1 - x0)) is set to sdrift by one neighbour, the        */
+/* diagonal closes the row.  Two fixes against the reference: the jump  */
+/* weights use the scaled sizes jm x0, jp x0 both in the normalisation  */
+/* and in the assignment (CSVModel.cs:296-297 vs 315,321; sgsvg.cu:111- */
+/* 112 uses the minus size for both sides), and the drift correction    */
+/* added to the down neighbour is (drift - sdrift)/(x0 - x1) > 0 (the   */
+/* reference divides by x1 - x0, CSVModel.cs:356-357, sgsvg.cu:186,     */
+/* which doubles the excess instead of removing it).  Entries are       */
+/* summed in state order.  gen: d x d column-major, gen[y0 + d*y1].     */
+/* ------------------------------------------------------------------ */
+void or_sv_generator(int nx, int nr, const double *xval, const double *rval,
+                     const double *sdrift, const double *svol, const double *volvol,
+                     const double *voldrift, const double *jm, const double *jp,
+                     double *gen)
+{
+    int d = nx * nr;
+    memset(gen, 0, sizeof(double) * (size_t)d * d);
+    for (int r0 = 0; r0 < nr; r0++) {
+        for (int x0 = 0; x0 < nx; x0++) {
+            int y0 = x0 + nx * r0;
+            double *row = gen + y0; /* row[d*y1] */
+            double X0 = xval[y0];
+            /* x block: [dm dp; dm^2 dp^2] [am ap]' = [0 s2]' (interior x) */
+            if (x0 > 0 && x0 < nx - 1) {
+                double dm = xval[y0 - 1] - X0, dp = xval[y0 + 1] - X0;
+                double s2 = svol[y0] * svol[y0];
+                double det = dm * dp * dp - dp * dm * dm;
+                row[(size_t)d * (y0 - 1)] = (-dp * s2) / det;
+                row[(size_t)d * (y0 + 1)] = (dm * s2) / det;
+            }
+            /* r block */
+            if (nr > 1) {
+                double bm = 0.0, bp = 0.0;
+                if (r0 > 0 && r0 < nr - 1) {
+                    double em = rval[y0 - nx] - rval[y0], ep = rval[y0 + nx] - rval[y0];
+                    double v2 = volvol[y0] * volvol[y0], mu = voldrift[y0];
+                    double det = em * ep * ep - ep * em * em;
+                    bm = (ep * ep * mu - ep * v2) / det;
+                    bp = (-em * em * mu + em * v2) / det;
+                } else if (r0 == 0) {
+                    bp = voldrift[y0] / (rval[y0 + nx] - rval[y0]);
+                } else {
+                    bm = voldrift[y0] / (rval[y0 - nx] - rval[y0]);
+                }
+                if (r0 > 0) row[(size_t)d * (y0 - nx)] = bm;
+                if (r0 < nr - 1) row[(size_t)d * (y0 + nx)] = bp;
+                /* jumps into regime r0+1 mirroring the rate up */
+                if (r0 < nr - 1 && bp > 0.0) {
+                    double sm = jm[y0] * X0, sp = jp[y0] * X0, sum0 = 0.0;
+                    for (int x1 = 0; x1 < x0 - 1; x1++)
+                        if (sm > 0.0) sum0 += exp((xval[x1 + nx * (r0 + 1)] - X0) / sm);
+                    for (int x1 = x0 + 1; x1 < nx; x1++)
+                        if (sp > 0.0) sum0 += exp(-(xval[x1 + nx * (r0 + 1)] - X0) / sp);
+                    if (sum0 > 0.0) {
+                        double ratio = bp / sum0;
+                        for (int x1 = 0; x1 < x0 - 1; x1++)
+                            if (sm > 0.0) {
+                                int y1 = x1 + nx * (r0 + 1);
+                                row[(size_t)d * y1] = ratio * exp((xval[y1] - X0) / sm);
+                            }
+                        for (int x1 = x0 + 1; x1 < nx; x1++)
+                            if (sp > 0.0) {
+                                int y1 = x1 + nx * (r0 + 1);
+                                row[(size_t)d * y1] = ratio * exp(-(xval[y1] - X0) / sp);
+                            }
+                    }
+                }
+            }
+            /* clip, drift and mass in state order */
+            double drift = 0.0, sum0 = 0.0;
+            for (int y1 = 0; y1 < d; y1++) {
+                if (y1 == y0) continue;
+                double v = row[(size_t)d * y1];
+                if (v < 0.0) {
+                    row[(size_t)d * y1] = 0.0;
+                } else {
+                    drift += v * (xval[y1] - X0);
+                    sum0 += v;
+                }
+            }
+            double drift0 = sdrift[y0];
+            if (drift > drift0) {
+                if (x0 > 0) {
+                    double ratio = (drift - drift0) / (X0 - xval[y0 - 1]);
+                    row[(size_t)d * (y0 - 1)] += ratio;
+                    sum0 += ratio;
+                }
+            } else if (x0 < nx - 1) {
+                double ratio = (drift0 - drift) / (xval[y0 + 1] - X0);
+                row[(size_t)d * (y0 + 1)] += ratio;
+                sum0 += ratio;
+            }
+            row[(size_t)d * y0] = -sum0;
+        }
+    }
+}

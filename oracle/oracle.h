@@ -16,6 +16,9 @@
  *   or_stencil_invm      closed-form 2x2 inverse, A*invm = I, tests/test_oracle.py
  *   or_lv_generator      closed-form uniform-grid rates, zero row sums
  *   or_rs_generator      zero row sums; regime marginal = expm(Q t) (closed form)
+ *   or_sv_generator      zero row sums, nonnegative rates, row drift = sdrift,
+ *                        second moment = svol^2 + the drift fix's share,
+ *                        jump profile and mass (tests/test_oracle.py)
  *   or_first_step*       exact formula, row sums 1
  *   or_dgemm / squaring  numpy matmul; scipy expm fixtures (tests/golden)
  *   or_sck_f32           closed form C[.,d-1]=1, monotone rows, float64 cumsum
@@ -47,6 +50,12 @@ void or_rs_generator(int nx, int nr, const double *invm, const double *xval,
                      double *gen);
 void or_lv_params_c1(int nx, const double *xval, double rate, double *drift,
                      double *vol);
+/* stochastic-volatility generator (CSVModel.cs:187-379), two fixes (jump
+ * sizes, drift-correction sign); per-state arrays of d = nx*nr */
+void or_sv_generator(int nx, int nr, const double *xval, const double *rval,
+                     const double *sdrift, const double *svol, const double *volvol,
+                     const double *voldrift, const double *jm, const double *jp,
+                     double *gen);
 void or_lv_params_cev(int nx, const double *xval, double rate, double sigma0,
                       double beta, double skew, double curv, double *drift,
                       double *vol);

@@ -42,6 +42,7 @@ def _load():
         "or_stencil_invm": (None, [I, _dp, _dp]),
         "or_lv_generator": (None, [I, _dp, _dp, _dp, _dp]),
         "or_rs_generator": (None, [I, I, _dp, _dp, _dp, D, D, _dp]),
+        "or_sv_generator": (None, [I, I] + [_dp] * 9),
         "or_lv_params_c1": (None, [I, _dp, D, _dp, _dp]),
         "or_lv_params_cev": (None, [I, _dp, D, D, D, D, D, _dp, _dp]),
         "or_plan_niter": (I, [D, D, D, C.POINTER(D)]),
@@ -114,6 +115,19 @@ def rs_generator(x, sigma_r, rate, q_off):
     g = np.zeros(d * d)
     lib.or_rs_generator(nx, nr, invm(x), np.ascontiguousarray(x),
                         np.ascontiguousarray(sigma_r, dtype=np.float64), rate, q_off, g)
+    return g.reshape(d, d, order="F")
+
+
+def sv_generator(nx, nr, xval, rval, sdrift, svol, volvol, voldrift, jm, jp):
+    """FP64 stochastic-volatility generator, d x d with d = nx*nr; every
+    argument a per-state array of d values (state y = x + nx*r)"""
+    d = nx * nr
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in
+            (xval, rval, sdrift, svol, volvol, voldrift, jm, jp)]
+    for a in arrs:
+        assert a.shape == (d,)
+    g = np.zeros(d * d)
+    lib.or_sv_generator(nx, nr, *arrs, g)
     return g.reshape(d, d, order="F")
 
 

@@ -605,3 +605,29 @@ def test_spot_prices(H, plus):
     got = out.download().reshape(len(rows), ns)
     ref = 0.97 * (U[rows].astype(np.float64) @ pay.astype(np.float64) + plus * pay[rows])
     assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("out_kind", [4, 1, 3, 0, 2])
+def test_sv_generator_vs_oracle(H, out_kind):
+    """ophip_dgen_sv (FP64 on the device: a row table, then one thread per
+    element) against the oracle's restatement of CSVModel.host_d_gen:
+    FP64 outputs within 1e-13 of the largest rate (exp and division
+    rounding), FP32 outputs one rounding of the FP64 value"""
+    banner(f"dgen_sv out_kind={out_kind}")
+    from _cases import sv_case
+    nx, nr, *arrs = case = sv_case(nx=48, nr=6, seed=3)
+    d = nx * nr
+    L = O.sv_generator(*case)
+    dt = 2.0 ** -16
+    hd = [dev(H, np.ascontiguousarray(a, dtype=np.float64)) for a in arrs]
+    dbl = out_kind in (1, 3, 4)
+    out = H.DeviceArray.empty((d, d), dtype=np.float64 if dbl else np.float32)
+    H.check(H.lib.ophip_dgen_sv(nx, nr, *[a.h for a in hd], dt, out_kind, out.h))
+    got = out.download()
+    want = {4: L, 1: np.eye(d) + dt * L, 3: dt * L, 0: np.eye(d) + dt * L, 2: dt * L}[out_kind]
+    scale = np.abs(want - (np.eye(d) if out_kind in (0, 1) else 0)).max()
+    if dbl:
+        assert np.abs(got - want).max() <= 1e-13 * max(scale, 1.0)
+    else:
+        # one FP32 rounding of the FP64 value (plus the FP64 rounding slack)
+        assert np.all(np.abs(got.astype(np.float64) - want) <= np.abs(want) * 2.0 ** -24 + 1e-13 * scale)

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+from _cases import sv_case
 from oplib_amd import opmodel as M
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
@@ -348,3 +349,65 @@ def test_bbf_vol_closed_forms():
         p = [0.3, beta, 0.0, 0.0, 0.0]
         inv = ((K / 100.0) ** (1 - beta) - 1.0) / (0.3 * (1 - beta) * math.log(K / 100.0))
         assert O.bbf_vol(p, 100.0, K) == pytest.approx(1.0 / inv, rel=1e-12)
+
+
+def test_sv_generator_invariants():
+    """rows sum to zero, off-diagonal rates are nonnegative, and every row
+    with a neighbour to correct through carries the prescribed drift"""
+    nx, nr, xv, rv, sd, sv, vv, vd, jm, jp = case = sv_case()
+    g = O.sv_generator(*case)
+    d = nx * nr
+    off = g - np.diag(np.diag(g))
+    assert off.min() >= 0.0
+    assert np.abs(g.sum(axis=1)).max() <= 1e-12 * np.abs(g).max()
+    drift = (g * (xv[None, :] - xv[:, None])).sum(axis=1)
+    xi = np.arange(d) % nx
+    interior = (xi > 0) & (xi < nx - 1)
+    assert np.allclose(drift[interior], sd[interior], rtol=1e-10, atol=1e-12 * np.abs(sd).max())
+
+
+def test_sv_generator_second_moment_closed_form():
+    """without jumps and regime dynamics each interior row solves the 2x2
+    system (zero drift, variance svol^2) and then moves the drift onto one
+    neighbour: sum L (x1-x0)^2 = svol^2 + sdrift * (dp if sdrift > 0 else dm)"""
+    nx, nr, xv, rv, sd, sv, vv, vd, jm, jp = case = sv_case(jumps=False, regimes=False)
+    g = O.sv_generator(*case)
+    d = nx * nr
+    for y in range(d):
+        x = y % nx
+        if x == 0 or x == nx - 1:
+            continue
+        dm, dp = xv[y - 1] - xv[y], xv[y + 1] - xv[y]
+        m2 = (g[y] * (xv - xv[y]) ** 2).sum()
+        want = sv[y] ** 2 + sd[y] * (dp if sd[y] > 0 else dm)
+        assert abs(m2 - want) <= 1e-10 * want, (y, m2, want)
+        # only the two price neighbours and the diagonal
+        assert np.count_nonzero(g[y]) <= 3
+
+
+def test_sv_generator_regime_rates_and_jumps():
+    """interior regimes: the two regime rates carry the regime drift and
+    variance; a rate up to regime r+1 is mirrored by jumps into regime r+1
+    of the same total, with the exponential profile of the scaled sizes"""
+    nx, nr, xv, rv, sd, sv, vv, vd, jm, jp = case = sv_case()
+    g = O.sv_generator(*case)
+    seen = 0
+    for y in range(nx * nr):
+        x, r = y % nx, y // nx
+        if not (0 < r < nr - 1):
+            continue
+        bm, bp = g[y, y - nx], g[y, y + nx]
+        em, ep = rv[y - nx] - rv[y], rv[y + nx] - rv[y]
+        if bm > 0 and bp > 0:
+            assert abs(bm * em + bp * ep - vd[y]) <= 1e-12 * (abs(vd[y]) + 1)
+            assert abs(bm * em ** 2 + bp * ep ** 2 - vv[y] ** 2) <= 1e-12
+        if bp > 0 and 2 < x < nx - 2:
+            up = g[y, nx * (r + 1):nx * (r + 2)].copy()
+            jumps = up.sum() - up[x] - up[x - 1]
+            assert up[x - 1] == 0.0
+            assert abs(jumps - bp) <= 1e-12 * bp, (y, jumps, bp)
+            h = xv[1] - xv[0]
+            assert np.isclose(up[x + 2] / up[x + 1], np.exp(-h / (jp[y] * xv[y])), rtol=1e-12)
+            assert np.isclose(up[x - 3] / up[x - 2], np.exp(-h / (jm[y] * xv[y])), rtol=1e-12)
+            seen += 1
+    assert seen > 10
