@@ -325,6 +325,17 @@ int ophip_mc1f_rows(unsigned mt, int y0, unsigned y_sk, int nscen, int nk, int d
  * min{j : b/256 < C[j]}; the guided sampler returns the same states as
  * ophip_mc1f_rows, bit for bit, with fewer dependent table probes */
 int ophip_mc_guide(int d, int nm, unsigned crow, unsigned guide);
+/* The reference's GPU generator (smc.cu:87-156, 664-736; MersenneTwister.h):
+ * 4096 twisted-GFSR lanes of period 2^607-1 (19-word states, lane-major
+ * st[lane + 4096*i]) with per-lane parameters in MersenneTwister.dat's format
+ * (4096 x {matrix_a, mask_b, mask_c, seed} uint32; ophip_dcmt_load returns 1
+ * on a size mismatch).  ophip_dcmt_setseed: states from 4096 host seeds;
+ * ophip_dcmt_uniforms: nscen (% 4096 == 0) uniforms x/2^32, lane-interleaved
+ * unif[lane + 4096*c], as opcuda_mt_benchmark's stream in the reference. */
+int ophip_dcmt_load(const void *table, long bytes);
+int ophip_dcmt_status_sz(void);
+int ophip_dcmt_setseed(const unsigned *seeds, unsigned st);
+int ophip_dcmt_uniforms(unsigned st, unsigned unif, int nscen);
 /* Monte-Carlo payoffs evaluated on the device instead of after a copy of
  * every batch of states to the host (smc.cu:653, CModel Device.cs:352-363;
  * VanillaPayoffEvaluator OPPricer.cs:22-53, discounting OPPricer.cs:143-146):

@@ -27,6 +27,10 @@
 
 #include <vector>
 
+extern "C" unsigned opcuda_mem_alloc(unsigned sz);
+extern "C" unsigned opcuda_get_status(void);
+extern "C" int opcuda_memcpy_h2d(unsigned d, unsigned long h, unsigned sz);
+
 using namespace ophip;
 
 namespace {
@@ -486,5 +490,114 @@ OPHIP_EXPORT int ophip_mc_payoff(unsigned y_skp, int nscen, int nk, int nsteps, 
                        P<const int16_t>(y_skp), (size_t)nscen, P<const float>(xvalp), d, a, nchunk, part);
     hipLaunchKernelGGL(k_mc_pay_sum, dim3((unsigned)nsteps), dim3(64), 0, D.stream, (size_t)nscen, a, nchunk,
                        (const double *)part, P<double>(outp));
+    return (int)check_launch();
+}
+
+// ---------------------------------------------------------------------
+// The reference's GPU generator: 4096 twisted-GFSR lanes of period
+// 2^607 - 1 (DCMT-607: 19-word states, MersenneTwister.h:8-29) with
+// per-lane parameters {matrix_a, mask_b, mask_c, seed} in the on-disk
+// format of MersenneTwister.dat (16 bytes per lane, 4096 lanes).  The
+// reference's loader copies its argument onto itself (smc.cu:87-96), so
+// its lanes run with zero parameters; here the table is loaded.  Seeding
+// (smc.cu:118-156) and the uniform stream (smc.cu:664-736: lane-
+// interleaved unif[lane + 4096 c] = x / 2^32) follow the reference,
+// including its circular-buffer index restarting at 0 on every call.  The
+// state buffer is lane-major, st[lane + 4096 i], 19 * 4096 words.
+// ---------------------------------------------------------------------
+namespace {
+
+constexpr int kDcN = 19, kDcM = 9, kDcLanes = 4096;
+
+struct DcmtParam {
+    uint32_t matrix_a, mask_b, mask_c, seed;
+};
+unsigned g_dcmt[16] = {0};  // per device: arena handle of the loaded table
+
+__global__ void k_dcmt_seed(const uint32_t *__restrict__ seeds, uint32_t *__restrict__ st) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= kDcLanes) return;
+    uint32_t v = seeds[l];
+    st[l] = v;
+    for (int i = 1; i < kDcN; i++) {
+        v = 1812433253u * (v ^ (v >> 30)) + (uint32_t)i;
+        st[l + kDcLanes * i] = v;
+    }
+}
+
+// one thread per lane; the circular buffer is walked in whole turns of 19
+// with compile-time indices (the state stays in registers)
+__global__ __launch_bounds__(64) void k_dcmt_unif(const DcmtParam *__restrict__ prm, uint32_t *__restrict__ st,
+                                                  int count, float *__restrict__ unif) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= kDcLanes) return;
+    const DcmtParam p = prm[l];
+    uint32_t s[kDcN];
+#pragma unroll
+    for (int i = 0; i < kDcN; i++) s[i] = st[l + kDcLanes * i];
+    for (int c0 = 0; c0 < count; c0 += kDcN) {
+#pragma unroll
+        for (int i = 0; i < kDcN; i++) {
+            if (c0 + i < count) {
+                const uint32_t y = (s[i] & 0xfffffffeu) | (s[(i + 1) % kDcN] & 0x1u);
+                uint32_t x = s[(i + kDcM) % kDcN] ^ (y >> 1) ^ ((y & 1u) ? p.matrix_a : 0u);
+                s[i] = x;
+                x ^= x >> 12;
+                x ^= (x << 7) & p.mask_b;
+                x ^= (x << 15) & p.mask_c;
+                x ^= x >> 18;
+                unif[(size_t)l + (size_t)kDcLanes * (c0 + i)] = (float)x / 4294967296.0f;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kDcN; i++) st[l + kDcLanes * i] = s[i];
+}
+
+}  // namespace
+
+// the parameter table, MersenneTwister.dat's format: 4096 records of four
+// uint32 {matrix_a, mask_b, mask_c, seed}; bytes must be 65536.  Returns 0,
+// or 1 on a size mismatch (the reference's convention, smc.cu:87-96).
+OPHIP_EXPORT int ophip_dcmt_load(const void *table, long bytes) {
+    if (!table || bytes != (long)(sizeof(DcmtParam) * kDcLanes)) return 1;
+    Device &D = dev();
+    if (D.id < 0 || D.id >= 16) return (int)set_status(kErrNoDevice);
+    unsigned &h = g_dcmt[D.id];
+    if (!h) h = opcuda_mem_alloc((unsigned)bytes);
+    if (!h) return (int)opcuda_get_status();
+    return opcuda_memcpy_h2d(h, (unsigned long)(uintptr_t)table, (unsigned)bytes);
+}
+
+// words of a lane state buffer (19 per lane, 4096 lanes)
+OPHIP_EXPORT int ophip_dcmt_status_sz(void) { return kDcN * kDcLanes; }
+
+// lane states from 4096 host seeds (smc.cu:118-156)
+OPHIP_EXPORT int ophip_dcmt_setseed(const unsigned *seeds, unsigned st) {
+    if (!seeds) return (int)set_status(kErrBadArg);
+    if (!handle_range_ok(st, 4 * (size_t)kDcN * kDcLanes)) return (int)set_status(kErrBadHandle);
+    uint32_t *sd = (uint32_t *)workspace(4 * (size_t)kDcLanes);
+    if (!sd) return (int)kErrOutOfArena;
+    Device &D = dev();
+    hipError_t e = hipMemcpyAsync(sd, seeds, 4 * (size_t)kDcLanes, hipMemcpyHostToDevice, D.stream);
+    if (e != hipSuccess) return (int)set_status(hip_status(e));
+    hipLaunchKernelGGL(k_dcmt_seed, dim3(kDcLanes / 64), dim3(64), 0, D.stream, (const uint32_t *)sd, P<uint32_t>(st));
+    e = hipStreamSynchronize(D.stream);  // the seeds' staging buffer is reused
+    if (e != hipSuccess) return (int)set_status(hip_status(e));
+    return (int)check_launch();
+}
+
+// nscen uniforms x / 2^32 from the loaded lanes, lane-interleaved:
+// unif[lane + 4096 c], c < nscen / 4096 (opcuda_mt_benchmark's stream,
+// smc.cu:664-736).  nscen % 4096 == 0.
+OPHIP_EXPORT int ophip_dcmt_uniforms(unsigned st, unsigned unif, int nscen) {
+    Device &D = dev();
+    if (nscen <= 0 || nscen % kDcLanes) return (int)set_status(kErrBadArg);
+    const unsigned tab = (D.id >= 0 && D.id < 16) ? g_dcmt[D.id] : 0;
+    if (!tab) return (int)set_status(kErrBadArg);  // no table loaded
+    if (!handle_range_ok(st, 4 * (size_t)kDcN * kDcLanes) || !handle_range_ok(unif, 4 * (size_t)nscen))
+        return (int)set_status(kErrBadHandle);
+    hipLaunchKernelGGL(k_dcmt_unif, dim3(kDcLanes / 64), dim3(64), 0, D.stream, P<const DcmtParam>(tab),
+                       P<uint32_t>(st), nscen / kDcLanes, P<float>(unif));
     return (int)check_launch();
 }

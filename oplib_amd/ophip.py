@@ -131,6 +131,10 @@ SIGNATURES = {
     "ophip_mc_set_nrng": (I32, [I32]),
     "ophip_mc1f_rows": (I32, [U32, I32, U32, I32, I32, I32, U32, U32, ULONG]),
     "ophip_mc_guide": (I32, [I32, I32, U32, U32]),
+    "ophip_dcmt_load": (I32, [C.c_void_p, C.c_long]),
+    "ophip_dcmt_status_sz": (I32, []),
+    "ophip_dcmt_setseed": (I32, [C.POINTER(U32), U32]),
+    "ophip_dcmt_uniforms": (I32, [U32, U32, I32]),
     "ophip_mc_payoff": (I32, [U32, I32, I32, I32, C.POINTER(I32), C.POINTER(F64), U32, I32,
                               C.POINTER(F32), I32, U32]),
     "ophip_mc1f_guided": (I32, [U32, I32, U32, I32, I32, I32, U32, U32, U32]),

@@ -805,3 +805,54 @@ void or_sv_generator(int nx, int nr, const double *xval, const double *rval,
         }
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* Twisted GFSR lanes of the reference's GPU generator (smc.cu:118-133, */
+/* 664-728; MersenneTwister.h:8-29): per lane a state of n words, the   */
+/* recurrence x_i = x_{i+m} ^ ((x_i & umask | x_{i+1} & lmask) >> 1) ^  */
+/* (odd ? matrix_a : 0) and the tempering with shifts (s0, 7, 15, s1)    */
+/* and masks (mask_b, mask_c), stepped through a circular buffer whose  */
+/* index restarts at 0 on every call (the reference keeps no index).   */
+/* The DCMT-607 lanes use n = 19, m = 9, umask = 0xfffffffe, lmask = 1, */
+/* s0 = 12, s1 = 18; with n = 624, m = 397, umask 0x80000000, s0 = 11   */
+/* and the MT19937 constants the same code is MT19937 (pinned by its    */
+/* known answer).  state: lane-major, st[lane + nlanes * i].            */
+/* ------------------------------------------------------------------ */
+void or_tgfsr_run(int nlanes, int n, int m, uint32_t umask, uint32_t lmask, int s0, int s1,
+                  const uint32_t *matrix_a, const uint32_t *mask_b, const uint32_t *mask_c,
+                  uint32_t *st, int count, uint32_t *out)
+{
+    uint32_t *s = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)n);
+    for (int l = 0; l < nlanes; l++) {
+        for (int i = 0; i < n; i++) s[i] = st[l + (size_t)nlanes * i];
+        int i = 0;
+        for (int c = 0; c < count; c++) {
+            int i1 = i + 1 >= n ? i + 1 - n : i + 1;
+            int im = i + m >= n ? i + m - n : i + m;
+            uint32_t y = (s[i] & umask) | (s[i1] & lmask);
+            uint32_t x = s[im] ^ (y >> 1) ^ ((y & 1u) ? matrix_a[l] : 0u);
+            s[i] = x;
+            i = i1;
+            x ^= x >> s0;
+            x ^= (x << 7) & mask_b[l];
+            x ^= (x << 15) & mask_c[l];
+            x ^= x >> s1;
+            out[l + (size_t)nlanes * c] = x;
+        }
+        for (int k = 0; k < n; k++) st[l + (size_t)nlanes * k] = s[k];
+    }
+    free(s);
+}
+
+/* lane states from seeds (smc.cu:118-133): st[l] = seed[l], then the
+ * init_genrand recurrence over n words, lane-major */
+void or_tgfsr_seed(int nlanes, int n, const uint32_t *seed, uint32_t *st)
+{
+    for (int l = 0; l < nlanes; l++) {
+        st[l] = seed[l];
+        for (int i = 1; i < n; i++) {
+            uint32_t p = st[l + (size_t)nlanes * (i - 1)];
+            st[l + (size_t)nlanes * i] = 1812433253u * (p ^ (p >> 30)) + (uint32_t)i;
+        }
+    }
+}

@@ -28,6 +28,7 @@
  *   or_bs_call           bstest's published value (S=K=100,r=.05,s=.25,T=1:
  *                        12.336) and put-call parity
  *   or_mlp_*             finite differences and the closed-form Adam step
+ *   or_tgfsr_*           with MT19937's parameters the MT19937 known answers
  */
 #ifndef OPLIB_ORACLE_H
 #define OPLIB_ORACLE_H
@@ -116,6 +117,12 @@ void or_adam_step(int n, double *p, const double *g, double *m, double *v,
                   double lr, double beta1, double beta2, double eps, int t);
 double or_gelu(double x);
 double or_gelu_grad(double x);
+
+/* ---- twisted GFSR lanes of the reference GPU generator (F4) ---- */
+void or_tgfsr_run(int nlanes, int n, int m, uint32_t umask, uint32_t lmask, int s0, int s1,
+                  const uint32_t *matrix_a, const uint32_t *mask_b, const uint32_t *mask_c,
+                  uint32_t *st, int count, uint32_t *out);
+void or_tgfsr_seed(int nlanes, int n, const uint32_t *seed, uint32_t *st);
 
 #ifdef __cplusplus
 }

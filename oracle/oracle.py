@@ -60,6 +60,8 @@ def _load():
         "or_sck_rows_f32": (None, [I, I, _fp, _fp]),
         "or_mt19937_init": (None, [U, _up]),
         "or_mt19937_next": (U, [_up]),
+        "or_tgfsr_run": (None, [I, I, I, U, U, I, I, _up, _up, _up, _up, I, _up]),
+        "or_tgfsr_seed": (None, [I, I, _up, _up]),
         "or_u01_from_bits": (F, [U]),
         "or_mt_uniforms": (None, [U, I, _fp]),
         "or_sample_linear": (I, [_fp, I, F]),
@@ -190,6 +192,28 @@ def mt_raw(seed, n):
     st = np.zeros(625, dtype=np.uint32)
     lib.or_mt19937_init(seed, st)
     return np.array([lib.or_mt19937_next(st) for _ in range(n)], dtype=np.uint32)
+
+
+DCMT607 = dict(n=19, m=9, umask=0xFFFFFFFE, lmask=0x1, s0=12, s1=18)
+MT19937 = dict(n=624, m=397, umask=0x80000000, lmask=0x7FFFFFFF, s0=11, s1=18)
+
+
+def tgfsr_seed(seeds, n):
+    """lane states (lane-major) from per-lane seeds"""
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
+    st = np.zeros(len(seeds) * n, dtype=np.uint32)
+    lib.or_tgfsr_seed(len(seeds), n, seeds, st)
+    return st
+
+
+def tgfsr_run(params, st, count, matrix_a, mask_b, mask_c):
+    """count raw outputs per lane, out[c][lane]; st (lane-major) is advanced"""
+    nl = len(matrix_a)
+    out = np.zeros(nl * count, dtype=np.uint32)
+    ma, mb, mc = (np.ascontiguousarray(a, dtype=np.uint32) for a in (matrix_a, mask_b, mask_c))
+    lib.or_tgfsr_run(nl, params["n"], params["m"], params["umask"], params["lmask"], params["s0"],
+                     params["s1"], ma, mb, mc, st, count, out)
+    return out.reshape(count, nl)
 
 
 def mc_chain(nstreams, seed0, groups, nk, d, y0, ctpk_flat, m_k, layout_rows=0):

@@ -631,3 +631,35 @@ def test_sv_generator_vs_oracle(H, out_kind):
     else:
         # one FP32 rounding of the FP64 value (plus the FP64 rounding slack)
         assert np.all(np.abs(got.astype(np.float64) - want) <= np.abs(want) * 2.0 ** -24 + 1e-13 * scale)
+
+
+def test_dcmt_lanes_bit_exact(H):
+    """the reference's 4096 DCMT-607 lanes from a parameter table in
+    MersenneTwister.dat's format: seeding, two calls of uniforms (25 per
+    lane: not a whole turn of the 19-word buffer, whose index restarts at 0
+    on every call) and the saved states, bit for bit against the oracle"""
+    banner("dcmt lanes")
+    import ctypes as C
+    rng = np.random.default_rng(607)
+    n = 4096
+    table = np.zeros((n, 4), dtype=np.uint32)
+    table[:, 0] = rng.integers(0, 2**31, n, dtype=np.uint32) | np.uint32(0x80000000)  # matrix_a
+    table[:, 1] = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)       # mask_b
+    table[:, 2] = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)       # mask_c
+    table[:, 3] = np.arange(n, dtype=np.uint32) + 4172
+    assert H.lib.ophip_dcmt_load(table.ctypes.data, table.nbytes - 16) == 1
+    assert H.lib.ophip_dcmt_load(table.ctypes.data, table.nbytes) == 0
+    assert H.lib.ophip_dcmt_status_sz() == 19 * n
+    seeds = (5489 + np.arange(n)).astype(np.uint32)
+    st = H.DeviceArray.empty(19 * n, dtype=np.uint32)
+    H.check(H.lib.ophip_dcmt_setseed(seeds.ctypes.data_as(C.POINTER(C.c_uint)), st.h))
+    ost = O.tgfsr_seed(seeds, 19)
+    assert np.array_equal(st.download(), ost)
+    per = 25
+    u = H.DeviceArray.empty(n * per)
+    for call in range(2):
+        H.check(H.lib.ophip_dcmt_uniforms(st.h, u.h, n * per))
+        raw = O.tgfsr_run(O.DCMT607, ost, per, table[:, 0], table[:, 1], table[:, 2])
+        want = raw.astype(np.float32) / np.float32(4294967296.0)
+        assert np.array_equal(u.download().reshape(per, n), want), call
+        assert np.array_equal(st.download(), ost), call

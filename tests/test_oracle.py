@@ -411,3 +411,14 @@ def test_sv_generator_regime_rates_and_jumps():
             assert np.isclose(up[x - 3] / up[x - 2], np.exp(-h / (jm[y] * xv[y])), rtol=1e-12)
             seen += 1
     assert seen > 10
+
+
+def test_tgfsr_is_mt19937_with_its_parameters():
+    """the generic twisted-GFSR lane code with MT19937's parameters is
+    MT19937: the C++-standard known answers and the oracle's own stream"""
+    st = O.tgfsr_seed([5489], 624)
+    out = O.tgfsr_run(O.MT19937, st, 10000, [0x9908B0DF], [0x9D2C5680], [0xEFC60000])[:, 0]
+    assert out[0] == 3499211612 and out[9999] == 4123659995
+    st1 = O.tgfsr_seed([1], 624)
+    assert np.array_equal(O.tgfsr_run(O.MT19937, st1, 2000, [0x9908B0DF], [0x9D2C5680], [0xEFC60000])[:, 0],
+                          O.mt_raw(1, 2000))
