@@ -67,8 +67,8 @@ def traffic_from_profiles(kernel):
 
 def roofline_backward(best, methods, wbk, bi_gbs, bi_us, bi_bytes):
     """roofline entry of the headline method's dominant kernel: the FP16 x3
-    band-sparse backward step (per launch: the nonzero fragments of every
-    lattice's hi/lo copies and the live value columns), or the FP32
+    band-sparse backward step (per launch: every lattice's visited hi
+    fragments and read lo fragments, and the live value columns), or the FP32
     skinny step"""
     if best.startswith("wform_fp16x3"):
         nb = methods[best].get("group", 1)
@@ -79,7 +79,8 @@ def roofline_backward(best, methods, wbk, bi_gbs, bi_us, bi_bytes):
                 "traffic": traffic_from_profiles(key),
                 "bytes_per_launch": w["bytes_per_launch"], "avg_launch_us": w["avg_launch_us"],
                 "dense_bytes_per_launch": w["dense_bytes_per_launch"],
-                "nonzero_fragment_fraction": w["nonzero_fragment_fraction"]}
+                "visited_fragment_fraction": w["visited_fragment_fraction"],
+                "lo_read_fraction": w["lo_read_fraction"]}
     k = "backward step: k_skinny<3> + k_skinny_combine"
     return {"bound": "hbm", "kernel": k, "achieved": bi_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": bi_gbs / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": bi_bytes,
@@ -142,7 +143,7 @@ def main():
                                    f"{H.lib.ophip_wsq_bwd_kernel_for(nb, 2048, 48)}) x20, "
                                    "k_copy4 (64 MiB) x20",
                           "step_bytes_nonzero": wb.bytes, "step_bytes_dense": wb.bytes_dense,
-                          "nonzero_fragment_fraction": wb.nz_frac,
+                          "visited_fragment_fraction": wb.nz_frac, "lo_read_fraction": wb.lo_frac,
                           "copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
         return
     log(f"[bench] workload rs2048 d={d} nsq={cfg.nsq} steps={cfg.nsteps} strikes={cfg.nstrikes}")
@@ -234,12 +235,12 @@ def main():
                    "kernel_id": kid,
                    "avg_launch_us": us, "bytes_per_launch": wb.bytes,
                    "dense_bytes_per_launch": wb.bytes_dense,
-                   "nonzero_fragment_fraction": wb.nz_frac,
+                   "visited_fragment_fraction": wb.nz_frac, "lo_read_fraction": wb.lo_frac,
                    "GBps": wb.bytes / (us * 1e-6) / 1e9,
                    "dense_equivalent_GBps": wb.bytes_dense / (us * 1e-6) / 1e9}
-        log(f"[bench] FP16x3 backward step x{nb}: {us:.2f} us, {wbk[nb]['GBps']:.0f} GB/s of nonzero "
-            f"fragments ({wb.nz_frac:.3f} of the matrix), {wbk[nb]['dense_equivalent_GBps']:.0f} GB/s "
-            "dense-equivalent")
+        log(f"[bench] FP16x3 backward step x{nb}: {us:.2f} us, {wbk[nb]['GBps']:.0f} GB/s of visited "
+            f"fragments ({wb.nz_frac:.3f} of the matrix, lo parts {wb.lo_frac:.3f}), "
+            f"{wbk[nb]['dense_equivalent_GBps']:.0f} GB/s dense-equivalent")
         del wb
 
     # ---- precision study: the same generator to the 12 monthly maturities
@@ -262,6 +263,7 @@ def main():
 
     # ---- Monte Carlo on the daily table ----
     mc = None
+    U_host = None
     if not args.no_mc:
         # the daily kernel of the FP16 x3 method (within 3e-6 of FP64), so
         # that the Monte Carlo prices can be held against the FP64 oracle
@@ -270,7 +272,8 @@ def main():
         sw = M.Solver(cw)
         sw.solve()
         Uw = H.DeviceArray.empty(d * d)
-        Uw.upload(np.ascontiguousarray(sw.kernel().reshape(-1, order="F")))
+        U_host = sw.kernel()
+        Uw.upload(np.ascontiguousarray(U_host.reshape(-1, order="F")))
         sw.close()
         mc = bench_mc(H, Uw, d, cfg, gold[1, 0, :])
 
@@ -285,7 +288,7 @@ def main():
     # ---- CPU baseline: the oracle's FP64 solve of the same scenario ----
     cpu = None
     if not args.no_cpu:
-        cpu = bench_cpu(cfg)
+        cpu = bench_cpu(cfg, U_host)
 
     # the headline: the fastest method on this workload whose prices are
     # within 1e-4 of FP64 (the plain FP32 method is reported beside it)
@@ -355,7 +358,7 @@ class WformBackward:
     """the FP16 x3 method's dominant kernel on its real operands: each
     lattice (a 2048-state regime scenario) squared 16 times in the W form,
     W's fragment copies and tile lists laid out, then the band-sparse
-    backward step (k_wbwd_sp) of all of them in one launch, alternating two
+    backward step (k_wbwd_blk or k_wbwd_sp) of all of them in one launch, alternating two
     value buffers per lattice"""
 
     def __init__(self, H, M, cfgs, ncp=48):
@@ -382,13 +385,16 @@ class WformBackward:
         cfg = cfgs[0]
         self.buf = cfg.nsq & 1
         nkt = d // 32
-        self.nz = 0  # nonzero 16 x 32 fragments over all lattices
+        self.nz = 0  # visited 16 x 32 fragments over all lattices
+        self.nlo = 0  # of which the lo part is read
         for st in self.states:
             H.check(H.lib.ophip_wsq_bwd_layout(st.h, d, mode, self.buf))
             tl = np.empty((d // 16) * (nkt + 1), dtype=np.int32)
             H.check(H.lib.opcuda_memcpy_d2h(H.lib.ophip_wsq_array(st.h, d, mode, 0, 7),
                                             tl.ctypes.data, tl.nbytes))
-            self.nz += int(tl.reshape(d // 16, nkt + 1)[:, 0].sum())
+            tl = tl.reshape(d // 16, nkt + 1)
+            self.nz += int(tl[:, 0].sum())
+            self.nlo += sum(int((tl[i, 1:1 + tl[i, 0]] >= 65536).sum()) for i in range(d // 16))
         H.sync()
         xs = np.tile(x, cfg.nr)
         pay = np.zeros((d, ncp), np.float32)
@@ -407,11 +413,12 @@ class WformBackward:
         self.t = 0
         # bytes per launch: dense algorithmic (the hi/lo copies of every W,
         # 4 d^2 each, and the live value columns read and written) and the
-        # part the band-sparse kernel must read (the nonzero fragments, 2 KB
-        # each with hi and lo)
+        # part the band-sparse kernel must read (1 KB per visited fragment's
+        # hi part and per read lo part)
         self.bytes_dense = self.n * (4.0 * d * d + 4.0 * d * cfg.nstrikes * 2)
-        self.bytes = 2048.0 * self.nz + self.n * 4.0 * d * cfg.nstrikes * 2
+        self.bytes = 1024.0 * (self.nz + self.nlo) + self.n * 4.0 * d * cfg.nstrikes * 2
         self.nz_frac = self.nz / (self.n * (d // 16) * nkt)
+        self.lo_frac = self.nlo / (self.n * (d // 16) * nkt)
 
     def step(self):
         a, b = self.t & 1, (self.t + 1) & 1
@@ -682,7 +689,7 @@ def bench_surrogate(H, M, ntrain, steps, log_fn):
     return out
 
 
-def bench_cpu(cfg):
+def bench_cpu(cfg, U_daily=None):
     from oracle import oracle as O
     nth = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
     O.set_threads(nth)
@@ -695,10 +702,62 @@ def bench_cpu(cfg):
         if t >= 10.0 or k >= 8:
             break
     log(f"[bench] CPU FP64 oracle: {k} solves in {t:.2f} s on {nth} threads")
-    return {"value": k / t, "unit": "scenarios/s", "cores": nth, "kind": "port",
-            "sample": f"{k} full FP64 solves of the same scenario (assembly, first step, "
-                      "16 DGEMM squarings at d=2048, 252 backward steps x 41 strikes) by the "
-                      "C oracle with OpenMP", "seconds": t}
+    out = {"value": k / t, "unit": "scenarios/s", "cores": nth, "kind": "port",
+           "sample": f"{k} full FP64 solves of the same scenario (assembly, first step, "
+                     "16 DGEMM squarings at d=2048, 252 backward steps x 41 strikes) by the "
+                     "C oracle with OpenMP", "seconds": t}
+    out["components"] = cpu_components(O, nth, U_daily)
+    return out
+
+
+def cpu_components(O, nth, U_daily):
+    """the reference's CPU benchmark programs restated, on the host cores:
+    cmt (MT uniforms), csgemm (SGEMM d = 576 x 100, the host BLAS) and the
+    clv-style chain sampler (bracket tables + bisection, rangen.cpp:96-153)
+    on the daily 2048-state kernel"""
+    import ctypes as C
+    import threading
+    comp = {}
+    # cmt: 2^20 uniforms per thread, one MT19937 stream each
+    n = 1 << 20
+    bufs = [np.zeros(n, dtype=np.float32) for _ in range(nth)]
+    ths = [threading.Thread(target=O.lib.or_mt_uniforms, args=(5489 + i, n, bufs[i])) for i in range(nth)]
+    t0 = time.perf_counter()
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    t = time.perf_counter() - t0
+    comp["cmt"] = {"value": nth * n / t / 1e6, "unit": "M uniforms/s", "threads": nth,
+                   "sample": f"{nth} x 2^20 MT19937 uniforms"}
+    # csgemm: d = 576 SGEMM x 100 on the host BLAS (numpy)
+    a = np.random.default_rng(0).standard_normal((576, 576)).astype(np.float32)
+    b = a.T.copy()
+    a @ b
+    t0 = time.perf_counter()
+    for _ in range(100):
+        c = a @ b
+    t = time.perf_counter() - t0
+    comp["csgemm"] = {"value": 100 * 2.0 * 576 ** 3 / t / 1e9, "unit": "GFLOP/s",
+                      "sample": "SGEMM d=576 x 100 through numpy's BLAS"}
+    # clv-style sampler on the daily kernel: 64 blocks x 2048 paths x 252 steps
+    if U_daily is not None:
+        d = U_daily.shape[0]
+        t0 = time.perf_counter()
+        y, ct, hs = O.mc_hash(U_daily[None].astype(np.float64), 1, 64, 64 + 256, [0], 1)
+        tb = time.perf_counter() - t0
+        mk = np.zeros(252, dtype=np.int32)
+        nb, ns = 64, 2048
+        ys = np.zeros(nb * 252 * ns, dtype=np.int16)
+        t0 = time.perf_counter()
+        O.lib.or_mc_hash_run(d, ct, hs, nb, ns, 252, 64 + 256, mk, 5489, ys)
+        t = time.perf_counter() - t0
+        comp["clv_mc"] = {"value": nb * ns * 252 / t / 1e6, "unit": "M transitions/s", "threads": nth,
+                          "table_build_s": tb,
+                          "sample": f"{nb} x {ns} paths x 252 steps on the daily 2048-state kernel, "
+                                    "bracket tables + bisection, MT19937 per path block"}
+    log(f"[bench] CPU components: {comp}")
+    return comp
 
 
 if __name__ == "__main__":

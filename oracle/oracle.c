@@ -856,3 +856,82 @@ void or_tgfsr_seed(int nlanes, int n, const uint32_t *seed, uint32_t *st)
         }
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* The reference's CPU chain sampler (F4): hash-bracket tables           */
+/* (CModel Host Double.cs:403-458) and the sampler (rangen.cpp:96-153).  */
+/* ctpk[m][y1 + d*y2]: running FP64 row sums divided by the row total,  */
+/* capped at 1, column 0 left as the raw first entry (the reference     */
+/* starts the normalisation at y2 = 1); hash[y1 + d*s + d*d*m] = lb |    */
+/* ub << 16 brackets the states whose cumulative value crosses          */
+/* [s/d, (s+1)/d).  A draw u = x / 2^32 takes bracket s = (int)(u d),   */
+/* state 0 when u < ctpk[y][0] and lb == 0, else bisects (lb, ub] to ub. */
+/* Uniforms: MT19937 streams, one per block of nscen paths (seed0 + b;  */
+/* the reference's MKL MT2203 streams are not available), nscen words   */
+/* per step, step-major (viRngUniformBits per step, rangen.cpp:118).    */
+/* ------------------------------------------------------------------ */
+void or_mc_hash_build(int d, int nm, const double *tpk, double *ctpk, int *hash)
+{
+#pragma omp parallel for schedule(static)
+    for (int my = 0; my < nm * d; my++) {
+        int m = my / d, y1 = my % d;
+        const double *t = tpk + (size_t)d * d * m;
+        double *c = ctpk + (size_t)d * d * m;
+        double acc = t[y1];
+        c[y1] = acc;
+        for (int y2 = 1; y2 < d; y2++) {
+            acc += t[y1 + (size_t)d * y2];
+            c[y1 + (size_t)d * y2] = acc;
+        }
+        for (int y2 = 1; y2 < d; y2++) {
+            double v = c[y1 + (size_t)d * y2] / acc;
+            c[y1 + (size_t)d * y2] = v < 1.0 ? v : 1.0;
+        }
+        int lb = 0, ub = 0;
+        for (int s = 0; s < d; s++) {
+            for (; lb < d; lb++)
+                if (c[y1 + (size_t)d * lb] > (double)((float)s / (float)d)) break;
+            if (lb > 0) lb -= 1;
+            for (; ub < d; ub++)
+                if (c[y1 + (size_t)d * ub] > (double)((float)(s + 1) / (float)d)) break;
+            if (ub == d) ub -= 1;
+            if (c[y1 + (size_t)d * lb] >= 1.0) ub = lb;
+            hash[y1 + (size_t)d * s + (size_t)d * d * m] = lb + (ub << 16);
+        }
+    }
+}
+
+void or_mc_hash_run(int d, const double *ctpk, const int *hash, int nblocks, int nscen, int nk, int y0,
+                    const int *m_k, uint32_t seed0, int16_t *y_sk)
+{
+    const int niter = (int)(log((double)d) / log(2.0)) + 1;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < nblocks; b++) {
+        uint32_t st[625];
+        or_mt19937_init(seed0 + (uint32_t)b, st);
+        int16_t *yb = y_sk + (size_t)b * nk * nscen;
+        for (int k = 0; k < nk; k++) {
+            const int m = m_k[k];
+            const int *hs = hash + (size_t)d * d * m;
+            const double *cm = ctpk + (size_t)d * d * m;
+            for (int sc = 0; sc < nscen; sc++) {
+                const double u = (double)or_mt19937_next(st) / 4294967296.0;
+                const int yp = k > 0 ? yb[(size_t)(k - 1) * nscen + sc] : y0;
+                const int s = (int)(u * d);
+                const int h = hs[yp + (size_t)d * s];
+                int ub = h >> 16, lb = h - (ub << 16);
+                if (lb == 0 && u < cm[yp]) ub = 0;
+                if (ub - lb > 1) {
+                    const double *cp = cm + yp;
+                    for (int it = 0; it < niter; it++) {
+                        const int mid = (ub + lb) / 2;
+                        if (u < cp[(size_t)d * mid]) ub = mid;
+                        else lb = mid;
+                        if (ub - lb <= 1) break;
+                    }
+                }
+                yb[(size_t)k * nscen + sc] = (int16_t)ub;
+            }
+        }
+    }
+}

@@ -124,6 +124,11 @@ void or_tgfsr_run(int nlanes, int n, int m, uint32_t umask, uint32_t lmask, int 
                   uint32_t *st, int count, uint32_t *out);
 void or_tgfsr_seed(int nlanes, int n, const uint32_t *seed, uint32_t *st);
 
+/* ---- the reference's CPU hash-bracket chain sampler (F4) ---- */
+void or_mc_hash_build(int d, int nm, const double *tpk, double *ctpk, int *hash);
+void or_mc_hash_run(int d, const double *ctpk, const int *hash, int nblocks, int nscen, int nk, int y0,
+                    const int *m_k, uint32_t seed0, int16_t *y_sk);
+
 #ifdef __cplusplus
 }
 #endif

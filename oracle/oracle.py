@@ -62,6 +62,8 @@ def _load():
         "or_mt19937_next": (U, [_up]),
         "or_tgfsr_run": (None, [I, I, I, U, U, I, I, _up, _up, _up, _up, I, _up]),
         "or_tgfsr_seed": (None, [I, I, _up, _up]),
+        "or_mc_hash_build": (None, [I, I, _dp, _dp, _ip]),
+        "or_mc_hash_run": (None, [I, _dp, _ip, I, I, I, I, _ip, U, _sp]),
         "or_u01_from_bits": (F, [U]),
         "or_mt_uniforms": (None, [U, I, _fp]),
         "or_sample_linear": (I, [_fp, I, F]),
@@ -214,6 +216,20 @@ def tgfsr_run(params, st, count, matrix_a, mask_b, mask_c):
     lib.or_tgfsr_run(nl, params["n"], params["m"], params["umask"], params["lmask"], params["s0"],
                      params["s1"], ma, mb, mc, st, count, out)
     return out.reshape(count, nl)
+
+
+def mc_hash(tpk, nblocks, nscen, y0, m_k, seed0):
+    """the reference's CPU sampler: tpk (nm, d, d) FP64 with [m][y0, y]
+    semantics; returns (states [nblocks][nk][nscen], ctpk, hash)"""
+    nm, d, _ = tpk.shape
+    t = np.ascontiguousarray(np.stack([a.reshape(-1, order="F") for a in tpk]).reshape(-1), dtype=np.float64)
+    ct = np.zeros_like(t)
+    hs = np.zeros(nm * d * d, dtype=np.int32)
+    lib.or_mc_hash_build(d, nm, t, ct, hs)
+    mk = np.ascontiguousarray(m_k, dtype=np.int32)
+    y = np.zeros(nblocks * len(mk) * nscen, dtype=np.int16)
+    lib.or_mc_hash_run(d, ct, hs, nblocks, nscen, len(mk), y0, mk, seed0, y)
+    return y.reshape(nblocks, len(mk), nscen), ct, hs
 
 
 def mc_chain(nstreams, seed0, groups, nk, d, y0, ctpk_flat, m_k, layout_rows=0):

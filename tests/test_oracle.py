@@ -422,3 +422,27 @@ def test_tgfsr_is_mt19937_with_its_parameters():
     st1 = O.tgfsr_seed([1], 624)
     assert np.array_equal(O.tgfsr_run(O.MT19937, st1, 2000, [0x9908B0DF], [0x9D2C5680], [0xEFC60000])[:, 0],
                           O.mt_raw(1, 2000))
+
+
+def test_mc_hash_sampler_is_the_inverse_cdf():
+    """the reference's CPU sampler (bracket tables + bisection) returns the
+    inverse-CDF state min{j : u < C[y, j]} of the same uniforms: replayed
+    draw by draw with a linear scan on random stochastic tables"""
+    rng = np.random.default_rng(11)
+    nm, d, nk, nscen, nb = 2, 37, 5, 64, 3
+    P = rng.random((nm, d, d)) ** 6
+    P /= P.sum(axis=2, keepdims=True)
+    m_k = np.array([0, 1, 1, 0, 1], dtype=np.int32)
+    y, ct, _ = O.mc_hash(P, nb, nscen, 7, m_k, 1234)
+    C = np.cumsum(P, axis=2)
+    C = np.minimum(C / C[:, :, -1:], 1.0)
+    for b in range(nb):
+        st = O.mt_raw(1234 + b, nk * nscen).reshape(nk, nscen)
+        yp = np.full(nscen, 7)
+        for k in range(nk):
+            u = st[k].astype(np.float64) / 4294967296.0
+            row = C[m_k[k]][yp]
+            want = np.minimum((u[:, None] < row).argmax(axis=1) + 0 * yp, d - 1)
+            want = np.where((u[:, None] < row).any(axis=1), want, d - 1)
+            assert np.array_equal(y[b, k], want), (b, k)
+            yp = want
