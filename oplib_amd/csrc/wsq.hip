@@ -493,9 +493,15 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
 // Live items of a sparse squaring round, one wavefront per work item of
 // the round's static list (lane c: cell column c of A / cell row c of B).
 // An item is live if it has K cells, if its own input cells are nonzero
-// (W' = 2W + ...) or, when wskip is 0 (a lattice's first round: the
-// destination's flags are not yet those of its contents) or its
-// destination cells are nonzero (they must be overwritten with zeros).
+// (W' = 2W + ...), when wskip is 0, or if its destination cells are
+// flagged (they must be overwritten with zeros).  Buffer 1's flags start
+// as all ones when a plan is made (k_fill8) and every round keeps both
+// buffers' flags a superset of their nonzero cells, so a lattice's first
+// round of a later solve writes only where the previous solve left
+// values.  (The backward layout writes its fragment copies into the
+// column-major copy slots of the buffer the squarings did not end in:
+// those slots are rewritten by the round that next flags their cells, and
+// nothing reads an unflagged cell.)
 // Live items are appended to the list of their XCD (static position j % 8:
 // the xcd_order placement); the list order within an XCD does not change
 // any result.  Requires d <= 4096 (64 cells).
@@ -547,6 +553,11 @@ __global__ __launch_bounds__(1024) void k_wlive(char *__restrict__ base, const W
         for (int q = 0; q < w; q++) rank += slive[q];
         live[(size_t)x * cap + sbase + rank] = WLive{it.s, it.tm, it.tn, 0, cells, 0};
     }
+}
+
+// n bytes at p set to v (cell flags)
+__global__ __launch_bounds__(256) void k_fill8(uint8_t *__restrict__ p, int n, uint8_t v) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
 // diagonal correction: W'_ii -= sum_j W'_ij (fixed summation order).
@@ -831,43 +842,55 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd1(int d, float df, const uint16
 
 // ---------------------------------------------------------------------
 // Band-sparse backward step.  Outside a band around the diagonal of each
-// regime block the squared W is below 2^-51, where its FP16 hi and lo
-// copies are both zero: at the 2048-state lattice (d = 2048, U(1/252))
-// only 42% of the 16 x 32 fragments hold a nonzero value.  k_wbwd_flags
-// marks the fragments whose H or L part is nonzero and k_wbwd_lists lists
-// their k tiles; the step kernels visit only those (a skipped fragment's
-// products are exact zeros).
+// regime block the squared W is tiny: at the 2048-state lattice (d = 2048,
+// U(1/252)) 57% of the 16 x 32 fragments of its FP16 hi/lo copies are zero
+// and a third of the rest hold values below 2^-40.  k_wbwd_flags marks, per
+// fragment, whether a step visits it (some |W| >= 2^-40: its hi part is
+// read) and whether it also reads the lo part (some |W| >= 2^-28 in FP16,
+// 2^-32 in BF16); k_wbwd_lists lists the visited k tiles, each entry kt +
+// 65536 lo.  What a step leaves out is at most 2^-40 per entry of W: an
+// entry under 2^-40, or the lo part of a smaller entry, which is at most
+// half an ulp of its hi part (FP16 hi parts W 2^15 under 2^-13 are
+// multiples of 2^-24: 2^-25 2^-15; BF16 under 2^-32: 2^-9 2^-32).  A row
+// of the step then differs from the full three-product sum by at most
+// (entries left out) 2^-40 max|X|.
 // ---------------------------------------------------------------------
 // Lists: per 16-row tile (d/16 lists), then per 64-row block (d/64: the
 // union of its four tiles' lists) and per 32-row block (d/32: two tiles),
-// each nkt + 1 ints: the count, then the k tiles in increasing order.
+// each nkt + 1 ints: the count, then the k tiles in increasing order (a
+// block reads the lo parts of a k tile if one of its row tiles does).
 __device__ inline size_t tl_list4(int d) { return (size_t)(d / 16) * (d / 32 + 1); }
 __device__ inline size_t tl_list2(int d) { return tl_list4(d) + (size_t)(d / 64) * (d / 32 + 1); }
 
-// k_wbwd_flags: byte flags[it][kt] (d/16 x d/32) = 1 where the hi or lo
-// fragment (it, kt) holds a value other than +-0; a wave per (it, four k
-// tiles), all eight 16-byte loads in flight.
-__global__ __launch_bounds__(256) void k_wbwd_flags(int d, const uint16_t *__restrict__ Hf,
-                                                    const uint16_t *__restrict__ Lf,
-                                                    uint8_t *__restrict__ flags) {
+// the hi parts' magnitude bits (sign dropped) at which a fragment is
+// visited and at which its lo part is read.  FP16 (hi = W 2^15): 2^-40
+// 2^15 is below the smallest subnormal, so every nonzero hi part is
+// visited; 2^-28 2^15 = 2^-13, exponent field 2.  BF16 (hi = W): exponent
+// fields 87 (2^-40) and 95 (2^-32).
+constexpr uint32_t kVisitF16 = 1u, kLoF16 = 2u << 10;
+constexpr uint32_t kVisitBF16 = 87u << 7, kLoBF16 = 95u << 7;
+
+// k_wbwd_flags: byte flags[it][kt] (d/16 x d/32): bit 0 = the fragment is
+// visited, bit 1 = its lo part is read; a wave per (it, four k tiles), the
+// four 16-byte loads in flight.
+__global__ __launch_bounds__(256) void k_wbwd_flags(int d, const uint16_t *__restrict__ Hf, uint32_t tvis,
+                                                    uint32_t tlo, uint8_t *__restrict__ flags) {
     const int nkt = d / 32, lane = threadIdx.x & 63;
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);  // (it, k0 / 4)
     if (w >= d / 16 * (nkt / 4)) return;  // wave-uniform
     const int it = w / (nkt / 4), k0 = 4 * (w % (nkt / 4));
-    uint4 h[4], l[4];
+    uint4 h[4];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-        const size_t o = frag_off(it, k0 + u, lane, nkt);
-        h[u] = *(const uint4 *)(Hf + o);
-        l[u] = *(const uint4 *)(Lf + o);
-    }
+    for (int u = 0; u < 4; u++) h[u] = *(const uint4 *)(Hf + frag_off(it, k0 + u, lane, nkt));
     unsigned word = 0;
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-        // +-0 counts as zero: the sign bits of both 16-bit halves dropped
-        const unsigned v =
-            (h[u].x | h[u].y | h[u].z | h[u].w | l[u].x | l[u].y | l[u].z | l[u].w) & 0x7fff7fffu;
-        word |= (__any(v != 0u) ? 1u : 0u) << (8 * u);
+        const uint32_t q[4] = {h[u].x, h[u].y, h[u].z, h[u].w};
+        uint32_t m = 0;  // the largest magnitude bits of the lane's eight values
+#pragma unroll
+        for (int c = 0; c < 4; c++) m = max(m, max(q[c] & 0x7fffu, (q[c] >> 16) & 0x7fffu));
+        const unsigned f = (__any(m >= tvis) ? 1u : 0u) | (__any(m >= tlo) ? 2u : 0u);
+        word |= f << (8 * u);
     }
     if (lane == 0) *(unsigned *)(flags + (size_t)it * nkt + k0) = word;
 }
@@ -892,8 +915,8 @@ __global__ __launch_bounds__(256) void k_wbwd_lists(int d, const uint8_t *__rest
         int f = 0;
         if (kt < nkt)
             for (int q = 0; q < nr; q++) f |= flags[(size_t)(r0 + q) * nkt + kt];
-        const uint64_t m = __ballot(f != 0);
-        if (f) out[1 + n + __popcll(m & ((1ull << lane) - 1))] = kt;
+        const uint64_t m = __ballot((f & 1) != 0);
+        if (f & 1) out[1 + n + __popcll(m & ((1ull << lane) - 1))] = kt + ((f & 2) ? 65536 : 0);
         n += __popcll(m);
     }
     if (lane == 0) out[0] = n;
@@ -943,10 +966,10 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd_sp(char *__restrict__ base, co
         s8 a[E][4];
 #pragma unroll
         for (int e = 0; e < E; e++) {
-            const int kt = __builtin_amdgcn_readfirstlane(lst[1 + min(e0 + e * NW, cnt - 1)]);
-            const size_t k = (size_t)512 * kt;
+            const int ent = __builtin_amdgcn_readfirstlane(lst[1 + min(e0 + e * NW, cnt - 1)]);
+            const size_t k = (size_t)512 * (ent & 0xffff);
             a[e][0] = *(const s8 *)(ha + k);
-            a[e][1] = *(const s8 *)(la + k);
+            a[e][1] = (ent >> 16) ? *(const s8 *)(la + k) : s8{};  // uniform
             a[e][2] = *(const s8 *)(xh + k);
             a[e][3] = *(const s8 *)(xl + k);
         }
@@ -1052,12 +1075,13 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd_blk(char *__restrict__ base, c
         s8 w[E][2 * RB], x[E][2 * NT];
 #pragma unroll
         for (int e = 0; e < E; e++) {
-            const int kt = __builtin_amdgcn_readfirstlane(lst[1 + min(e0 + e * NW, cnt - 1)]);
-            const size_t k = (size_t)512 * kt;
+            const int ent = __builtin_amdgcn_readfirstlane(lst[1 + min(e0 + e * NW, cnt - 1)]);
+            const size_t k = (size_t)512 * (ent & 0xffff);
+            const bool lo = (ent >> 16) != 0;  // uniform
 #pragma unroll
             for (int i = 0; i < RB; i++) {
                 w[e][i] = *(const s8 *)(ha + i * rstride + k);
-                w[e][RB + i] = *(const s8 *)(la + i * rstride + k);
+                w[e][RB + i] = lo ? *(const s8 *)(la + i * rstride + k) : s8{};
             }
 #pragma unroll
             for (int j = 0; j < NT; j++) {
@@ -1402,6 +1426,19 @@ static WsqPlan *make_plan(int n, const unsigned *states, const int *ds, const in
         set_status(hip_status(e));
         return nullptr;
     }
+    // buffer 1's cell flags: "may be nonzero" everywhere until a round
+    // writes that buffer (ordered after earlier work on the states)
+    for (int i = 0; i < n; i++) {
+        const int nc = ds[i] / 64;
+        hipLaunchKernelGGL(k_fill8, dim3(cdiv(nc * nc, 256)), dim3(256), 0, D.stream,
+                           reinterpret_cast<uint8_t *>(ptr(states[i] + (unsigned)wsq_layout(ds[i], mode).fl[1])),
+                           nc * nc, (uint8_t)1);
+    }
+    if (check_launch()) {
+        (void)hipFree(P->dev);
+        delete P;
+        return nullptr;
+    }
     set_status(kOk);
     return P;
 }
@@ -1473,7 +1510,7 @@ OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
                 if (g > 8) g -= g % 8;
                 if (sparse) g = std::max(g, 8);
                 // work items carry s = src*n + i (the full table); the
-                // destination's flags describe its contents from round 1 on
+                // destination's flags cover its nonzero cells (k_wlive)
                 const WLive *lv = nullptr;
                 const int *cn = nullptr;
                 if (sparse) {
@@ -1481,12 +1518,10 @@ OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
                     cn = P->counts + 8 * c;
                     if (c == 1)
                         hipLaunchKernelGGL((k_wlive<2>), dim3(8 * cdiv(cdiv(nit, 8), 16)), dim3(1024), 0, D.stream, D.base,
-                                           P->sc, its, nit, P->live[c], P->cap[c], P->counts + 8 * c,
-                                           rr > 0 ? 1 : 0);
+                                           P->sc, its, nit, P->live[c], P->cap[c], P->counts + 8 * c, 1);
                     else
                         hipLaunchKernelGGL((k_wlive<1>), dim3(8 * cdiv(cdiv(nit, 8), 16)), dim3(1024), 0, D.stream, D.base,
-                                           P->sc, its, nit, P->live[c], P->cap[c], P->counts + 8 * c,
-                                           rr > 0 ? 1 : 0);
+                                           P->sc, its, nit, P->live[c], P->cap[c], P->counts + 8 * c, 1);
                 }
                 if (c == 1)
                     hipLaunchKernelGGL((k_wsq<BF, LO, 128, 8>), dim3(g), dim3(512), 0, D.stream, D.base,
@@ -1694,12 +1729,13 @@ OPHIP_EXPORT int ophip_wsq_bwd_prep(int d, int ncp, int mode, unsigned X, unsign
 }
 
 namespace {
-// the tile lists of fragment copies Hf/Lf (after their layout)
-void bwd_tile_lists(Device &D, const WsqLayout &L, unsigned state, int d, const uint16_t *Hf,
-                    const uint16_t *Lf) {
+// the tile lists of the fragment copies (after their layout), from the hi copy Hf
+void bwd_tile_lists(Device &D, const WsqLayout &L, unsigned state, int d, int mode, const uint16_t *Hf) {
     uint8_t *flags = P<uint8_t>(state + (unsigned)L.tf);
     const int waves = d / 16 * (d / 128);
-    hipLaunchKernelGGL(k_wbwd_flags, dim3(cdiv(waves, 4)), dim3(256), 0, D.stream, d, Hf, Lf, flags);
+    const uint32_t tvis = mode == 2 ? kVisitBF16 : kVisitF16, tlo = mode == 2 ? kLoBF16 : kLoF16;
+    hipLaunchKernelGGL(k_wbwd_flags, dim3(cdiv(waves, 4)), dim3(256), 0, D.stream, d, (const uint16_t *)Hf,
+                       tvis, tlo, flags);
     hipLaunchKernelGGL(k_wbwd_lists, dim3(cdiv(d / 16 + d / 64 + d / 32, 4)), dim3(256), 0, D.stream, d,
                        (const uint8_t *)flags, P<int>(state + (unsigned)L.tl));
 }
@@ -1720,7 +1756,7 @@ OPHIP_EXPORT int ophip_wsq_bwd_layout(unsigned state, int d, int mode, int buf) 
     const size_t nf = (size_t)d * d / 8;
     hipLaunchKernelGGL((k_wfrag<true>), dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, D.stream, d, Hr,
                        Lr, Hf, Lf);
-    bwd_tile_lists(D, L, state, d, Hf, Lf);
+    bwd_tile_lists(D, L, state, d, mode, Hf);
     return (int)check_launch();
 }
 
@@ -1742,7 +1778,7 @@ OPHIP_EXPORT int ophip_wsq_bwd_layout_perm(unsigned state, int d, int mode, int 
     const size_t nf = (size_t)d * d / 8;
     hipLaunchKernelGGL(k_wfragp, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, D.stream, d, Hr, Lr,
                        P<const int>(perm), Hf, Lf);
-    bwd_tile_lists(D, L, state, d, Hf, Lf);
+    bwd_tile_lists(D, L, state, d, mode, Hf);
     return (int)check_launch();
 }
 
@@ -1867,7 +1903,7 @@ OPHIP_EXPORT int ophip_wsq_bwd_layout_regime(unsigned state, int d, int mode, in
     uint16_t *Lf = P<uint16_t>(state + (unsigned)L.H[1 - buf][3]);
     hipLaunchKernelGGL(k_wfragk, dim3((unsigned)((d / 64) * (d / 64))), dim3(256), 0, D.stream, d, nx, nr, Hr,
                        Lr, Hf, Lf);
-    bwd_tile_lists(D, L, state, d, Hf, Lf);
+    bwd_tile_lists(D, L, state, d, mode, Hf);
     return (int)check_launch();
 }
 

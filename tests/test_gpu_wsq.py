@@ -286,20 +286,22 @@ BWD_KERNELS = {0: "dense", 2: "16-row tiles", 3: "64-row blocks", 4: "32-row blo
 @pytest.mark.parametrize("mode", [1, 2])
 def test_wform_backward_step(hip, mode, kernel):
     """Y = df (X + W X) from the state's hi/lo copies (fragment-major): against
-    the same arithmetic in numpy (FP64 accumulation, 1e-6 of max|Y|),
-    against df (X + W X) with the FP32 W (2e-5), and X's / Y's 16-bit
-    copies bit-exact; every step kernel (the dense one and the band-sparse
-    ones visiting the listed fragments only)"""
+    the same arithmetic in numpy (FP64 accumulation, 1e-6 of max|Y|; the
+    band-sparse kernels read the hi parts of the fragments reaching 2^-40
+    and the lo parts of those reaching 2^-28 (FP16) or 2^-32 (BF16), which
+    leaves out at most 2^-40 per entry of W), against df (X + W X) with
+    the FP32 W (2e-5), and
+    X's / Y's 16-bit copies bit-exact; every step kernel"""
     banner(f"wsq backward step mode={mode} kernel={BWD_KERNELS[kernel]}")
     H = hip
     old = H.lib.ophip_wsq_set_bwd_kernel(kernel)
     try:
-        _backward_step_case(H, mode)
+        _backward_step_case(H, mode, kernel == 0)
     finally:
         H.lib.ophip_wsq_set_bwd_kernel(old)
 
 
-def _backward_step_case(H, mode):
+def _backward_step_case(H, mode, dense):
     L = Lattice(H, 128, 2, mode)  # d = 256
     p = plan(H, [L], 128)
     L.first_step()
@@ -318,6 +320,8 @@ def _backward_step_case(H, mode):
     YT = H.DeviceArray.empty(2 * ncp * d, dtype=np.uint16)
     H.check(H.lib.ophip_wsq_bwd_layout(L.state.h, d, mode, buf))
     H.check(H.lib.ophip_wsq_bwd_prep(d, ncp, mode, Xd.h, XT.h))
+    vis, lo = (np.kron(m, np.ones((16, 32))) for m in
+               tile_flags(L.arr(buf, 1, np.uint16, "C"), d, mode))
     df = 0.9999
     H.check(H.lib.ophip_wsq_bwd_step(L.state.h, d, mode, buf, ncp, np.float32(df), Xd.h, XT.h,
                                      Yd.h, YT.h))
@@ -329,7 +333,15 @@ def _backward_step_case(H, mode):
     assert np.array_equal(xt[0], xh) and np.array_equal(xt[1], xl)
     Y = Yd.download().reshape(d, ncp)
     Xh, Xl = dec16(xh, mode).T.astype(np.float64), dec16(xl, mode).T.astype(np.float64)
-    ref = df * (X.astype(np.float64) + (Hr @ Xh + (Hr @ Xl + Lr @ Xh) / G) / S)
+    full = df * (X.astype(np.float64) + (Hr @ Xh + (Hr @ Xl + Lr @ Xh) / G) / S)
+    Hv, Lv = Hr * vis, Lr * lo  # the visited fragments' hi parts, the read lo parts
+    ref = df * (X.astype(np.float64) + (Hv @ Xh + (Hv @ Xl + Lv @ Xh) / G) / S)
+    # what the band rule leaves out: at most 2^-40 per entry of W
+    left = (np.abs(Hr + Lr / G) * (1 - vis) + np.abs(Lr / G) * (vis - lo)) / S
+    assert left.max() <= 2.0**-40 * (1 + 2.0**-10)
+    assert np.abs(ref - full).max() <= 2.0**-40 * 1.01 * (left > 0).sum(axis=1).max() * 300.0
+    if dense:  # the dense kernel reads every fragment
+        ref = full
     assert np.abs(Y - ref).max() <= 1e-6 * np.abs(ref).max()
     exact = df * (X.astype(np.float64) + W @ X.astype(np.float64))
     assert np.abs(Y - exact).max() <= 2e-5 * np.abs(exact).max()
@@ -343,12 +355,24 @@ def _backward_step_case(H, mode):
                                        int(yl[tuple(b)])) for b in bad[:4]])
 
 
-def tile_lists(hf, lf, d):
-    """numpy: per 16-row tile, the k tiles (32 wide) whose H or L fragment
-    holds a value other than +-0"""
-    nz = ((unfrag(hf, d, d) & 0x7FFF) != 0) | ((unfrag(lf, d, d) & 0x7FFF) != 0)
-    t = nz.reshape(d // 16, 16, d // 32, 32).any(axis=(1, 3))
-    return [np.nonzero(t[i])[0] for i in range(d // 16)]
+# hi-part magnitude bits at which a fragment is visited (|W| >= 2^-40) and
+# its lo part read (|W| >= 2^-28 in FP16, whose copies hold W 2^15; 2^-32
+# in BF16, whose copies hold W)
+VISIT_BITS = {1: 1, 2: 87 << 7}
+LO_BITS = {1: 2 << 10, 2: 95 << 7}
+
+
+def tile_flags(hbits, d, mode):
+    """numpy: per (16-row tile, 32-column k tile) of the row-major hi copy,
+    whether a step visits it and whether it reads the lo part"""
+    m = (hbits.astype(np.uint32) & 0x7FFF).reshape(d // 16, 16, d // 32, 32).max(axis=(1, 3))
+    return m >= VISIT_BITS[mode], m >= LO_BITS[mode]
+
+
+def tile_lists(hf, d, mode):
+    """numpy: per 16-row tile, the visited k tiles and their lo flags"""
+    vis, lo = tile_flags(unfrag(hf, d, d), d, mode)
+    return [(np.nonzero(vis[i])[0], lo[i]) for i in range(d // 16)]
 
 
 def backward_lattice(H, nsq, mode=1, nx=128, nr=2):
@@ -363,8 +387,9 @@ def backward_lattice(H, nsq, mode=1, nx=128, nr=2):
 
 @pytest.mark.parametrize("nsq", [2, 9])
 def test_wform_backward_tile_lists(hip, nsq):
-    """the tile lists built with the fragment layout are the nonzero 16 x 32
-    fragments of W's hi/lo copies (numpy on the downloaded fragments); a
+    """the tile lists built with the fragment layout are the 16 x 32
+    fragments whose hi part reaches 2^-40, each flagged where it reaches
+    2^-28 (its lo part is read), numpy on the downloaded fragments; a
     banded W after few squarings skips most of them"""
     banner(f"wsq backward tile lists nsq={nsq}")
     H, mode = hip, 1
@@ -373,8 +398,7 @@ def test_wform_backward_tile_lists(hip, nsq):
     H.check(H.lib.ophip_wsq_bwd_layout(L.state.h, d, mode, buf))
     H.sync()
     hf = L.arr(1 - buf, 2, np.uint16, "F").reshape(-1, order="F")
-    lf = L.arr(1 - buf, 4, np.uint16, "F").reshape(-1, order="F")
-    ref = tile_lists(hf, lf, d)
+    ref = tile_lists(hf, d, mode)
     h = H.lib.ophip_wsq_array(L.state.h, d, mode, 0, 7)
     nkt = d // 32
     nl = d // 16 + d // 64 + d // 32  # lists per 16-row tile, 64-row block, 32-row block
@@ -385,11 +409,14 @@ def test_wform_backward_tile_lists(hip, nsq):
     blocks += [list(range(4 * b, 4 * b + 4)) for b in range(d // 64)]
     blocks += [[2 * b, 2 * b + 1] for b in range(d // 32)]
     for i, rows in enumerate(blocks):
-        want = functools.reduce(np.union1d, [ref[k] for k in rows])
+        want = functools.reduce(np.union1d, [ref[k][0] for k in rows])
+        lo = functools.reduce(np.logical_or, [ref[k][1] for k in rows])
+        want = want + 65536 * lo[want]
         assert tl[i, 0] == len(want), (i, tl[i, 0], len(want))
         assert np.array_equal(tl[i, 1:1 + tl[i, 0]], want), i
-    frac = sum(len(r) for r in ref) / ((d // 16) * nkt)
-    print(f"nonzero fragment fraction {frac:.3f}")
+    frac = sum(len(r[0]) for r in ref) / ((d // 16) * nkt)
+    frac_lo = sum(int(r[1].sum()) for r in ref) / ((d // 16) * nkt)
+    print(f"visited fragment fraction {frac:.3f}, lo parts read {frac_lo:.3f}")
     if nsq == 2:
         assert frac < 0.5
 
@@ -623,6 +650,47 @@ def test_zero_cell_skip_same_bits(hip, mode, tile):
     finally:
         H.lib.ophip_wsq_set_skip(old)
         H.lib.ophip_wsq_plan_free(p)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_later_solve_same_bits(hip, mode):
+    """a plan's later solve, whose first round writes only where the previous
+    solve left values (the destination's flags), gives the bits of a fresh
+    state's solve: after a wide-band solve (10 squarings) and a backward
+    layout (fragment copies written into the other buffer's column-major
+    slots), a narrow one (4 squarings); both buffers' FP32 W and row-major
+    copies, and the column-major copies of every flagged cell"""
+    banner(f"wsq later solve after a wide one and a layout, mode={mode}")
+    H = hip
+    A, B = Lattice(H, 128, 2, mode), Lattice(H, 128, 2, mode)
+    d = A.d
+    pa, pb = plan(H, [A], 128), plan(H, [B], 128)
+    try:
+        A.first_step()
+        H.check(H.lib.ophip_wsq_exec(pa, 10))
+        H.check(H.lib.ophip_wsq_bwd_layout(A.state.h, d, mode, 0))
+        A.first_step()
+        H.check(H.lib.ophip_wsq_exec(pa, 4))
+        B.first_step()
+        H.check(H.lib.ophip_wsq_exec(pb, 4))
+        H.sync()
+        nc = d // 64
+        for buf in (0, 1):
+            fa = np.empty(nc * nc, dtype=np.uint8)
+            H.check(H.lib.opcuda_memcpy_d2h(H.lib.ophip_wsq_array(A.state.h, d, mode, buf, 6),
+                                            fa.ctypes.data, fa.nbytes))
+            cell = np.kron(fa.reshape(nc, nc) != 0, np.ones((64, 64), dtype=bool))
+            for which in (0, 1, 2, 3, 4):
+                dt = np.uint32 if which == 0 else np.uint16
+                order = "F" if which in (2, 4) else "C"
+                a, b = A.arr(buf, which, dt, order), B.arr(buf, which, dt, order)
+                if which in (2, 4):
+                    assert np.array_equal(a[cell], b[cell]), (buf, which)
+                else:
+                    assert np.array_equal(a, b), (buf, which)
+    finally:
+        H.lib.ophip_wsq_plan_free(pa)
+        H.lib.ophip_wsq_plan_free(pb)
 
 
 def test_forward_steps_skip_zero_cells(hip):
