@@ -223,15 +223,22 @@ int ophip_wsq_bwd_layout_perm(unsigned state, int d, int mode, int buf, unsigned
  * squaring, reference-order steps) without a permutation array, through
  * LDS tiles: nr in {1, 2, 4, 8}, nx % 8 == 0, nx*nr = d */
 int ophip_wsq_bwd_layout_regime(unsigned state, int d, int mode, int buf, int nx, int nr);
+/* the same for n lattices of one d, nx and nr in three launches (per 32
+ * lattices): layout, flags, lists.  No reference counterpart (the
+ * reference lays out nothing; it steps one model at a time). */
+int ophip_wsq_bwd_layout_regime_batch(int n, const unsigned *states, int d, int mode, int buf, int nx,
+                                      int nr);
 int ophip_wsq_bwd_prep(int d, int ncp, int mode, unsigned X, unsigned XT);
 int ophip_wsq_bwd_step(unsigned state, int d, int mode, int buf, int ncp, float df, unsigned X,
                        unsigned XT, unsigned Y, unsigned YT);
 /* The backward step of nb lattices of one d in one launch (per 32): Y[i] =
- * df[i] (X[i] + W_i X[i]), W_i from states[i] laid out as above.  Both
- * step entry points visit only the 16 x 32 fragments that hold a nonzero
- * hi or lo value (lists built by ophip_wsq_bwd_layout*): outside the band
- * of a squared generator both copies flush to zero, and skipped products
- * are exact zeros.  ophip_wsq_set_bwd_kernel(k) selects the kernel: 0 the
+ * df[i] (X[i] + W_i X[i]), W_i from states[i] laid out as above.  The
+ * band-sparse step kernels read the hi part of a 16 x 32 fragment only
+ * where some |W| >= 2^-40 and its lo part only where some |W| >= 2^-28
+ * (FP16) or 2^-32 (BF16) (lists built by ophip_wsq_bwd_layout*): what they
+ * leave out is at most 2^-40 per entry of W, so a row of Y moves by at most
+ * (entries left out) 2^-40 max|X|; the dense kernel reads every fragment.
+ * ophip_wsq_set_bwd_kernel(k) selects the kernel: 0 the
  * dense one (every fragment), 1 automatic (default), 2 per 16-row tile, 3
  * per 64-row block, 4 per 32-row block (3 and 4: ncp 48); it returns the
  * previous setting and applies to kernels launched afterwards.  No

@@ -105,6 +105,7 @@ struct opm_solver {
     Buf U[2];            // FP32 d x d ping-pong for the squarings
     Buf V[2], pay0;      // backward induction: d x ncp row-major
     Buf VT[2];           // W form (x3 modes): hi/lo copies of V for the matrix-core step
+    Buf payXT;           // the same copies of pay0 (made once: a group's first step reads them)
     // W form, propagation: the spot rows' forward chain p_k = p_{k-1} (I + W)
     Buf fP, fx, fout;
     unsigned long fplan = 0;
@@ -220,7 +221,7 @@ opm_solver *opm_create(const opm_config *cfg) {
         s->n_mat = 1;
         size_t vb = 4 * (size_t)d * s->ncp;
         ok = alloc(s->V[0], vb) && alloc(s->V[1], vb) && alloc(s->pay0, vb);
-        if (ok && s->wmode == 1) ok = alloc(s->VT[0], vb) && alloc(s->VT[1], vb);
+        if (ok && s->wmode == 1) ok = alloc(s->VT[0], vb) && alloc(s->VT[1], vb) && alloc(s->payXT, vb);
         if (ok && s->wmode == 1 && s->nm) {
             s->v_ref = true;
             std::vector<int> pm((size_t)d);
@@ -236,6 +237,8 @@ opm_solver *opm_create(const opm_config *cfg) {
                     p[(size_t)y * s->ncp + k] = (float)(v > 0.0 ? v : 0.0);
                 }
             ok = !upload(s->pay0, p.data(), vb);
+            if (ok && s->wmode == 1 && d % 128 == 0 && d <= 4096)  // sizes the matrix-core step takes
+                ok = ophip_wsq_bwd_prep(d, s->ncp, s->wmode, s->pay0.h, s->payXT.h) == 0;
         }
     } else if (ok && s->wmode >= 0) {
         // W form: the forward chain of each regime's spot row, p_k =
@@ -299,7 +302,7 @@ void opm_destroy(opm_solver *s) {
     if (s->wplan) ophip_wsq_plan_free(s->wplan);
     if (s->fplan) ophip_fwd_plan_free(s->fplan);
     Buf *all[] = {&s->x, &s->drift, &s->vol, &s->U[0], &s->U[1], &s->V[0], &s->V[1],
-                  &s->VT[0], &s->VT[1], &s->pay0, &s->Uk[0], &s->Uk[1], &s->payT, &s->out,
+                  &s->VT[0], &s->VT[1], &s->payXT, &s->pay0, &s->Uk[0], &s->Uk[1], &s->payT, &s->out,
                   &s->wstate, &s->fP, &s->fx, &s->fout, &s->perm};
     for (Buf *b : all) release(*b);
     delete s;
@@ -631,36 +634,52 @@ static int group_body(opm_group *g) {
     st = ophip_wsq_exec(g->wplan, nsq);
     if (st) return st;
     const int buf = nsq & 1;
-    std::vector<unsigned> states(n), X[2], XT[2];
+    std::vector<unsigned> states(n), X[2], XT[2], P0(n), PT0(n);
     std::vector<float> df(n);
     for (int i = 0; i < n; i++) {
         opm_solver *s = g->s[i];
         s->u_final = buf;
         s->kernel_h = ophip_wsq_w(s->wstate.h, d, wmode, buf);
         s->v_final = nsteps & 1;
-        st = opcuda_scopy1(s->V[0].h, s->pay0.h, (unsigned)((size_t)d * s->ncp));
-        if (st) return st;
+        if (wmode != 1) {
+            st = opcuda_scopy1(s->V[0].h, s->pay0.h, (unsigned)((size_t)d * s->ncp));
+            if (st) return st;
+        }
         states[i] = s->wstate.h;
         df[i] = (float)std::exp(-s->cfg.rate * s->cfg.DeltaT);
+        P0[i] = s->pay0.h;
+        PT0[i] = s->payXT.h;
         for (int p = 0; p < 2; p++) {
             X[p].push_back(s->V[p].h);
             XT[p].push_back(s->VT[p].h);
         }
     }
     if (wmode == 1) {
-        // FP16 x3 on the matrix cores: every lattice's step in one launch
-        for (opm_solver *s : g->s) {
-            st = bwd_layout(s, d, buf);
+        // FP16 x3 on the matrix cores: every lattice's fragment layout and
+        // tile lists in three launches, every lattice's step in one launch;
+        // the first step reads the payoff and its copies (made at creation)
+        const int nx = a->cfg.nx, nr = a->cfg.nr;
+        if (a->v_ref && nx % 8 == 0 && (nr == 1 || nr == 2 || nr == 4 || nr == 8)) {
+            st = ophip_wsq_bwd_layout_regime_batch(n, states.data(), d, wmode, buf, nx, nr);
             if (st) return st;
-            st = ophip_wsq_bwd_prep(d, s->ncp, wmode, s->V[0].h, s->VT[0].h);
-            if (st) return st;
+        } else {
+            for (opm_solver *s : g->s) {
+                st = bwd_layout(s, d, buf);
+                if (st) return st;
+            }
         }
         for (int t = 0; t < nsteps; t++) {
             const int p = t & 1, q = p ^ 1;
-            st = ophip_wsq_bwd_step_batch(n, states.data(), d, wmode, buf, a->ncp, df.data(), X[p].data(),
-                                          XT[p].data(), X[q].data(), XT[q].data());
+            st = ophip_wsq_bwd_step_batch(n, states.data(), d, wmode, buf, a->ncp, df.data(),
+                                          t ? X[p].data() : P0.data(), t ? XT[p].data() : PT0.data(),
+                                          X[q].data(), XT[q].data());
             if (st) return st;
         }
+        if (nsteps == 0)
+            for (opm_solver *s : g->s) {
+                st = opcuda_scopy1(s->V[0].h, s->pay0.h, (unsigned)((size_t)d * s->ncp));
+                if (st) return st;
+            }
     } else {
         for (int t = 0; t < nsteps; t++)
             for (int i = 0; i < n; i++) {

@@ -675,15 +675,30 @@ __global__ __launch_bounds__(256) void k_wfragp(int d, const uint16_t *__restric
     *(uint4 *)(Lf + f * 8) = *(const uint4 *)lo;
 }
 
+// one lattice of a batched fragment layout and its tile lists (arena
+// offsets): row-major copies in, fragment copies, flags and lists out
+struct LayScen {
+    uint64_t Hr, Lr, Hf, Lf, flags, tl;
+};
+constexpr int kLayBatch = 32;
+struct LayBatch {
+    LayScen s[kLayBatch];
+};
+
 // The same for the regime permutation of a lattice squared node-major
 // (copies' state nr x + r) stepped in the reference order (fragment state
 // x + nx r), through LDS: a workgroup reads one 64 x 64 block of the
 // row-major copies (64/nr nodes x nr regimes per side) with 16-byte loads
 // and writes every 8-column run of its fragment rows as one 16-byte store.
-// nr in {1, 2, 4, 8}, nx % 8 == 0, d = nx nr, d % 64 == 0.
-__global__ __launch_bounds__(256) void k_wfragk(int d, int nx, int nr, const uint16_t *__restrict__ Hr,
-                                                const uint16_t *__restrict__ Lr, uint16_t *__restrict__ Hf,
-                                                uint16_t *__restrict__ Lf) {
+// nr in {1, 2, 4, 8}, nx % 8 == 0, d = nx nr, d % 64 == 0.  Lattice
+// blockIdx.y of the batch.
+__global__ __launch_bounds__(256) void k_wfragk(int d, int nx, int nr, char *__restrict__ base,
+                                                const LayBatch bt) {
+    const LayScen &S = bt.s[blockIdx.y];
+    const uint16_t *__restrict__ Hr = at<const uint16_t>(base, S.Hr);
+    const uint16_t *__restrict__ Lr = at<const uint16_t>(base, S.Lr);
+    uint16_t *__restrict__ Hf = at<uint16_t>(base, S.Hf);
+    uint16_t *__restrict__ Lf = at<uint16_t>(base, S.Lf);
     __shared__ __attribute__((aligned(16))) uint16_t sh[2][64][72];
     const int nb = d / 64, bi = blockIdx.x / nb, bj = blockIdx.x % nb, t = threadIdx.x;
     const int nkt = d / 32, NB = 64 / nr;
@@ -872,9 +887,11 @@ constexpr uint32_t kVisitBF16 = 87u << 7, kLoBF16 = 95u << 7;
 
 // k_wbwd_flags: byte flags[it][kt] (d/16 x d/32): bit 0 = the fragment is
 // visited, bit 1 = its lo part is read; a wave per (it, four k tiles), the
-// four 16-byte loads in flight.
-__global__ __launch_bounds__(256) void k_wbwd_flags(int d, const uint16_t *__restrict__ Hf, uint32_t tvis,
-                                                    uint32_t tlo, uint8_t *__restrict__ flags) {
+// four 16-byte loads in flight; lattice blockIdx.y of the batch.
+__global__ __launch_bounds__(256) void k_wbwd_flags(int d, char *__restrict__ base, const LayBatch bt,
+                                                    uint32_t tvis, uint32_t tlo) {
+    const uint16_t *__restrict__ Hf = at<const uint16_t>(base, bt.s[blockIdx.y].Hf);
+    uint8_t *__restrict__ flags = at<uint8_t>(base, bt.s[blockIdx.y].flags);
     const int nkt = d / 32, lane = threadIdx.x & 63;
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);  // (it, k0 / 4)
     if (w >= d / 16 * (nkt / 4)) return;  // wave-uniform
@@ -897,9 +914,11 @@ __global__ __launch_bounds__(256) void k_wbwd_flags(int d, const uint16_t *__res
 
 // k_wbwd_lists: one wave per list (d/16 row tiles, then d/64 64-row
 // blocks, then d/32 32-row blocks), lanes over k tiles: the union of the
-// list's row tiles' flags, compacted in k order with a ballot prefix
-__global__ __launch_bounds__(256) void k_wbwd_lists(int d, const uint8_t *__restrict__ flags,
-                                                    int *__restrict__ tl) {
+// list's row tiles' flags, compacted in k order with a ballot prefix;
+// lattice blockIdx.y of the batch
+__global__ __launch_bounds__(256) void k_wbwd_lists(int d, char *__restrict__ base, const LayBatch bt) {
+    const uint8_t *__restrict__ flags = at<const uint8_t>(base, bt.s[blockIdx.y].flags);
+    int *__restrict__ tl = at<int>(base, bt.s[blockIdx.y].tl);
     const int nkt = d / 32, n16 = d / 16, n64 = d / 64, n32 = d / 32;
     const int lane = threadIdx.x & 63;
     const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -1729,15 +1748,35 @@ OPHIP_EXPORT int ophip_wsq_bwd_prep(int d, int ncp, int mode, unsigned X, unsign
 }
 
 namespace {
-// the tile lists of the fragment copies (after their layout), from the hi copy Hf
-void bwd_tile_lists(Device &D, const WsqLayout &L, unsigned state, int d, int mode, const uint16_t *Hf) {
-    uint8_t *flags = P<uint8_t>(state + (unsigned)L.tf);
+// one lattice's entry of a batched layout: the copies of buffer buf, the
+// fragment copies in the other buffer's column-major slots
+LayScen lay_scen(const WsqLayout &L, unsigned state, int buf) {
+    LayScen S;
+    S.Hr = state + L.H[buf][0];
+    S.Lr = state + L.H[buf][2];
+    S.Hf = state + L.H[1 - buf][1];
+    S.Lf = state + L.H[1 - buf][3];
+    S.flags = state + L.tf;
+    S.tl = state + L.tl;
+    return S;
+}
+
+// the tile lists of m lattices' fragment copies (after their layout), from
+// the hi copies
+void bwd_tile_lists(Device &D, int d, int mode, const LayBatch &bt, int m) {
     const int waves = d / 16 * (d / 128);
     const uint32_t tvis = mode == 2 ? kVisitBF16 : kVisitF16, tlo = mode == 2 ? kLoBF16 : kLoF16;
-    hipLaunchKernelGGL(k_wbwd_flags, dim3(cdiv(waves, 4)), dim3(256), 0, D.stream, d, (const uint16_t *)Hf,
-                       tvis, tlo, flags);
-    hipLaunchKernelGGL(k_wbwd_lists, dim3(cdiv(d / 16 + d / 64 + d / 32, 4)), dim3(256), 0, D.stream, d,
-                       (const uint8_t *)flags, P<int>(state + (unsigned)L.tl));
+    hipLaunchKernelGGL(k_wbwd_flags, dim3(cdiv(waves, 4), m), dim3(256), 0, D.stream, d, D.base, bt, tvis,
+                       tlo);
+    hipLaunchKernelGGL(k_wbwd_lists, dim3(cdiv(d / 16 + d / 64 + d / 32, 4), m), dim3(256), 0, D.stream, d,
+                       D.base, bt);
+}
+
+// the same for one lattice
+void bwd_tile_lists(Device &D, const WsqLayout &L, unsigned state, int d, int mode, int buf) {
+    LayBatch bt{};
+    bt.s[0] = lay_scen(L, state, buf);
+    bwd_tile_lists(D, d, mode, bt, 1);
 }
 }  // namespace
 
@@ -1756,7 +1795,7 @@ OPHIP_EXPORT int ophip_wsq_bwd_layout(unsigned state, int d, int mode, int buf) 
     const size_t nf = (size_t)d * d / 8;
     hipLaunchKernelGGL((k_wfrag<true>), dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, D.stream, d, Hr,
                        Lr, Hf, Lf);
-    bwd_tile_lists(D, L, state, d, mode, Hf);
+    bwd_tile_lists(D, L, state, d, mode, buf);
     return (int)check_launch();
 }
 
@@ -1778,7 +1817,7 @@ OPHIP_EXPORT int ophip_wsq_bwd_layout_perm(unsigned state, int d, int mode, int 
     const size_t nf = (size_t)d * d / 8;
     hipLaunchKernelGGL(k_wfragp, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, D.stream, d, Hr, Lr,
                        P<const int>(perm), Hf, Lf);
-    bwd_tile_lists(D, L, state, d, mode, Hf);
+    bwd_tile_lists(D, L, state, d, mode, buf);
     return (int)check_launch();
 }
 
@@ -1890,21 +1929,35 @@ OPHIP_EXPORT int ophip_wsq_bwd_step_batch(int nb, const unsigned *states, int d,
 // ophip_wsq_bwd_layout_perm for the regime permutation perm[x + nx r] =
 // nr x + r (a lattice squared node-major, stepped in the reference order)
 // without a permutation array: nr in {1, 2, 4, 8}, nx % 8 == 0, nx nr = d.
+OPHIP_EXPORT int ophip_wsq_bwd_layout_regime_batch(int n, const unsigned *states, int d, int mode, int buf,
+                                                   int nx, int nr);
 OPHIP_EXPORT int ophip_wsq_bwd_layout_regime(unsigned state, int d, int mode, int buf, int nx, int nr) {
+    return ophip_wsq_bwd_layout_regime_batch(1, &state, d, mode, buf, nx, nr);
+}
+
+// ophip_wsq_bwd_layout_regime for n lattices of the same d, nx and nr in
+// one layout launch and one launch each for the flags and the lists (per
+// 32 lattices)
+OPHIP_EXPORT int ophip_wsq_bwd_layout_regime_batch(int n, const unsigned *states, int d, int mode, int buf,
+                                                   int nx, int nr) {
     if (!bwd_args_ok(d, 16, mode) || buf < 0 || buf > 1 || nx <= 0 || nx % 8 || nx * nr != d ||
-        (nr != 1 && nr != 2 && nr != 4 && nr != 8))
+        (nr != 1 && nr != 2 && nr != 4 && nr != 8) || n <= 0 || !states)
         return (int)set_status(kErrBadArg);
     const WsqLayout L = wsq_layout(d, mode);
-    if (!handle_range_ok(state, L.total)) return (int)set_status(kErrBadHandle);
+    for (int i = 0; i < n; i++)
+        if (!handle_range_ok(states[i], L.total)) return (int)set_status(kErrBadHandle);
     Device &D = dev();
-    const uint16_t *Hr = P<const uint16_t>(state + (unsigned)L.H[buf][0]);
-    const uint16_t *Lr = P<const uint16_t>(state + (unsigned)L.H[buf][2]);
-    uint16_t *Hf = P<uint16_t>(state + (unsigned)L.H[1 - buf][1]);
-    uint16_t *Lf = P<uint16_t>(state + (unsigned)L.H[1 - buf][3]);
-    hipLaunchKernelGGL(k_wfragk, dim3((unsigned)((d / 64) * (d / 64))), dim3(256), 0, D.stream, d, nx, nr, Hr,
-                       Lr, Hf, Lf);
-    bwd_tile_lists(D, L, state, d, mode, Hf);
-    return (int)check_launch();
+    for (int i0 = 0; i0 < n; i0 += kLayBatch) {
+        const int m = std::min(kLayBatch, n - i0);
+        LayBatch bt{};
+        for (int i = 0; i < m; i++) bt.s[i] = lay_scen(L, states[i0 + i], buf);
+        hipLaunchKernelGGL(k_wfragk, dim3((unsigned)((d / 64) * (d / 64)), m), dim3(256), 0, D.stream, d, nx, nr,
+                           D.base, bt);
+        bwd_tile_lists(D, d, mode, bt, m);
+        const unsigned e = check_launch();
+        if (e) return (int)e;
+    }
+    return 0;
 }
 
 // Y = df (X + W X) with W laid out by ophip_wsq_bwd_layout(state, .., buf),

@@ -98,6 +98,7 @@ SIGNATURES = {
     "ophip_wsq_bwd_layout": (I32, [U32, I32, I32, I32]),
     "ophip_wsq_bwd_layout_perm": (I32, [U32, I32, I32, I32, U32]),
     "ophip_wsq_bwd_layout_regime": (I32, [U32, I32, I32, I32, I32, I32]),
+    "ophip_wsq_bwd_layout_regime_batch": (I32, [I32, C.POINTER(U32), I32, I32, I32, I32, I32]),
     "ophip_wsq_bwd_prep": (I32, [I32, I32, I32, U32, U32]),
     "ophip_wsq_bwd_step": (I32, [U32, I32, I32, I32, I32, F32, U32, U32, U32, U32]),
     "ophip_wsq_bwd_step_batch": (I32, [I32, C.POINTER(U32), I32, I32, I32, I32, C.POINTER(F32),

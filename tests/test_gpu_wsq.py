@@ -757,3 +757,32 @@ def test_wform_backward_layout_regime(hip, nx, nr):
         out.append((L.arr(1 - buf, 2, np.uint16, "F").copy(), L.arr(1 - buf, 4, np.uint16, "F").copy(), tl))
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b)
+
+
+def test_wform_backward_layout_regime_batch(hip):
+    """one batched regime layout over three lattices (different squaring
+    counts) gives each the fragment copies and tile lists of its own
+    single-lattice layout"""
+    banner("wsq backward layout, regime permutation, batch of 3 vs single")
+    import ctypes as C
+    H, mode, buf, nx, nr = hip, 1, 1, 64, 2
+    lats = [backward_lattice(H, n, mode, nx=nx, nr=nr) for n in (3, 5, 9)]  # odd counts: buffer 1
+    d = lats[0].d
+    nkt = d // 32
+    nl = d // 16 + d // 64 + d // 32
+
+    def grab(L):
+        tl = np.empty(nl * (nkt + 1), dtype=np.int32)
+        H.check(H.lib.opcuda_memcpy_d2h(H.lib.ophip_wsq_array(L.state.h, d, mode, 0, 7), tl.ctypes.data,
+                                        tl.nbytes))
+        return (L.arr(1 - buf, 2, np.uint16, "F").copy(), L.arr(1 - buf, 4, np.uint16, "F").copy(), tl)
+
+    st = (C.c_uint * 3)(*[L.state.h for L in lats])
+    H.check(H.lib.ophip_wsq_bwd_layout_regime_batch(3, st, d, mode, buf, nx, nr))
+    H.sync()
+    got = [grab(L) for L in lats]
+    for L, g in zip(lats, got):
+        H.check(H.lib.ophip_wsq_bwd_layout_regime(L.state.h, d, mode, buf, nx, nr))
+        H.sync()
+        for a, b in zip(g, grab(L)):
+            assert np.array_equal(a, b)
