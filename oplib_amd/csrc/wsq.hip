@@ -124,7 +124,9 @@ template <> struct Num<true> {
 };
 
 // ---------------------------------------------------------------------
-// k_wsplit: 64x64 tile per workgroup (64 threads), LDS transpose
+// k_wsplit: 64x64 tile per workgroup (64 threads), LDS transpose.  A tile
+// whose copies are all zero and whose destination cell is not flagged
+// (it already holds zeros) stores no copies.
 // ---------------------------------------------------------------------
 template <bool BF, bool LO>
 __global__ __launch_bounds__(64) void k_wsplit(char *__restrict__ base,
@@ -140,6 +142,8 @@ __global__ __launch_bounds__(64) void k_wsplit(char *__restrict__ base,
     const int d = sc.d, lane = threadIdx.x;
     const int i0 = it.tm * 64, j0 = it.tn * 64;
     const int i = i0 + lane;
+    uint8_t *flag = at<uint8_t>(base, sc.fln) + (size_t)it.tm * (d / 64) + it.tn;
+    const bool was = *flag != 0;  // wave-uniform
     double rs = 0.0;
     unsigned nz = 0;
     float *Wm = at<float>(base, sc.W);  // W0 itself (flushed in place)
@@ -162,18 +166,22 @@ __global__ __launch_bounds__(64) void k_wsplit(char *__restrict__ base,
             w = 0.0f;
         }
         rs += (double)w;
-        Hcn[e] = h;
         th[lane][c] = h;
+        if (LO) tl[lane][c] = l;
         nz |= nzv;
-        if (LO) {
-            Lcn[e] = l;
-            tl[lane][c] = l;
-        }
     }
     at<double>(base, sc.part)[(size_t)it.tn * d + i] = rs;  // slot = 64-column block
     // the tile is one cell
     const bool any = __ballot(nz != 0) != 0;
-    if (lane == 0) at<uint8_t>(base, sc.fln)[(size_t)it.tm * (d / 64) + it.tn] = any ? 1 : 0;
+    if (lane == 0) *flag = any ? 1 : 0;
+    if (!any && !was) return;  // wave-uniform: the cell holds zeros already
+    // column-major copies: lane = row, from its own LDS row
+#pragma unroll 8
+    for (int c = 0; c < 64; c++) {
+        const size_t e = (size_t)i + (size_t)d * (j0 + c);
+        Hcn[e] = th[lane][c];
+        if (LO) Lcn[e] = tl[lane][c];
+    }
     __syncthreads();
     // row-major copies: row r of the tile, lanes over columns
     for (int r = 0; r < 64; r++) {
@@ -494,11 +502,11 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
 // the round's static list (lane c: cell column c of A / cell row c of B).
 // An item is live if it has K cells, if its own input cells are nonzero
 // (W' = 2W + ...), when wskip is 0, or if its destination cells are
-// flagged (they must be overwritten with zeros).  Buffer 1's flags start
-// as all ones when a plan is made (k_fill8) and every round keeps both
-// buffers' flags a superset of their nonzero cells, so a lattice's first
-// round of a later solve writes only where the previous solve left
-// values.  (The backward layout writes its fragment copies into the
+// flagged (they must be overwritten with zeros).  Both buffers' flags start
+// as all ones when a plan is made (k_fill8) and the split and every round
+// keep them a superset of their buffers' nonzero cells, so a lattice's
+// first round (and split) of a later solve writes only where the previous
+// solve left values.  (The backward layout writes its fragment copies into the
 // column-major copy slots of the buffer the squarings did not end in:
 // those slots are rewritten by the round that next flags their cells, and
 // nothing reads an unflagged cell.)
@@ -1445,14 +1453,16 @@ static WsqPlan *make_plan(int n, const unsigned *states, const int *ds, const in
         set_status(hip_status(e));
         return nullptr;
     }
-    // buffer 1's cell flags: "may be nonzero" everywhere until a round
-    // writes that buffer (ordered after earlier work on the states)
-    for (int i = 0; i < n; i++) {
-        const int nc = ds[i] / 64;
-        hipLaunchKernelGGL(k_fill8, dim3(cdiv(nc * nc, 256)), dim3(256), 0, D.stream,
-                           reinterpret_cast<uint8_t *>(ptr(states[i] + (unsigned)wsq_layout(ds[i], mode).fl[1])),
-                           nc * nc, (uint8_t)1);
-    }
+    // both buffers' cell flags: "may be nonzero" everywhere until the
+    // split or a round writes that buffer (ordered after earlier work on
+    // the states)
+    for (int i = 0; i < n; i++)
+        for (int b = 0; b < 2; b++) {
+            const int nc = ds[i] / 64;
+            hipLaunchKernelGGL(k_fill8, dim3(cdiv(nc * nc, 256)), dim3(256), 0, D.stream,
+                               reinterpret_cast<uint8_t *>(ptr(states[i] + (unsigned)wsq_layout(ds[i], mode).fl[b])),
+                               nc * nc, (uint8_t)1);
+        }
     if (check_launch()) {
         (void)hipFree(P->dev);
         delete P;
