@@ -134,14 +134,17 @@ def main():
         wb = WformBackward(H, M, M.config_rs2048_scenarios(nb))
         for _ in range(20):
             wb.step()
+        sp = M.Solver(M.config_rs2048())  # the plain FP32 solve: 16 k_sgemm squarings, 252 steps
+        sp.solve()
         n = 16 << 20  # floats: 64 MiB copied by k_copy4 (opcuda_scopy1)
         A, B = H.DeviceArray.empty(n), H.DeviceArray.empty(n)
         for _ in range(20):
             H.check(H.lib.opcuda_scopy1(B.h, A.h, n), "scopy1")
         H.sync()
-        print(json.dumps({"probe": f"FP16 x3 backward step ({nb} lattices per launch, kernel "
-                                   f"{H.lib.ophip_wsq_bwd_kernel_for(nb, 2048, 48)}) x20, "
-                                   "k_copy4 (64 MiB) x20",
+        print(json.dumps({"probe": f"FP16 x3 squarings of {nb} lattices (16 rounds each, one lattice per "
+                                   f"launch), FP16 x3 backward step ({nb} lattices per launch, kernel "
+                                   f"{H.lib.ophip_wsq_bwd_kernel_for(nb, 2048, 48)}) x20, one plain FP32 "
+                                   "solve (16 squarings, 252 steps), k_copy4 (64 MiB) x20",
                           "step_bytes_nonzero": wb.bytes, "step_bytes_dense": wb.bytes_dense,
                           "visited_fragment_fraction": wb.nz_frac, "lo_read_fraction": wb.lo_frac,
                           "copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
