@@ -64,6 +64,23 @@ double opm_max_rate(const opm_config *cfg);
  * dt = min(0.5/max|L_yy|, dt0_cap), niter = ceil(log2(DeltaT/dt)) */
 int opm_plan_nsq(const opm_config *cfg, double dt0_cap);
 
+/* the SV generator's per-state inputs from the model's term structure,
+ * CSVModel.mkgen (CSVModel.cs:103-172): for each generator date i (t_years[i]
+ * = (t_i - today) in days / 365.25) the eight curves c (tau: 8 x 4
+ * row-major {A, B, c2, c3}) give xi_c = A + (B - A)(1 + c3 t/c2) exp(-t/c2)
+ * (A when c2 = 0), plus rho[8 i + c] when rho is given: vol, low beta, high
+ * beta, vol of vol, vol mean-reversion rate and level, jump sizes - and +.
+ * Per state y = x + nx r (xval[nx] nodes, rval[nr] regime values, y0 the
+ * spot state): beta = (high (sqrt x_y - sqrt x_min) + low (sqrt x_max -
+ * sqrt x_y)) / (sqrt x_max - sqrt x_min), svol = rval_y vol (x_y /
+ * x_y0)^beta, volvol, voldrift = rate (level - rval_y), jm, jp; outputs
+ * ni x d, date-major, FP64 on the host.  The arrays feed ophip_dgen_sv.
+ * Returns 0, or 1 on a bad argument or a non-finite curve value (where
+ * the reference throws). */
+int opm_sv_params(int nx, int nr, const double *xval, const double *rval, int y0, int ni, const double *t_years,
+                  const double *tau, const double *rho, double *svol, double *volvol, double *voldrift,
+                  double *jm, double *jp);
+
 /* a solver owns device buffers for one configuration */
 typedef struct opm_solver opm_solver;
 opm_solver *opm_create(const opm_config *cfg);

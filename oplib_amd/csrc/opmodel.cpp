@@ -186,6 +186,46 @@ int opm_plan_nsq(const opm_config *c, double dt0_cap) {
     return (int)std::ceil(std::log(c->DeltaT / dt) / std::log(2.0));
 }
 
+int opm_sv_params(int nx, int nr, const double *xval, const double *rval, int y0, int ni, const double *t_years,
+                  const double *tau, const double *rho, double *svol, double *volvol, double *voldrift,
+                  double *jm, double *jp) {
+    if (nx < 2 || nr < 1 || ni < 1 || !xval || !rval || !t_years || !tau || !svol || !volvol || !voldrift ||
+        !jm || !jp || y0 < 0 || y0 >= nx * nr)
+        return 1;
+    const int d = nx * nr;
+    // per-state price and regime value, state y = x + nx r
+    auto xs = [&](int y) { return xval[y % nx]; };
+    auto rs = [&](int y) { return rval[y / nx]; };
+    const double sx_max = std::sqrt(xs(nx - 1)), sx_min = std::sqrt(xs(0)), x_spot = xs(y0);
+    if (!(sx_max > sx_min) || !(x_spot > 0.0)) return 1;
+    for (int i = 0; i < ni; i++) {
+        // the eight curves at t_i: A + (B - A)(1 + c3 t / c2) exp(-t / c2), or A when c2 = 0,
+        // plus the date's offset
+        double v[8];
+        const double t = t_years[i];
+        for (int c = 0; c < 8; c++) {
+            const double *q = tau + 4 * c;
+            double xi = q[0];
+            if (q[2] != 0.0) xi = q[0] + (q[1] - q[0]) * (1.0 + q[3] * t / q[2]) * std::exp(-t / q[2]);
+            if (rho) xi += rho[8 * i + c];
+            if (!std::isfinite(xi)) return 1;
+            v[c] = xi;
+        }
+        const double vol = v[0], lowbeta = v[1], highbeta = v[2], vv = v[3], volmrr = v[4], volmrl = v[5];
+        for (int y = 0; y < d; y++) {
+            const double sx = std::sqrt(xs(y));
+            const double beta = (highbeta * (sx - sx_min) + lowbeta * (sx_max - sx)) / (sx_max - sx_min);
+            const size_t o = (size_t)i * d + y;
+            svol[o] = rs(y) * vol * std::pow(xs(y) / x_spot, beta);
+            volvol[o] = vv;
+            voldrift[o] = volmrr * (volmrl - rs(y));
+            jm[o] = v[6];
+            jp[o] = v[7];
+        }
+    }
+    return 0;
+}
+
 opm_solver *opm_create(const opm_config *cfg) {
     if (!valid(cfg)) return nullptr;
     if (opcuda_cublas_init() != 0) return nullptr;

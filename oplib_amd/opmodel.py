@@ -84,8 +84,32 @@ lib.opm_plan_nsq.argtypes = [C.POINTER(Config), C.c_double]
 lib.opm_plan_nsq.restype = C.c_int
 
 
+D64 = C.POINTER(C.c_double)
+lib.opm_sv_params.argtypes = [C.c_int, C.c_int, D64, D64, C.c_int, C.c_int, D64, D64, D64,
+                              D64, D64, D64, D64, D64]
+lib.opm_sv_params.restype = C.c_int
+
+
 def max_rate(cfg):
     return lib.opm_max_rate(C.byref(cfg))
+
+
+def sv_params(xval, rval, y0, t_years, tau, rho=None):
+    """CSVModel.mkgen (CSVModel.cs:103-172): the SV generator's per-state
+    inputs (svol, volvol, voldrift, jm, jp), each ni x d (state y = x + nx r)"""
+    xv = np.ascontiguousarray(xval, dtype=np.float64)
+    rv = np.ascontiguousarray(rval, dtype=np.float64)
+    ts = np.ascontiguousarray(t_years, dtype=np.float64)
+    tau = np.ascontiguousarray(tau, dtype=np.float64).reshape(8, 4)
+    nx, nr, ni = len(xv), len(rv), len(ts)
+    rh = None if rho is None else np.ascontiguousarray(rho, dtype=np.float64).reshape(ni, 8)
+    out = [np.zeros((ni, nx * nr)) for _ in range(5)]
+    ptr = lambda a: a.ctypes.data_as(D64)  # noqa: E731
+    st = lib.opm_sv_params(nx, nr, ptr(xv), ptr(rv), int(y0), ni, ptr(ts), ptr(tau),
+                           ptr(rh) if rh is not None else None, *[ptr(a) for a in out])
+    if st:
+        raise ValueError("opm_sv_params: bad argument or non-finite curve value")
+    return out
 
 
 def plan_nsq(cfg, dt0_cap):

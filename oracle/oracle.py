@@ -135,6 +135,36 @@ def sv_generator(nx, nr, xval, rval, sdrift, svol, volvol, voldrift, jm, jp):
     return g.reshape(d, d, order="F")
 
 
+def sv_params(xval, rval, y0, t_years, tau, rho=None):
+    """CSVModel.mkgen (CSVModel.cs:103-172) in numpy: per date i and state
+    y = x + nx r the SV generator inputs (svol, volvol, voldrift, jm, jp),
+    each ni x d, from eight curves tau[c] = {A, B, c2, c3} and the dates'
+    offsets rho[i][c]"""
+    xval, rval = np.asarray(xval, float), np.asarray(rval, float)
+    tau = np.asarray(tau, float).reshape(8, 4)
+    nx, nr = len(xval), len(rval)
+    xs, rs = np.tile(xval, nr), np.repeat(rval, nx)
+    sx, smin, smax = np.sqrt(xs), math.sqrt(xs[0]), math.sqrt(xs[nx - 1])
+    out = [[], [], [], [], []]
+    for i, t in enumerate(float(q) for q in t_years):
+        v = []
+        for c in range(8):
+            a, b, c2, c3 = (float(q) for q in tau[c])
+            xi = a if c2 == 0 else a + (b - a) * (1 + c3 * t / c2) * math.exp(-t / c2)
+            if rho is not None:
+                xi += rho[i][c]
+            if not math.isfinite(xi):
+                raise ValueError("non-finite curve value")
+            v.append(xi)
+        beta = (v[2] * (sx - smin) + v[1] * (smax - sx)) / (smax - smin)
+        out[0].append(rs * v[0] * (xs / xs[y0]) ** beta)
+        out[1].append(np.full(nx * nr, v[3]))
+        out[2].append(v[4] * (v[5] - rs))
+        out[3].append(np.full(nx * nr, v[6]))
+        out[4].append(np.full(nx * nr, v[7]))
+    return [np.array(a) for a in out]
+
+
 def lv_params_c1(x, rate):
     n = len(x)
     dr, vo = np.zeros(n), np.zeros(n)

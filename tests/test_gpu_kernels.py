@@ -663,3 +663,31 @@ def test_dcmt_lanes_bit_exact(H):
         want = raw.astype(np.float32) / np.float32(4294967296.0)
         assert np.array_equal(u.download().reshape(per, n), want), call
         assert np.array_equal(st.download(), ost), call
+
+
+def test_sv_generator_from_term_structure(H):
+    """the SV model's generator from its term structure: opm_sv_params
+    (CSVModel.mkgen) for the second of two dates, then ophip_dgen_sv on the
+    device, against the oracle's mkgen and generator (FP64 L within 1e-13
+    of its largest rate)"""
+    banner("dgen_sv from term-structure curves")
+    from oplib_amd import opmodel as M
+    nx, nr = 48, 6
+    x, x0 = O.xgrid(nx, 10.0, 400.0, 100.0)
+    rv = np.linspace(0.1, 0.6, nr)
+    tau = np.array([[0.2, 0.3, 1.0, 0.5], [-0.5, -0.2, 2.0, 0.0], [0.4, 0.1, 0.5, 1.0],
+                    [0.3, 0.3, 0.0, 0.0], [2.0, 1.0, 1.5, -0.5], [0.35, 0.3, 1.0, 0.0],
+                    [0.1, 0.15, 3.0, 0.2], [0.05, 0.02, 3.0, 0.2]])
+    ts, y0 = [0.25, 1.0], x0 + nx * 2
+    got = [a[1] for a in M.sv_params(x, rv, y0, ts, tau)]
+    want = [a[1] for a in O.sv_params(x, rv, y0, ts, tau)]
+    xs, rs = np.tile(x, nr), np.repeat(rv, nx)
+    sd = 0.02 * xs
+    L = O.sv_generator(nx, nr, xs, rs, sd, *want)
+    d = nx * nr
+    hd = [dev(H, np.ascontiguousarray(a, dtype=np.float64)) for a in (xs, rs, sd, *got)]
+    out = H.DeviceArray.empty((d, d), dtype=np.float64)
+    H.check(H.lib.ophip_dgen_sv(nx, nr, *[a.h for a in hd], 2.0 ** -16, 4, out.h))
+    scale = np.abs(L).max()
+    assert np.abs(out.download() - L).max() <= 1e-13 * scale
+    assert np.allclose(L.sum(axis=1), 0.0, atol=1e-12 * scale)
