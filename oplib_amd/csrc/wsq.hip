@@ -880,8 +880,9 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd1(int d, float df, const uint16
 // ---------------------------------------------------------------------
 // Lists: per 16-row tile (d/16 lists), then per 64-row block (d/64: the
 // union of its four tiles' lists) and per 32-row block (d/32: two tiles),
-// each nkt + 1 ints: the count, then the k tiles in increasing order (a
-// block reads the lo parts of a k tile if one of its row tiles does).
+// each nkt + 1 ints: the count, then the k tiles in increasing order; a
+// block's entry carries bit 17 + q where its row tile q visits the k tile
+// (the others skip it) and reads the lo parts if one of its row tiles does.
 __device__ inline size_t tl_list4(int d) { return (size_t)(d / 16) * (d / 32 + 1); }
 __device__ inline size_t tl_list2(int d) { return tl_list4(d) + (size_t)(d / 64) * (d / 32 + 1); }
 
@@ -939,11 +940,16 @@ __global__ __launch_bounds__(256) void k_wbwd_lists(int d, char *__restrict__ ba
     int n = 0;
     for (int k0 = 0; k0 < nkt; k0 += 64) {  // nkt <= 128
         const int kt = k0 + lane;
-        int f = 0;
+        int f = 0, vm = 0;  // union of the flags; which row tiles visit kt
         if (kt < nkt)
-            for (int q = 0; q < nr; q++) f |= flags[(size_t)(r0 + q) * nkt + kt];
+            for (int q = 0; q < nr; q++) {
+                const int fq = flags[(size_t)(r0 + q) * nkt + kt];
+                f |= fq;
+                vm |= (fq & 1) << q;
+            }
         const uint64_t m = __ballot((f & 1) != 0);
-        if (f & 1) out[1 + n + __popcll(m & ((1ull << lane) - 1))] = kt + ((f & 2) ? 65536 : 0);
+        if (f & 1)
+            out[1 + n + __popcll(m & ((1ull << lane) - 1))] = kt + ((f & 2) ? 65536 : 0) + (nr > 1 ? vm << 17 : 0);
         n += __popcll(m);
     }
     if (lane == 0) out[0] = n;
@@ -1104,11 +1110,12 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd_blk(char *__restrict__ base, c
         for (int e = 0; e < E; e++) {
             const int ent = __builtin_amdgcn_readfirstlane(lst[1 + min(e0 + e * NW, cnt - 1)]);
             const size_t k = (size_t)512 * (ent & 0xffff);
-            const bool lo = (ent >> 16) != 0;  // uniform
+            const bool lo = ((ent >> 16) & 1) != 0;  // uniform
 #pragma unroll
             for (int i = 0; i < RB; i++) {
-                w[e][i] = *(const s8 *)(ha + i * rstride + k);
-                w[e][RB + i] = lo ? *(const s8 *)(la + i * rstride + k) : s8{};
+                const bool vis = ((ent >> (17 + i)) & 1) != 0;  // row tile i visits this k tile
+                w[e][i] = vis ? *(const s8 *)(ha + i * rstride + k) : s8{};
+                w[e][RB + i] = vis && lo ? *(const s8 *)(la + i * rstride + k) : s8{};
             }
 #pragma unroll
             for (int j = 0; j < NT; j++) {

@@ -389,7 +389,8 @@ def backward_lattice(H, nsq, mode=1, nx=128, nr=2):
 def test_wform_backward_tile_lists(hip, nsq):
     """the tile lists built with the fragment layout are the 16 x 32
     fragments whose hi part reaches 2^-40, each flagged where it reaches
-    2^-28 (its lo part is read), numpy on the downloaded fragments; a
+    2^-28 (its lo part is read) and, in a block's list, with the row tiles
+    that visit it, numpy on the downloaded fragments; a
     banded W after few squarings skips most of them"""
     banner(f"wsq backward tile lists nsq={nsq}")
     H, mode = hip, 1
@@ -411,7 +412,8 @@ def test_wform_backward_tile_lists(hip, nsq):
     for i, rows in enumerate(blocks):
         want = functools.reduce(np.union1d, [ref[k][0] for k in rows])
         lo = functools.reduce(np.logical_or, [ref[k][1] for k in rows])
-        want = want + 65536 * lo[want]
+        vm = sum(np.isin(want, ref[k][0]).astype(np.int64) << q for q, k in enumerate(rows))
+        want = want + 65536 * lo[want] + (vm << 17 if len(rows) > 1 else 0)
         assert tl[i, 0] == len(want), (i, tl[i, 0], len(want))
         assert np.array_equal(tl[i, 1:1 + tl[i, 0]], want), i
     frac = sum(len(r[0]) for r in ref) / ((d // 16) * nkt)
