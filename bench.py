@@ -118,9 +118,9 @@ def main():
                          "64 MiB copy kernel 20 times, for counter collection")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
-              "design (scenario sharding, one all-gather of prices) is described in "
-              "DESIGN.md and not measured.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); a multi-GPU run "
+              "would shard the scenarios over the GPUs and gather the prices once; it is "
+              "not built or measured here.", file=sys.stderr)
         sys.exit(2)
 
     from oplib_amd import ophip as H

@@ -16,7 +16,7 @@
 // fixed chunks whose partials are added in chunk order; bias gradients are
 // summed in a fixed lane/offset order.  The same inputs give the same bits.
 //
-// No counterpart in the reference (the north-star surrogate).  The CPU
+// No counterpart in the reference (the learned surrogate).  The CPU
 // oracle (or_mlp_forward / or_mlp_backward / or_adam_step) restates the
 // same network in FP64 with W_l stored [out][in] row-major, the layout of
 // the master weights here.

@@ -91,7 +91,7 @@ WFORM_BOUNDS = {
 @pytest.mark.parametrize("name,prec", sorted(WFORM_BOUNDS))
 def test_wform_mixed_precision_vs_fp64(M, name, prec):
     """U = I + W squarings on FP16/BF16 matrix cores with row-sum
-    renormalisation, against the FP64 oracle (the north-star bar is 1e-4)."""
+    renormalisation, against the FP64 oracle (the target bar is 1e-4)."""
     bound = WFORM_BOUNDS[(name, prec)]
     banner(f"wform {name} prec={prec}")
     cfg = {"rs2048": M.config_rs2048, "lv1d": M.config_lv1d,

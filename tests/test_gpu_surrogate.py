@@ -315,7 +315,7 @@ def test_surrogate_gain_and_fallback(S, hip):
     """trained on solver prices, the network beats the control variate
     alone (the method without it) on held-out draws by a wide margin, and
     every flagged draw gets the solver's own prices"""
-    banner("surrogate: train 4096 draws, judge 256 held-out draws")
+    banner("surrogate: train 4096 draws, check 256 held-out draws")
     sol = S.PriceSolver(nb=64)
     ptr = S.draw_params(4096, S.SEED_TRAIN)
     ytr = sol.solve(ptr)
