@@ -183,8 +183,8 @@ unsigned      ophip_wsq_w(unsigned state, int d, int mode, int buf);
  * that 64 x 64 cell is +-0), 7 backward-step tile lists (d/16 + d/64 +
  * d/32 lists of d/32 + 1 ints, per 16-row tile, 64-row block and 32-row
  * block: count, then the visited k tiles, kt + 65536 where the lo part is
- * read too, + 2^(17+q) in a block's list where its row tile q visits kt;
- * buf ignored)
+ * read too, + 2^(17+q) in a block of nr row tiles where its tile q visits
+ * kt and + 2^(17+nr+q) where that tile reads the lo part; buf ignored)
  * (0 when the mode has no such array) */
 unsigned      ophip_wsq_array(unsigned state, int d, int mode, int buf, int which);
 /* Zero-cell skipping: the squaring visits only the K cells whose A and B

@@ -882,7 +882,8 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd1(int d, float df, const uint16
 // union of its four tiles' lists) and per 32-row block (d/32: two tiles),
 // each nkt + 1 ints: the count, then the k tiles in increasing order; a
 // block's entry carries bit 17 + q where its row tile q visits the k tile
-// (the others skip it) and reads the lo parts if one of its row tiles does.
+// (the others skip it) and bit 17 + nr + q where that tile reads the lo
+// part (bit 16: some tile of the block does).
 __device__ inline size_t tl_list4(int d) { return (size_t)(d / 16) * (d / 32 + 1); }
 __device__ inline size_t tl_list2(int d) { return tl_list4(d) + (size_t)(d / 64) * (d / 32 + 1); }
 
@@ -940,16 +941,18 @@ __global__ __launch_bounds__(256) void k_wbwd_lists(int d, char *__restrict__ ba
     int n = 0;
     for (int k0 = 0; k0 < nkt; k0 += 64) {  // nkt <= 128
         const int kt = k0 + lane;
-        int f = 0, vm = 0;  // union of the flags; which row tiles visit kt
+        int f = 0, vm = 0, lm = 0;  // union of the flags; which row tiles visit kt, read its lo parts
         if (kt < nkt)
             for (int q = 0; q < nr; q++) {
                 const int fq = flags[(size_t)(r0 + q) * nkt + kt];
                 f |= fq;
                 vm |= (fq & 1) << q;
+                lm |= ((fq >> 1) & 1) << q;
             }
         const uint64_t m = __ballot((f & 1) != 0);
         if (f & 1)
-            out[1 + n + __popcll(m & ((1ull << lane) - 1))] = kt + ((f & 2) ? 65536 : 0) + (nr > 1 ? vm << 17 : 0);
+            out[1 + n + __popcll(m & ((1ull << lane) - 1))] =
+                kt + ((f & 2) ? 65536 : 0) + (nr > 1 ? (vm << 17) + (lm << (17 + nr)) : 0);
         n += __popcll(m);
     }
     if (lane == 0) out[0] = n;
@@ -1110,12 +1113,11 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd_blk(char *__restrict__ base, c
         for (int e = 0; e < E; e++) {
             const int ent = __builtin_amdgcn_readfirstlane(lst[1 + min(e0 + e * NW, cnt - 1)]);
             const size_t k = (size_t)512 * (ent & 0xffff);
-            const bool lo = ((ent >> 16) & 1) != 0;  // uniform
 #pragma unroll
-            for (int i = 0; i < RB; i++) {
-                const bool vis = ((ent >> (17 + i)) & 1) != 0;  // row tile i visits this k tile
+            for (int i = 0; i < RB; i++) {  // uniform: does row tile i visit this k tile, read its lo part
+                const bool vis = ((ent >> (17 + i)) & 1) != 0, lo = ((ent >> (17 + RB + i)) & 1) != 0;
                 w[e][i] = vis ? *(const s8 *)(ha + i * rstride + k) : s8{};
-                w[e][RB + i] = vis && lo ? *(const s8 *)(la + i * rstride + k) : s8{};
+                w[e][RB + i] = lo ? *(const s8 *)(la + i * rstride + k) : s8{};
             }
 #pragma unroll
             for (int j = 0; j < NT; j++) {

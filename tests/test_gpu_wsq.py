@@ -413,7 +413,9 @@ def test_wform_backward_tile_lists(hip, nsq):
         want = functools.reduce(np.union1d, [ref[k][0] for k in rows])
         lo = functools.reduce(np.logical_or, [ref[k][1] for k in rows])
         vm = sum(np.isin(want, ref[k][0]).astype(np.int64) << q for q, k in enumerate(rows))
-        want = want + 65536 * lo[want] + (vm << 17 if len(rows) > 1 else 0)
+        lm = sum(ref[k][1][want].astype(np.int64) << q for q, k in enumerate(rows))
+        nr = len(rows)
+        want = want + 65536 * lo[want] + ((vm << 17) + (lm << (17 + nr)) if nr > 1 else 0)
         assert tl[i, 0] == len(want), (i, tl[i, 0], len(want))
         assert np.array_equal(tl[i, 1:1 + tl[i, 0]], want), i
     frac = sum(len(r[0]) for r in ref) / ((d // 16) * nkt)
