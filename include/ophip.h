@@ -91,6 +91,13 @@ void opcuda_sekd_lambda(int d, unsigned gen, float dt, unsigned tpk, unsigned r,
                         unsigned lambda);                               /* sek.cu:129 */
 void opcuda_sck(int d, int nm, unsigned tpk, unsigned ctpk);           /* sck.cu:88 */
 void opcuda_sglvg(unsigned gen, int ni, int d, unsigned invm, unsigned pars); /* sglvg.cu:104 */
+/* sgsvg.cu:215 (bound by CSVModel.cs:383-384): ni FP32 SV generators from
+ * the caller's 4x4 stencil inverses (16 floats per state) and pars = xval
+ * (d), then per generator SDrift, SVol, VolDrift, VolVol, Jumpsz_minus,
+ * Jumpsz_plus (ni*d each); evaluated in FP64 as ophip_dgen_sv (same two
+ * fixes), rounded once; writes every entry of each d x d block.
+ * nx >= 3, nr >= 2. */
+void opcuda_sgsvg(unsigned gen, int ni, int nx, int nr, unsigned invm, unsigned pars);
 
 /* ---- Monte Carlo (smc.cu) ---- */
 int  opcuda_mc_load_mt_gpu(char *stream, long sz);                     /* smc.cu:87 (MT19937: no-op) */
@@ -179,8 +186,10 @@ int ophip_saxpby(int n, float a, unsigned A, float b, unsigned B, unsigned C);
 unsigned long ophip_wsq_bytes(int d, int mode);
 unsigned      ophip_wsq_w(unsigned state, int d, int mode, int buf);
 /* any array of a state: which = 0 W, 1 Hr, 2 Hc, 3 Lr, 4 Lc, 5 partial
- * row sums, 6 cell flags (d/64 x d/64 bytes; 0 = every 16-bit copy value of
- * that 64 x 64 cell is +-0), 7 backward-step tile lists (d/16 + d/64 +
+ * row sums, 6 cell flags (d/64 x d/64 bytes; 0 = every FP32 W and
+ * row-major copy value of that 64 x 64 cell is +-0, and so is every
+ * column-major copy value unless a backward layout has since written its
+ * fragment copies into that buffer), 7 backward-step tile lists (d/16 + d/64 +
  * d/32 lists of d/32 + 1 ints, per 16-row tile, 64-row block and 32-row
  * block: count, then the visited k tiles, kt + 65536 where the lo part is
  * read too, + 2^(17+q) in a block of nr row tiles where its tile q visits

@@ -332,9 +332,9 @@ OPHIP_EXPORT int ophip_sek_rs_batch(int n, const int *nx, const int *nr, const u
 // ---------------------------------------------------------------------
 
 // Drop-in layout (sck.cu:43-84): ctpk[m*d*d + row + d*y].  One thread per
-// (m,row): sequential FP32 running sum along the row, then divide by the
-// total and cap at 1.  Negative entries count as zero (a rounding-level
-// negative would break monotonicity; FP32 plain kernels have none).
+// (m,row): sequential FP32 running sum of the raw entries along the row (a
+// negative entry lowers the sum, as in the reference), then divide by the
+// total and cap at 1.
 __global__ void k_sck(int d, int nm, const float *__restrict__ tpk, float *__restrict__ ctpk) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nm * d) return;
@@ -344,8 +344,7 @@ __global__ void k_sck(int d, int nm, const float *__restrict__ tpk, float *__res
     float *cp = ctpk + base;
     float s = 0.0f;
     for (int y = 0; y < d; y++) {
-        float v = tp[(size_t)d * y];
-        s += (v > 0.0f) ? v : 0.0f;
+        s += tp[(size_t)d * y];
         cp[(size_t)d * y] = s;
     }
     for (int y = 0; y < d; y++) {
@@ -362,78 +361,97 @@ OPHIP_EXPORT void opcuda_sck(int d, int nm, unsigned tpk, unsigned ctpk) {
     check_launch();
 }
 
-// Row-contiguous table crow[m*d*d + row*d + y], the same values bit for bit
-// (the same sequential FP32 order per row): k_sck_scan forms each row's
-// running sums of the positive parts (its last one is the row total the
-// reference forms in a first pass with the same adds), one wavefront per
-// 64 rows; k_sck_norm divides by the total and clamps to 1 over the whole
-// table in parallel.
-__global__ __launch_bounds__(64) void k_sck_scan(int d, const float *__restrict__ tpk,
-                                                  float *__restrict__ crow, float *__restrict__ tot,
-                                                  int plus_identity) {
-    // lane = row: each 64-column step loads the lane's own 64 values with
-    // 64 coalesced loads (64 consecutive rows per column), the next step's
-    // loads issued before this step's sums; the running sums go through
-    // an LDS tile so that the row-contiguous stores are coalesced too.
-    // Rows and columns past d are clamped reads that are never used.
-    __shared__ float tile[64][65];
-    const int lane = threadIdx.x;
-    const int row0 = blockIdx.x * 64, row = row0 + lane;
-    const size_t mb = (size_t)blockIdx.y * d * d;
-    const float *src = tpk + mb + min(row, d - 1);
-    const int last = (d - 1) / 64 * 64;  // first column of the last step
-    float cur[64], nxt[64];
+// Row-contiguous table crow[m*d*d + row*d + y] in a fixed two-level
+// order: each row is cut into 64-column chunks; chunk q holds a sequential
+// FP32 running sum l_q(y) of its entries (negative entries count as zero:
+// a rounding-level negative would make the row non-monotone; with
+// plus_identity 1 is added to the diagonal entry first), the chunk totals
+// t_q = l_q(last) are chained in order (carry_0 = 0, carry_{q+1} =
+// carry_q + t_q), the running sum is S(y) = carry_q + l_q(y), the row total
+// T = S(d-1), and C(y) = min(1, S(y)/T): C(d-1) = 1 exactly, rows are
+// non-decreasing, and the values are within a few ulps of the sequential
+// order (bit for bit where every partial sum is exact).
+//
+// k_sck_part: one wavefront per 64 rows x one chunk (lane = row: the 64
+// values of its chunk with 64 coalesced loads, 64 consecutive rows per
+// column), writes t_q.  k_sck_fin: the same wavefronts again, each chains
+// its rows' totals (the same adds in the same order in every wavefront,
+// so all of them see the same T), re-forms l_q (the same adds as
+// k_sck_part) and writes C through an LDS tile, so the row-contiguous
+// stores are coalesced.  Grid (row blocks, chunks, tables): 1024
+// wavefronts at d = 2048.  Rows and columns past d are clamped reads that
+// are never used.
+constexpr int kSckChunk = 64;
+
+__device__ inline void sck_chunk_load(const float *__restrict__ src, int d, int y0, float (&v)[kSckChunk]) {
 #pragma unroll
-    for (int c = 0; c < 64; c++) cur[c] = src[(size_t)d * min(c, d - 1)];
-    float s = 0.0f;
-    for (int y0 = 0; y0 < d; y0 += 64) {
-        const int yn = min(y0 + 64, last);  // the last step reloads itself (unused)
-#pragma unroll
-        for (int c = 0; c < 64; c++) nxt[c] = src[(size_t)d * min(yn + c, d - 1)];
-        const int ny = min(64, d - y0);
-#pragma unroll
-        for (int c = 0; c < 64; c++) {
-            if (c < ny) {
-                float v = cur[c];
-                if (plus_identity && row == y0 + c) v += 1.0f;
-                s += (v > 0.0f) ? v : 0.0f;
-            }
-            tile[lane][c] = s;
-        }
-        __syncthreads();
-        for (int r = 0; r < 64 && row0 + r < d; r++)
-            if (lane < ny) crow[mb + (size_t)(row0 + r) * d + y0 + lane] = tile[r][lane];
-        __syncthreads();
-#pragma unroll
-        for (int c = 0; c < 64; c++) cur[c] = nxt[c];
-    }
-    if (row < d) tot[(size_t)blockIdx.y * d + row] = s;
+    for (int c = 0; c < kSckChunk; c++) v[c] = src[(size_t)d * min(y0 + c, d - 1)];
 }
 
-// C = min(1, running sum / total), in place, one thread per 4 entries
-__global__ void k_sck_norm(int d, size_t n, float *__restrict__ crow, const float *__restrict__ tot) {
-    const size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+__device__ inline float sck_chunk_sum(const float (&v)[kSckChunk], int ny, int row, int y0, int plus_identity,
+                                      float (*run)[kSckChunk + 1], int lane) {
+    float l = 0.0f;
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-        if (i + e < n) {
-            const float q = crow[i + e] / tot[(i + e) / d];
-            crow[i + e] = q < 1.0f ? q : 1.0f;
+    for (int c = 0; c < kSckChunk; c++) {
+        if (c < ny) {
+            float x = v[c];
+            if (plus_identity && row == y0 + c) x += 1.0f;
+            l += (x > 0.0f) ? x : 0.0f;
         }
+        if (run) run[lane][c] = l;
     }
+    return l;
+}
+
+__global__ __launch_bounds__(64) void k_sck_part(int d, const float *__restrict__ tpk, float *__restrict__ tq,
+                                                  int plus_identity) {
+    const int lane = threadIdx.x, nq = gridDim.y;
+    const int row = blockIdx.x * 64 + lane, q = blockIdx.y, y0 = q * kSckChunk;
+    const size_t mb = (size_t)blockIdx.z * d * d;
+    float v[kSckChunk];
+    sck_chunk_load(tpk + mb + min(row, d - 1), d, y0, v);
+    const float l = sck_chunk_sum(v, min(kSckChunk, d - y0), row, y0, plus_identity, nullptr, lane);
+    if (row < d) tq[((size_t)blockIdx.z * nq + q) * d + row] = l;
+}
+
+__global__ __launch_bounds__(64) void k_sck_fin(int d, const float *__restrict__ tpk,
+                                                 const float *__restrict__ tq, float *__restrict__ crow,
+                                                 int plus_identity) {
+    __shared__ float tile[64][kSckChunk + 1];
+    const int lane = threadIdx.x, nq = gridDim.y;
+    const int row0 = blockIdx.x * 64, row = row0 + lane, q = blockIdx.y, y0 = q * kSckChunk;
+    const size_t mb = (size_t)blockIdx.z * d * d;
+    float v[kSckChunk];
+    sck_chunk_load(tpk + mb + min(row, d - 1), d, y0, v);
+    const float *t = tq + (size_t)blockIdx.z * nq * d + min(row, d - 1);
+    float carry = 0.0f;
+    for (int p = 0; p < q; p++) carry = carry + t[(size_t)p * d];
+    float T = carry;
+    for (int p = q; p < nq; p++) T = T + t[(size_t)p * d];
+    const int ny = min(kSckChunk, d - y0);
+    sck_chunk_sum(v, ny, row, y0, plus_identity, tile, lane);
+#pragma unroll
+    for (int c = 0; c < kSckChunk; c++) {
+        const float s = (carry + tile[lane][c]) / T;
+        tile[lane][c] = s < 1.0f ? s : 1.0f;
+    }
+    __syncthreads();
+    for (int r = 0; r < 64 && row0 + r < d; r++)
+        if (lane < ny) crow[mb + (size_t)(row0 + r) * d + y0 + lane] = tile[r][lane];
 }
 
 static int sck_rows(int d, int nm, unsigned tpk, unsigned crow, int plus_identity) {
     Device &D = dev();
-    if (d <= 0 || nm <= 0 || !ptr(tpk) || !ptr(crow)) return (int)set_status(kErrBadArg);
+    if (d <= 0 || nm <= 0 || nm > 65535 || !ptr(tpk) || !ptr(crow)) return (int)set_status(kErrBadArg);
     if (!handle_range_ok(tpk, 4 * (size_t)nm * d * d) || !handle_range_ok(crow, 4 * (size_t)nm * d * d))
         return (int)set_status(kErrBadArg);
-    float *tot = (float *)workspace(sizeof(float) * (size_t)nm * d);
-    if (!tot) return (int)kErrOutOfArena;
-    hipLaunchKernelGGL(k_sck_scan, dim3(cdiv(d, 64), nm), dim3(64), 0, D.stream, d, P<const float>(tpk),
-                       P<float>(crow), tot, plus_identity);
-    const size_t n = (size_t)nm * d * d, n4 = (n + 3) / 4;
-    hipLaunchKernelGGL(k_sck_norm, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, D.stream, d, n,
-                       P<float>(crow), (const float *)tot);
+    const int nq = cdiv(d, kSckChunk);
+    float *tq = (float *)workspace(sizeof(float) * (size_t)nm * nq * d);
+    if (!tq) return (int)kErrOutOfArena;
+    const dim3 g(cdiv(d, 64), nq, nm);
+    hipLaunchKernelGGL(k_sck_part, g, dim3(64), 0, D.stream, d, P<const float>(tpk), tq, plus_identity);
+    hipLaunchKernelGGL(k_sck_fin, g, dim3(64), 0, D.stream, d, P<const float>(tpk), (const float *)tq,
+                       P<float>(crow), plus_identity);
     return (int)check_launch();
 }
 
@@ -475,6 +493,11 @@ namespace {
 struct SVLattice {
     int nx, nr;
     const double *xval, *rval, *sdrift, *svol, *volvol, *voldrift, *jm, *jp;
+    // the caller's per-state 4x4 stencil inverses (S2DGrid.pregen's invm,
+    // S2DGrid.cs:123-245: invm[16 y + j + 4 i] maps the moments (x drift,
+    // x variance, r drift, r variance) to the rates to x-1, x+1, r-1, r+1,
+    // as sgsvg.cu:72-103 applies it); nullptr: the closed form from rval
+    const float *invm;
 };
 
 // row table: am, ap, bm, bp (after clipping and the drift fix), jump ratio,
@@ -488,7 +511,24 @@ __global__ void k_sv_rows(SVLattice L, double *__restrict__ rt) {
     const int x0 = y0 % nx, r0 = y0 / nx;
     const double X0 = L.xval[y0];
     double am = 0.0, ap = 0.0, bm = 0.0, bp = 0.0, ratio = 0.0, sm = 0.0, sp = 0.0;
-    if (x0 > 0 && x0 < nx - 1) {
+    if (L.invm) {
+        // rates = invm * (0, svol^2 on interior x, voldrift, volvol^2 on
+        // interior r), the moments sgsvg.cu:58-68 sets, in FP64
+        const bool ix = x0 > 0 && x0 < nx - 1, ir = r0 > 0 && r0 < nr - 1;
+        const double rhs[4] = {0.0, ix ? L.svol[y0] * L.svol[y0] : 0.0, L.voldrift[y0],
+                               ir ? L.volvol[y0] * L.volvol[y0] : 0.0};
+        const float *iv = L.invm + (size_t)16 * y0;
+        double rate[4];
+        for (int j = 0; j < 4; j++) {
+            double s = 0.0;
+            for (int i = 0; i < 4; i++) s += (double)iv[j + 4 * i] * rhs[i];
+            rate[j] = s;
+        }
+        am = rate[0];
+        ap = rate[1];
+        bm = rate[2];
+        bp = rate[3];
+    } else if (x0 > 0 && x0 < nx - 1) {
         const double dm = L.xval[y0 - 1] - X0, dp = L.xval[y0 + 1] - X0;
         const double s2 = L.svol[y0] * L.svol[y0];
         const double det = dm * dp * dp - dp * dm * dm;
@@ -496,7 +536,9 @@ __global__ void k_sv_rows(SVLattice L, double *__restrict__ rt) {
         ap = (dm * s2) / det;
     }
     if (nr > 1) {
-        if (r0 > 0 && r0 < nr - 1) {
+        if (L.invm) {
+            // rates from the caller's stencil (above)
+        } else if (r0 > 0 && r0 < nr - 1) {
             const double em = L.rval[y0 - nx] - L.rval[y0], ep = L.rval[y0 + nx] - L.rval[y0];
             const double v2 = L.volvol[y0] * L.volvol[y0], mu = L.voldrift[y0];
             const double det = em * ep * ep - ep * em * em;
@@ -631,4 +673,68 @@ OPHIP_EXPORT int ophip_dgen_sv(int nx, int nr, unsigned xval, unsigned rval, uns
     default: hipLaunchKernelGGL((k_sv_fill<0, double>), g, b, 0, D.stream, L, (const double *)rt, dt, P<double>(out)); break;
     }
     return (int)check_launch();
+}
+
+namespace {
+
+// the reference's FP32 parameter block of generator i to FP64: xval (d,
+// shared), then SDrift, SVol, VolDrift, VolVol, Jumpsz_minus, Jumpsz_plus
+// (ni*d each, generator i at offset i*d; sgsvg.cu:222-228); out: seven
+// arrays of d doubles in the order xval, sdrift, svol, voldrift, volvol,
+// jm, jp
+__global__ void k_sv_pars(int d, int ni, int i, const float *__restrict__ pars, double *__restrict__ out) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= d) return;
+    const size_t nd = (size_t)ni * d, gi = (size_t)i * d + y;
+    out[y] = pars[y];
+    out[(size_t)d + y] = pars[d + gi];                      // SDrift
+    out[(size_t)2 * d + y] = pars[d + nd + gi];             // SVol
+    out[(size_t)3 * d + y] = pars[d + 2 * nd + gi];         // VolDrift
+    out[(size_t)4 * d + y] = pars[d + 3 * nd + gi];         // VolVol
+    out[(size_t)5 * d + y] = pars[d + 4 * nd + gi];         // Jumpsz_minus
+    out[(size_t)6 * d + y] = pars[d + 5 * nd + gi];         // Jumpsz_plus
+}
+
+}  // namespace
+
+// The reference's device entry point for the stochastic-volatility
+// generators (sgsvg.cu:215-236, bound by CSVModel.cs:383-384 and called
+// by CSVModel.device_sgen, CSVModel.cs:417): ni FP32 generators
+// gen[i*d*d + y0 + d*y1], d = nx*nr, from the caller's stencil inverses
+// invm (16 floats per state) and the parameter block pars (d + 6*ni*d
+// floats: xval, then per generator SDrift, SVol, VolDrift, VolVol,
+// Jumpsz_minus, Jumpsz_plus).  Every entry is evaluated in FP64 by the
+// kernels of ophip_dgen_sv (rates from invm, jumps into the next regime,
+// clipping, drift fix, diagonal; with that generator's two fixes) and
+// rounded once to FP32; the whole d x d block of each generator is
+// written (the reference writes the nonzeros into a block its caller
+// zeroed first, CSVModel.cs:414).  Bad sizes or handles set the status
+// (the reference returns silently when d % 32 != 0, sgsvg.cu:218).
+OPHIP_EXPORT void opcuda_sgsvg(unsigned gen, int ni, int nx, int nr, unsigned invm, unsigned pars) {
+    const long dl = (long)nx * nr;
+    if (nx < 3 || nr < 2 || ni <= 0 || dl > 32767 || (long)ni * dl * dl * 4 > 0xFFFFFFFFL) {
+        set_status(kErrBadArg);
+        return;
+    }
+    const size_t d = (size_t)dl;
+    if (!handle_range_ok(gen, (size_t)4 * ni * d * d) || !handle_range_ok(invm, (size_t)64 * d) ||
+        !handle_range_ok(pars, (size_t)4 * (d + 6 * (size_t)ni * d))) {
+        set_status(kErrBadHandle);
+        return;
+    }
+    double *ws = (double *)workspace(sizeof(double) * (7 + kSVRow) * d);
+    if (!ws) return;
+    double *rt = ws + 7 * d;
+    Device &D = dev();
+    SVLattice L{nx, nr, ws, nullptr, ws + d, ws + 2 * d, ws + 4 * d, ws + 3 * d, ws + 5 * d, ws + 6 * d,
+                P<const float>(invm)};
+    const dim3 g(grid1d(d * d, 256)), b(256);
+    for (int i = 0; i < ni; i++) {
+        hipLaunchKernelGGL(k_sv_pars, dim3(cdiv((int)d, 64)), dim3(64), 0, D.stream, (int)d, ni, i,
+                           P<const float>(pars), ws);
+        hipLaunchKernelGGL(k_sv_rows, dim3(cdiv((int)d, 64)), dim3(64), 0, D.stream, L, rt);
+        hipLaunchKernelGGL((k_sv_fill<0, float>), g, b, 0, D.stream, L, (const double *)rt, 0.0,
+                           P<float>(gen) + (size_t)i * d * d);
+    }
+    check_launch();
 }

@@ -16,13 +16,18 @@
 // words produced by the previous chunk).  State buffer: stream s occupies
 // words [625*s, 625*s + 625): 624 state words and the output index.
 //
-// Path mapping (fixed; the CPU oracle reproduces it): stream s serves G
-// groups of 64 paths, scenario = (s*G + g)*64 + lane.  Within a group the
-// draws are step-major, lane-minor: at step k lane l consumes the stream's
-// next output number k*64 + l.  Uniform u = (x >> 8) * 2^-24 in [0,1).
+// Path mapping (fixed; the CPU oracle reproduces it): with nscen paths
+// over nrng streams, stream s serves per = nscen/nrng paths, scenario
+// s*per + j, in groups of 64 consecutive paths (the last group holds
+// per mod 64 when that is not 0).  A group's chains run step by step; at
+// step k its n paths take the stream's next n outputs in path order, so a
+// call consumes exactly per*nk outputs of each stream (the reference's
+// lanes consume one output per path and step too, smc.cu:195-250).
+// Uniform u = (x >> 8) * 2^-24 in [0,1).
 // Chain: y_{-1} = y0, y_k = min{ j : u_k < C[m_k[k]][y_{k-1}, j] } clamped
-// to d-1 (the inverse-CDF of rangen.cpp:128-149).  Output is time-major
-// int16 y_sk[k*nscen + scen] (smc.cu:222).
+// to d-1 (the inverse CDF; the reference's fixed-count bisection from
+// lb = 0, smc.cu:224-238, returns the same state, 0 included).  Output is
+// time-major int16 y_sk[k*nscen + scen] (smc.cu:222).
 #include "ophip_internal.h"
 
 #include <vector>
@@ -79,23 +84,25 @@ struct WaveMT {
         for (int i = lane; i < kMTN; i += 64) st[i] = b[i];
         if (lane == 0) st[kMTN] = (uint32_t)idx;
     }
-    // 64 consecutive outputs, lane l gets output idx + l.  Whenever this
-    // call reaches the end of the current state (idx + 64 >= 624, including
-    // landing exactly on 624) the next state is twisted first, because the
+    // n <= 64 consecutive outputs, lane l < n gets output idx + l (lanes
+    // >= n read a value they must not use).  Whenever this call reaches
+    // the end of the current state (idx + n >= 624, including landing
+    // exactly on 624) the next state is twisted first, because the
     // buffers swap at the end of the call.
-    __device__ uint32_t next64(int lane) {
+    __device__ uint32_t next(int lane, int n) {
         uint32_t *c = cur ? buf1 : buf0;
-        uint32_t *n = cur ? buf0 : buf1;
-        if (idx + 64 >= kMTN) mt_twist(c, n, lane);
+        uint32_t *nb = cur ? buf0 : buf1;
+        if (idx + n >= kMTN) mt_twist(c, nb, lane);
         int pos = idx + lane;
-        uint32_t x = (pos < kMTN) ? c[pos] : n[pos - kMTN];
-        idx += 64;
+        uint32_t x = (pos < kMTN) ? c[pos] : nb[pos - kMTN < kMTN ? pos - kMTN : 0];
+        idx += n;
         if (idx >= kMTN) {
             idx -= kMTN;
             cur ^= 1;
         }
         return mt_temper(x);
     }
+    __device__ uint32_t next64(int lane) { return next(lane, 64); }
 };
 
 __device__ inline float u01(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
@@ -115,7 +122,7 @@ __global__ void k_mt_seed(int nrng, const uint32_t *__restrict__ seeds, uint32_t
 }
 
 template <bool ROWS>
-__global__ __launch_bounds__(64) void k_mc1f(uint32_t *__restrict__ mt, int G, int nk, int d,
+__global__ __launch_bounds__(64) void k_mc1f(uint32_t *__restrict__ mt, int per, int nk, int d,
                                              int y0, const float *__restrict__ ctpk,
                                              const int *__restrict__ m_k,
                                              int16_t *__restrict__ y_sk, size_t nscen, int niter) {
@@ -126,11 +133,13 @@ __global__ __launch_bounds__(64) void k_mc1f(uint32_t *__restrict__ mt, int G, i
     uint32_t *st = mt + (size_t)s * kMTW;
     w.load(st, lane);
     const size_t d2 = (size_t)d * d;
-    for (int g = 0; g < G; g++) {
+    for (int j0 = 0; j0 < per; j0 += 64) {
+        const int n = min(64, per - j0);  // paths of this group
+        const bool live = lane < n;
         int y = y0;
-        const size_t scen = ((size_t)s * G + g) * 64 + lane;
+        const size_t scen = (size_t)s * per + j0 + lane;
         for (int k = 0; k < nk; k++) {
-            const float u = u01(w.next64(lane));
+            const float u = u01(w.next(lane, n));
             const float *tab = ctpk + (size_t)m_k[k] * d2;
             const float *row = ROWS ? tab + (size_t)y * d : tab + y;
             const size_t stride = ROWS ? 1 : (size_t)d;
@@ -142,8 +151,8 @@ __global__ __launch_bounds__(64) void k_mc1f(uint32_t *__restrict__ mt, int G, i
                     else lo = mid;
                 }
             }
-            y = hi;
-            y_sk[(size_t)k * nscen + scen] = (int16_t)y;
+            y = live ? hi : y0;
+            if (live) y_sk[(size_t)k * nscen + scen] = (int16_t)y;
         }
     }
     w.store(st, lane);
@@ -175,7 +184,7 @@ __global__ void k_mc_guide(int d, size_t nrows, const float *__restrict__ crow,
     guide[i] = (int16_t)hi;
 }
 
-__global__ __launch_bounds__(64) void k_mc1f_guided(uint32_t *__restrict__ mt, int G, int nk, int d,
+__global__ __launch_bounds__(64) void k_mc1f_guided(uint32_t *__restrict__ mt, int per, int nk, int d,
                                                     int y0, const float *__restrict__ crow,
                                                     const int16_t *__restrict__ guide,
                                                     const int *__restrict__ m_k,
@@ -187,11 +196,13 @@ __global__ __launch_bounds__(64) void k_mc1f_guided(uint32_t *__restrict__ mt, i
     uint32_t *st = mt + (size_t)s * kMTW;
     w.load(st, lane);
     const size_t d2 = (size_t)d * d, gstride = (size_t)d * (kGuideNB + 1);
-    for (int g = 0; g < G; g++) {
+    for (int j0 = 0; j0 < per; j0 += 64) {
+        const int n = min(64, per - j0);
+        const bool live = lane < n;
         int y = y0;
-        const size_t scen = ((size_t)s * G + g) * 64 + lane;
+        const size_t scen = (size_t)s * per + j0 + lane;
         for (int k = 0; k < nk; k++) {
-            const float u = u01(w.next64(lane));
+            const float u = u01(w.next(lane, n));
             const int m = m_k[k];
             const float *row = crow + (size_t)m * d2 + (size_t)y * d;
             const int16_t *gd = guide + (size_t)m * gstride + (size_t)y * (kGuideNB + 1);
@@ -202,14 +213,16 @@ __global__ __launch_bounds__(64) void k_mc1f_guided(uint32_t *__restrict__ mt, i
                 if (u < row[mid]) hi = mid;
                 else lo = mid;
             }
-            y = hi;
-            y_sk[(size_t)k * nscen + scen] = (int16_t)y;
+            y = live ? hi : y0;
+            if (live) y_sk[(size_t)k * nscen + scen] = (int16_t)y;
         }
     }
     w.store(st, lane);
 }
 
-__global__ __launch_bounds__(64) void k_mt_bench(uint32_t *__restrict__ mt, int per_stream,
+// stream s writes its next per outputs to unif[s + nrng*j], j < per
+// (lane-interleaved, the reference's layout, smc.cu:712), 64 at a time
+__global__ __launch_bounds__(64) void k_mt_bench(uint32_t *__restrict__ mt, int per, int nrng,
                                                  float *__restrict__ unif) {
     __shared__ uint32_t b0[kMTN], b1[kMTN];
     const int lane = threadIdx.x;
@@ -217,8 +230,11 @@ __global__ __launch_bounds__(64) void k_mt_bench(uint32_t *__restrict__ mt, int 
     WaveMT w{b0, b1, 0, 0};
     uint32_t *st = mt + (size_t)s * kMTW;
     w.load(st, lane);
-    float *o = unif + (size_t)s * per_stream;
-    for (int j = 0; j < per_stream; j += 64) o[j + lane] = u01(w.next64(lane));
+    for (int j0 = 0; j0 < per; j0 += 64) {
+        const int n = min(64, per - j0);
+        const float u = u01(w.next(lane, n));
+        if (lane < n) unif[(size_t)s + (size_t)nrng * (j0 + lane)] = u;
+    }
     w.store(st, lane);
 }
 
@@ -232,8 +248,7 @@ int mc_run(bool rows, unsigned mtptr, int y0, unsigned y_skp, int nscen, int nk,
            unsigned ctpk, unsigned m_kp, unsigned long yhost) {
     Device &D = dev();
     const int nrng = g_nrng;
-    if (nscen <= 0 || nk <= 0 || d <= 0 || d > 32767 || y0 < 0 || y0 >= d ||
-        nscen % (64 * nrng) != 0)
+    if (nscen <= 0 || nk <= 0 || d <= 0 || d > 32767 || y0 < 0 || y0 >= d || nscen % nrng != 0)
         return (int)set_status(kErrBadArg);
     uint32_t *mt = P<uint32_t>(mtptr);
     int16_t *ysk = P<int16_t>(y_skp);
@@ -243,13 +258,14 @@ int mc_run(bool rows, unsigned mtptr, int y0, unsigned y_skp, int nscen, int nk,
     if (!handle_range_ok(mtptr, (size_t)nrng * kMTW * 4) ||
         !handle_range_ok(y_skp, (size_t)nscen * nk * 2) || !handle_range_ok(m_kp, (size_t)nk * 4))
         return (int)set_status(kErrBadHandle);
-    const int G = nscen / (64 * nrng);
+    if (!handle_range_ok(ctpk, (size_t)4 * d * d)) return (int)set_status(kErrBadHandle);
+    const int per = nscen / nrng;
     const int niter = ceil_log2(d);
     if (rows)
-        hipLaunchKernelGGL(k_mc1f<true>, dim3(nrng), dim3(64), 0, D.stream, mt, G, nk, d, y0, tab,
+        hipLaunchKernelGGL(k_mc1f<true>, dim3(nrng), dim3(64), 0, D.stream, mt, per, nk, d, y0, tab,
                            mk, ysk, (size_t)nscen, niter);
     else
-        hipLaunchKernelGGL(k_mc1f<false>, dim3(nrng), dim3(64), 0, D.stream, mt, G, nk, d, y0,
+        hipLaunchKernelGGL(k_mc1f<false>, dim3(nrng), dim3(64), 0, D.stream, mt, per, nk, d, y0,
                            tab, mk, ysk, (size_t)nscen, niter);
     unsigned st = check_launch();
     if (st) return (int)st;
@@ -307,8 +323,8 @@ OPHIP_EXPORT void opcuda_mc_setseed(unsigned long host_seedptr, unsigned mtptr) 
 
 // Sample nk steps of nscen chains on the reference's column-major table
 // layout ctpk[m*d*d + yprev + d*y].  nscen must be a multiple of
-// 64*opcuda_mc_nrng() (the reference requires a multiple of its 4096
-// lanes, CModel Device.cs:345).  d up to 32767 (int16 states; the
+// opcuda_mc_nrng() (4096 by default: the reference's requirement,
+// CModel Device.cs:345, smc.cu:616).  d up to 32767 (int16 states; the
 // reference stops at d < 1024, smc.cu:622).  yhost = 0 leaves the states
 // on the device (no host round trip).
 OPHIP_EXPORT int opcuda_mc1f(unsigned mtptr, int y0, unsigned y_sk, int nscen, int nk, int d,
@@ -323,19 +339,20 @@ OPHIP_EXPORT int ophip_mc1f_rows(unsigned mtptr, int y0, unsigned y_sk, int nsce
     return mc_run(true, mtptr, y0, y_sk, nscen, nk, d, crow, m_k, yhost);
 }
 
-// nscen uniforms: stream s writes its next nscen/nrng outputs to
-// unif[s*(nscen/nrng) + j] (smc.cu:664-736 writes lane-interleaved;
-// here each stream's outputs are contiguous).  nscen % (64*nrng) == 0.
+// nscen uniforms: stream s writes its next per = nscen/nrng outputs to
+// unif[s + nrng*j], j < per (lane-interleaved, as smc.cu:712 writes
+// unif_s[iRng + nRng*scen]).  nscen % nrng == 0; each call consumes
+// exactly per outputs of every stream.
 OPHIP_EXPORT int opcuda_mt_benchmark(unsigned mtptr, unsigned unif, int nscen) {
     Device &D = dev();
     const int nrng = g_nrng;
-    if (nscen <= 0 || nscen % (64 * nrng)) return (int)set_status(kErrBadArg);
+    if (nscen <= 0 || nscen % nrng) return (int)set_status(kErrBadArg);
     uint32_t *mt = P<uint32_t>(mtptr);
     float *u = P<float>(unif);
     if (!mt || !u || !handle_range_ok(unif, (size_t)nscen * 4) ||
         !handle_range_ok(mtptr, (size_t)nrng * kMTW * 4))
         return (int)set_status(kErrBadHandle);
-    hipLaunchKernelGGL(k_mt_bench, dim3(nrng), dim3(64), 0, D.stream, mt, nscen / nrng, u);
+    hipLaunchKernelGGL(k_mt_bench, dim3(nrng), dim3(64), 0, D.stream, mt, nscen / nrng, nrng, u);
     return (int)check_launch();
 }
 
@@ -358,7 +375,7 @@ OPHIP_EXPORT int ophip_mc1f_guided(unsigned mtptr, int y0, unsigned y_skp, int n
                                    unsigned crow, unsigned guide, unsigned m_kp) {
     Device &D = dev();
     const int nrng = g_nrng;
-    if (nscen <= 0 || nk <= 0 || d <= 0 || d > 32767 || y0 < 0 || y0 >= d || nscen % (64 * nrng) != 0)
+    if (nscen <= 0 || nk <= 0 || d <= 0 || d > 32767 || y0 < 0 || y0 >= d || nscen % nrng != 0)
         return (int)set_status(kErrBadArg);
     uint32_t *mt = P<uint32_t>(mtptr);
     int16_t *ysk = P<int16_t>(y_skp);
@@ -369,9 +386,10 @@ OPHIP_EXPORT int ophip_mc1f_guided(unsigned mtptr, int y0, unsigned y_skp, int n
     if (!handle_range_ok(mtptr, (size_t)nrng * kMTW * 4) ||
         !handle_range_ok(y_skp, (size_t)nscen * nk * 2) || !handle_range_ok(m_kp, (size_t)nk * 4))
         return (int)set_status(kErrBadHandle);
-    const int G = nscen / (64 * nrng);
-    hipLaunchKernelGGL(k_mc1f_guided, dim3(nrng), dim3(64), 0, D.stream, mt, G, nk, d, y0, tab, gd, mk,
-                       ysk, (size_t)nscen);
+    if (!handle_range_ok(crow, (size_t)4 * d * d) || !handle_range_ok(guide, (size_t)2 * d * (kGuideNB + 1)))
+        return (int)set_status(kErrBadHandle);
+    hipLaunchKernelGGL(k_mc1f_guided, dim3(nrng), dim3(64), 0, D.stream, mt, nscen / nrng, nk, d, y0, tab, gd,
+                       mk, ysk, (size_t)nscen);
     return (int)check_launch();
 }
 

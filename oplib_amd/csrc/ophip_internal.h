@@ -38,6 +38,9 @@ struct Device {
 
 // current device context (creates the arena and stream on first use)
 Device &dev();
+// the current device context if its arena and stream exist, else nullptr;
+// never creates them (no allocation)
+Device *dev_ready();
 // handle -> device address; nullptr for 0 or out-of-arena handles
 void *ptr(uint32_t h);
 template <class T> inline T *P(uint32_t h) { return reinterpret_cast<T *>(ptr(h)); }

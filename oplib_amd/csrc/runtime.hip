@@ -90,6 +90,15 @@ Device &dev() {
     return g_dev[t_dev];
 }
 
+Device *dev_ready() {
+    int id = t_dev;
+    if (id < 0 && hipGetDevice(&id) != hipSuccess) return nullptr;
+    if (id < 0 || id >= kMaxDev) return nullptr;
+    std::lock_guard<std::mutex> lk(g_mu);
+    Device &d = g_dev[id];
+    return d.base ? &d : nullptr;
+}
+
 void *ptr(uint32_t h) {
     if (h == 0) return nullptr;
     Device &d = dev();

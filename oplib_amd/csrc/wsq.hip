@@ -124,14 +124,17 @@ template <> struct Num<true> {
 };
 
 // ---------------------------------------------------------------------
-// k_wsplit: 64x64 tile per workgroup (64 threads), LDS transpose.  A tile
-// whose copies are all zero and whose destination cell is not flagged
-// (it already holds zeros) stores no copies.
+// k_wsplit: 64x64 tile per workgroup (64 threads), LDS transpose.  With
+// the sparse round loop (dense = 0) a tile whose copies are all zero and
+// whose destination cell is not flagged stores no copies: that loop reads
+// no unflagged cell.  The dense loop reads every cell's copies, and a
+// backward layout may have left fragment copies in the column-major slots
+// of an unflagged cell, so with dense = 1 every tile stores its copies.
 // ---------------------------------------------------------------------
 template <bool BF, bool LO>
 __global__ __launch_bounds__(64) void k_wsplit(char *__restrict__ base,
                                                const WScen *__restrict__ scen,
-                                               const WItem *__restrict__ items) {
+                                               const WItem *__restrict__ items, int dense) {
     using N = Num<BF>;
     __shared__ uint16_t th[64][66], tl[64][66];
     const WItem it = items[blockIdx.x];
@@ -174,7 +177,7 @@ __global__ __launch_bounds__(64) void k_wsplit(char *__restrict__ base,
     // the tile is one cell
     const bool any = __ballot(nz != 0) != 0;
     if (lane == 0) *flag = any ? 1 : 0;
-    if (!any && !was) return;  // wave-uniform: the cell holds zeros already
+    if (!any && !was && !dense) return;  // wave-uniform: no flagged cell to overwrite
     // column-major copies: lane = row, from its own LDS row
 #pragma unroll 8
     for (int c = 0; c < 64; c++) {
@@ -1526,11 +1529,11 @@ OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
     const dim3 fixgrid(cdiv(P->dmax, 64), P->n);
     return (int)launch_mode(P->mode, [&](auto bf, auto lo) {
         constexpr bool BF = decltype(bf)::value, LO = decltype(lo)::value;
-        hipLaunchKernelGGL((k_wsplit<BF, LO>), dim3(P->n_split), dim3(64), 0, D.stream, D.base,
-                           P->sc_split, P->it_split);
         // sparse rounds: live-item compaction before each squaring launch;
         // every diagonal fix clears the counters for the next round
         const bool sparse = g_wsq_skip && P->sparse_ok;
+        hipLaunchKernelGGL((k_wsplit<BF, LO>), dim3(P->n_split), dim3(64), 0, D.stream, D.base,
+                           P->sc_split, P->it_split, sparse ? 0 : 1);
         int *counts = sparse ? P->counts : nullptr;
         hipLaunchKernelGGL((k_wfix<BF, LO>), fixgrid, dim3(64), 0, D.stream, D.base, P->sc_split,
                            P->n, (const int *)nullptr, 0, counts);
@@ -1597,7 +1600,10 @@ OPHIP_EXPORT unsigned ophip_wsq_w(unsigned state, int d, int mode, int buf) {
 // handle of one array of a state: which = 0 W (FP32), 1 Hr, 2 Hc, 3 Lr,
 // 4 Lc (16-bit copies, row-/column-major), 5 partial row sums (FP64),
 // 6 cell flags of the copies (d/64 x d/64 bytes, row-major; 0 = every
-// value of the 64 x 64 cell is +-0); 0 for arrays the mode does not have
+// FP32 W and row-major copy value of the 64 x 64 cell is +-0, and so is
+// every column-major copy value unless a backward layout has written its
+// fragment copies into that buffer's column-major slots since); 0 for
+// arrays the mode does not have
 OPHIP_EXPORT unsigned ophip_wsq_array(unsigned state, int d, int mode, int buf, int which) {
     if (d <= 0 || d % 64 || mode < 0 || mode > 2 || buf < 0 || buf > 1 || which < 0 || which > 7)
         return 0;

@@ -61,6 +61,7 @@ SIGNATURES = {
     "opcuda_sekd_lambda": (V, [I32, U32, F32, U32, U32, U32]),
     "opcuda_sck": (V, [I32, I32, U32, U32]),
     "opcuda_sglvg": (V, [U32, I32, I32, U32, U32]),
+    "opcuda_sgsvg": (V, [U32, I32, I32, I32, U32, U32]),
     "opcuda_mc_load_mt_gpu": (I32, [C.c_char_p, LONG]),
     "opcuda_mc_nrng": (I32, []),
     "opcuda_mc_status_sz": (I32, []),

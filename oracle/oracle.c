@@ -355,27 +355,35 @@ void or_row_renorm(int d, double *U)
 /* gather GEMV (A10)                                                    */
 /* ------------------------------------------------------------------ */
 
-/* global_sgemv3 semantics (sgemv3.cu:92-159): for every listed column c,
+/* global_sgemv3 / global_sgemv4 semantics (sgemv3.cu:92-159,
+ * sgemv4.cu:90-170): for every listed column c,
  * Z[col1[c] + i] = sum_k A[i,k] * Z[col0[c] + k], i < d.  Column offsets are
- * element offsets into Z.  Processing stops at the first col1 >= sentinel
- * (sgemv3.cu:141-157).  All reads happen before any write. */
+ * element offsets into Z.  The list is read in groups of 16 (one thread
+ * block's columns); inside a group the stores stop at the first
+ * col1 >= sentinel (sgemv3.cu:141-157, sgemv4.cu:155-170: `return` at the
+ * first sentinel), the next group starts afresh.  All reads happen before
+ * any write.  FP64 accumulation (the checker of an FP32 kernel). */
 void or_sgemv_gather(int d, const float *A, float *Z, int ncol,
                      const int *col0, const int *col1, int sentinel)
 {
-    double *out = (double *)calloc((size_t)d * ncol, sizeof(double));
-    int nvalid = 0;
-    for (int c = 0; c < ncol; c++) {
-        if (col1[c] >= sentinel) break;
-        nvalid++;
-        for (int kk = 0; kk < d; kk++) {
-            double z = Z[col0[c] + kk];
-            const float *a = A + (size_t)d * kk;
-            for (int i = 0; i < d; i++) out[(size_t)d * c + i] += (double)a[i] * z;
+    double *out = (double *)calloc((size_t)d * (ncol > 0 ? ncol : 1), sizeof(double));
+    char *live = (char *)calloc((size_t)(ncol > 0 ? ncol : 1), 1);
+    for (int g0 = 0; g0 < ncol; g0 += 16) {
+        for (int c = g0; c < ncol && c < g0 + 16; c++) {
+            if (col1[c] >= sentinel) break;
+            live[c] = 1;
+            for (int kk = 0; kk < d; kk++) {
+                double z = Z[col0[c] + kk];
+                const float *a = A + (size_t)d * kk;
+                for (int i = 0; i < d; i++) out[(size_t)d * c + i] += (double)a[i] * z;
+            }
         }
     }
-    for (int c = 0; c < nvalid; c++)
-        for (int i = 0; i < d; i++) Z[col1[c] + i] = (float)out[(size_t)d * c + i];
+    for (int c = 0; c < ncol; c++)
+        if (live[c])
+            for (int i = 0; i < d; i++) Z[col1[c] + i] = (float)out[(size_t)d * c + i];
     free(out);
+    free(live);
 }
 
 /* Backward induction in FP64: V <- df * U * V, nsteps times, V is d x ncol. */
@@ -394,10 +402,10 @@ void or_backward_induction(int d, const double *U, int nsteps, double df,
 /* cumulative tables (A11)                                              */
 /* ------------------------------------------------------------------ */
 
-/* global_sck (sck.cu:43-84): per (m,row) a sequential FP32 running sum along
- * the row (stride d), then every entry divided by the row total and capped
- * at 1.  Entries below zero are taken as zero (a rounding-level negative
- * would make the row non-monotone; FP32 plain kernels have none). */
+/* global_sck (sck.cu:43-84): per (m,row) a sequential FP32 running sum of
+ * the row's raw entries (stride d; a negative entry lowers the sum, as in
+ * the reference), then every entry divided by the row total and capped at
+ * 1 (sck.cu:74-80). */
 void or_sck_f32(int d, int nm, const float *tpk, float *ctpk)
 {
     size_t d2 = (size_t)d * d;
@@ -407,8 +415,7 @@ void or_sck_f32(int d, int nm, const float *tpk, float *ctpk)
             float *c = ctpk + m * d2 + row;
             float s = 0.0f;
             for (int y = 0; y < d; y++) {
-                float v = t[(size_t)d * y];
-                s += (v > 0.0f) ? v : 0.0f;
+                s += t[(size_t)d * y];
                 c[(size_t)d * y] = s;
             }
             for (int y = 0; y < d; y++) {
@@ -418,7 +425,11 @@ void or_sck_f32(int d, int nm, const float *tpk, float *ctpk)
         }
 }
 
-/* Same table, stored row-contiguous: crow[m*d*d + row*d + y]. */
+/* The same table stored row-contiguous, crow[m*d*d + row*d + y], with
+ * entries below zero taken as zero (a rounding-level negative would make
+ * the row non-monotone and the sampler's bisection ill-defined): the
+ * sequential order of or_sck_f32, so on nonnegative tables this is that
+ * table transposed, bit for bit. */
 void or_sck_rows_f32(int d, int nm, const float *tpk, float *crow)
 {
     size_t d2 = (size_t)d * d;
@@ -434,6 +445,41 @@ void or_sck_rows_f32(int d, int nm, const float *tpk, float *crow)
             }
             for (int y = 0; y < d; y++) {
                 float v = c[y] / s;
+                c[y] = (v < 1.0f) ? v : 1.0f;
+            }
+        }
+}
+
+/* The row-contiguous table in a fixed two-level order (the GPU's
+ * ophip_sck_rows): each row is cut into chunks of `chunk` columns; chunk q
+ * holds a sequential FP32 running sum l_q(y) of its clamped entries from 0
+ * (plus_identity: 1 added to the diagonal entry first); the chunk totals
+ * t_q = l_q(last) are chained in order, carry_0 = 0, carry_{q+1} =
+ * carry_q + t_q; the running sum is S(y) = carry_q + l_q(y) and the row
+ * total T = S(d-1); C(y) = min(1, S(y) / T), so C(d-1) = 1 exactly and rows
+ * are non-decreasing.  Where every partial sum is exact (e.g. dyadic
+ * entries) this equals the sequential order bit for bit. */
+void or_sck_rows2_f32(int d, int nm, int chunk, int plus_identity, const float *tpk, float *crow)
+{
+    size_t d2 = (size_t)d * d;
+    for (int m = 0; m < nm; m++)
+        for (int row = 0; row < d; row++) {
+            const float *t = tpk + m * d2 + row;
+            float *c = crow + m * d2 + (size_t)row * d;
+            float carry = 0.0f;
+            for (int y0 = 0; y0 < d; y0 += chunk) {
+                float l = 0.0f;
+                for (int y = y0; y < d && y < y0 + chunk; y++) {
+                    float v = t[(size_t)d * y];
+                    if (plus_identity && y == row) v += 1.0f;
+                    l += (v > 0.0f) ? v : 0.0f;
+                    c[y] = carry + l;
+                }
+                carry = carry + l;
+            }
+            const float T = c[d - 1];
+            for (int y = 0; y < d; y++) {
+                float v = c[y] / T;
                 c[y] = (v < 1.0f) ? v : 1.0f;
             }
         }
@@ -524,35 +570,56 @@ static int sample_bisect(const float *base, size_t stride, int d, float u)
     return hi;
 }
 
-/* The build's Monte Carlo mapping (one MT19937 stream per wavefront of 64
- * paths): stream s (seed seed0+s) serves groups g = 0..G-1 of 64 paths;
- * scenario index = (s*G + g)*64 + lane.  Within a group, draws are
- * step-major, lane-minor: draw index (within the group's segment of the
- * stream) = k*64 + lane.  Chain: y_{-1} = y0, y_k = inverse CDF of row
- * y_{k-1} of table m_k[k] (smc.cu:195-255 restarts at y0 per scenario).
- * Output time-major y_sk[k*nscen + scen] (smc.cu:222, rangen.cpp:122).
+/* The reference's GPU search (smc.cu:224-238, 336-347, 447-460,
+ * 560-572): lb = 0, ub = d-1, a fixed number of halvings (10 for
+ * 512 < d <= 1024, 9 for 256 < d <= 512, 8 for 128 < d <= 256, 7 for
+ * d <= 128), mid = (ub+lb)/2, ub = mid when u < C[mid] else lb = mid,
+ * result ub.  Column-major table (C[yprev + d*j]).  The count of halvings
+ * is the deepest path to ub = 0 for that size bucket, so the result is the
+ * inverse CDF min{j : u < C[j]} clamped to d-1 (state 0 included: with
+ * ub = 1, lb = 0 the last halving tests C[0]). */
+int or_sample_ref_bisect(const float *ctpk_m, int d, int yprev, float u)
+{
+    int niter = d > 512 ? 10 : d > 256 ? 9 : d > 128 ? 8 : 7;
+    int ub = d - 1, lb = 0;
+    for (int it = 0; it < niter; it++) {
+        int mid = (ub + lb) / 2;
+        if (u < ctpk_m[yprev + (size_t)d * mid]) ub = mid;
+        else lb = mid;
+    }
+    return ub;
+}
+
+/* The build's Monte Carlo mapping (one MT19937 stream per wavefront):
+ * stream s (seed seed0+s) serves `per` paths, scenario index s*per + j,
+ * in groups of 64 consecutive paths (the last group holds per mod 64 when
+ * that is not 0).  For every group the chains run step by step; at step k
+ * the group's n paths take the stream's next n outputs in path order (so
+ * a stream consumes exactly per*nk outputs per call).  Chain:
+ * y_{-1} = y0, y_k = inverse CDF of row y_{k-1} of table m_k[k]
+ * (smc.cu:195-255 restarts at y0 per scenario).  Output time-major
+ * y_sk[k*nscen + scen] (smc.cu:222, rangen.cpp:122), nscen = nstreams*per.
  * layout_rows = 0: reference layout ctpk[m*d*d + yprev + d*y];
  * layout_rows = 1: row-contiguous ctpk[m*d*d + yprev*d + y].
  * d <= 64 uses the linear scan (the definition); larger d bisect, which is
  * identical on non-decreasing rows. */
-void or_mc_chain(int nstreams, uint32_t seed0, int groups_per_stream,
-                 int nk, int d, int y0, const float *ctpk, const int *m_k,
-                 int layout_rows, int16_t *y_sk)
+void or_mc_chain(int nstreams, uint32_t seed0, int per, int nk, int d, int y0,
+                 const float *ctpk, const int *m_k, int layout_rows, int16_t *y_sk)
 {
-    int G = groups_per_stream;
-    size_t nscen = (size_t)nstreams * G * 64;
+    size_t nscen = (size_t)nstreams * per;
     size_t d2 = (size_t)d * d;
-#pragma omp parallel for num_threads(nthreads()) schedule(dynamic, 1)
+#pragma omp parallel for num_threads(nthreads()) schedule(dynamic, 16)
     for (int s = 0; s < nstreams; s++) {
         uint32_t st[625];
         or_mt19937_init(seed0 + (uint32_t)s, st);
         int ycur[64];
-        for (int g = 0; g < G; g++) {
-            for (int l = 0; l < 64; l++) ycur[l] = y0;
-            size_t sbase = ((size_t)s * G + g) * 64;
+        for (int j0 = 0; j0 < per; j0 += 64) {
+            const int n = per - j0 < 64 ? per - j0 : 64;
+            for (int l = 0; l < n; l++) ycur[l] = y0;
+            size_t sbase = (size_t)s * per + j0;
             for (int k = 0; k < nk; k++) {
                 const float *tab = ctpk + (size_t)m_k[k] * d2;
-                for (int l = 0; l < 64; l++) {
+                for (int l = 0; l < n; l++) {
                     float u = or_u01_from_bits(or_mt19937_next(st));
                     int yp = ycur[l], y;
                     if (layout_rows) {

@@ -21,7 +21,17 @@
  *                        jump profile and mass (tests/test_oracle.py)
  *   or_first_step*       exact formula, row sums 1
  *   or_dgemm / squaring  numpy matmul; scipy expm fixtures (tests/golden)
- *   or_sck_f32           closed form C[.,d-1]=1, monotone rows, float64 cumsum
+ *   or_sck_f32           closed form C[.,d-1]=1, monotone rows, float64 cumsum;
+ *                        a negative entry lowers the running sum (hand-worked row)
+ *   or_sck_rows_f32      bit-equal to the transpose of or_sck_f32 on
+ *                        nonnegative tables
+ *   or_sck_rows2_f32     bit-equal to or_sck_rows_f32 where every sum is exact
+ *                        (dyadic entries); within float64 cumsum rounding
+ *                        otherwise; C[.,d-1] = 1, monotone rows
+ *   or_sgemv_gather      numpy A @ Z[:, col0] with the per-16-group sentinel rule
+ *   or_lv_params_cev     closed forms: sigma(100) = sigma0, the factor at
+ *                        100 e and 100/e, the 1e-4 floor
+ *   or_sample_ref_bisect equals the linear-scan inverse CDF for every d <= 1023
  *   or_mt19937_*         C++ std::mt19937 known answers (5489 -> 3499211612,
  *                        10000th output 4123659995)
  *   or_sample_linear     definition (min j: u < C[j]) checked by brute force
@@ -87,6 +97,7 @@ void or_backward_induction(int d, const double *U, int nsteps, double df,
 /* ---- cumulative tables (A11) ---- */
 void or_sck_f32(int d, int nm, const float *tpk, float *ctpk);
 void or_sck_rows_f32(int d, int nm, const float *tpk, float *crow);
+void or_sck_rows2_f32(int d, int nm, int chunk, int plus_identity, const float *tpk, float *crow);
 
 /* ---- MT19937 (A12, A14) ---- */
 void or_mt19937_init(uint32_t seed, uint32_t *state /* 625 words */);
@@ -97,9 +108,9 @@ void or_mt_uniforms(uint32_t seed, int n, float *out);
 /* ---- chain sampler (A13) ---- */
 int or_sample_linear(const float *crow, int d, float u);
 int or_sample_colmajor(const float *ctpk_m, int d, int yprev, float u);
-void or_mc_chain(int nstreams, uint32_t seed0, int groups_per_stream,
-                 int nk, int d, int y0, const float *ctpk, const int *m_k,
-                 int layout_rows, int16_t *y_sk);
+int or_sample_ref_bisect(const float *ctpk_m, int d, int yprev, float u);
+void or_mc_chain(int nstreams, uint32_t seed0, int per, int nk, int d, int y0,
+                 const float *ctpk, const int *m_k, int layout_rows, int16_t *y_sk);
 
 /* ---- payoff / pricing (A15) ---- */
 double or_bs_call(double S, double K, double r, double sigma, double T);

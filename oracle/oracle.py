@@ -58,6 +58,7 @@ def _load():
         "or_backward_induction": (None, [I, _dp, I, D, I, _dp]),
         "or_sck_f32": (None, [I, I, _fp, _fp]),
         "or_sck_rows_f32": (None, [I, I, _fp, _fp]),
+        "or_sck_rows2_f32": (None, [I, I, I, I, _fp, _fp]),
         "or_mt19937_init": (None, [U, _up]),
         "or_mt19937_next": (U, [_up]),
         "or_tgfsr_run": (None, [I, I, I, U, U, I, I, _up, _up, _up, _up, I, _up]),
@@ -67,6 +68,8 @@ def _load():
         "or_u01_from_bits": (F, [U]),
         "or_mt_uniforms": (None, [U, I, _fp]),
         "or_sample_linear": (I, [_fp, I, F]),
+        "or_sample_colmajor": (I, [_fp, I, I, F]),
+        "or_sample_ref_bisect": (I, [_fp, I, I, F]),
         "or_mc_chain": (None, [I, U, I, I, I, I, _fp, _ip, I, _sp]),
         "or_bs_call": (D, [D, D, D, D, D]),
         "or_ncdf": (D, [D]),
@@ -214,6 +217,16 @@ def sck_f32(tpk):
     return out
 
 
+def sck_rows2_f32(tpk, chunk=64, plus_identity=0):
+    """the row-contiguous table in the two-level order; tpk: (nm, d, d)
+    float32 with [m][y0, y] semantics; returns (nm, d, d) [m][row][y]"""
+    nm, d, _ = tpk.shape
+    src = np.ascontiguousarray(np.stack([t.reshape(-1, order="F") for t in tpk]).reshape(-1))
+    out = np.zeros_like(src)
+    lib.or_sck_rows2_f32(d, nm, chunk, plus_identity, src, out)
+    return out.reshape(nm, d, d)
+
+
 def mt_uniforms(seed, n):
     out = np.zeros(n, dtype=np.float32)
     lib.or_mt_uniforms(seed, n, out)
@@ -262,10 +275,13 @@ def mc_hash(tpk, nblocks, nscen, y0, m_k, seed0):
     return y.reshape(nblocks, len(mk), nscen), ct, hs
 
 
-def mc_chain(nstreams, seed0, groups, nk, d, y0, ctpk_flat, m_k, layout_rows=0):
-    nscen = nstreams * groups * 64
+def mc_chain(nstreams, seed0, groups, nk, d, y0, ctpk_flat, m_k, layout_rows=0, per=None):
+    """states [nk][nscen] of the build's stream mapping: stream s serves
+    `per` paths (default groups * 64), scenario s*per + j"""
+    per = groups * 64 if per is None else per
+    nscen = nstreams * per
     out = np.zeros(nscen * nk, dtype=np.int16)
-    lib.or_mc_chain(nstreams, seed0, groups, nk, d, y0, np.ascontiguousarray(ctpk_flat,
+    lib.or_mc_chain(nstreams, seed0, per, nk, d, y0, np.ascontiguousarray(ctpk_flat,
                     dtype=np.float32), np.ascontiguousarray(m_k, dtype=np.int32),
                     layout_rows, out)
     return out.reshape(nk, nscen)

@@ -352,6 +352,83 @@ def test_sgemv3_gather_and_sentinel(H):
     assert np.array_equal(got[:, 15], Z[:, 15])  # the column after the sentinel
 
 
+def gsgemv4_tables(H, rng, d, nz, ncols):
+    """device buffers in the layout gsgemv4/Program.cs:170-300 builds: Z (2 nz
+    vectors of d) then the ni matrices in one float buffer; in one int
+    buffer the nblocks descriptor handles, the 10-uint descriptors
+    {i, bx, by, d, nz, ncol, A, Z, col0, col1} and per matrix its col0 and
+    col1 lists padded to a multiple of 16 with the sentinel 2 nz d; every
+    vector index used once (input or output)"""
+    ni = len(ncols)
+    vals = (0.01 * rng.integers(-1000, 1000, ni * d * d + 2 * d * nz + 1)).astype(np.float32)
+    fbuf = dev(H, vals)
+    Zh = fbuf.h
+    Ah = [fbuf.h + 4 * (2 * nz * d + i * d * d) for i in range(ni)]
+    perm = rng.permutation(2 * nz)
+    used = 0
+    cols = []
+    for nc in ncols:
+        v0, v1 = perm[used:used + nc], perm[used + nc:used + 2 * nc]
+        used += 2 * nc
+        pad = 16 * ((nc + 15) // 16)
+        c0 = np.full(pad, 2 * nz * d, np.uint32)
+        c1 = np.full(pad, 2 * nz * d, np.uint32)
+        c0[:nc], c1[:nc] = v0 * d, v1 * d
+        cols.append((c0, c1))
+    blocks = [(i, bx, by) for i in range(ni) for bx in range((d + 63) // 64)
+              for by in range((ncols[i] + 15) // 16)]
+    nb = len(blocks)
+    isz = 11 * nb + sum(2 * len(c[0]) for c in cols)
+    ibuf = H.DeviceArray.empty(isz, dtype=np.uint32)
+    host = np.zeros(isz, np.uint32)
+    colh, off = [], 11 * nb
+    for c0, c1 in cols:
+        host[off:off + len(c0)] = c0
+        host[off + len(c0):off + 2 * len(c0)] = c1
+        colh.append((ibuf.h + 4 * off, ibuf.h + 4 * (off + len(c0))))
+        off += 2 * len(c0)
+    for b, (i, bx, by) in enumerate(blocks):
+        o = nb + 10 * b
+        host[b] = ibuf.h + 4 * o
+        host[o:o + 10] = [i, bx, by, d, nz, ncols[i], Ah[i], Zh, colh[i][0], colh[i][1]]
+    ibuf.upload(host)
+    return fbuf, vals, ibuf, host, nb, cols
+
+
+@pytest.mark.parametrize("ncol_field", [True, False])
+def test_sgemv4_descriptors(H, ncol_field):
+    """opcuda_sgemv4 on gsgemv4-style descriptor tables (several matrices,
+    ragged column counts, sentinel padding; d = 200 leaves a partial 64-row
+    tile) against the pinned gather oracle at the reference's threshold
+    |a - b|/(1 + |a|) <= 1e-3 (gsgemv4/Program.cs:470-482) and at 1e-5 of
+    the result's scale; vectors no column writes stay untouched.  The
+    descriptor's ncol field is read by the reference's host code only:
+    0 there stops at the sentinels alone"""
+    banner(f"sgemv4 descriptors ncol_field={ncol_field}")
+    rng = np.random.default_rng(31)
+    d, nz, ncols = 200, 48, [5, 16, 23]
+    fbuf, vals, ibuf, host, nb, cols = gsgemv4_tables(H, rng, d, nz, ncols)
+    if not ncol_field:
+        for b in range(nb):
+            host[nb + 10 * b + 5] = 0
+        ibuf.upload(host)
+    H.lib.opcuda_sgemv4(nb, ibuf.h)
+    H.check()
+    H.sync()
+    got = fbuf.download()[:2 * nz * d]
+    Z = vals[:2 * nz * d].copy()
+    for i, (c0, c1) in enumerate(cols):
+        A = np.ascontiguousarray(vals[2 * nz * d + i * d * d: 2 * nz * d + (i + 1) * d * d])
+        O.lib.or_sgemv_gather(d, A, Z, len(c0), c0.astype(np.int32), c1.astype(np.int32), 2 * nz * d)
+    assert ref_tol(Z, got) <= 1e-3
+    assert np.abs(got - Z).max() <= 1e-5 * np.abs(Z).max()
+    written = np.concatenate([c1[c1 < 2 * nz * d] // d for _, c1 in cols])
+    untouched = np.setdiff1d(np.arange(2 * nz), written)
+    g2, v2 = got.reshape(2 * nz, d), vals[:2 * nz * d].reshape(2 * nz, d)
+    assert np.array_equal(g2[untouched], v2[untouched])
+    assert not np.array_equal(g2[written], v2[written])
+
+
 def test_skinny_backward_step(H):
     banner("skinny")
     rng = np.random.default_rng(9)
@@ -380,12 +457,13 @@ def stochastic(rng, nm, d, power=6):
     return P.astype(np.float32)
 
 
-def test_sck_bit_exact_both_layouts(H):
-    banner("sck")
+@pytest.mark.parametrize("d", [200, 2048])
+def test_sck_bit_exact_both_layouts(H, d):
+    banner(f"sck d={d}")
     rng = np.random.default_rng(11)
-    nm, d = 2, 200
+    nm = 2 if d < 1024 else 1
     P = stochastic(rng, nm, d)
-    P[0, 3, 50] = -1e-9  # a rounding-level negative counts as zero
+    P[0, 3, 50] = -1e-9  # the reference layout sums it raw; the row layout counts it as zero
     flat = np.ascontiguousarray(np.stack([p.reshape(-1, order="F") for p in P]).reshape(-1))
     t = dev(H, flat)
     c = H.DeviceArray.empty(nm * d * d)
@@ -396,9 +474,14 @@ def test_sck_bit_exact_both_layouts(H):
     assert np.array_equal(c.download(), want)
     cr = H.DeviceArray.empty(nm * d * d)
     H.check(H.lib.ophip_sck_rows(d, nm, t.h, cr.h))
-    want_r = np.zeros_like(flat)
-    O.lib.or_sck_rows_f32(d, nm, flat, want_r)
-    assert np.array_equal(cr.download(), want_r)
+    assert np.array_equal(cr.download().reshape(nm, d, d), O.sck_rows2_f32(P, 64, 0))
+    # the W form's table (1 added to each diagonal entry first)
+    Wt = P.copy()
+    Wt[:, np.arange(d), np.arange(d)] -= 1.0
+    wflat = np.ascontiguousarray(np.stack([p.reshape(-1, order="F") for p in Wt]).reshape(-1))
+    tw = dev(H, wflat)
+    H.check(H.lib.ophip_sck_rows_ex(d, nm, tw.h, cr.h, 1))
+    assert np.array_equal(cr.download().reshape(nm, d, d), O.sck_rows2_f32(Wt, 64, 1))
 
 
 def test_mt19937_streams_bit_exact(H):
@@ -413,14 +496,14 @@ def test_mt19937_streams_bit_exact(H):
         seeds = (5489 + np.arange(nrng)).astype(np.uint32)
         H.lib.opcuda_mc_setseed(seeds.ctypes.data, mt.h)
         H.check()
-        per = 64 * 30  # crosses several twists (624 words each)
+        per = 64 * 30 + 17  # crosses several twists (624 words each); a partial 64-block
         u = H.DeviceArray.empty(nrng * per)
-        for rep in range(2):  # the state carries over between launches
+        for rep in range(2):  # the state carries over between launches, exactly per outputs
             H.check(H.lib.opcuda_mt_benchmark(mt.h, u.h, nrng * per))
-            got = u.download().reshape(nrng, per)
+            got = u.download().reshape(per, nrng)  # unif[s + nrng*j], smc.cu:712
             for s in (0, 1, 7, 15):
                 want = O.mt_uniforms(5489 + s, per * (rep + 1))[per * rep:]
-                assert np.array_equal(got[s], want), (s, rep)
+                assert np.array_equal(got[:, s], want), (s, rep)
     finally:
         H.lib.ophip_mc_set_nrng(4096)
 
@@ -445,7 +528,9 @@ def test_mc_chain_bit_exact(H, d, nrng, groups, nk):
     H.check(H.lib.ophip_mc_set_nrng(nrng))
     try:
         y0 = d // 3
-        want = O.mc_chain(nrng, 777, groups, nk, d, y0, ctab, m_k, 0)
+        crows = cr.download()
+        want_c = O.mc_chain(nrng, 777, groups, nk, d, y0, ctab, m_k, 0)
+        want = O.mc_chain(nrng, 777, groups, nk, d, y0, crows, m_k, 1)
         for rows in (False, True):
             mt = H.DeviceArray.empty(625 * nrng, dtype=np.uint32)
             seeds = (777 + np.arange(nrng)).astype(np.uint32)
@@ -456,7 +541,7 @@ def test_mc_chain_bit_exact(H, d, nrng, groups, nk):
             fn = H.lib.ophip_mc1f_rows if rows else H.lib.opcuda_mc1f
             H.check(fn(mt.h, y0, ysk.h, nscen, nk, d, (cr if rows else ct).h, mkd.h,
                        host.ctypes.data))
-            assert np.array_equal(host.reshape(nk, nscen), want), rows
+            assert np.array_equal(host.reshape(nk, nscen), want if rows else want_c), rows
         # guide tables: the same states from the indexed search
         gd = H.DeviceArray.empty(nm * d * 257, dtype=np.int16)
         H.check(H.lib.ophip_mc_guide(d, nm, cr.h, gd.h))
@@ -473,6 +558,148 @@ def test_mc_chain_bit_exact(H, d, nrng, groups, nk):
         ysk = H.DeviceArray.empty(nscen * nk, dtype=np.int16)
         H.check(H.lib.ophip_mc1f_guided(mt.h, y0, ysk.h, nscen, nk, d, cr.h, gd.h, mkd.h))
         assert np.array_equal(ysk.download().reshape(nk, nscen), want)
+    finally:
+        H.lib.ophip_mc_set_nrng(4096)
+
+
+@pytest.mark.parametrize("nscen", [4096, 4096 * 25])
+def test_mc1f_reference_batch_sizes(H, nscen):
+    """the reference drivers' batch sizes at the default 4096 streams
+    (glv/Program.cs:73: 4096; gsv and clv/Program.cs:48: 4096*25): per =
+    nscen/4096 paths per stream, bit-exact against the oracle's mapping on
+    the reference table layout (opcuda_mc1f) and the row layout (guided);
+    each call consumes exactly per*nk outputs of every stream"""
+    banner(f"mc1f reference batch nscen={nscen}")
+    assert H.lib.opcuda_mc_nrng() == 4096
+    nrng, per = 4096, nscen // 4096
+    rng = np.random.default_rng(nscen)
+    d, nk = 128, 40  # clv's lattice and weekly steps (clv/Program.cs:46-60)
+    P = stochastic(rng, 1, d, power=8)
+    flat = np.ascontiguousarray(P[0].reshape(-1, order="F"))
+    t = dev(H, flat)
+    ct = H.DeviceArray.empty(d * d)
+    H.lib.opcuda_sck(d, 1, t.h, ct.h)
+    H.check()
+    cr = H.DeviceArray.empty(d * d)
+    H.check(H.lib.ophip_sck_rows(d, 1, t.h, cr.h))
+    gd = H.DeviceArray.empty(d * 257, dtype=np.int16)
+    H.check(H.lib.ophip_mc_guide(d, 1, cr.h, gd.h))
+    mkd = dev(H, np.zeros(nk, dtype=np.int32))
+    y0 = 40
+    seeds = (5489 + np.arange(nrng)).astype(np.uint32)
+    want_c = O.mc_chain(nrng, 5489, 0, nk, d, y0, ct.download(), np.zeros(nk, np.int32), 0, per=per)
+    want_r = O.mc_chain(nrng, 5489, 0, nk, d, y0, cr.download(), np.zeros(nk, np.int32), 1, per=per)
+    ysk = H.DeviceArray.empty(nscen * nk, dtype=np.int16)
+    mt = H.DeviceArray.empty(H.lib.opcuda_mc_status_sz(), dtype=np.uint32)
+    H.lib.opcuda_mc_setseed(seeds.ctypes.data, mt.h)
+    H.check()
+    host = np.zeros(nscen * nk, dtype=np.int16)
+    H.check(H.lib.opcuda_mc1f(mt.h, y0, ysk.h, nscen, nk, d, ct.h, mkd.h, host.ctypes.data))
+    assert np.array_equal(host.reshape(nk, nscen), want_c)
+    # the stream continues exactly per*nk outputs later
+    u = H.DeviceArray.empty(4096 * 3)
+    H.check(H.lib.opcuda_mt_benchmark(mt.h, u.h, 4096 * 3))
+    got = u.download().reshape(3, 4096)
+    for s in (0, 1, 2049, 4095):
+        assert np.array_equal(got[:, s], O.mt_uniforms(5489 + s, per * nk + 3)[per * nk:]), s
+    H.lib.opcuda_mc_setseed(seeds.ctypes.data, mt.h)
+    H.check()
+    H.check(H.lib.ophip_mc1f_guided(mt.h, y0, ysk.h, nscen, nk, d, cr.h, gd.h, mkd.h))
+    assert np.array_equal(ysk.download().reshape(nk, nscen), want_r)
+
+
+def test_mc_c1_lv_kernel_bit_exact(H):
+    """the 1-D workload's Monte Carlo leg at its size: 2^16 paths x 16 steps
+    on its own kernel U(1/16) (plain FP32 solve, reference table layout),
+    one MT19937 stream seeded 5489 serving every path (scenario j = path
+    j, groups of 64, step-major), bit-exact against the oracle; the paths'
+    call prices at T = 1 yr within 5 standard errors of the FP64 oracle's"""
+    banner("mc C1: LV d=256, 2^16 x 16, seed 5489")
+    import ctypes as C
+    import os
+    from oplib_amd import opmodel as M
+    cfg = M.config_lv1d()
+    s = M.Solver(cfg)
+    s.solve()
+    U = s.kernel()  # U(1/16), column-major
+    s.close()
+    d, nk, nscen = cfg.d, 16, 1 << 16
+    x, x0 = M.grid(cfg)
+    Ud = dev(H, U)
+    ct = H.DeviceArray.empty(d * d)
+    H.lib.opcuda_sck(d, 1, Ud.h, ct.h)
+    H.check()
+    mk = np.zeros(nk, dtype=np.int32)
+    want = O.mc_chain(1, 5489, 0, nk, d, x0, ct.download(), mk, 0, per=nscen)
+    H.check(H.lib.ophip_mc_set_nrng(1))
+    try:
+        mt = H.DeviceArray.empty(625, dtype=np.uint32)
+        seed = np.array([5489], dtype=np.uint32)
+        H.lib.opcuda_mc_setseed(seed.ctypes.data, mt.h)
+        H.check()
+        ysk = H.DeviceArray.empty(nscen * nk, dtype=np.int16)
+        mkd = dev(H, mk)
+        H.check(H.lib.opcuda_mc1f(mt.h, x0, ysk.h, nscen, nk, d, ct.h, mkd.h, 0))
+        got = ysk.download().reshape(nk, nscen)
+        assert np.array_equal(got, want)
+    finally:
+        H.lib.ophip_mc_set_nrng(4096)
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "prices_lv1d.npz"))["prices"]
+    p64 = gold.reshape(-1, len(cfg.strike_list()))[-1]  # T = 16/16 yr
+    xs = x.astype(np.float32)[got[-1]]
+    ks = np.array(cfg.strike_list(), dtype=np.float64)
+    pay = np.maximum(xs[:, None].astype(np.float64) - ks[None, :], 0.0)
+    df = M.df(cfg, 1.0)
+    mc, se = df * pay.mean(0), df * pay.std(0) / math.sqrt(nscen)
+    z = np.abs(mc - p64) / se
+    print(f"[mc C1] max |MC - FP64 operator| = {z.max():.2f} SE", flush=True)
+    assert z.max() < 5.0
+
+
+def test_mc_c2_full_size_bit_exact(H):
+    """the regime workload's Monte Carlo leg at its size: 2^20 paths x 252
+    daily steps on the 2048-state daily kernel U(1/252) (FP16 x3 solve,
+    row-contiguous table), 16384 MT19937 streams, one per wavefront, seeded
+    5489 + i, 64 paths each: every sampled state bit-exact against the
+    oracle; the guided sampler and the plain bisection agree"""
+    banner("mc C2: d=2048, 2^20 x 252, 16384 streams seeded 5489+i")
+    from oplib_amd import opmodel as M
+    cfg = M.config_rs2048()
+    cfg.precision = M.OPM_PREC_F16X3
+    s = M.Solver(cfg)
+    s.solve()
+    U = s.kernel()
+    s.close()
+    d, nk, nscen, nrng = cfg.d, 252, 1 << 20, 16384
+    x, x0 = M.grid(cfg)
+    y0 = x0 + cfg.nx * 1  # regime 1's spot
+    Ud = dev(H, U)
+    cr = H.DeviceArray.empty(d * d)
+    H.check(H.lib.ophip_sck_rows(d, 1, Ud.h, cr.h))
+    del Ud
+    gd = H.DeviceArray.empty(d * 257, dtype=np.int16)
+    H.check(H.lib.ophip_mc_guide(d, 1, cr.h, gd.h))
+    mk = np.zeros(nk, dtype=np.int32)
+    mkd = dev(H, mk)
+    seeds = (5489 + np.arange(nrng)).astype(np.uint32)
+    H.check(H.lib.ophip_mc_set_nrng(nrng))
+    try:
+        mt = H.DeviceArray.empty(625 * nrng, dtype=np.uint32)
+        H.lib.opcuda_mc_setseed(seeds.ctypes.data, mt.h)
+        H.check()
+        ysk = H.DeviceArray.empty(nscen * nk, dtype=np.int16)
+        H.check(H.lib.ophip_mc1f_guided(mt.h, y0, ysk.h, nscen, nk, d, cr.h, gd.h, mkd.h))
+        got = ysk.download().reshape(nk, nscen)
+        print("[mc C2] GPU done, oracle running", flush=True)
+        want = O.mc_chain(nrng, 5489, 0, nk, d, y0, cr.download(), mk, 1, per=64)
+        bad = np.count_nonzero(got != want)
+        print(f"[mc C2] {got.size} states, {bad} differ", flush=True)
+        assert bad == 0
+        del want
+        H.lib.opcuda_mc_setseed(seeds.ctypes.data, mt.h)
+        H.check()
+        H.check(H.lib.ophip_mc1f_rows(mt.h, y0, ysk.h, nscen, nk, d, cr.h, mkd.h, 0))
+        assert np.array_equal(ysk.download().reshape(nk, nscen), got)
     finally:
         H.lib.ophip_mc_set_nrng(4096)
 
@@ -580,7 +807,7 @@ def test_mc1f_rejects_bad_shapes(H):
     banner("mc1f args")
     H.check(H.lib.ophip_mc_set_nrng(4))
     try:
-        st = H.lib.opcuda_mc1f(256, 0, 256, 100, 4, 8, 256, 256, 0)  # nscen % (64*nrng) != 0
+        st = H.lib.opcuda_mc1f(256, 0, 256, 102, 4, 8, 256, 256, 0)  # nscen % nrng != 0
         assert st != 0
     finally:
         H.lib.ophip_mc_set_nrng(4096)
@@ -631,6 +858,78 @@ def test_sv_generator_vs_oracle(H, out_kind):
     else:
         # one FP32 rounding of the FP64 value (plus the FP64 rounding slack)
         assert np.all(np.abs(got.astype(np.float64) - want) <= np.abs(want) * 2.0 ** -24 + 1e-13 * scale)
+
+
+def sv_invm(nx, nr, xv, rv):
+    """S2DGrid.pregen (S2DGrid.cs:123-245): per state the 4 x 4 system of
+    (x drift, x variance, r drift, r variance) over the neighbours (x-1,
+    x+1, r-1, r+1), identity-like rows at the edges, solved; stored
+    invm[16 y + i + 4 j] = inverse[i][j]"""
+    d = nx * nr
+    inv = np.zeros(16 * d)
+    for a0 in range(nr):
+        for x0 in range(nx):
+            y = x0 + nx * a0
+            m = np.zeros((4, 4))
+            m[0, 0] = xv[y - 1] - xv[y] if x0 > 0 else 0.0
+            m[0, 1] = xv[y + 1] - xv[y] if x0 < nx - 1 else 0.0
+            if 0 < x0 < nx - 1:
+                m[1, 0], m[1, 1] = m[0, 0] ** 2, m[0, 1] ** 2
+            elif x0 == 0:
+                m[1, 0] = 1.0
+            else:
+                m[1, 1] = 1.0
+            m[2, 2] = rv[y - nx] - rv[y] if a0 > 0 else 0.0
+            m[2, 3] = rv[y + nx] - rv[y] if a0 < nr - 1 else 0.0
+            if 0 < a0 < nr - 1:
+                m[3, 2], m[3, 3] = m[2, 2] ** 2, m[2, 3] ** 2
+            elif a0 == 0:
+                m[3, 2] = 1.0
+            else:
+                m[3, 3] = 1.0
+            inv[16 * y:16 * y + 16] = np.linalg.inv(m).reshape(-1, order="F")
+    return inv
+
+
+def test_sgsvg_dropin_vs_oracle(H):
+    """opcuda_sgsvg with the reference's arguments (CSVModel.device_sgen,
+    CSVModel.cs:386-417: the stencil inverses of S2DGrid.pregen and the
+    pars block xval | SDrift | SVol | VolDrift | VolVol | Jm | Jp over ni
+    generators) against the oracle's FP64 generator of the same (FP32)
+    inputs: every row within 4 (nx - 1) 2^-24 of its largest rate (the
+    FP32 invm carries 2^-24 per entry into each rate, and the drift fix
+    sums rates times distances of up to nx - 1 grid steps and divides by
+    one step), one FP32 rounding of the result; the whole block is
+    written"""
+    banner("sgsvg drop-in")
+    from _cases import sv_case
+    nx, nr, xv, rv, sd, sv, vv, vd, jm, jp = sv_case(nx=48, nr=6, seed=5)
+    d, ni = nx * nr, 2
+    gens = [(sd, sv, vd, vv, jm, jp), (0.5 * sd, 1.2 * sv, 0.7 * vd, 1.3 * vv, 0.5 * jm, 2.0 * jp)]
+    f32 = lambda a: np.asarray(a, dtype=np.float32)  # noqa: E731
+    pars = np.zeros(d + 6 * ni * d, np.float32)
+    pars[:d] = xv
+    for i, g in enumerate(gens):
+        for k, a in enumerate(g):
+            pars[d + k * ni * d + i * d: d + k * ni * d + (i + 1) * d] = a
+    inv = sv_invm(nx, nr, xv, rv).astype(np.float32)
+    gd = H.DeviceArray.from_numpy(np.full(ni * d * d, np.nan, np.float32))
+    invd, pd = dev(H, inv), dev(H, pars)
+    H.lib.opcuda_sgsvg(gd.h, ni, nx, nr, invd.h, pd.h)
+    H.check()
+    got = gd.download().reshape(ni, d, d).transpose(0, 2, 1)  # [i][y0][y1]
+    assert np.all(np.isfinite(got))
+    for i, (s_d, s_v, v_d, v_v, j_m, j_p) in enumerate(gens):
+        want = O.sv_generator(nx, nr, f32(xv).astype(np.float64), rv, f32(s_d), f32(s_v), f32(v_v),
+                              f32(v_d), f32(j_m), f32(j_p))
+        scale = np.abs(want).max(axis=1, keepdims=True)
+        err = (np.abs(got[i].astype(np.float64) - want) / scale).max()
+        print(f"[sgsvg] generator {i}: max row-relative error {err:.2e}", flush=True)
+        assert err <= 4 * (nx - 1) * 2.0**-24, (i, err)
+        assert np.all(got[i][want == 0] == 0)  # structural zeros stay zero
+    # the reference's entry point refuses what its kernel cannot build
+    H.lib.opcuda_sgsvg(gd.h, ni, nx, 1, invd.h, pd.h)
+    assert H.lib.opcuda_get_status() != 0
 
 
 def test_dcmt_lanes_bit_exact(H):

@@ -267,6 +267,24 @@ def test_group_rs2048_vs_fp64(M):
     g.close()
 
 
+@pytest.mark.parametrize("n", [16, 32])
+def test_group_rs2048_headline_sizes_vs_fp64(M, n):
+    """the headline method at its group sizes: 16 and 32 scenarios of the
+    2048-state regime lattice in one group launch, every one of the
+    n x 8 x 41 prices within 1e-4 of the FP64 oracle (fixture of 32 FP64
+    solves), replayed from the captured graph"""
+    banner(f"group rs2048 x{n} FP16x3 (headline)")
+    gold = np.load(os.path.join(GOLD, "prices_rs2048_scenarios.npz"))["prices"]
+    assert len(gold) >= n
+    g = M.Group(M.config_rs2048_scenarios(n))
+    g.capture()
+    g.launch()
+    pg = g.prices()
+    e = report(f"group rs2048 x{n}", pg, gold[:n])
+    assert np.all(np.isfinite(pg)) and e.max() <= 1e-4
+    g.close()
+
+
 def test_mcplan_exec_vs_fp64(M):
     """the reference's step plan executed on the device (first steps,
     squaring rounds and chain products through opcuda_ssqmm pointer tables)

@@ -697,6 +697,43 @@ def test_later_solve_same_bits(hip, mode):
         H.lib.ophip_wsq_plan_free(pb)
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_dense_solve_after_layout_same_bits(hip, mode):
+    """the dense K loop (ophip_wsq_set_skip(0)) reads every cell's copies:
+    after an odd-count solve and a backward layout (fragment copies written
+    into buffer 0's column-major slots, where many cells are flagged zero),
+    a dense solve gives a fresh state's bits in every array of both
+    buffers"""
+    banner(f"wsq dense solve after a layout, mode={mode}")
+    H = hip
+    A, B = Lattice(H, 512, 1, mode), Lattice(H, 512, 1, mode)  # one band: zero cells
+    d = A.d
+    pa, pb = plan(H, [A], 128), plan(H, [B], 128)
+    old = H.lib.ophip_wsq_set_skip(1)
+    try:
+        A.first_step()
+        H.check(H.lib.ophip_wsq_exec(pa, 3))
+        H.check(H.lib.ophip_wsq_bwd_layout(A.state.h, d, mode, 1))
+        H.sync()
+        H.lib.ophip_wsq_set_skip(0)
+        for nsq in (1, 2):
+            A.first_step()
+            H.check(H.lib.ophip_wsq_exec(pa, nsq))
+            B.first_step()
+            H.check(H.lib.ophip_wsq_exec(pb, nsq))
+            H.sync()
+            for buf in (0, 1):
+                for which in (0, 1, 2, 3, 4):
+                    dt = np.uint32 if which == 0 else np.uint16
+                    assert np.array_equal(A.arr(buf, which, dt), B.arr(buf, which, dt)), (nsq, buf, which)
+            H.check(H.lib.ophip_wsq_bwd_layout(A.state.h, d, mode, nsq & 1))
+            H.check(H.lib.ophip_wsq_bwd_layout(B.state.h, d, mode, nsq & 1))
+    finally:
+        H.lib.ophip_wsq_set_skip(old)
+        H.lib.ophip_wsq_plan_free(pa)
+        H.lib.ophip_wsq_plan_free(pb)
+
+
 def test_forward_steps_skip_zero_cells(hip):
     """the forward steps p_k = p_{k-1} (I + W) visiting only W's flagged
     cells (ophip_fwd_set_cells) give the bits of the dense steps, on a

@@ -184,6 +184,143 @@ def test_sck_properties_and_float64():
         assert np.abs(C - ref).max() < 1e-5
 
 
+def test_sck_raw_sum_lowers_on_negative_entry():
+    # sck.cu:64-71 adds the raw entries: a negative one lowers the running
+    # sum (hand-worked row 0.5, -0.25, 0.75 -> sums 0.5, 0.25, 1.0)
+    d = 3
+    t = np.zeros((1, d, d), np.float32)
+    t[0, 0] = [0.5, -0.25, 0.75]
+    t[0, 1] = [0.0, 1.0, 0.0]
+    t[0, 2] = [0.25, 0.25, 0.5]
+    c = O.sck_f32(t).reshape(d, d).T  # [row][y]
+    assert np.array_equal(c[0], np.float32([0.5, 0.25, 1.0]))
+    assert np.array_equal(c[1], np.float32([0.0, 1.0, 1.0]))
+    assert np.array_equal(c[2], np.float32([0.25, 0.5, 1.0]))
+
+
+def test_sck_rows_is_the_transposed_table():
+    rng = np.random.default_rng(21)
+    nm, d = 2, 77
+    t = (rng.random((nm, d, d)) ** 3).astype(np.float32)
+    t[1, 4, :] = 0
+    t[1, 4, 9] = 2.0
+    col = O.sck_f32(t).reshape(nm, d, d)  # [m][y][row]
+    rows = np.zeros(nm * d * d, np.float32)
+    src = np.ascontiguousarray(np.stack([a.reshape(-1, order="F") for a in t]).reshape(-1))
+    O.lib.or_sck_rows_f32(d, nm, src, rows)
+    assert np.array_equal(rows.reshape(nm, d, d), np.transpose(col, (0, 2, 1)))
+
+
+@pytest.mark.parametrize("d,plus", [(200, 0), (130, 1), (64, 0), (33, 1)])
+def test_sck_rows_two_level_exact_on_dyadic(d, plus):
+    # entries k * 2^-12 with k < 64: every partial sum of a row is exact in
+    # FP32, so any summation order gives the same bits as the sequential one
+    rng = np.random.default_rng(d)
+    t = (rng.integers(0, 64, (2, d, d)) * 2.0**-12).astype(np.float32)
+    if plus:
+        t[:, np.arange(d), np.arange(d)] -= 1.0  # W = U - I: plus_identity restores U
+    t[0, 3, :] = 0
+    t[0, 3, 5] = 2.0**-12 if not plus else 2.0**-12
+    two = O.sck_rows2_f32(t, 64, plus)
+    base = t.astype(np.float64) + (plus * np.eye(d))[None]
+    want = np.zeros(2 * d * d, np.float32)
+    src = np.ascontiguousarray(np.stack([a.reshape(-1, order="F") for a in base.astype(np.float32)])
+                               .reshape(-1))
+    O.lib.or_sck_rows_f32(d, 2, src, want)
+    assert np.array_equal(two, want.reshape(2, d, d))
+
+
+def test_sck_rows_two_level_properties():
+    rng = np.random.default_rng(23)
+    nm, d = 2, 300
+    t = (rng.random((nm, d, d)) ** 6).astype(np.float32)
+    t[0, 7, 100] = -1e-9  # a rounding-level negative counts as zero
+    c = O.sck_rows2_f32(t, 64)
+    assert np.all(c[:, :, -1] == 1.0)
+    assert np.all(np.diff(c, axis=2) >= 0)
+    t64 = np.maximum(t.astype(np.float64), 0.0)
+    ref = np.cumsum(t64, axis=2) / t64.sum(2)[:, :, None]
+    assert np.abs(c - ref).max() < 4 * d * 2.0**-24
+    # chunk = d is the sequential order
+    seq = O.sck_rows2_f32(t, d)
+    rows = np.zeros(nm * d * d, np.float32)
+    src = np.ascontiguousarray(np.stack([a.reshape(-1, order="F") for a in t]).reshape(-1))
+    O.lib.or_sck_rows_f32(d, nm, src, rows)
+    assert np.array_equal(seq, rows.reshape(nm, d, d))
+
+
+def test_sgemv_gather_vs_numpy():
+    # the gsgemv4 layout (gsgemv4/Program.cs:236-300): Z holds 2*nz vectors
+    # of d, column lists are element offsets, padded with the sentinel
+    # 2*nz*d; inside a 16-group the stores stop at the first sentinel
+    rng = np.random.default_rng(24)
+    d, nz = 70, 12
+    A = rng.standard_normal((d, d)).astype(np.float32)  # not symmetric
+    Z = rng.standard_normal((d, 2 * nz)).astype(np.float32)
+    sent = 2 * nz * d
+    col0 = np.full(32, sent, np.int32)
+    col1 = np.full(32, sent, np.int32)
+    pairs = {0: (1, 13), 1: (4, 14), 2: (2, 15), 3: (9, 16),         # group 0: 4 live,
+             5: (3, 17),                                             # then a sentinel at 4
+             16: (5, 18), 17: (0, 19), 18: (11, 20), 19: (6, 21)}    # group 1: 4 live
+    for c, (a, b) in pairs.items():
+        col0[c], col1[c] = a * d, b * d
+    want = Z.astype(np.float64).copy()
+    for c in (0, 1, 2, 3, 16, 17, 18, 19):
+        a, b = pairs[c]
+        want[:, b] = A.astype(np.float64) @ Z[:, a].astype(np.float64)
+    got = np.ascontiguousarray(Z.reshape(-1, order="F"))
+    O.lib.or_sgemv_gather(d, np.ascontiguousarray(A.reshape(-1, order="F")), got, 32, col0, col1, sent)
+    got = got.reshape(d, 2 * nz, order="F")
+    assert np.abs(got - want).max() < 1e-5 * np.abs(want).max()
+    assert np.array_equal(got[:, 17], Z[:, 17])  # after the group's first sentinel
+    # a transposed operand fails
+    bad = A.T.astype(np.float64) @ Z[:, 1].astype(np.float64)
+    assert np.abs(got[:, 13] - bad).max() > 1e-2
+
+
+def test_lv_params_cev_closed_forms():
+    e = math.e
+    x = np.array([0.0, 100.0 / e, 100.0, 100.0 * e, 400.0])
+    s0, beta, r, a, b = 0.3, 0.6, 0.03, 0.2, 0.1
+    dr, vo = O.lv_params_cev(x, r, s0, beta, a, b)
+    assert np.allclose(dr, r * x, rtol=1e-15, atol=0)
+    sig = np.divide(vo, x, out=np.zeros_like(vo), where=x > 0)
+    assert vo[0] == 0.0
+    assert sig[2] == pytest.approx(s0, rel=1e-15)                                  # sigma(100) = sigma0
+    assert sig[3] == pytest.approx(s0 * math.exp(beta - 1) * (1 + a + b), rel=1e-13)  # l = 1
+    assert sig[1] == pytest.approx(s0 * math.exp(1 - beta) * (1 - a + b), rel=1e-13)  # l = -1
+    l4 = math.log(4.0)
+    assert sig[4] == pytest.approx(s0 * 4.0 ** (beta - 1) * (1 + a * l4 + b * l4 * l4), rel=1e-13)
+    # a negative factor is floored at 1e-4 (sigma, in price units 1e-4 x)
+    dr, vo = O.lv_params_cev(x, r, s0, beta, -2.0, 0.0)
+    assert vo[3] == pytest.approx(1e-4 * x[3], rel=1e-15)
+    assert vo[1] > 1e-4 * x[1]
+
+
+@pytest.mark.parametrize("d", [1, 2, 3, 64, 127, 128, 129, 255, 256, 257, 511, 512, 513, 700, 1023])
+def test_reference_gpu_bisection_is_the_inverse_cdf(d):
+    # smc.cu:224-238 (and the 512/256/128 variants): the fixed-count
+    # bisection from lb = 0 returns min{j : u < C[j]} clamped to d-1,
+    # state 0 included, the definition the build's sampler implements
+    rng = np.random.default_rng(d)
+    P = rng.random((d, d)) ** 4
+    P[:, rng.integers(0, d, d // 3 + 1)] = 0.0  # zero-mass states
+    P[0, :] = 0.0
+    P[0, 0] = 1.0  # row 0: all mass on state 0
+    P[P.sum(1) == 0, 0] = 1.0
+    c = O.sck_f32(P.astype(np.float32)[None])
+    us = list(rng.random(400).astype(np.float32)) + [np.float32(0.0), np.float32(1.0 - 2.0**-24)]
+    for yp in sorted({0, d // 2, d - 1}):
+        row = c[yp::d]
+        extra = [row[j] for j in range(0, d, max(1, d // 7))]  # u exactly on a table value
+        for u in us + extra:
+            got = O.lib.or_sample_ref_bisect(c, d, yp, float(u))
+            want = O.lib.or_sample_colmajor(c, d, yp, float(u))
+            assert got == want, (d, yp, float(u))
+    assert O.lib.or_sample_ref_bisect(c, d, 0, 0.5) == 0  # state 0 is reachable
+
+
 def test_sampler_linear_is_inverse_cdf():
     rng = np.random.default_rng(3)
     d = 17
@@ -216,20 +353,25 @@ def test_mc_chain_bisect_equals_linear():
 
 
 def test_mc_chain_stream_mapping():
-    # identity transition: a state-dependent table makes each draw visible
+    # uniform rows: y = floor(u * 64) exactly, so every draw is visible.
+    # Stream s serves `per` paths (scenario s*per + j) in groups of 64; at
+    # step k a group's n paths take the stream's next n outputs in order
     d = 64
-    P = np.zeros((d, d))
-    for i in range(d):
-        P[i] = 1.0 / d  # uniform rows: y = floor(u * 64) exactly
+    P = np.full((d, d), 1.0 / d)
     c = O.sck_f32(P.astype(np.float32)[None])
-    y = O.mc_chain(2, 100, 2, 3, d, 0, c, np.zeros(3, dtype=np.int32))
-    for s in range(2):
-        u = O.mt_uniforms(100 + s, 2 * 3 * 64)
-        for g in range(2):
-            for k in range(3):
-                for lane in range(64):
-                    want = int(np.floor(u[g * 192 + k * 64 + lane] * 64))
-                    assert y[k, (s * 2 + g) * 64 + lane] == want
+    for nstreams, per, nk in ((2, 128, 3), (3, 25, 4), (2, 100, 2)):
+        y = O.mc_chain(nstreams, 100, 0, nk, d, 0, c, np.zeros(nk, dtype=np.int32), per=per)
+        assert y.shape == (nk, nstreams * per)
+        for s in range(nstreams):
+            u = O.mt_uniforms(100 + s, per * nk)
+            pos = 0
+            for j0 in range(0, per, 64):
+                n = min(64, per - j0)
+                for k in range(nk):
+                    want = np.floor(u[pos:pos + n] * 64).astype(np.int16)
+                    assert np.array_equal(y[k, s * per + j0:s * per + j0 + n], want)
+                    pos += n
+            assert pos == per * nk
 
 
 # ---- pricing -------------------------------------------------------------
