@@ -366,6 +366,19 @@ int ophip_mc_payoff(unsigned y_sk, int nscen, int nk, int nsteps, const int *ste
                     unsigned xval, int d, const float *strikes, int nstrikes, unsigned out);
 int ophip_mc1f_guided(unsigned mt, int y0, unsigned y_sk, int nscen, int nk, int d, unsigned crow,
                       unsigned guide, unsigned m_k);
+/* ophip_mc_payoff without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops).  The partial sums live in a fixed
+ * 4 MiB array of the library on each device it runs on (held until the
+ * process ends): ceil(nscen / 4096) * nsteps <= 4096 (e.g. 2^20 paths at
+ * 16 listed steps), larger calls return OPHIP_ERR_BAD_ARG; the library
+ * must already be initialised on the device (OPHIP_ERR_NO_DEVICE
+ * otherwise).  Same arguments, kernels and bits as ophip_mc_payoff.  Calls
+ * on one device share the array: calls that overlap on one device
+ * (issued from several host threads at once) race. */
+int ophip_mc_payoff_noalloc(unsigned y_sk, int nscen, int nk, int nsteps, const int *steps,
+                            const double *df, unsigned xval, int d, const float *strikes, int nstrikes,
+                            unsigned out);
 
 /* graphs, events, workspace */
 int           ophip_reserve_workspace(unsigned long bytes);

@@ -30,6 +30,7 @@
 // time-major int16 y_sk[k*nscen + scen] (smc.cu:222).
 #include "ophip_internal.h"
 
+#include <mutex>
 #include <vector>
 
 extern "C" unsigned opcuda_mem_alloc(unsigned sz);
@@ -507,6 +508,94 @@ OPHIP_EXPORT int ophip_mc_payoff(unsigned y_skp, int nscen, int nk, int nsteps, 
     hipLaunchKernelGGL(k_mc_pay_part, dim3((unsigned)nchunk, (unsigned)nsteps), dim3(256), 0, D.stream,
                        P<const int16_t>(y_skp), (size_t)nscen, P<const float>(xvalp), d, a, nchunk, part);
     hipLaunchKernelGGL(k_mc_pay_sum, dim3((unsigned)nsteps), dim3(64), 0, D.stream, (size_t)nscen, a, nchunk,
+                       (const double *)part, P<double>(outp));
+    return (int)check_launch();
+}
+
+// ---------------------------------------------------------------------
+// ophip_mc_payoff_noalloc: the same payoff evaluation for callers that
+// cannot allocate device memory (code under HIP graph capture, serving
+// loops): the partial sums live in `scratch`, a fixed array of this
+// library (kPayScratchBytes on every device the library runs on, held
+// until the process ends) instead of the library workspace, which is
+// grown with hipMalloc on demand.  Its device address is looked up once
+// per device (hipGetSymbolAddress with that device current) and passed to
+// the same kernels.  k_mc_pay_part writes every partial slot that
+// k_mc_pay_sum reads, so nothing needs clearing between calls.
+// ---------------------------------------------------------------------
+constexpr size_t kPayScratchBytes = (size_t)4 << 20;
+static __device__ __align__(256) double scratch[kPayScratchBytes / sizeof(double)];
+
+namespace {
+
+double *g_scratch[16] = {};  // per device: the device address of `scratch`
+std::mutex g_scratch_mu;
+
+double *scratch_ptr(Device &D) {
+    if (D.id < 0 || D.id >= 16) {
+        set_status(kErrNoDevice);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[D.id]) {
+        int cur = -1;
+        hipError_t e0 = hipGetDevice(&cur);
+        if (e0 == hipSuccess && cur != D.id) e0 = hipSetDevice(D.id);
+        if (e0 != hipSuccess) {
+            set_status(hip_status(e0));
+            return nullptr;
+        }
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != D.id) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p) {
+            set_status(hip_status(e != hipSuccess ? e : hipErrorInvalidSymbol));
+            return nullptr;
+        }
+        g_scratch[D.id] = (double *)p;
+    }
+    return g_scratch[D.id];
+}
+
+}  // namespace
+
+// ophip_mc_payoff's arguments and results (the same kernels, the same
+// bits); the partial sums (2 * 64 doubles per chunk of 4096 paths and
+// listed step) must fit kPayScratchBytes: ceil(nscen / 4096) * nsteps <=
+// 4096, e.g. 2^20 paths at 16 steps; larger calls are refused with
+// OPHIP_ERR_BAD_ARG.  The library must be initialised on the device
+// (opcuda_cublas_init, as every handle the caller passes implies); the
+// call allocates nothing.  Calls on one device share the array: calls that
+// overlap on one device (issued from several host threads at once) race.
+OPHIP_EXPORT int ophip_mc_payoff_noalloc(unsigned y_skp, int nscen, int nk, int nsteps, const int *steps,
+                                         const double *df, unsigned xvalp, int d, const float *strikes,
+                                         int nstrikes, unsigned outp) {
+    Device *Dp = dev_ready();
+    if (!Dp) return (int)set_status(kErrNoDevice);
+    if (nscen <= 0 || nk <= 0 || nsteps <= 0 || nsteps > kPayMax || nstrikes <= 0 || nstrikes > kPayMax ||
+        d <= 0 || d > 32767 || !steps || !df || !strikes)
+        return (int)set_status(kErrBadArg);
+    const size_t nchunk = ((size_t)nscen + kPayChunk - 1) / kPayChunk;
+    if (sizeof(double) * 2 * kPayMax * nchunk * (size_t)nsteps > kPayScratchBytes)
+        return (int)set_status(kErrBadArg);  // beyond the fixed partial-sum array
+    PayArgs a{};
+    a.nsteps = nsteps;
+    a.nstrikes = nstrikes;
+    for (int i = 0; i < nsteps; i++) {
+        if (steps[i] < 0 || steps[i] >= nk) return (int)set_status(kErrBadArg);
+        a.steps[i] = steps[i];
+        a.df[i] = df[i];
+    }
+    for (int c = 0; c < nstrikes; c++) a.strikes[c] = strikes[c];
+    if (!handle_range_ok(y_skp, (size_t)nscen * nk * 2) || !handle_range_ok(xvalp, (size_t)d * 4) ||
+        !handle_range_ok(outp, (size_t)2 * nsteps * nstrikes * 8))
+        return (int)set_status(kErrBadHandle);
+    Device &D = *Dp;
+    double *part = scratch_ptr(D);
+    if (!part) return (int)opcuda_get_status();
+    hipLaunchKernelGGL(k_mc_pay_part, dim3((unsigned)nchunk, (unsigned)nsteps), dim3(256), 0, D.stream,
+                       P<const int16_t>(y_skp), (size_t)nscen, P<const float>(xvalp), d, a, (int)nchunk, part);
+    hipLaunchKernelGGL(k_mc_pay_sum, dim3((unsigned)nsteps), dim3(64), 0, D.stream, (size_t)nscen, a, (int)nchunk,
                        (const double *)part, P<double>(outp));
     return (int)check_launch();
 }
