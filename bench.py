@@ -65,26 +65,54 @@ def traffic_from_profiles(kernel):
     return None
 
 
-def roofline_backward(best, methods, wbk, bi_gbs, bi_us, bi_bytes):
+def roofline_backward(best, methods, wbk, bi_gbs, bi_us, bi_bytes, mall=None):
     """roofline entry of the headline method's dominant kernel: the FP16 x3
     band-sparse backward step (per launch: every lattice's visited hi
     fragments and read lo fragments, and the live value columns), or the FP32
-    skinny step"""
+    skinny step; beside HBM's peak, the measured Infinity-Cache read
+    bandwidth (the visited fragments of a group fit the 256 MB cache)"""
     if best.startswith("wform_fp16x3"):
         nb = methods[best].get("group", 1)
         w = wbk.get(nb) or wbk[1]
         key = f"{w['kernel'].split('<')[0]}/{nb}"
-        return {"bound": "hbm", "kernel": w["kernel"], "achieved": w["GBps"], "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": w["GBps"] / HBM_PEAK_GBS,
-                "traffic": traffic_from_profiles(key),
-                "bytes_per_launch": w["bytes_per_launch"], "avg_launch_us": w["avg_launch_us"],
-                "dense_bytes_per_launch": w["dense_bytes_per_launch"],
-                "visited_fragment_fraction": w["visited_fragment_fraction"],
-                "lo_read_fraction": w["lo_read_fraction"]}
+        r = {"bound": "hbm", "kernel": w["kernel"], "achieved": w["GBps"], "peak": HBM_PEAK_GBS,
+             "unit": "GB/s", "frac": w["GBps"] / HBM_PEAK_GBS,
+             "traffic": traffic_from_profiles(key),
+             "bytes_per_launch": w["bytes_per_launch"], "avg_launch_us": w["avg_launch_us"],
+             "dense_bytes_per_launch": w["dense_bytes_per_launch"],
+             "visited_fragment_fraction": w["visited_fragment_fraction"],
+             "lo_read_fraction": w["lo_read_fraction"]}
+        if mall:
+            r["mall_read_GBps_measured"] = mall["mall_read_GBps"]
+            r["frac_of_mall_read"] = w["GBps"] / mall["mall_read_GBps"]
+        return r
     k = "backward step: k_skinny<3> + k_skinny_combine"
     return {"bound": "hbm", "kernel": k, "achieved": bi_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": bi_gbs / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": bi_bytes,
             "avg_launch_us": bi_us}
+
+
+def bench_mall(H):
+    """read bandwidth of the Infinity Cache (MALL) and of HBM: one streaming
+    read (ophip_read_probe, 16-byte loads) of a buffer read just before, 20
+    times between events; a buffer below the 256 MB cache stays resident,
+    1 GiB does not"""
+    out = H.DeviceArray.empty(8192, dtype=np.uint32)  # 32 KiB the probe may touch
+    res = {}
+    for mb in (32, 64, 104, 128, 192, 1024):
+        nbytes = mb << 20
+        buf = H.DeviceArray(nbytes, np.uint8)
+        H.lib.opcuda_ssetall(buf.h, nbytes // 4, np.float32(1.0), 1)
+        for _ in range(3):
+            H.check(H.lib.ophip_read_probe(buf.h, nbytes, out.h))
+        ms = timed_events(H, lambda: H.lib.ophip_read_probe(buf.h, nbytes, out.h), 20)
+        H.check()
+        res[str(mb)] = nbytes / (ms * 1e-3) / 1e9
+        buf.free()
+    log(f"[bench] read bandwidth by working set (MB -> GB/s): "
+        + ", ".join(f"{k}: {v:.0f}" for k, v in res.items()))
+    return {"kernel": "k_read16 (ophip_read_probe)", "read_GBps_by_MB": res,
+            "mall_read_GBps": res["104"], "hbm_read_GBps": res["1024"]}
 
 
 def timed_events(H, fn, reps, a=0, b=1):
@@ -215,11 +243,14 @@ def main():
             continue
         methods[name] = bench_method(H, M, prec, args.steps, args.warmup, gold)
         log(f"[bench] {name}: {methods[name]}")
-    # many scenarios of the same lattice priced together (FP16 x3)
+    # many scenarios of the same lattice priced together (FP16 x3): `steps`
+    # timed group launches after `warmup` untimed ones
     for nb in args.groups:
         name = f"wform_fp16x3_group{nb}"
-        methods[name] = bench_group(H, M, nb, max(2, args.steps // 4), 1)
+        methods[name] = bench_group(H, M, nb, args.steps, args.warmup)
         log(f"[bench] {name}: {methods[name]}")
+    # Infinity-Cache and HBM read bandwidth (the backward step's ceiling)
+    mall = bench_mall(H)
 
     # ---- the FP16 x3 backward step (dominant kernel of the x3 methods), on
     # one lattice and on the largest group, HIP events on the library stream
@@ -312,9 +343,11 @@ def main():
         "value": bm["scenarios_per_s"],
         "unit": "scenarios/s",
         "n_gpus": 1,
+        # a step: one graph launch of the headline method (for a group, all of
+        # its scenarios); `steps` launches timed after `warmup` untimed ones
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": bm["ms"],
+        "ms_per_step": bm.get("ms_per_group", bm["ms"]),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -337,7 +370,8 @@ def main():
         # dominant kernel of the headline method: the backward-induction
         # step (252 per scenario) streams the 2048 x 2048 matrix's hi/lo
         # copies (FP16 x3 W form; the FP32 methods' step is in "gemv")
-        "roofline": roofline_backward(best, methods, wbk, bi_gbs, bi_us, bi_bytes),
+        "roofline": roofline_backward(best, methods, wbk, bi_gbs, bi_us, bi_bytes, mall),
+        "read_bandwidth": mall,
         "roofline_squaring": {"bound": "mfma", "kernel": "k_sgemm<128,128> (plain FP32 squaring)",
                               "achieved": gemm_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                               "frac": gemm_tflops / FP32_PEAK_TFLOPS,
@@ -523,13 +557,7 @@ def bench_mc(H, U, d, cfg, op_prices):
     sck_gbs = 8.0 * d * d / (sck_ms * 1e-3) / 1e9  # 4 B read + 4 B written per entry
     log(f"[bench] cumulative table {sck_ms * 1e3:.1f} us, {sck_gbs:.0f} GB/s")
     nk = 252
-    nrng = 4096
     nscen = 1 << 20
-    H.check(H.lib.ophip_mc_set_nrng(nrng))
-    mt = H.DeviceArray.empty(625 * nrng, dtype=np.uint32)
-    seeds = (5489 + np.arange(nrng)).astype(np.uint32)
-    H.lib.opcuda_mc_setseed(seeds.ctypes.data, mt.h)
-    H.check()
     mk = H.DeviceArray.from_numpy(np.zeros(nk, dtype=np.int32))
     ysk = H.DeviceArray.empty(nscen * nk, dtype=np.int16)
     y0 = 64 + 256 * 1
@@ -538,18 +566,32 @@ def bench_mc(H, U, d, cfg, op_prices):
     H.check(H.lib.ophip_mc_guide(d, 1, ct.h, gd.h))
     guide_us = timed_events(H, lambda: H.lib.ophip_mc_guide(d, 1, ct.h, gd.h), 5) * 1e3
     log(f"[bench] guide table {guide_us:.1f} us")
+    # one stream per wavefront of 64 paths (16384 streams seeded 5489 + i,
+    # the workload's mapping), and 4096 streams of 256 paths beside it
+    by_streams = {}
+    for nrng in (4096, 16384):
+        H.check(H.lib.ophip_mc_set_nrng(nrng))
+        mt = H.DeviceArray.empty(625 * nrng, dtype=np.uint32)
+        seeds = (5489 + np.arange(nrng)).astype(np.uint32)
+        H.lib.opcuda_mc_setseed(seeds.ctypes.data, mt.h)
+        H.check()
 
-    def run_bisect():
-        H.lib.ophip_mc1f_rows(mt.h, y0, ysk.h, nscen, nk, d, ct.h, mk.h, 0)
-    run_bisect()
-    H.sync()
-    ms_bisect = timed_events(H, run_bisect, 3)
+        def run_bisect():
+            H.lib.ophip_mc1f_rows(mt.h, y0, ysk.h, nscen, nk, d, ct.h, mk.h, 0)
+        run_bisect()
+        H.sync()
+        ms_bisect = timed_events(H, run_bisect, 3)
 
-    def run():
-        H.lib.ophip_mc1f_guided(mt.h, y0, ysk.h, nscen, nk, d, ct.h, gd.h, mk.h)
-    run()
-    H.sync()
-    ms = timed_events(H, run, 3)
+        def run():
+            H.lib.ophip_mc1f_guided(mt.h, y0, ysk.h, nscen, nk, d, ct.h, gd.h, mk.h)
+        run()
+        H.sync()
+        ms = timed_events(H, run, 3)
+        by_streams[nrng] = {"ms_guided": ms, "ms_full_bisection": ms_bisect}
+        log(f"[bench] MC {nrng} streams: guided {ms:.2f} ms, bisection {ms_bisect:.2f} ms per 2^20 x 252")
+    H.check(H.lib.ophip_mc_set_nrng(4096))
+    nrng = 16384
+    ms, ms_bisect = by_streams[nrng]["ms_guided"], by_streams[nrng]["ms_full_bisection"]
     trans = float(nscen) * nk
     log(f"[bench] MC {ms:.2f} ms per 2^20 x 252, {trans / ms / 1e6:.2f} G transitions/s")
     # the paths' call payoffs at T = 1 yr evaluated on the device, against the
@@ -586,7 +628,8 @@ def bench_mc(H, U, d, cfg, op_prices):
                                               "sample the FP16 x3 method's U(1/252))"},
             "cumulative_table_us": sck_ms * 1e3, "cumulative_table_GBps": sck_gbs,
             "steps": nk, "ms": ms, "transitions_per_s": trans / (ms * 1e-3),
-            "streams": nrng, "d": d}
+            "streams": nrng, "paths_per_stream": nscen // nrng, "d": d,
+            "by_streams": {str(k): v for k, v in by_streams.items()}}
 
 
 def bench_sv(H, cfg):
@@ -692,9 +735,29 @@ def bench_surrogate(H, M, ntrain, steps, log_fn):
     return out
 
 
+def cpu_info():
+    """the host's CPU model and thread counts"""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    return {"model": model, "os_cpu_count": os.cpu_count(), "threads_allowed": aff,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
 def bench_cpu(cfg, U_daily=None):
     from oracle import oracle as O
-    nth = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    # the host threads this process may use: OMP_NUM_THREADS where it is set
+    # (the GPU's share of the host's cores), else every thread of the
+    # affinity mask (the reference's Environment.ProcessorCount,
+    # cmt/Program.cs:55)
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    nth = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or aff
     O.set_threads(nth)
     t0 = time.perf_counter()
     k = 0
@@ -708,8 +771,155 @@ def bench_cpu(cfg, U_daily=None):
     out = {"value": k / t, "unit": "scenarios/s", "cores": nth, "kind": "port",
            "sample": f"{k} full FP64 solves of the same scenario (assembly, first step, "
                      "16 DGEMM squarings at d=2048, 252 backward steps x 41 strikes) by the "
-                     "C oracle with OpenMP", "seconds": t}
+                     "C oracle with OpenMP", "seconds": t, "host": cpu_info()}
+    O.set_threads(1)
+    t0 = time.perf_counter()
+    O.solve_fp64(cfg)
+    t1 = time.perf_counter() - t0
+    O.set_threads(nth)
+    out["single_thread"] = {"value": 1.0 / t1, "unit": "scenarios/s", "cores": 1,
+                            "sample": "one full FP64 solve of the same scenario on one thread"}
+    log(f"[bench] CPU FP64 oracle, one thread: {t1:.2f} s per solve")
     out["components"] = cpu_components(O, nth, U_daily)
+    out["components"]["clv"] = cpu_clv(O, nth)
+    out["components"]["csv"] = cpu_csv(O, nth)
+    return out
+
+
+def plan_days(step_days, gen_days):
+    """the reference's step plan (CModel.make_mc_plan, CModel Common.cs:163-
+    644) for steps ending at step_days (days) and generators valid up to
+    gen_days: each step cut at the generator dates into pieces (generator,
+    days), the pieces deduplicated; returns the pieces and each step's chain
+    of piece indices"""
+    pieces, chains, index = [], [], {}
+    prev = 0
+    for t in step_days:
+        chain, a = [], prev
+        while a < t:
+            i = next(j for j, g in enumerate(gen_days) if g > a)
+            b = min(t, gen_days[i])
+            key = (i, b - a)
+            if key not in index:
+                index[key] = len(pieces)
+                pieces.append(key)
+            chain.append(index[key])
+            a = b
+        chains.append(chain)
+        prev = t
+    return pieces, chains
+
+
+def cpu_plan_exec(O, gens, pieces, chains):
+    """the plan in FP64 on the host (CModel Host Double.cs:89-305): per piece
+    dt = min(0.5/max|L_yy|, 1 day), niter = ceil(log2(days/dt)) (the
+    oracle's planner formula), U0 = I + L dt_j, niter DGEMM squarings; per
+    step the product of its chain; returns the step kernels and the DGEMM
+    flop count (2 d^3 per product)"""
+    import ctypes as C
+    d = gens[0].shape[0]
+    flop, Uj = 0.0, []
+    for i, days in pieces:
+        L = gens[i]
+        dtj = C.c_double()
+        niter = O.lib.or_plan_niter(days / 365.25, float(np.abs(np.diag(L)).max()), 1.0 / 365.25,
+                                    C.byref(dtj))
+        Uj.append(O.square_chain(O.first_step(L, dtj.value), niter))
+        flop += niter * 2.0 * d ** 3
+    steps = []
+    for ch in chains:
+        U = Uj[ch[0]]
+        for j in ch[1:]:
+            U = O.dgemm(U, Uj[j])
+            flop += 2.0 * d ** 3
+        steps.append(U)
+    return steps, flop
+
+
+def cpu_mc_hash(O, steps, y0, nblocks, nscen, chunk=8):
+    """the reference's CPU sampler (hash-bracket tables, CModel Host
+    Double.cs:403-458; rangen.cpp:96-153) over the step kernels: table build
+    and nblocks x nscen paths x len(steps) steps, in chunks of blocks;
+    returns (build seconds, sampling seconds)"""
+    nm, d = len(steps), steps[0].shape[0]
+    t = np.ascontiguousarray(np.stack([U.reshape(-1, order="F") for U in steps]).reshape(-1))
+    ct = np.zeros_like(t)
+    hs = np.zeros(nm * d * d, dtype=np.int32)
+    t0 = time.perf_counter()
+    O.lib.or_mc_hash_build(d, nm, t, ct, hs)
+    tb = time.perf_counter() - t0
+    mk = np.arange(nm, dtype=np.int32)
+    ys = np.zeros(chunk * nm * nscen, dtype=np.int16)
+    ts = 0.0
+    for b0 in range(0, nblocks, chunk):
+        nb = min(chunk, nblocks - b0)
+        t0 = time.perf_counter()
+        O.lib.or_mc_hash_run(d, ct, hs, nb, nscen, nm, y0, mk, 5489 + b0, ys)
+        ts += time.perf_counter() - t0
+    return tb, ts
+
+
+def cpu_clv(O, nth):
+    """clv (benchmarks/clv/Program.cs:46-133) restated: local vol on 128
+    nodes, 10 generators valid to 30, 60, ..., 300 days (r = 0.05,
+    sigma = 0.25 x, CLVModel.cs:86-136 in FP64), the plan of 40 weekly steps
+    in FP64 (first steps, DGEMM squarings, chain products), then 4096*25
+    paths x 120 batches x 40 steps through the hash sampler.  The
+    reference's piecewise-uniform pivot grid is replaced by a uniform grid
+    on [0, 400] of the same size (the same work)."""
+    x, x0 = O.xgrid(128, 0.0, 400.0, 100.0)
+    gens = [O.lv_generator(x, 0.05 * x, 0.25 * x) for _ in range(10)]
+    pieces, chains = plan_days([7 * (k + 1) for k in range(40)], [30 * (i + 1) for i in range(10)])
+    t0 = time.perf_counter()
+    steps, flop = cpu_plan_exec(O, gens, pieces, chains)
+    tp = time.perf_counter() - t0
+    nscen, nb = 4096 * 25, 120
+    tb, ts = cpu_mc_hash(O, steps, x0, nb, nscen)
+    O.set_threads(1)
+    _, ts1 = cpu_mc_hash(O, steps, x0, 2, nscen, chunk=2)
+    O.set_threads(nth)
+    out = {"gflops": flop / tp / 1e9, "plan_seconds": tp, "kernels": len(pieces),
+           "mc_M_eval_per_s": 40.0 * nb * nscen / ts / 1e6, "threads": nth,
+           "mc_M_eval_per_s_one_thread": 40.0 * 2 * nscen / ts1 / 1e6, "table_build_s": tb,
+           "sample": "LV d=128, 10 monthly generators, 40 weekly steps (FP64 DGEMM squarings and "
+                     "chains), 4096*25 paths x 120 batches x 40 steps, hash-bracket sampler; one "
+                     "thread: 2 batches"}
+    log(f"[bench] CPU clv: {out['gflops']:.1f} GF/s, {out['mc_M_eval_per_s']:.0f} M eval/s "
+        f"({out['mc_M_eval_per_s_one_thread']:.0f} on one thread)")
+    return out
+
+
+def cpu_csv(O, nth):
+    """csv (benchmarks/csv/Program.cs:48-147) restated: stochastic vol on 64
+    price nodes x 8 vol regimes (0.55 .. 2.0, spot regime 4), 10 generators
+    from the term structure (vol 0.25, beta 0.5, volvol 0.5, mean reversion
+    0.5 to 3, jump sizes -3 / 0: CSVModel.mkgen, CSVModel.cs:103-172; the
+    generator CSVModel.host_d_gen, CSVModel.cs:187-379, in FP64), the plan of
+    40 weekly steps in FP64, then 4096*25 paths x 100 batches x 40 steps
+    through the hash sampler.  Uniform price grid on [0, 400] of the same
+    size in place of the pivot grid."""
+    nx, nr = 64, 8
+    x, x0 = O.xgrid(nx, 0.0, 400.0, 100.0)
+    rval = np.array([0.55, 0.75, 0.90, 1.0, 1.1, 1.3, 1.6, 2.0])
+    y0 = x0 + nx * 4
+    tau = np.array([[v, v, v, 0.0] for v in (0.25, 0.5, 0.5, 0.5, 0.5, 3.0, -3.0, 0.0)])
+    gen_days = [30 * (i + 1) for i in range(10)]
+    svol, volvol, voldrift, jm, jp = O.sv_params(x, rval, y0, [g / 365.25 for g in gen_days], tau)
+    xs, rs = np.tile(x, nr), np.repeat(rval, nx)
+    gens = [O.sv_generator(nx, nr, xs, rs, 0.05 * xs, svol[i], volvol[i], voldrift[i], jm[i], jp[i])
+            for i in range(10)]
+    pieces, chains = plan_days([7 * (k + 1) for k in range(40)], gen_days)
+    t0 = time.perf_counter()
+    steps, flop = cpu_plan_exec(O, gens, pieces, chains)
+    tp = time.perf_counter() - t0
+    nscen, nb = 4096 * 25, 100
+    tb, ts = cpu_mc_hash(O, steps, y0, nb, nscen)
+    out = {"gflops": flop / tp / 1e9, "plan_seconds": tp, "kernels": len(pieces),
+           "mc_M_eval_per_s": 40.0 * nb * nscen / ts / 1e6, "threads": nth, "table_build_s": tb,
+           "sample": "SV 64 x 8 = 512 states, 10 monthly generators, 40 weekly steps (FP64 DGEMM "
+                     "squarings and chains), 4096*25 paths x 100 batches x 40 steps, hash-bracket "
+                     "sampler"}
+    log(f"[bench] CPU csv: {out['gflops']:.1f} GF/s, {out['mc_M_eval_per_s']:.0f} M eval/s")
     return out
 
 

@@ -380,6 +380,13 @@ int ophip_mc_payoff_noalloc(unsigned y_sk, int nscen, int nk, int nsteps, const 
                             const double *df, unsigned xval, int d, const float *strikes, int nstrikes,
                             unsigned out);
 
+/* Read-bandwidth probe (a measurement aid; no reference counterpart): one
+ * streaming read of `bytes` (a multiple of 16) at handle x on the library
+ * stream; out: 32 KiB scratch.  Timed with ophip_event_*, a buffer that
+ * fits the 256 MB Infinity Cache and was just read gives the cache's read
+ * bandwidth, a larger one HBM's. */
+int ophip_read_probe(unsigned x, unsigned long bytes, unsigned out);
+
 /* graphs, events, workspace */
 int           ophip_reserve_workspace(unsigned long bytes);
 int           ophip_capture_begin(void);

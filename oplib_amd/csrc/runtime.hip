@@ -343,6 +343,33 @@ __global__ void k_copy1(size_t n, size_t from, const float *__restrict__ x, floa
     if (j < n) y[j] = x[j];
 }
 
+// read-bandwidth probe: every 16-byte word of x read once, four loads in
+// flight per thread, folded by XOR; the store depends on the data (a
+// pattern random inputs essentially never give), so no load is dead and
+// almost nothing is written
+constexpr int kProbeGrid = 2048;
+__global__ __launch_bounds__(256) void k_read16(size_t n16, const uint4 *__restrict__ x,
+                                                uint4 *__restrict__ out) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    uint4 a = {0u, 0u, 0u, 0u};
+    for (; j + 3 * stride < n16; j += 4 * stride) {
+        const uint4 v0 = x[j], v1 = x[j + stride], v2 = x[j + 2 * stride], v3 = x[j + 3 * stride];
+        a.x ^= v0.x ^ v1.x ^ v2.x ^ v3.x;
+        a.y ^= v0.y ^ v1.y ^ v2.y ^ v3.y;
+        a.z ^= v0.z ^ v1.z ^ v2.z ^ v3.z;
+        a.w ^= v0.w ^ v1.w ^ v2.w ^ v3.w;
+    }
+    for (; j < n16; j += stride) {
+        const uint4 v = x[j];
+        a.x ^= v.x;
+        a.y ^= v.y;
+        a.z ^= v.z;
+        a.w ^= v.w;
+    }
+    if ((a.x ^ a.y ^ a.z ^ a.w) == 0x9e3779b9u) out[blockIdx.x] = a;
+}
+
 __global__ void k_sset(int n, float c, float *__restrict__ x, int incx) {
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
         x[(size_t)j * incx] = c;
@@ -386,6 +413,21 @@ OPHIP_EXPORT int opcuda_scopy1(unsigned dst, unsigned src, unsigned n) {
         hipLaunchKernelGGL(k_copy1, dim3(grid_for(rest, 256)), dim3(256), 0, d.stream,
                            (size_t)n, n4 * 4, xs, ys);
     }
+    return (int)check_launch();
+}
+
+// Read-bandwidth probe (measurement; no reference counterpart): reads the
+// bytes (a multiple of 16) at handle x once on the library stream; out:
+// 32 KiB that a data-dependent store may touch.  Timed between events, a
+// buffer that fits the Infinity Cache and was read just before gives the
+// cache's read bandwidth, a larger one HBM's.
+OPHIP_EXPORT int ophip_read_probe(unsigned x, unsigned long bytes, unsigned out) {
+    Device &d = dev();
+    if (bytes == 0 || bytes % 16) return (int)set_status(kErrBadArg);
+    if (!handle_range_ok(x, bytes) || !handle_range_ok(out, (size_t)16 * kProbeGrid))
+        return (int)set_status(kErrBadHandle);
+    hipLaunchKernelGGL(k_read16, dim3(kProbeGrid), dim3(256), 0, d.stream, (size_t)bytes / 16,
+                       P<const uint4>(x), P<uint4>(out));
     return (int)check_launch();
 }
 

@@ -142,6 +142,7 @@ SIGNATURES = {
     "ophip_mc1f_guided": (I32, [U32, I32, U32, I32, I32, I32, U32, U32, U32]),
     "ophip_mc_payoff_noalloc": (I32, [U32, I32, I32, I32, C.POINTER(I32), C.POINTER(F64), U32, I32,
                                       C.POINTER(F32), I32, U32]),
+    "ophip_read_probe": (I32, [U32, ULONG, U32]),
     "ophip_reserve_workspace": (I32, [ULONG]),
     "ophip_capture_begin": (I32, []),
     "ophip_capture_end": (ULONG, []),

@@ -224,6 +224,44 @@ void or_sek_f32(int d, const float *gen, float dt, float *U)
         }
 }
 
+/* global_sekd / global_sekd_lambda (sek.cu:62-103): (id + dt gen) /
+ * (1 + dt r[y1]) and / (1 + dt r[y1] lambda), FP32, with the contraction
+ * of a product into the add that follows it (one FMA each, as sek_f32)
+ * and an IEEE division.  lambda = NULL: the sekd form. */
+void or_sekd_f32(int d, const float *gen, float dt, const float *r, const float *lambda, float *U)
+{
+    for (int y2 = 0; y2 < d; y2++)
+        for (int y1 = 0; y1 < d; y1++) {
+            size_t yy = (size_t)y1 + (size_t)d * y2;
+            float num = fmaf(dt, gen[yy], (y1 == y2) ? 1.0f : 0.0f);
+            float den = lambda ? fmaf(dt * r[y1], lambda[0], 1.0f) : fmaf(dt, r[y1], 1.0f);
+            U[yy] = num / den;
+        }
+}
+
+/* global_sglvg (sglvg.cu:42-99), FP32: ni generators gen[i d^2 + y0 + d y1]
+ * zeroed, then for interior rows y0 (0 < y0 < d-1) with rhs = (drift,
+ * vol^2): rate down xi = 0 + invm[4 y0] drift, then + invm[4 y0 + 2] vol^2
+ * (each product contracted into its add: the second step is one FMA), rate
+ * up likewise with invm[4 y0 + 1] and invm[4 y0 + 3], diagonal -(down + up).
+ * pars: drift (ni d) then vol (ni d). */
+void or_sglvg_f32(int ni, int d, const float *invm, const float *pars, float *gen)
+{
+    memset(gen, 0, sizeof(float) * (size_t)ni * d * d);
+    for (int i = 0; i < ni; i++)
+        for (int y0 = 1; y0 < d - 1; y0++) {
+            const float r0 = pars[y0 + (size_t)d * i];
+            const float v = pars[(size_t)ni * d + y0 + (size_t)d * i];
+            const float r1 = v * v;
+            const float lo = fmaf(invm[4 * y0 + 2], r1, invm[4 * y0 + 0] * r0);
+            const float hi = fmaf(invm[4 * y0 + 3], r1, invm[4 * y0 + 1] * r0);
+            float *g = gen + (size_t)i * d * d;
+            g[y0 + (size_t)d * (y0 - 1)] = lo;
+            g[y0 + (size_t)d * (y0 + 1)] = hi;
+            g[y0 + (size_t)d * y0] = -(lo + hi);
+        }
+}
+
 /* ------------------------------------------------------------------ */
 /* GEMM (A6-A9)                                                         */
 /* ------------------------------------------------------------------ */
@@ -939,7 +977,7 @@ void or_tgfsr_seed(int nlanes, int n, const uint32_t *seed, uint32_t *st)
 /* ------------------------------------------------------------------ */
 void or_mc_hash_build(int d, int nm, const double *tpk, double *ctpk, int *hash)
 {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for num_threads(nthreads()) schedule(static)
     for (int my = 0; my < nm * d; my++) {
         int m = my / d, y1 = my % d;
         const double *t = tpk + (size_t)d * d * m;
@@ -972,7 +1010,7 @@ void or_mc_hash_run(int d, const double *ctpk, const int *hash, int nblocks, int
                     const int *m_k, uint32_t seed0, int16_t *y_sk)
 {
     const int niter = (int)(log((double)d) / log(2.0)) + 1;
-#pragma omp parallel for schedule(dynamic, 1)
+#pragma omp parallel for num_threads(nthreads()) schedule(dynamic, 1)
     for (int b = 0; b < nblocks; b++) {
         uint32_t st[625];
         or_mt19937_init(seed0 + (uint32_t)b, st);

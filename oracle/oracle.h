@@ -20,6 +20,8 @@
  *                        second moment = svol^2 + the drift fix's share,
  *                        jump profile and mass (tests/test_oracle.py)
  *   or_first_step*       exact formula, row sums 1
+ *   or_sekd_f32, or_sglvg_f32  every FP32 step one correct rounding of its
+ *                        exact value (rational arithmetic, tests/test_oracle.py)
  *   or_dgemm / squaring  numpy matmul; scipy expm fixtures (tests/golden)
  *   or_sck_f32           closed form C[.,d-1]=1, monotone rows, float64 cumsum;
  *                        a negative entry lowers the running sum (hand-worked row)
@@ -78,6 +80,8 @@ double or_max_abs_diag(int d, const double *gen);
 /* ---- first step (A5) ---- */
 void or_first_step(int d, const double *gen, double dt, double *U);
 void or_sek_f32(int d, const float *gen, float dt, float *U);
+void or_sekd_f32(int d, const float *gen, float dt, const float *r, const float *lambda, float *U);
+void or_sglvg_f32(int ni, int d, const float *invm, const float *pars, float *gen);
 
 /* ---- GEMM / squaring / propagation (A6-A9) ---- */
 void or_dgemm(int m, int n, int k, const double *A, int lda, const double *B,

@@ -49,6 +49,8 @@ def _load():
         "or_max_abs_diag": (D, [I, _dp]),
         "or_first_step": (None, [I, _dp, D, _dp]),
         "or_sek_f32": (None, [I, _fp, F, _fp]),
+        "or_sekd_f32": (None, [I, _fp, F, _fp, C.c_void_p, _fp]),
+        "or_sglvg_f32": (None, [I, I, _fp, _fp, _fp]),
         "or_dgemm": (None, [I, I, I, _dp, I, _dp, I, _dp, I]),
         "or_sgemm_f64acc": (None, [I, I, I, F, _fp, I, _fp, I, F, _fp, I]),
         "or_square_chain": (None, [I, I, _dp, _dp]),

@@ -101,15 +101,20 @@ def test_sek_bit_exact(H):
     assert np.array_equal(out.download(), ref.reshape(d, d, order="F"))
     r = rng.random(d).astype(np.float32) * 0.05
     rd = dev(H, r)
+    gf = np.ascontiguousarray(g.reshape(-1, order="F"))
     H.lib.opcuda_sekd(d, gd.h, np.float32(1e-3), out.h, rd.h)
     H.check()
+    ref = np.zeros(d * d, dtype=np.float32)
+    O.lib.or_sekd_f32(d, gf, np.float32(1e-3), r, None, ref)
+    assert np.array_equal(out.download(), ref.reshape(d, d, order="F"))
     want = (np.eye(d) + 1e-3 * g.astype(np.float64)) / (1 + 1e-3 * r[:, None].astype(np.float64))
     assert np.max(np.abs(out.download() - want)) < 1e-6
-    lam = dev(H, np.array([0.5], dtype=np.float32))
+    lamv = np.array([0.5], dtype=np.float32)
+    lam = dev(H, lamv)
     H.lib.opcuda_sekd_lambda(d, gd.h, np.float32(1e-3), out.h, rd.h, lam.h)
     H.check()
-    want = (np.eye(d) + 1e-3 * g.astype(np.float64)) / (1 + 0.5e-3 * r[:, None].astype(np.float64))
-    assert np.max(np.abs(out.download() - want)) < 1e-6
+    O.lib.or_sekd_f32(d, gf, np.float32(1e-3), r, lamv.ctypes.data, ref)
+    assert np.array_equal(out.download(), ref.reshape(d, d, order="F"))
 
 
 def test_sglvg_vs_fp64(H):
@@ -124,6 +129,9 @@ def test_sglvg_vs_fp64(H):
     H.lib.opcuda_sglvg(gen.h, ni, d, invd.h, parsd.h)
     H.check()
     got = gen.download().reshape(ni, d * d)
+    ref32 = np.zeros(ni * d * d, np.float32)
+    O.lib.or_sglvg_f32(ni, d, inv.astype(np.float32), pars, ref32)
+    assert np.array_equal(got.reshape(-1), ref32)  # the reference's FP32 steps, bit for bit
     for i, (a, b) in enumerate(((dr, vo), (1.3 * dr, 0.9 * vo))):
         ref = O.lv_generator(x, a, b)
         scale = np.abs(ref).max()

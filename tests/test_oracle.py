@@ -167,6 +167,75 @@ def test_sek_f32_formula():
     assert np.array_equal(out.reshape(d, d, order="F"), ref)
 
 
+def rnd32(q):
+    """a rational rounded to the nearest float32, ties to even (exact)"""
+    from fractions import Fraction
+    f = np.float32(float(q))
+    best = None
+    for c in (np.nextafter(f, np.float32(-np.inf)), f, np.nextafter(f, np.float32(np.inf))):
+        e = abs(Fraction(float(c)) - q)
+        if best is None or e < best[0] or (e == best[0] and int(c.view(np.uint32)) % 2 == 0):
+            best = (e, c)
+    return best[1]
+
+
+def test_sekd_f32_each_step_one_rounding():
+    # sek.cu:62-103: num = dt gen + id (one FMA), den = dt r + 1 (one FMA;
+    # lambda: (dt r) lambda + 1), an IEEE quotient; checked in exact
+    # rational arithmetic on a sample of entries
+    from fractions import Fraction as Fr
+    rng = np.random.default_rng(12)
+    d = 40
+    g = (rng.standard_normal((d, d)) * 50).astype(np.float32)
+    r = (rng.random(d) * 0.05).astype(np.float32)
+    dt, lam = np.float32(1e-3), np.float32(0.7)
+    gf = np.ascontiguousarray(g.reshape(-1, order="F"))
+    out = np.zeros(d * d, np.float32)
+    outl = np.zeros(d * d, np.float32)
+    O.lib.or_sekd_f32(d, gf, dt, r, None, out)
+    lamv = np.array([lam], np.float32)
+    O.lib.or_sekd_f32(d, gf, dt, r, lamv.ctypes.data, outl)
+    for e in rng.integers(0, d * d, 150).tolist() + [0, d + 1, 5 * d + 5]:
+        y1, y2 = e % d, e // d
+        num = rnd32(Fr(float(dt)) * Fr(float(gf[e])) + (1 if y1 == y2 else 0))
+        den = rnd32(Fr(float(dt)) * Fr(float(r[y1])) + 1)
+        assert out[e] == rnd32(Fr(float(num)) / Fr(float(den))), e
+        dtr = rnd32(Fr(float(dt)) * Fr(float(r[y1])))
+        denl = rnd32(Fr(float(dtr)) * Fr(float(lam)) + 1)
+        assert outl[e] == rnd32(Fr(float(num)) / Fr(float(denl))), e
+
+
+def test_sglvg_f32_each_step_one_rounding():
+    # sglvg.cu:42-99: zeroed blocks, boundary rows zero, interior rows
+    # down = invm[4y+2] v^2 + (invm[4y] drift) and up likewise (one FMA
+    # each after the rounded first product), diagonal -(down + up)
+    from fractions import Fraction as Fr
+    d, ni = 50, 2
+    x, _ = O.xgrid(d, 0.0, 400.0, 100.0)
+    inv = O.invm(x).astype(np.float32)
+    dr, vo = O.lv_params_c1(x, 0.02)
+    pars = np.concatenate([dr, 1.3 * dr, vo, 0.9 * vo]).astype(np.float32)
+    gen = np.full(ni * d * d, np.nan, np.float32)
+    O.lib.or_sglvg_f32(ni, d, inv, pars, gen)
+    G = gen.reshape(ni, d, d).transpose(0, 2, 1)  # [i][y0][y1]
+    assert np.all(G[:, 0] == 0) and np.all(G[:, -1] == 0)
+    for i in range(ni):
+        for y in range(1, d - 1):
+            r0, v = pars[y + d * i], pars[ni * d + y + d * i]
+            r1 = rnd32(Fr(float(v)) * Fr(float(v)))
+            F = lambda a: Fr(float(a))  # noqa: E731
+            lo = rnd32(F(inv[4 * y + 2]) * F(r1) + F(rnd32(F(inv[4 * y]) * F(r0))))
+            hi = rnd32(F(inv[4 * y + 3]) * F(r1) + F(rnd32(F(inv[4 * y + 1]) * F(r0))))
+            assert G[i, y, y - 1] == lo and G[i, y, y + 1] == hi, (i, y)
+            assert G[i, y, y] == -rnd32(F(lo) + F(hi))
+            row = G[i, y].copy()
+            row[[y - 1, y, y + 1]] = 0
+            assert np.all(row == 0)
+    # and it is the FP64 generator to FP32 accuracy
+    ref = O.lv_generator(x, dr, vo)
+    assert np.abs(G[0] - ref).max() <= 1e-6 * np.abs(ref).max()
+
+
 # ---- cumulative tables and sampler ---------------------------------------
 
 def test_sck_properties_and_float64():
