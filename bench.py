@@ -144,6 +144,9 @@ def main():
     ap.add_argument("--probe", action="store_true",
                     help="only launch the dominant kernel (W-form backward step) 20 times and a "
                          "64 MiB copy kernel 20 times, for counter collection")
+    ap.add_argument("--profile-group", type=int, default=0, metavar="N",
+                    help="only price one group of N scenarios of the 2048-state lattice: three "
+                         "graph launches (for a kernel trace of one launch's kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); a multi-GPU run "
@@ -157,6 +160,19 @@ def main():
     H.init()
     cfg = M.config_rs2048()
     d = cfg.d
+    if args.profile_group:
+        g = M.Group(M.config_rs2048_scenarios(args.profile_group))
+        g.capture()
+        for rep in range(3):
+            g.launch()
+            H.sync()
+            log(f"[bench] group launch {rep + 1} of 3 done")
+        gold = np.load(os.path.join(ROOT, "tests", "golden", "prices_rs2048_scenarios.npz"))["prices"]
+        e = rel_err(g.prices(), gold[:args.profile_group])
+        print(json.dumps({"profile_group": args.profile_group, "launches": 3,
+                          "max_rel_err": float(e.max())}), flush=True)
+        g.close()
+        return
     if args.probe:
         nb = args.probe_group
         wb = WformBackward(H, M, M.config_rs2048_scenarios(nb))
