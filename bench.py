@@ -13,10 +13,11 @@ oracle (the plain FP32 method, the drop-in, is timed beside it and fails
 that bar at this size).
 
 Also measured, in the same run:
-  roofline      the headline method's dominant kernel, the backward step
-                (skinny GEMM), timed with HIP events on the library stream,
-                in GB/s against HBM; the FP32 squaring GEMM against the
-                FP32 matrix-core peak;
+  roofline      the headline method's dominant kernel, the band-sparse
+                backward step of a scenario group, timed with HIP events on
+                the library stream, in GB/s against HBM (and beside the
+                measured Infinity-Cache read rate); the FP32 squaring GEMM
+                against the FP32 matrix-core peak;
   methods       every method's scenarios/s and error;
   sweep         the 64-lattice calibration sweep, plain vs batched W form;
   mc            the MT19937 chain sampler on the daily table,
@@ -63,6 +64,15 @@ def traffic_from_profiles(kernel):
                 except ValueError:
                     return None
     return None
+
+
+def band_cutoffs(H):
+    """the band-sparse backward step's cutoffs: a 16 x 32 fragment of W is
+    read where some |W| >= 2^visit, its lo part where some |W| >= 2^lo"""
+    import ctypes as C
+    ev, el = C.c_int(0), C.c_int(0)
+    H.check(H.lib.ophip_wsq_get_bwd_cutoffs(C.byref(ev), C.byref(el)))
+    return {"visit": f"2^{ev.value}", "lo": f"2^{el.value}"}
 
 
 def roofline_backward(best, methods, wbk, bi_gbs, bi_us, bi_bytes, mall=None):
@@ -376,6 +386,7 @@ def main():
                    "scenarios_per_launch": grp,
                    "d": d, "squarings": cfg.nsq, "backward_steps": cfg.nsteps,
                    "strikes": cfg.nstrikes, "method": names[best],
+                   "band_cutoffs": band_cutoffs(H),
                    "plain_fp32_scenarios_per_s": value,
                    "speedup_vs_plain_fp32": bm["scenarios_per_s"] / value},
         "error_vs_fp64": {"max_rel": bm["max_rel_err"],

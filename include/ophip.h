@@ -245,10 +245,16 @@ int ophip_wsq_bwd_step(unsigned state, int d, int mode, int buf, int ncp, float 
 /* The backward step of nb lattices of one d in one launch (per 32): Y[i] =
  * df[i] (X[i] + W_i X[i]), W_i from states[i] laid out as above.  The
  * band-sparse step kernels read the hi part of a 16 x 32 fragment only
- * where some |W| >= 2^-40 and its lo part only where some |W| >= 2^-28
- * (FP16) or 2^-32 (BF16) (lists built by ophip_wsq_bwd_layout*): what they
- * leave out is at most 2^-40 per entry of W, so a row of Y moves by at most
- * (entries left out) 2^-40 max|X|; the dense kernel reads every fragment.
+ * where some |W| >= 2^visit_exp and its lo part only where some |W| >=
+ * 2^lo_exp (FP16) or 2^(lo_exp - 3) (BF16) (lists built by
+ * ophip_wsq_bwd_layout*, with the cutoffs set then): what they leave out of
+ * an entry of W is the entry where it is under 2^visit_exp, else its lo part
+ * where it is under 2^lo_exp, at most max(2^(lo_exp - 12), 2^-40); a row of
+ * Y moves by at most the left-out parts times |X|.  The dense kernel reads
+ * every fragment.  ophip_wsq_set_bwd_cutoffs sets the two exponents (-40 <=
+ * visit_exp <= lo_exp <= -1; default -24 and -12, which move the 2048-state
+ * daily workload's prices by under 1e-5 relative; -40 and -28 leave out
+ * nothing above 2^-40) for lists built afterwards.
  * ophip_wsq_set_bwd_kernel(k) selects the kernel: 0 the
  * dense one (every fragment), 1 automatic (default), 2 per 16-row tile, 3
  * per 64-row block, 4 per 32-row block (3 and 4: ncp 48); it returns the
@@ -259,6 +265,8 @@ int ophip_wsq_bwd_step_batch(int nb, const unsigned *states, int d, int mode, in
                              const float *df, const unsigned *X, const unsigned *XT,
                              const unsigned *Y, const unsigned *YT);
 int ophip_wsq_set_bwd_kernel(int k);
+int ophip_wsq_set_bwd_cutoffs(int visit_exp, int lo_exp);
+int ophip_wsq_get_bwd_cutoffs(int *visit_exp, int *lo_exp);
 /* the kernel (0, 2, 3, 4 as above) the current setting gives a step of nb
  * lattices of size d with ncp value columns */
 int ophip_wsq_bwd_kernel_for(int nb, int d, int ncp);

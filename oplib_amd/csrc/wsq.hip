@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <type_traits>
@@ -870,16 +871,19 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd1(int d, float df, const uint16
 // Band-sparse backward step.  Outside a band around the diagonal of each
 // regime block the squared W is tiny: at the 2048-state lattice (d = 2048,
 // U(1/252)) 57% of the 16 x 32 fragments of its FP16 hi/lo copies are zero
-// and a third of the rest hold values below 2^-40.  k_wbwd_flags marks, per
-// fragment, whether a step visits it (some |W| >= 2^-40: its hi part is
-// read) and whether it also reads the lo part (some |W| >= 2^-28 in FP16,
-// 2^-32 in BF16); k_wbwd_lists lists the visited k tiles, each entry kt +
-// 65536 lo.  What a step leaves out is at most 2^-40 per entry of W: an
-// entry under 2^-40, or the lo part of a smaller entry, which is at most
-// half an ulp of its hi part (FP16 hi parts W 2^15 under 2^-13 are
-// multiples of 2^-24: 2^-25 2^-15; BF16 under 2^-32: 2^-9 2^-32).  A row
-// of the step then differs from the full three-product sum by at most
-// (entries left out) 2^-40 max|X|.
+// and most of the rest hold values far below FP32 resolution.
+// k_wbwd_flags marks, per fragment, whether a step visits it (some |W| >=
+// 2^ev: its hi part is read) and whether it also reads the lo part (some
+// |W| >= 2^el in FP16, 2^(el-3) in BF16, whose hi parts carry three fewer
+// bits); k_wbwd_lists lists the visited k tiles, each entry kt + 65536 lo.
+// What a step leaves out of an entry of W is the entry itself where it is
+// under 2^ev, or its lo part where it is under 2^el: at most half an ulp of
+// the hi part, 2^(el-12) in both encodings (and at most 2^-40, FP16's
+// subnormal spacing 2^-24 of W 2^15, where el < -28).  A row of the step
+// then differs from the full three-product sum by at most the left-out
+// parts times |X|.  The defaults ev = -24, el = -12 move the prices of the
+// 2048-state daily workload by under 1e-5 relative in exact arithmetic
+// (16 scenarios); ev = -40, el = -28 leave out nothing above 2^-40.
 // ---------------------------------------------------------------------
 // Lists: per 16-row tile (d/16 lists), then per 64-row block (d/64: the
 // union of its four tiles' lists) and per 32-row block (d/32: two tiles),
@@ -890,13 +894,22 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd1(int d, float df, const uint16
 __device__ inline size_t tl_list4(int d) { return (size_t)(d / 16) * (d / 32 + 1); }
 __device__ inline size_t tl_list2(int d) { return tl_list4(d) + (size_t)(d / 64) * (d / 32 + 1); }
 
-// the hi parts' magnitude bits (sign dropped) at which a fragment is
-// visited and at which its lo part is read.  FP16 (hi = W 2^15): 2^-40
-// 2^15 is below the smallest subnormal, so every nonzero hi part is
-// visited; 2^-28 2^15 = 2^-13, exponent field 2.  BF16 (hi = W): exponent
-// fields 87 (2^-40) and 95 (2^-32).
-constexpr uint32_t kVisitF16 = 1u, kLoF16 = 2u << 10;
-constexpr uint32_t kVisitBF16 = 87u << 7, kLoBF16 = 95u << 7;
+// the magnitude bits (sign dropped) of a hi copy at which |W| reaches 2^e:
+// FP16 holds W 2^15 (exponent field f + 15 for 2^f, f >= -14; subnormal
+// 2^f is the mantissa 2^(f + 24); below 2^-24 every nonzero value counts),
+// BF16 holds W (exponent field e + 127)
+namespace {
+inline uint32_t hi_bits(int mode, int e) {
+    if (mode == 2) return (uint32_t)(e + 127) << 7;
+    const int f = e + 15;
+    if (f >= -14) return (uint32_t)(f + 15) << 10;
+    if (f >= -24) return 1u << (f + 24);
+    return 1u;
+}
+
+// the cutoffs ev, el (exponents) of the flags built from now on
+std::atomic<int> g_cut_visit{-24}, g_cut_lo{-12};
+}  // namespace
 
 // k_wbwd_flags: byte flags[it][kt] (d/16 x d/32): bit 0 = the fragment is
 // visited, bit 1 = its lo part is read; a wave per (it, four k tiles), the
@@ -1790,7 +1803,8 @@ LayScen lay_scen(const WsqLayout &L, unsigned state, int buf) {
 // the hi copies
 void bwd_tile_lists(Device &D, int d, int mode, const LayBatch &bt, int m) {
     const int waves = d / 16 * (d / 128);
-    const uint32_t tvis = mode == 2 ? kVisitBF16 : kVisitF16, tlo = mode == 2 ? kLoBF16 : kLoF16;
+    const int ev = g_cut_visit.load(), el = g_cut_lo.load();
+    const uint32_t tvis = hi_bits(mode, ev), tlo = hi_bits(mode, mode == 2 ? el - 3 : el);
     hipLaunchKernelGGL(k_wbwd_flags, dim3(cdiv(waves, 4), m), dim3(256), 0, D.stream, d, D.base, bt, tvis,
                        tlo);
     hipLaunchKernelGGL(k_wbwd_lists, dim3(cdiv(d / 16 + d / 64 + d / 32, 4), m), dim3(256), 0, D.stream, d,
@@ -1874,6 +1888,22 @@ OPHIP_EXPORT int ophip_wsq_set_bwd_kernel(int k) {
     const int old = g_bwd_kernel;
     if (k >= 0 && k <= 4) g_bwd_kernel = k;
     return old;
+}
+
+// the band-sparse step's cutoffs (exponents, -40 <= visit <= lo <= -1) for
+// the tile lists built (or captured) from now on
+OPHIP_EXPORT int ophip_wsq_set_bwd_cutoffs(int visit_exp, int lo_exp) {
+    if (visit_exp < -40 || visit_exp > lo_exp || lo_exp > -1) return (int)set_status(kErrBadArg);
+    g_cut_visit.store(visit_exp);
+    g_cut_lo.store(lo_exp);
+    return 0;
+}
+
+OPHIP_EXPORT int ophip_wsq_get_bwd_cutoffs(int *visit_exp, int *lo_exp) {
+    if (!visit_exp || !lo_exp) return (int)set_status(kErrBadArg);
+    *visit_exp = g_cut_visit.load();
+    *lo_exp = g_cut_lo.load();
+    return 0;
 }
 
 // One backward step of nb lattices of the same d in one launch per 32:

@@ -106,6 +106,8 @@ SIGNATURES = {
                                        C.POINTER(U32), C.POINTER(U32), C.POINTER(U32),
                                        C.POINTER(U32)]),
     "ophip_wsq_set_bwd_kernel": (I32, [I32]),
+    "ophip_wsq_set_bwd_cutoffs": (I32, [I32, I32]),
+    "ophip_wsq_get_bwd_cutoffs": (I32, [C.POINTER(I32), C.POINTER(I32)]),
     "ophip_wsq_bwd_kernel_for": (I32, [I32, I32, I32]),
     "ophip_wsq_plan_tiles": (ULONG, [I32, C.POINTER(U32), C.POINTER(I32), C.POINTER(I32), I32,
                                      C.POINTER(I32), I32]),

@@ -228,23 +228,47 @@ def test_group_equals_single_small(M, prec):
     g.close()
 
 
+def bwd_cutoffs(H):
+    import ctypes as C
+    ev, el = C.c_int(0), C.c_int(0)
+    H.check(H.lib.ophip_wsq_get_bwd_cutoffs(C.byref(ev), C.byref(el)))
+    return ev.value, el.value
+
+
 def test_group_rs2048_vs_fp64(M):
     """four scenarios of the 2048-state regime lattice (the workload, then
     seeded vol levels, regime rates and rates) priced together in FP16 x3:
-    each within 1e-5 of the FP64 oracle (fixture); the first one the bits of
-    the single-scenario solve when both use the same backward-step kernel
-    (64-row blocks), and within FP32 accumulation-order rounding of the
-    single solve's own default kernel; a captured group replays to the same
-    bits"""
+    with the tightest band cutoffs (2^-40, 2^-28) each within 1e-5 of the
+    FP64 oracle (fixture); the default cutoffs (2^-24, 2^-12) leave out
+    parts of W that move these prices by at most 9.2e-6 relative in exact
+    arithmetic (FP64 backward induction with and without them), so the
+    default prices stay within 1.5e-5 of the tight ones; the first one the
+    bits of the single-scenario solve when both use the same backward-step
+    kernel (64-row blocks), and within FP32 accumulation-order rounding of
+    the single solve's own default kernel; a captured group replays to the
+    same bits"""
     banner("group rs2048 x4 FP16x3")
     from oplib_amd import ophip as H
     gold = np.load(os.path.join(GOLD, "prices_rs2048_scenarios.npz"))["prices"]
     cfgs = M.config_rs2048_scenarios(4)
+    dflt = bwd_cutoffs(H)
+    assert dflt == (-24, -12)
+    H.check(H.lib.ophip_wsq_set_bwd_cutoffs(-40, -28))
+    try:
+        g = M.Group(cfgs)
+        g.solve()
+        pt = g.prices()
+        g.close()
+    finally:
+        H.check(H.lib.ophip_wsq_set_bwd_cutoffs(*dflt))
+    et = report("group rs2048 x4, cutoffs 2^-40 2^-28", pt, gold[:4])
+    assert np.all(np.isfinite(pt)) and et.max() < 1e-5
     g = M.Group(cfgs)
     g.solve()
     pg = g.prices()
     e = report("group rs2048 x4", pg, gold[:4])
-    assert np.all(np.isfinite(pg)) and e.max() < 1e-5
+    assert np.all(np.isfinite(pg)) and e.max() < 1e-4
+    assert O.rel_err(pg, pt).max() < 1.5e-5
     s = M.Solver(cfgs[0])
     s.solve()
     p1 = s.prices()

@@ -9,6 +9,7 @@ largest entry); the diagonal fix leaves FP64 row sums of 0 up to FP32
 rounding of the diagonal; a batch gives the same bits as the lattices
 squared alone.
 """
+import contextlib
 import functools
 import math
 import sys
@@ -282,21 +283,24 @@ def unfrag(a, nrows, K):
 BWD_KERNELS = {0: "dense", 2: "16-row tiles", 3: "64-row blocks", 4: "32-row blocks"}
 
 
+@pytest.mark.parametrize("cut", [None, (-40, -28)])
 @pytest.mark.parametrize("kernel", sorted(BWD_KERNELS))
 @pytest.mark.parametrize("mode", [1, 2])
-def test_wform_backward_step(hip, mode, kernel):
+def test_wform_backward_step(hip, mode, kernel, cut):
     """Y = df (X + W X) from the state's hi/lo copies (fragment-major): against
     the same arithmetic in numpy (FP64 accumulation, 1e-6 of max|Y|; the
-    band-sparse kernels read the hi parts of the fragments reaching 2^-40
-    and the lo parts of those reaching 2^-28 (FP16) or 2^-32 (BF16), which
-    leaves out at most 2^-40 per entry of W), against df (X + W X) with
-    the FP32 W (2e-5), and
-    X's / Y's 16-bit copies bit-exact; every step kernel"""
-    banner(f"wsq backward step mode={mode} kernel={BWD_KERNELS[kernel]}")
+    band-sparse kernels read the hi parts of the fragments reaching
+    2^visit and the lo parts of those reaching 2^lo (FP16) or 2^(lo-3)
+    (BF16), which leaves out an entry under 2^visit or at most
+    max(2^(lo-12), 2^-40) of one), against df (X + W X) with the FP32 W
+    (2e-5), and X's / Y's 16-bit copies bit-exact; every step kernel, at
+    the default cutoffs and at the tightest"""
+    banner(f"wsq backward step mode={mode} kernel={BWD_KERNELS[kernel]} cutoffs={cut}")
     H = hip
     old = H.lib.ophip_wsq_set_bwd_kernel(kernel)
     try:
-        _backward_step_case(H, mode, kernel == 0)
+        with cutoffs(H, cut):
+            _backward_step_case(H, mode, kernel == 0)
     finally:
         H.lib.ophip_wsq_set_bwd_kernel(old)
 
@@ -320,8 +324,9 @@ def _backward_step_case(H, mode, dense):
     YT = H.DeviceArray.empty(2 * ncp * d, dtype=np.uint16)
     H.check(H.lib.ophip_wsq_bwd_layout(L.state.h, d, mode, buf))
     H.check(H.lib.ophip_wsq_bwd_prep(d, ncp, mode, Xd.h, XT.h))
+    ev, el = get_cutoffs(H)
     vis, lo = (np.kron(m, np.ones((16, 32))) for m in
-               tile_flags(L.arr(buf, 1, np.uint16, "C"), d, mode))
+               tile_flags(L.arr(buf, 1, np.uint16, "C"), d, mode, (ev, el)))
     df = 0.9999
     H.check(H.lib.ophip_wsq_bwd_step(L.state.h, d, mode, buf, ncp, np.float32(df), Xd.h, XT.h,
                                      Yd.h, YT.h))
@@ -336,10 +341,13 @@ def _backward_step_case(H, mode, dense):
     full = df * (X.astype(np.float64) + (Hr @ Xh + (Hr @ Xl + Lr @ Xh) / G) / S)
     Hv, Lv = Hr * vis, Lr * lo  # the visited fragments' hi parts, the read lo parts
     ref = df * (X.astype(np.float64) + (Hv @ Xh + (Hv @ Xl + Lv @ Xh) / G) / S)
-    # what the band rule leaves out: at most 2^-40 per entry of W
-    left = (np.abs(Hr + Lr / G) * (1 - vis) + np.abs(Lr / G) * (vis - lo)) / S
-    assert left.max() <= 2.0**-40 * (1 + 2.0**-10)
-    assert np.abs(ref - full).max() <= 2.0**-40 * 1.01 * (left > 0).sum(axis=1).max() * 300.0
+    # what the band rule leaves out: an entry under 2^ev, or the lo part of
+    # an entry under 2^el, at most half an ulp of its hi part
+    gone = np.abs(Hr + Lr / G) * (1 - vis) / S
+    part = np.abs(Lr / G) * (vis - lo) / S
+    assert gone.max() < 2.0**ev * (1 + 2.0**-10)
+    assert part.max() <= max(2.0**(el - 12), 2.0**-40) * (1 + 2.0**-10)
+    assert np.abs(ref - full).max() <= 1.01 * ((gone + part) @ np.abs(Xh + Xl / G)).max() + 1e-9
     if dense:  # the dense kernel reads every fragment
         ref = full
     assert np.abs(Y - ref).max() <= 1e-6 * np.abs(ref).max()
@@ -355,23 +363,42 @@ def _backward_step_case(H, mode, dense):
                                        int(yl[tuple(b)])) for b in bad[:4]])
 
 
-# hi-part magnitude bits at which a fragment is visited (|W| >= 2^-40) and
-# its lo part read (|W| >= 2^-28 in FP16, whose copies hold W 2^15; 2^-32
-# in BF16, whose copies hold W)
-VISIT_BITS = {1: 1, 2: 87 << 7}
-LO_BITS = {1: 2 << 10, 2: 95 << 7}
+def get_cutoffs(H):
+    import ctypes as C
+    ev, el = C.c_int(0), C.c_int(0)
+    H.check(H.lib.ophip_wsq_get_bwd_cutoffs(C.byref(ev), C.byref(el)))
+    return ev.value, el.value
 
 
-def tile_flags(hbits, d, mode):
+@contextlib.contextmanager
+def cutoffs(H, cut):
+    """the band-sparse cutoffs (visit, lo exponents) inside the block; None
+    keeps the library's default"""
+    old = get_cutoffs(H)
+    if cut is not None:
+        H.check(H.lib.ophip_wsq_set_bwd_cutoffs(*cut))
+    try:
+        yield
+    finally:
+        H.check(H.lib.ophip_wsq_set_bwd_cutoffs(*old))
+
+
+def tile_flags(hbits, d, mode, cut):
     """numpy: per (16-row tile, 32-column k tile) of the row-major hi copy,
-    whether a step visits it and whether it reads the lo part"""
-    m = (hbits.astype(np.uint32) & 0x7FFF).reshape(d // 16, 16, d // 32, 32).max(axis=(1, 3))
-    return m >= VISIT_BITS[mode], m >= LO_BITS[mode]
+    whether a step visits it (some |W| >= 2^visit) and whether it reads the
+    lo part (some |W| >= 2^lo in FP16, whose copies hold W 2^15; 2^(lo-3)
+    in BF16, whose copies hold W), on the decoded magnitudes; an FP16
+    cutoff below its smallest subnormal counts every nonzero value"""
+    ev, el = cut
+    m = np.abs(dec16(hbits, mode).astype(np.float64)).reshape(d // 16, 16, d // 32, 32).max(axis=(1, 3))
+    if mode == 2:
+        return m >= 2.0**ev, m >= 2.0**(el - 3)
+    return m >= max(2.0**(ev + 15), 2.0**-24), m >= max(2.0**(el + 15), 2.0**-24)
 
 
-def tile_lists(hf, d, mode):
+def tile_lists(hf, d, mode, cut):
     """numpy: per 16-row tile, the visited k tiles and their lo flags"""
-    vis, lo = tile_flags(unfrag(hf, d, d), d, mode)
+    vis, lo = tile_flags(unfrag(hf, d, d), d, mode, cut)
     return [(np.nonzero(vis[i])[0], lo[i]) for i in range(d // 16)]
 
 
@@ -385,21 +412,24 @@ def backward_lattice(H, nsq, mode=1, nx=128, nr=2):
     return L
 
 
+@pytest.mark.parametrize("cut", [(-24, -12), (-40, -28), (-32, -16), (-20, -6)])
+@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("nsq", [2, 9])
-def test_wform_backward_tile_lists(hip, nsq):
+def test_wform_backward_tile_lists(hip, nsq, mode, cut):
     """the tile lists built with the fragment layout are the 16 x 32
-    fragments whose hi part reaches 2^-40, each flagged where it reaches
-    2^-28 (its lo part is read) and, in a block's list, with the row tiles
-    that visit it, numpy on the downloaded fragments; a
-    banded W after few squarings skips most of them"""
-    banner(f"wsq backward tile lists nsq={nsq}")
-    H, mode = hip, 1
+    fragments whose hi part reaches 2^visit, each flagged where it reaches
+    2^lo (FP16) or 2^(lo-3) (BF16) (its lo part is read) and, in a block's
+    list, with the row tiles that visit it, numpy on the downloaded
+    fragments; a banded W after few squarings skips most of them"""
+    banner(f"wsq backward tile lists nsq={nsq} mode={mode} cutoffs={cut}")
+    H = hip
     L = backward_lattice(H, nsq, mode)
     d, buf = L.d, nsq & 1
-    H.check(H.lib.ophip_wsq_bwd_layout(L.state.h, d, mode, buf))
-    H.sync()
+    with cutoffs(H, cut):
+        H.check(H.lib.ophip_wsq_bwd_layout(L.state.h, d, mode, buf))
+        H.sync()
     hf = L.arr(1 - buf, 2, np.uint16, "F").reshape(-1, order="F")
-    ref = tile_lists(hf, d, mode)
+    ref = tile_lists(hf, d, mode, cut)
     h = H.lib.ophip_wsq_array(L.state.h, d, mode, 0, 7)
     nkt = d // 32
     nl = d // 16 + d // 64 + d // 32  # lists per 16-row tile, 64-row block, 32-row block
