@@ -150,7 +150,7 @@ def main():
     ap.add_argument("--probe-group", type=int, default=16,
                     help="lattices per backward-step launch in --probe")
     ap.add_argument("--surr-train", type=int, default=262144, help="surrogate training draws")
-    ap.add_argument("--surr-steps", type=int, default=120000, help="surrogate Adam steps")
+    ap.add_argument("--surr-steps", type=int, default=240000, help="surrogate Adam steps")
     ap.add_argument("--probe", action="store_true",
                     help="only launch the dominant kernel (W-form backward step) 20 times and a "
                          "64 MiB copy kernel 20 times, for counter collection")
@@ -696,16 +696,17 @@ def bench_surrogate(H, M, ntrain, steps, log_fn):
            "members": 5, "train_draws": ntrain, "train_steps": steps}
     sol = S.PriceSolver(nb=64)
     t0 = time.perf_counter()
-    ytr = sol.solve(S.draw_params(ntrain, S.SEED_TRAIN))
+    ptr = S.draw_params(ntrain, S.SEED_TRAIN, S.TRAIN_MARGIN)
+    ytr = sol.solve(ptr)
     t_lab = time.perf_counter() - t0
     out["label_solves_per_s"] = ntrain / t_lab
     log_fn(f"[bench] surrogate labels: {ntrain} solves in {t_lab:.1f} s")
     pho = S.draw_params(4096, S.SEED_HELDOUT)
     yho = sol.solve(pho)
-    pcal = S.draw_params(2048, S.SEED_CAL)
+    pcal = S.draw_params(8192, S.SEED_CAL)
     ycal = sol.solve(pcal)
     net = S.Surrogate(members=5, cap=4096, seed=7)
-    net.fit(S.draw_params(ntrain, S.SEED_TRAIN), ytr, steps=steps, batch=4096, lr=3e-3,
+    net.fit(ptr, ytr, steps=steps, batch=4096, lr=3e-3,
             lr_end=1.5e-5)
     out["train_seconds"] = net.train_seconds
     log_fn(f"[bench] surrogate trained in {net.train_seconds:.1f} s")

@@ -36,6 +36,7 @@ typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef short s8 __attribute__((ext_vector_type(8)));
 
 constexpr int kIn = 5, kInP = 32, kHid = 256, kOut = 210, kOutP = 256;
+constexpr int kInLo = kIn;  // first input column of the parameters' BF16 remainders
 // padded master layout of one member (floats); weights W_l[o][i]
 constexpr int oW1 = 0;
 constexpr int ob1 = oW1 + kHid * kInP;
@@ -189,8 +190,9 @@ __global__ __launch_bounds__(256) void k_mlp_gemm(GemmOp op) {
     }
 }
 
-// normalized fp32 inputs -> bf16 X [b][32] (zero padded) and, for
-// training, X^T [64][B] (rows 5..63 zero).  idx (optional): member e's row
+// normalized fp32 inputs -> bf16 X [b][32] (the rounded inputs, their
+// rounded remainders in columns 5..9, zero padded) and, for training,
+// X^T [64][B] (the rounded inputs; rows 5..63 zero).  idx (optional): member e's row
 // b is source row idx[e*B + b]; without idx all members share row b.
 __global__ void k_mlp_inputs(int B, int E, const float *__restrict__ src, const int *__restrict__ idx,
                              uint16_t *__restrict__ X, long long sX, uint16_t *__restrict__ XT,
@@ -200,7 +202,13 @@ __global__ void k_mlp_inputs(int B, int E, const float *__restrict__ src, const 
     const int s = idx ? idx[(size_t)e * B + b] : b;
     uint16_t v[kInP];
 #pragma unroll
-    for (int i = 0; i < kInP; i++) v[i] = i < kIn ? f2bf(src[(size_t)s * kIn + i]) : 0;
+    for (int i = 0; i < kInP; i++) v[i] = 0;
+#pragma unroll
+    for (int i = 0; i < kIn; i++) {
+        const float x = src[(size_t)s * kIn + i];
+        v[i] = f2bf(x);
+        v[kInLo + i] = f2bf(x - bf2f(v[i]));  // exact difference, rounded once
+    }
     uint16_t *x = X + e * sX + (size_t)b * kInP;
 #pragma unroll
     for (int i = 0; i < kInP; i += 8) {
@@ -281,6 +289,13 @@ __global__ void k_mlp_wsum(int n, int ks, const float *__restrict__ part, long l
 
 // which layer a master index belongs to: weights -> (layer, o, i) and the
 // bf16 copies Wf[o][i] (forward operand) and Wb[i][o] (backward operand)
+// The first layer's input carries each parameter as a BF16 pair: column
+// i < 5 its rounded value, column 5 + i the rounded remainder (16
+// significant bits together; one BF16 input alone quantizes the box to
+// 2^-9 of its width).  Both columns take the same weight: the forward
+// copy Wf mirrors W1[o][i] into column 5 + i, whose master weight stays 0
+// (its gradient is 0: the transposed input keeps rows 5.. zero).
+
 __device__ inline void bf16_copies(int j, float w, uint16_t *Wf, uint16_t *Wb) {
     int base, in, out;
     if (j < ob1) { base = oW1; in = kInP; out = kHid; }
@@ -290,7 +305,9 @@ __device__ inline void bf16_copies(int j, float w, uint16_t *Wf, uint16_t *Wb) {
     else return;  // a bias
     const int l = j - base, o = l / in, i = l - o * in;
     const uint16_t h = f2bf(w);
+    if (base == oW1 && i >= kInLo && i < kInLo + kIn) return;  // written by column i - 5
     Wf[j] = h;
+    if (base == oW1 && i < kIn) Wf[j + kInLo] = h;
     Wb[base + (size_t)i * out + o] = h;
 }
 
@@ -390,9 +407,11 @@ __global__ void k_surr_base(int B, SurrGrid gr, const float *__restrict__ prm, f
 
 // one wave per draw: member outputs y_e (standardized relative residuals)
 // -> price_o = base_o + w_o (mu_o + sd_o mean_e(y_e)) with the residual
-// scale w_o = max(|base_o|, 1e-3 max|base|) (the error metric's
-// denominator, taken on the control variate); spread = max_o w_o sd_o
-// std_e(y_e) / max(|price_o|, 1e-3 max|price|); flag = spread > tol
+// scale w_o = sqrt(base_o^2 + (1e-3 max|base|)^2) (a smooth stand-in for
+// the error metric's denominator max(|p|, 1e-3 max|p|), taken on the
+// control variate: within a factor sqrt(2) of it, without its kink, which
+// the network cannot follow); spread = max_o w_o sd_o std_e(y_e) /
+// max(|price_o|, 1e-3 max|price|); flag = spread > tol
 __global__ __launch_bounds__(64) void k_surr_head(int B, int E, SurrGrid gr, const float *__restrict__ prm,
                                                   const float *__restrict__ Y, long long sY,
                                                   const float *__restrict__ mu, const float *__restrict__ sd,
@@ -430,7 +449,8 @@ __global__ __launch_bounds__(64) void k_surr_head(int B, int E, SurrGrid gr, con
             }
             const float mean = s / E;
             const float var = fmaxf(s2 / E - mean * mean, 0.0f);
-            const float w = fmaxf(fabsf(bs[u]), 1e-3f * bmax);
+            const float fl = 1e-3f * bmax;
+            const float w = sqrtf(bs[u] * bs[u] + fl * fl);
             pr[u] = bs[u] + w * (mu[o] + sd[o] * mean);
             sp[u] = w * sd[o] * sqrtf(var);
             pmax = fmaxf(pmax, fabsf(pr[u]));

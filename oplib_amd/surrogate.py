@@ -9,12 +9,14 @@ to the 10 x 21 grid of European call prices (maturities 0.1 .. 1.0 yr,
 strikes 80 .. 120 step 2, spot 100).  The networks learn the standardized
 relative residual of the operator solver's prices over a Black-Scholes
 control variate (the short-maturity implied vol of Berestycki-Busca-
-Florent), relative to the error metric's denominator evaluated on the
-control variate, so that the training loss weighs errors as the metric
-does; sigma0 enters on a log scale.  The
-control variate, the ensemble mean and the spread are evaluated on the GPU
+Florent), relative to a smooth stand-in for the error metric's
+denominator evaluated on the control variate, so that the training loss
+weighs errors as the metric does; sigma0 enters on a log scale, and each
+input reaches the first layer as a BF16 pair (value and remainder).  The
+training draws cover the box widened by 5% on each side.  The control
+variate, the ensemble mean and the spread are evaluated on the GPU
 (ophip_surr_eval).  Any draw whose relative ensemble spread exceeds the
-tolerance is priced by the operator solver instead.
+calibrated threshold is priced by the operator solver instead.
 
 Training labels come from the operator solver itself: the 1024-node
 lattice, U = I + W squared 20 times in FP16 with hi/lo split products
@@ -52,11 +54,27 @@ TOL = 1e-4
 SEED_TRAIN, SEED_HELDOUT, SEED_INFER, SEED_CAL = 1001, 2002, 3003, 4004
 
 
-def draw_params(n, seed):
+# training draws cover the box widened by this fraction of its width on
+# each side (sigma0 on its log scale), so that the box's corners lie inside
+# the training distribution
+TRAIN_MARGIN = 0.05
+
+
+def box(margin=0.0):
+    """the parameter box, widened by `margin` of its width on each side"""
+    lo = PARAM_LO - margin * (PARAM_HI - PARAM_LO)
+    hi = PARAM_HI + margin * (PARAM_HI - PARAM_LO)
+    ratio = PARAM_HI[0] / PARAM_LO[0]
+    lo[0], hi[0] = PARAM_LO[0] * ratio ** -margin, PARAM_HI[0] * ratio ** margin
+    return lo, hi
+
+
+def draw_params(n, seed, margin=0.0):
     """n draws, five MT19937(seed) uniforms x/2^32 each, in the order
-    sigma0, beta, r, a, b, mapped onto the box"""
+    sigma0, beta, r, a, b, mapped onto the box (widened by `margin`)"""
     u = M._mt_uniforms(seed, 5 * n).reshape(n, 5)
-    return PARAM_LO + (PARAM_HI - PARAM_LO) * u
+    lo, hi = box(margin)
+    return lo + (hi - lo) * u
 
 
 def config(p, precision=M.OPM_PREC_F16X3):
@@ -117,10 +135,13 @@ def normalize(params):
 
 
 def residual_scale(base):
-    """per-output residual scale: the error metric's denominator taken on
-    the control variate, max(|base|, 1e-3 max|base|) per draw"""
+    """per-output residual scale: sqrt(base^2 + (1e-3 max|base|)^2) per
+    draw, a smooth stand-in for the error metric's denominator
+    max(|base|, 1e-3 max|base|) on the control variate (within sqrt(2) of
+    it; the kink of the max is a feature the network cannot follow)"""
     base = np.asarray(base, dtype=np.float64)
-    return np.maximum(np.abs(base), 1e-3 * np.abs(base).max(axis=1, keepdims=True))
+    fl = 1e-3 * np.abs(base).max(axis=1, keepdims=True)
+    return np.sqrt(base * base + fl * fl)
 
 
 def control_variate(params):

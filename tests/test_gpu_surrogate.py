@@ -67,14 +67,22 @@ def unpack(flat):
 
 
 def emulate_forward(flat, xn):
-    """the kernels' arithmetic in numpy: returns y [B][256] and the cache"""
+    """the kernels' arithmetic in numpy: returns y [B][256] and the cache.
+    The first layer's input is each parameter as a BF16 pair (rounded value
+    in column i, rounded remainder in column 5 + i) under one weight: the
+    forward copy of W1 mirrors column i into column 5 + i; the weight
+    gradient sees the rounded values only (the cache's first input)"""
     B = xn.shape[0]
     a = np.zeros((B, 32), np.float32)
     a[:, :5] = bf16(xn)
-    cache = {"a": [a], "z": []}
+    cache = {"a": [a.copy()], "z": []}
+    a[:, 5:10] = bf16(np.asarray(xn, np.float32) - a[:, :5])
     L = unpack(flat)
     for l, (W, b, li, lo) in enumerate(L):
-        z = a.astype(np.float64) @ bf16(W).astype(np.float64).T + b
+        Wq = bf16(W).astype(np.float64)
+        if l == 0:
+            Wq[:, 5:10] = Wq[:, 0:5]
+        z = a.astype(np.float64) @ Wq.T + b
         if l < 3:
             cache["z"].append(z)
             a = bf16(gelu(z).astype(np.float32))
@@ -148,6 +156,18 @@ def test_forward_matches_emulation_and_oracle(S, hip):
         yo, _ = O.mlp_forward(Ws, bs, xn.astype(np.float64))
         rel = np.linalg.norm(y[e][:, :210] - yo) / np.linalg.norm(yo)
         assert rel < 3e-2, rel
+    # inputs closer than one BF16 step give different outputs (the remainder
+    # columns carry them), and the difference is the emulated one
+    xs = xn.copy()
+    xs[:, 0] = np.float32(0.75) + np.float32(2.0**-12) * np.arange(B, dtype=np.float32) / B
+    assert len(np.unique(bf16(xs[:, 0]))) == 1
+    Xs = hip.DeviceArray.from_numpy(xs, order="C")
+    hip.check(hip.lib.ophip_mlp_forward(net.net, B, Xs.h, Y.h), "forward")
+    ys = Y.download().reshape(E, B, 256)[0][:, :210]
+    ye, _ = emulate_forward(net.get(0, 0), xs)
+    assert np.abs(ys[-1] - ys[0]).max() > 0
+    ok, mx, fr = close_mostly(ys - ys[0], ye[:, :210] - ye[0, :210], np.abs(ye[:, :210] - ye[0, :210]).max())
+    assert ok, (mx, fr)
     net.close()
 
 
@@ -284,7 +304,12 @@ def test_eval_head_matches_members(S, hip):
     hip.check(hip.lib.ophip_mlp_forward(net.net, B, X.h, Y.h), "forward")
     y = Y.download().reshape(E, B, 256)[:, :, :210].astype(np.float64)
     base = O.surr_control_variate(prm, S.STRIKES, S.NMAT, S.DT, S.SPOT)
-    w = np.maximum(np.abs(base), 1e-3 * np.abs(base).max(axis=1, keepdims=True))
+    # the residual scale: sqrt(base^2 + (1e-3 max|base|)^2), the smooth
+    # stand-in for the metric's denominator
+    flo = 1e-3 * np.abs(base).max(axis=1, keepdims=True)
+    w = np.sqrt(base * base + flo * flo)
+    assert np.all(w >= np.maximum(np.abs(base), flo))
+    assert np.all(w <= np.sqrt(2) * np.maximum(np.abs(base), flo))
     mean = y.mean(0)
     ref = base + w * (net.mu[:210] + net.sd[:210] * mean)
     assert np.abs(pr - ref).max() <= 1e-4 * np.abs(ref).max()
