@@ -474,6 +474,9 @@ class WformBackward:
         self.df = arr(C.c_float, [0.9999] * self.n)
         self.X = [arr(C.c_uint, [v[p].h for v in self.V]) for p in (0, 1)]
         self.XT = [arr(C.c_uint, [v[p].h for v in self.VT]) for p in (0, 1)]
+        # the solves' steps carry the values as their hi/lo copies only (no
+        # FP32 values in or out): time the step the same way
+        self.none = arr(C.c_uint, [0] * self.n)
         self.t = 0
         # bytes per launch: dense algorithmic (the hi/lo copies of every W,
         # 4 d^2 each, and the live value columns read and written) and the
@@ -487,7 +490,7 @@ class WformBackward:
     def step(self):
         a, b = self.t & 1, (self.t + 1) & 1
         self.H.lib.ophip_wsq_bwd_step_batch(self.n, self.st, self.d, 1, self.buf, self.ncp, self.df,
-                                            self.X[a], self.XT[a], self.X[b], self.XT[b])
+                                            self.none, self.XT[a], self.none, self.XT[b])
         self.t += 1
 
 

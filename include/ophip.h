@@ -222,8 +222,11 @@ int           ophip_w8_exec(unsigned state, int d, int nsq);
  * fragment-major layout kept in the other buffer; X/Y are d x ncp
  * row-major FP32 (ncp 16, 32 or 48) with 16-bit hi/lo copies XT/YT
  * (2 * ncp * d values, fragment-major); ophip_wsq_bwd_prep writes XT from
- * X.  The reference's step is the FP32 SGEMV of U (sgemv.cu:44-51, driven
- * per step by CModel Device.cs). */
+ * X.  A step given X = 0 takes X from XT (hi + lo 2^-G: 22 significant bits
+ * in FP16, 16 in BF16); given Y = 0 it writes YT only, so a chain of steps
+ * moves the FP32 values only where they are wanted (the last step).  The
+ * reference's step is the FP32 SGEMV of U (sgemv.cu:44-51, driven per step
+ * by CModel Device.cs). */
 int ophip_wsq_bwd_layout(unsigned state, int d, int mode, int buf);
 /* the same with the states reordered: fragment state i is the copies'
  * state perm[i] (perm: handle of d ints, a permutation), e.g. a lattice

@@ -403,10 +403,12 @@ static int solve_body(opm_solver *s) {
             if (st) return st;
             st = ophip_wsq_bwd_prep(d, s->ncp, s->wmode, s->V[0].h, s->VT[0].h);
             if (st) return st;
+            // the steps carry the values as their FP16 hi/lo copies; the
+            // last step also writes the FP32 values
             for (int t = 0; t < c->nsteps; t++) {
                 const int a = t & 1, b = (t + 1) & 1;
-                st = ophip_wsq_bwd_step(s->wstate.h, d, s->wmode, s->u_final, s->ncp, df, s->V[a].h,
-                                        s->VT[a].h, s->V[b].h, s->VT[b].h);
+                st = ophip_wsq_bwd_step(s->wstate.h, d, s->wmode, s->u_final, s->ncp, df, 0u, s->VT[a].h,
+                                        t == c->nsteps - 1 ? s->V[b].h : 0u, s->VT[b].h);
                 if (st) return st;
             }
         } else {
@@ -708,11 +710,14 @@ static int group_body(opm_group *g) {
                 if (st) return st;
             }
         }
+        // the steps carry the values as their FP16 hi/lo copies (22
+        // significant bits); the last step also writes the FP32 values
+        const std::vector<unsigned> none(n, 0u);
         for (int t = 0; t < nsteps; t++) {
             const int p = t & 1, q = p ^ 1;
-            st = ophip_wsq_bwd_step_batch(n, states.data(), d, wmode, buf, a->ncp, df.data(),
-                                          t ? X[p].data() : P0.data(), t ? XT[p].data() : PT0.data(),
-                                          X[q].data(), XT[q].data());
+            st = ophip_wsq_bwd_step_batch(n, states.data(), d, wmode, buf, a->ncp, df.data(), none.data(),
+                                          t ? XT[p].data() : PT0.data(),
+                                          t == nsteps - 1 ? X[q].data() : none.data(), XT[q].data());
             if (st) return st;
         }
         if (nsteps == 0)

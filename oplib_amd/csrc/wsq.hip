@@ -644,6 +644,17 @@ __device__ inline size_t frag_off(int it, int kt, int l, int nkt) {
     return ((size_t)(it * nkt + kt) * 64 + l) * 8;
 }
 
+// value X[row][col] from its fragment-major hi/lo copies (column tile col /
+// 16, k tile row / 32, lane 16 (row % 32 / 8) + col % 16, element row % 8;
+// lo copies n elements on): hi + lo 2^-G in one FP32 rounding, the split's
+// own reconstruction (FP16: 22 significant bits, BF16: 16)
+template <class N>
+__device__ inline float x_from_copies(const uint16_t *xt, size_t n, int row, int col, int nkt) {
+    const int kk = row & 31;
+    const size_t o = frag_off(col >> 4, row >> 5, 16 * (kk >> 3) + (col & 15), nkt) + (kk & 7);
+    return N::dec(xt[o]) + N::dec(xt[o + n]) * N::IG;
+}
+
 // row-major H (and L) of a d x d matrix -> fragment-major; one thread per
 // 16-byte fragment
 template <bool LO>
@@ -844,11 +855,12 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd1(int d, float df, const uint16
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             const size_t o = (size_t)(row0 + 4 * g + e) * NC + col;
-            float y = df * (X[o] + sum[e]);
+            const float xv = X ? X[o] : x_from_copies<N>(XhT, (size_t)(XlT - XhT), row0 + 4 * g + e, col, nkt);
+            float y = df * (xv + sum[e]);
             // the copies split the rounded FP32 y (no fused multiply-convert,
             // whose single rounding differs from y's at FP16 ties)
             asm volatile("" : "+v"(y));
-            Y[o] = y;
+            if (Y) Y[o] = y;
             const uint16_t h = N::enc(y);
             th[col][4 * g + e] = h;
             tl[col][4 * g + e] = N::enc((y - N::dec(h)) * N::G);
@@ -977,7 +989,7 @@ __global__ __launch_bounds__(256) void k_wbwd_lists(int d, char *__restrict__ ba
 // one lattice of a batched backward step (arena offsets)
 struct BwdScen {
     uint64_t Hf, Lf, tl;    // W's fragment-major hi/lo copies, their tile lists
-    uint64_t X, XT, Y, YT;  // values and copies in, values and copies out
+    uint64_t X, XT, Y, YT;  // values and copies in, values and copies out (X 0: from XT; Y 0: not written)
     float df;
     int pad;
 };
@@ -998,6 +1010,7 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd_sp(char *__restrict__ base, co
     using N = Num<BF>;
     __shared__ f4 red[NW][64];
     __shared__ uint16_t th[16][16], tlo[16][16];
+    __shared__ __attribute__((aligned(16))) uint16_t xs[2][32][8];  // own rows' value copies (X = 0)
     const BwdScen &S = bt.s[blockIdx.y];
     const int t = threadIdx.x, lane = t & 63;
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -1013,6 +1026,16 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd_sp(char *__restrict__ base, co
     const uint16_t *la = at<const uint16_t>(base, S.Lf) + frag_off(it, 0, lane, nkt);
     const uint16_t *xh = at<const uint16_t>(base, S.XT) + frag_off(j, 0, lane, nkt);
     const uint16_t *xl = xh + n;
+    // without FP32 values: this block's rows of column tile j are groups
+    // g0, g0 + 1 of k tile row0 / 32, 32 fragment lanes per copy; one
+    // 16-byte load per thread of the first 64, stored after the K loop
+    const int g0 = (row0 % 32) / 8;
+    uint4 xstage = {0u, 0u, 0u, 0u};
+    if (!S.X && t < 64) {
+        const int arr = t >> 5, l = 16 * g0 + (t & 31);
+        xstage = *(const uint4 *)(at<const uint16_t>(base, S.XT) + (arr ? n : 0) +
+                                  frag_off(j, row0 / 32, l, nkt));
+    }
     f4 acc = {0.f, 0.f, 0.f, 0.f}, cor = {0.f, 0.f, 0.f, 0.f};
     for (int e0 = wave; e0 < cnt; e0 += NW * E) {
         s8 a[E][4];
@@ -1035,20 +1058,24 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd_sp(char *__restrict__ base, co
         }
     }
     red[wave][lane] = (acc + cor * N::IG) * N::IS;
+    if (!S.X && t < 64) *(uint4 *)&xs[t >> 5][t & 31][0] = xstage;
     __syncthreads();
     if (wave == 0) {
         f4 sum = red[0][lane];
 #pragma unroll
         for (int w = 1; w < NW; w++) sum += red[w][lane];
-        const float *X = at<const float>(base, S.X);
-        float *Y = at<float>(base, S.Y);
+        const float *X = S.X ? at<const float>(base, S.X) : nullptr;
+        float *Y = S.Y ? at<float>(base, S.Y) : nullptr;
         const int col = 16 * j + r;
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             const size_t o = (size_t)(row0 + 4 * g + e) * ncp + col;
-            float y = S.df * (X[o] + sum[e]);
+            // row 4g + e of the block: group g / 2 past g0, element 4 (g & 1) + e
+            const int fl = 16 * (g >> 1) + r, fe = 4 * (g & 1) + e;
+            const float xv = X ? X[o] : N::dec(xs[0][fl][fe]) + N::dec(xs[1][fl][fe]) * N::IG;
+            float y = S.df * (xv + sum[e]);
             asm volatile("" : "+v"(y));  // the copies split the rounded y
-            Y[o] = y;
+            if (Y) Y[o] = y;
             const uint16_t h = N::enc(y);
             th[r][4 * g + e] = h;
             tlo[r][4 * g + e] = N::enc((y - N::dec(h)) * N::G);
@@ -1085,6 +1112,8 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd_blk(char *__restrict__ base, c
     static_assert(RB == 2 || RB == 4, "row blocks of 32 or 64 rows");
     __shared__ f4 red[NW][NTILE][64];
     __shared__ uint16_t th[NC][ROWS], tlo[NC][ROWS];
+    constexpr int NKB = ROWS / 32;  // k tiles of this block's rows
+    __shared__ __attribute__((aligned(16))) uint16_t xs[2][NKB][NT][64][8];  // own rows' value copies (X = 0)
     const int nrb = d / ROWS;
     const int v = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (v >= nb * nrb) return;  // whole workgroup, before any barrier
@@ -1104,15 +1133,33 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd_blk(char *__restrict__ base, c
     const uint16_t *xl = xh + n;
     // the epilogue's X values (tiles wave, wave + NW, ..) loaded up front
     constexpr int TPW = (NTILE + NW - 1) / NW;
-    const float *X = at<const float>(base, S.X);
+    const float *X = S.X ? at<const float>(base, S.X) : nullptr;
     float xv[TPW][4];
+    // without FP32 values: the block's rows are k tiles row0 / 32 ..; their
+    // fragments (2 copies x NKB x NT x 64 lanes of 16 bytes) are loaded now,
+    // one per thread, and go to LDS after the K loop
+    constexpr int NXF = 2 * NKB * NT * 64, XPT = (NXF + 64 * NW - 1) / (64 * NW);
+    uint4 xstage[XPT];
+    if (X) {
 #pragma unroll
-    for (int q = 0; q < TPW; q++) {
-        const int tile = wave + q * NW;
-        if (tile < NTILE) {  // uniform
-            const int i = tile / NT, j = tile % NT;
+        for (int q = 0; q < TPW; q++) {
+            const int tile = wave + q * NW;
+            if (tile < NTILE) {  // uniform
+                const int i = tile / NT, j = tile % NT;
 #pragma unroll
-            for (int e = 0; e < 4; e++) xv[q][e] = X[(size_t)(row0 + 16 * i + 4 * g + e) * NC + 16 * j + r];
+                for (int e = 0; e < 4; e++) xv[q][e] = X[(size_t)(row0 + 16 * i + 4 * g + e) * NC + 16 * j + r];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < XPT; u++) {
+            const int f = t + 64 * NW * u;
+            xstage[u] = uint4{0u, 0u, 0u, 0u};
+            if (f < NXF) {
+                const int l = f & 63, j = (f >> 6) % NT, kb = (f >> 6) / NT % NKB, arr = (f >> 6) / (NT * NKB);
+                xstage[u] = *(const uint4 *)(at<const uint16_t>(base, S.XT) + (arr ? n : 0) +
+                                             frag_off(j, row0 / 32 + kb, l, nkt));
+            }
         }
     }
     f4 acc[RB][NT], cor[RB][NT];
@@ -1159,8 +1206,32 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd_blk(char *__restrict__ base, c
     for (int i = 0; i < RB; i++)
 #pragma unroll
         for (int j = 0; j < NT; j++) red[wave][i * NT + j][lane] = (acc[i][j] + cor[i][j] * N::IG) * N::IS;
+    if (!X) {
+#pragma unroll
+        for (int u = 0; u < XPT; u++) {
+            const int f = t + 64 * NW * u;
+            if (f < NXF) {
+                const int l = f & 63, j = (f >> 6) % NT, kb = (f >> 6) / NT % NKB, arr = (f >> 6) / (NT * NKB);
+                *(uint4 *)&xs[arr][kb][j][l][0] = xstage[u];
+            }
+        }
+    }
     __syncthreads();
-    float *Y = at<float>(base, S.Y);
+    if (!X)
+#pragma unroll
+        for (int q = 0; q < TPW; q++) {
+            const int tile = wave + q * NW;
+            if (tile < NTILE) {  // uniform
+                const int i = tile / NT, j = tile % NT;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    // local row 16 i + 4 g + e: k tile kb, group kg, element ke
+                    const int kk = 16 * i + 4 * g + e, kb = kk >> 5, fl = 16 * ((kk & 31) >> 3) + r, ke = kk & 7;
+                    xv[q][e] = N::dec(xs[0][kb][j][fl][ke]) + N::dec(xs[1][kb][j][fl][ke]) * N::IG;
+                }
+            }
+        }
+    float *Y = S.Y ? at<float>(base, S.Y) : nullptr;
 #pragma unroll
     for (int q = 0; q < TPW; q++) {
         const int tile = wave + q * NW;
@@ -1176,7 +1247,7 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd_blk(char *__restrict__ base, c
             const size_t o = (size_t)(row0 + row) * NC + col;
             float y = S.df * (xv[q][e] + sum[e]);
             asm volatile("" : "+v"(y));  // the copies split the rounded y
-            Y[o] = y;
+            if (Y) Y[o] = y;
             const uint16_t h = N::enc(y);
             th[col][row] = h;
             tlo[col][row] = N::enc((y - N::dec(h)) * N::G);
@@ -1919,8 +1990,8 @@ OPHIP_EXPORT int ophip_wsq_bwd_step_batch(int nb, const unsigned *states, int d,
     const WsqLayout L = wsq_layout(d, mode);
     const size_t n = (size_t)d * ncp;
     for (int i = 0; i < nb; i++)
-        if (!handle_range_ok(states[i], L.total) || !handle_range_ok(X[i], 4 * n) ||
-            !handle_range_ok(XT[i], 4 * n) || !handle_range_ok(Y[i], 4 * n) ||
+        if (!handle_range_ok(states[i], L.total) || (X[i] && !handle_range_ok(X[i], 4 * n)) ||
+            !handle_range_ok(XT[i], 4 * n) || (Y[i] && !handle_range_ok(Y[i], 4 * n)) ||
             !handle_range_ok(YT[i], 4 * n))
             return (int)set_status(kErrBadHandle);
     if (g_bwd_kernel == 0) {  // the dense single-lattice kernel, one launch each
@@ -2022,8 +2093,8 @@ OPHIP_EXPORT int ophip_wsq_bwd_step(unsigned state, int d, int mode, int buf, in
     if (!bwd_args_ok(d, ncp, mode) || buf < 0 || buf > 1) return (int)set_status(kErrBadArg);
     const WsqLayout L = wsq_layout(d, mode);
     const size_t n = (size_t)d * ncp;
-    if (!handle_range_ok(state, L.total) || !handle_range_ok(X, 4 * n) || !handle_range_ok(XT, 4 * n) ||
-        !handle_range_ok(Y, 4 * n) || !handle_range_ok(YT, 4 * n))
+    if (!handle_range_ok(state, L.total) || (X && !handle_range_ok(X, 4 * n)) || !handle_range_ok(XT, 4 * n) ||
+        (Y && !handle_range_ok(Y, 4 * n)) || !handle_range_ok(YT, 4 * n))
         return (int)set_status(kErrBadHandle);
     if (g_bwd_kernel) return ophip_wsq_bwd_step_batch(1, &state, d, mode, buf, ncp, &df, &X, &XT, &Y, &YT);
     Device &D = dev();

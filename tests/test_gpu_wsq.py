@@ -455,6 +455,50 @@ def test_wform_backward_tile_lists(hip, nsq, mode, cut):
         assert frac < 0.5
 
 
+@pytest.mark.parametrize("kernel", sorted(BWD_KERNELS))
+@pytest.mark.parametrize("mode", [1, 2])
+def test_wform_backward_values_from_copies(hip, mode, kernel):
+    """a step given X = 0 takes X from its copies: the same bits as a step
+    given X = hi + lo 2^-G (the copies' reconstruction, exact in FP32) as
+    FP32 values; given Y = 0 it writes the same copies YT and leaves Y's
+    buffer alone"""
+    banner(f"wsq backward step from the copies mode={mode} kernel={BWD_KERNELS[kernel]}")
+    H = hip
+    old = H.lib.ophip_wsq_set_bwd_kernel(kernel)
+    try:
+        L = backward_lattice(H, 3, mode)
+        d, ncp, buf = L.d, 48, 1
+        H.check(H.lib.ophip_wsq_bwd_layout(L.state.h, d, mode, buf))
+        S, G = (1.0, G_BF) if mode == 2 else (S16, G16)
+        X = np.random.RandomState(4).uniform(0.0, 300.0, size=(d, ncp)).astype(np.float32)
+        Xd = H.DeviceArray.from_numpy(X, order="C")
+        XT = H.DeviceArray.empty(2 * ncp * d, dtype=np.uint16)
+        H.check(H.lib.ophip_wsq_bwd_prep(d, ncp, mode, Xd.h, XT.h))
+        xt = XT.download().reshape(2, -1)
+        xh, xl = unfrag(xt[0], ncp, d), unfrag(xt[1], ncp, d)  # [col][row]
+        xrec = (dec16(xh, mode) + dec16(xl, mode) * np.float32(1.0 / G)).T.astype(np.float32)
+        Xr = H.DeviceArray.from_numpy(np.ascontiguousarray(xrec), order="C")
+        df = np.float32(0.9999)
+        outs = []
+        for xh_ in (Xr.h, 0):
+            Yd = H.DeviceArray.empty(d * ncp)
+            YT = H.DeviceArray.empty(2 * ncp * d, dtype=np.uint16)
+            H.check(H.lib.ophip_wsq_bwd_step(L.state.h, d, mode, buf, ncp, df, xh_, XT.h, Yd.h, YT.h))
+            H.sync()
+            outs.append((Yd.download(), YT.download()))
+        assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+        # Y = 0: the copies only
+        sentinel = np.full(d * ncp, 7.0, np.float32)
+        Yd = H.DeviceArray.from_numpy(sentinel)
+        YT = H.DeviceArray.empty(2 * ncp * d, dtype=np.uint16)
+        H.check(H.lib.ophip_wsq_bwd_step(L.state.h, d, mode, buf, ncp, df, 0, XT.h, 0, YT.h))
+        H.sync()
+        assert np.array_equal(YT.download(), outs[1][1])
+        assert np.array_equal(Yd.download(), sentinel)
+    finally:
+        H.lib.ophip_wsq_set_bwd_kernel(old)
+
+
 @pytest.mark.parametrize("kernel", [2, 3, 4])
 def test_wform_backward_batch_equals_single(hip, kernel):
     """one batched launch over three lattices (different squaring counts,
