@@ -1386,12 +1386,15 @@ struct WsqPlan {
 // launch runs on XCD b % 8 and the persistent grid is a multiple of 8, so
 // item j runs on XCD j % 8.  Tiles that share operand panels should share
 // an XCD (its L2), and the work of a banded W (most of it near the
-// diagonal while zero cells are skipped) should spread evenly: a lattice
-// with at least 8 tile rows is cut into 8 blocks of consecutive tile rows,
-// one per XCD (each holds its share of the band; its A panels stay in its
-// L2); smaller lattices go whole to the least-loaded XCD.  The per-XCD
-// lists are then balanced (moving items off the longest lists), sorted
-// longest item first and interleaved.
+// diagonal while zero cells are skipped) should spread evenly.  With 8
+// lattices or more, each lattice goes whole to the least-loaded XCD: its A
+// and B panels are then fetched into one L2 only (16 lattices of d = 2048:
+// the group's squarings take 7.33 instead of 7.68 ms per solve).  With
+// fewer, a lattice with at least 8 tile rows is cut into 8 blocks of
+// consecutive tile rows, one per XCD (each holds its share of the band;
+// its A panels stay in its L2), and smaller ones go whole to the
+// least-loaded XCD.  The per-XCD lists are then balanced (moving items off
+// the longest lists), sorted longest item first and interleaved.
 static std::vector<WItem> xcd_order(const std::vector<WItem> &v, const int *ds, int n, int T) {
     constexpr int X = 8;
     if (v.size() < 2 * X) return v;
@@ -1402,7 +1405,7 @@ static std::vector<WItem> xcd_order(const std::vector<WItem> &v, const int *ds, 
     for (int i = 0; i < n; i++) {
         if (by_lat[i].empty()) continue;
         const int nt = ds[i] / T;
-        if (nt >= X) {
+        if (nt >= X && n < X) {
             for (const WItem &w : by_lat[i]) {
                 const int x = w.tm * X / nt;
                 per[x].push_back(w);
