@@ -145,9 +145,9 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="skip the calibration-sweep leg")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate leg")
     ap.add_argument("--no-fp8", action="store_true", help="skip the FP8 W-form method")
-    ap.add_argument("--groups", type=int, nargs="*", default=[16],
+    ap.add_argument("--groups", type=int, nargs="*", default=[16, 32],
                     help="group sizes of the FP16 x3 scenario-group method")
-    ap.add_argument("--probe-group", type=int, default=16,
+    ap.add_argument("--probe-group", type=int, default=32,
                     help="lattices per backward-step launch in --probe")
     ap.add_argument("--surr-train", type=int, default=262144, help="surrogate training draws")
     ap.add_argument("--surr-steps", type=int, default=240000, help="surrogate Adam steps")

@@ -110,6 +110,10 @@ int  opcuda_mt_benchmark(unsigned mt, unsigned unif, int nscen);       /* smc.cu
 
 /* ==== MI355X additions (no reference counterpart unless cited) ==== */
 
+/* sets the calling thread's status, read by opcuda_get_status (a layer
+ * above the library reports its own failures through it); returns it */
+unsigned ophip_set_status(unsigned status);
+
 /* raw device address of a handle; arena size; library stream */
 unsigned long ophip_device_address(unsigned h);
 unsigned long ophip_arena_bytes(void);

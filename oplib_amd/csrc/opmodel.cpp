@@ -41,6 +41,12 @@ void release(Buf &b) {
 int upload(Buf &b, const void *p, size_t bytes) {
     return opcuda_memcpy_h2d(b.h, (unsigned long)(uintptr_t)p, (unsigned)bytes);
 }
+// a null result whose cause stays in the library's status: err (the status
+// a failing call left, read before any cleanup), or BAD_ARG
+template <class T> T *failed(unsigned err) {
+    ophip_set_status(err ? err : OPHIP_ERR_BAD_ARG);
+    return nullptr;
+}
 
 void make_grid(const opm_config *c, std::vector<double> &x, int &x0) {
     const int nx = c->nx;
@@ -227,8 +233,8 @@ int opm_sv_params(int nx, int nr, const double *xval, const double *rval, int y0
 }
 
 opm_solver *opm_create(const opm_config *cfg) {
-    if (!valid(cfg)) return nullptr;
-    if (opcuda_cublas_init() != 0) return nullptr;
+    if (!valid(cfg)) return failed<opm_solver>(OPHIP_ERR_BAD_ARG);
+    if (opcuda_cublas_init() != 0) return failed<opm_solver>(opcuda_get_status());
     opm_solver *s = new opm_solver();
     s->cfg = *cfg;
     const opm_config *c = &s->cfg;
@@ -330,8 +336,9 @@ opm_solver *opm_create(const opm_config *cfg) {
         }
     }
     if (!ok) {
+        const unsigned err = opcuda_get_status();
         opm_destroy(s);
-        return nullptr;
+        return failed<opm_solver>(err);
     }
     return s;
 }
@@ -588,17 +595,17 @@ static int group_body(opm_group *g);
 extern "C" {
 
 opm_group *opm_group_create(opm_solver *const *solvers, int n) {
-    if (!solvers || n <= 0) return nullptr;
+    if (!solvers || n <= 0) return failed<opm_group>(OPHIP_ERR_BAD_ARG);
     const opm_solver *a = solvers[0];
-    if (!a) return nullptr;
+    if (!a) return failed<opm_group>(OPHIP_ERR_BAD_ARG);
     for (int i = 0; i < n; i++) {
         const opm_solver *b = solvers[i];
         if (!b || b->wmode < 0 || b->wmode > 2 || b->cfg.price_kind != OPM_PRICE_BACKWARD ||
             b->d != a->d || b->wmode != a->wmode || b->cfg.nsq != a->cfg.nsq ||
             b->cfg.nsteps != a->cfg.nsteps || b->ncp != a->ncp || b->v_ref != a->v_ref || b->nm != a->nm)
-            return nullptr;
+            return failed<opm_group>(OPHIP_ERR_BAD_ARG);
         for (int k = 0; k < i; k++)
-            if (solvers[k] == b) return nullptr;  // each solver once
+            if (solvers[k] == b) return failed<opm_group>(OPHIP_ERR_BAD_ARG);  // each solver once
     }
     opm_group *g = new opm_group();
     g->s.assign(solvers, solvers + n);
@@ -633,8 +640,9 @@ opm_group *opm_group_create(opm_solver *const *solvers, int n) {
     }
     g->wplan = keep;
     if (!g->wplan) {
+        const unsigned err = opcuda_get_status();
         delete g;
-        return nullptr;
+        return failed<opm_group>(err);
     }
     return g;
 }
@@ -833,17 +841,17 @@ static bool batch_pick_plan(opm_batch *b) {
 }
 
 opm_batch *opm_batch_create(const opm_config *cfgs, int n, int tile) {
-    if (!cfgs || n <= 0) return nullptr;
-    if (opcuda_cublas_init() != 0) return nullptr;
+    if (!cfgs || n <= 0) return failed<opm_batch>(OPHIP_ERR_BAD_ARG);
+    if (opcuda_cublas_init() != 0) return failed<opm_batch>(opcuda_get_status());
     const opm_config &c0 = cfgs[0];
     for (int i = 0; i < n; i++) {
         const opm_config &c = cfgs[i];
         if (!valid(&c) || c.nr != 1 || c.precision != c0.precision ||
             c.nsteps != c0.nsteps || c.DeltaT != c0.DeltaT || c.nstrikes != c0.nstrikes ||
             std::memcmp(c.strikes, c0.strikes, sizeof(double) * c.nstrikes) != 0)
-            return nullptr;
-        if (c.precision != OPM_PREC_FP32 && c.nx % 128) return nullptr;
-        if (c.precision == OPM_PREC_F8X3) return nullptr;  // FP8: single lattices only
+            return failed<opm_batch>(OPHIP_ERR_BAD_ARG);
+        if (c.precision != OPM_PREC_FP32 && c.nx % 128) return failed<opm_batch>(OPHIP_ERR_BAD_ARG);
+        if (c.precision == OPM_PREC_F8X3) return failed<opm_batch>(OPHIP_ERR_BAD_ARG);  // single lattices only
     }
     opm_batch *b = new opm_batch();
     b->n = n;
@@ -950,8 +958,9 @@ opm_batch *opm_batch_create(const opm_config *cfgs, int n, int tile) {
     }
     if (ok && b->wmode >= 0 && tile == 0) ok = batch_pick_plan(b);
     if (!ok) {
+        const unsigned err = opcuda_get_status();
         opm_batch_destroy(b);
-        return nullptr;
+        return failed<opm_batch>(err);
     }
     return b;
 }
@@ -1098,14 +1107,16 @@ extern "C" {
 
 opm_mcplan *opm_mcplan_create(int nk, const int *t_k, int ni, const int *t_i, int dt0_days,
                               const double *maxdiag) {
-    if (nk <= 0 || ni <= 0 || !t_k || !t_i || !maxdiag || dt0_days <= 0) return nullptr;
+    if (nk <= 0 || ni <= 0 || !t_k || !t_i || !maxdiag || dt0_days <= 0)
+        return failed<opm_mcplan>(OPHIP_ERR_BAD_ARG);
     for (int k = 0; k < nk; k++)
-        if (t_k[k] <= 0 || (k > 0 && t_k[k] <= t_k[k - 1])) return nullptr;
+        if (t_k[k] <= 0 || (k > 0 && t_k[k] <= t_k[k - 1])) return failed<opm_mcplan>(OPHIP_ERR_BAD_ARG);
     for (int i = 0; i < ni; i++)
-        if (t_i[i] <= 0 || (i > 0 && t_i[i] <= t_i[i - 1]) || !(maxdiag[i] >= 0.0)) return nullptr;
+        if (t_i[i] <= 0 || (i > 0 && t_i[i] <= t_i[i - 1]) || !(maxdiag[i] >= 0.0))
+            return failed<opm_mcplan>(OPHIP_ERR_BAD_ARG);
     // the last step must lie inside the generators' span (the reference
     // would read a generator past its table)
-    if (t_k[nk - 1] > t_i[ni - 1]) return nullptr;
+    if (t_k[nk - 1] > t_i[ni - 1]) return failed<opm_mcplan>(OPHIP_ERR_BAD_ARG);
     opm_mcplan *p = new opm_mcplan();
     p->nk = nk;
     p->ni = ni;

@@ -251,6 +251,10 @@ OPHIP_EXPORT int opcuda_max_threads_per_block(int id) {
 
 OPHIP_EXPORT unsigned opcuda_get_status(void) { return t_status; }
 
+// sets the calling thread's status (a layer above the library reports its
+// own failures through it, e.g. after cleanup calls that succeeded)
+OPHIP_EXPORT unsigned ophip_set_status(unsigned s) { return set_status(s); }
+
 // ---------------------------------------------------------------------
 // memory (util.cu:182-236)
 // ---------------------------------------------------------------------

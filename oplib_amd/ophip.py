@@ -68,6 +68,7 @@ SIGNATURES = {
     "opcuda_mc_setseed": (V, [ULONG, U32]),
     "opcuda_mc1f": (I32, [U32, I32, U32, I32, I32, I32, U32, U32, ULONG]),
     "opcuda_mt_benchmark": (I32, [U32, U32, I32]),
+    "ophip_set_status": (U32, [U32]),
     "ophip_device_address": (ULONG, [U32]),
     "ophip_arena_bytes": (ULONG, []),
     "ophip_stream": (ULONG, []),

@@ -438,10 +438,10 @@ class Group:
         arr = (C.c_void_p * len(self.solvers))(*[s.p for s in self.solvers])
         self.p = lib.opm_group_create(arr, len(self.solvers))
         if not self.p:
+            err = ophip.lib.opcuda_get_status()  # before the solvers' cleanup resets it
             for s in self.solvers:
                 s.close()
-            raise ophip.OphipError(
-                f"opm_group_create failed: 0x{ophip.lib.opcuda_get_status():x}")
+            raise ophip.OphipError(f"opm_group_create failed: 0x{err:x}")
 
     def solve(self):
         ophip.check(lib.opm_group_solve(self.p), "opm_group_solve")

@@ -340,3 +340,20 @@ def test_mcplan_exec_vs_fp64(M):
         print(f"[plan] m={m} chain={list(map(int, ch))} max |U_m - FP64| = {err:.2e}", flush=True)
         assert err < 2e-5, (m, err)
     p.close()
+
+
+def test_create_failures_keep_their_status(M):
+    """a refused configuration, and a refused group, leave the reason in the
+    library's status (not the status of the cleanup after it)"""
+    banner("model-layer failures keep their status")
+    from oplib_amd import ophip as H
+    c = M.config_small(nx=64, nr=2)
+    c.nstrikes = 0
+    with pytest.raises(H.OphipError, match="0x10003"):
+        M.Solver(c)
+    a = M.config_small(nx=128, nr=2)
+    a.precision = M.OPM_PREC_F16X3
+    b = M.config_small(nx=128, nr=2, nsteps=9)
+    b.precision = M.OPM_PREC_F16X3
+    with pytest.raises(H.OphipError, match="0x10003"):
+        M.Group([a, b])  # different step counts

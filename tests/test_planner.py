@@ -71,5 +71,8 @@ def test_first_step_length_dedup(M):
     ([7], [30, 20]),         # generator dates not increasing
 ])
 def test_invalid_dates(M, t_k, t_i):
+    from oplib_amd import ophip
+    ophip.lib.ophip_set_status(0)
     with pytest.raises(ValueError):
         M.McPlan(t_k, t_i, 1, [1.0] * len(t_i))
+    assert ophip.lib.opcuda_get_status() == 0x10003  # OPHIP_ERR_BAD_ARG, kept for the caller
