@@ -193,7 +193,8 @@ unsigned      ophip_wsq_w(unsigned state, int d, int mode, int buf);
  * row sums, 6 cell flags (d/64 x d/64 bytes; 0 = every FP32 W and
  * row-major copy value of that 64 x 64 cell is +-0, and so is every
  * column-major copy value unless a backward layout has since written its
- * fragment copies into that buffer), 7 backward-step tile lists (d/16 + d/64 +
+ * fragment copies into that buffer; otherwise 96 + E with the cell's
+ * largest |W| below 2^E, or 255 where no bound is known), 7 backward-step tile lists (d/16 + d/64 +
  * d/32 lists of d/32 + 1 ints, per 16-row tile, 64-row block and 32-row
  * block: count, then the visited k tiles, kt + 65536 where the lo part is
  * read too, + 2^(17+q) in a block of nr row tiles where its tile q visits
@@ -203,8 +204,15 @@ unsigned      ophip_wsq_array(unsigned state, int d, int mode, int buf, int whic
 /* Zero-cell skipping: the squaring visits only the K cells whose A and B
  * cells are flagged nonzero (the skipped products are exact zeros: same
  * bits).  1 = on (default), 0 = every K-step; returns the previous value.
- * No reference counterpart (the reference's ssqmm.cu:119-170 is dense). */
+ * With skipping on, ophip_wsq_set_kcut(k) also leaves out the K cells whose
+ * operand cells' bounds (the flags above) multiply to at most 2^k: such a
+ * product moves no entry of W by more than 64 2^k (default k = -36; -192 or
+ * below keeps every cell that may be nonzero, the dense loop's bits);
+ * returns the previous k and applies to rounds launched or captured
+ * afterwards.  No reference counterpart (the reference's ssqmm.cu:119-170
+ * is dense). */
 int           ophip_wsq_set_skip(int on);
+int           ophip_wsq_set_kcut(int log2_bound);
 unsigned long ophip_wsq_plan(int n, const unsigned *states, const int *ds, const int *nsqs,
                              int mode, int tile, int grid);
 int           ophip_wsq_exec(unsigned long plan, int nsq);

@@ -67,12 +67,20 @@ struct WScen {
 
 // Cell flags.  The 16-bit copies are cut into 64 x 64 cells; flag byte
 // [cr][cc] (d/64 x d/64, row-major) is 0 only if every H (and L) value of
-// cell (rows 64 cr.., columns 64 cc..) is +-0.  FP16 H and L flush entries
+// cell (rows 64 cr.., columns 64 cc..) is +-0, else 96 + E with the cell's
+// largest |W| below 2^E (255: no bound known).  FP16 H and L flush entries
 // |W| < 2^-51 to zero, and a banded generator keeps most cells there for
 // many squarings, so the product's K loop visits only the cells k with
 // A(rows, k) and B(k, columns) both flagged: the skipped MFMAs would add
 // exact zeros, the result is the same bits.  Nonzero flags are only a "may
-// be nonzero" (the diagonal fix sets its cells' flags unconditionally).
+// be nonzero" (the diagonal fix flags its cells 255 unconditionally).
+// Beyond the zero cells, a K cell whose two operand bounds multiply to at
+// most 2^kcut (ophip_wsq_set_kcut; default 2^-36) is left out too: its
+// product changes no entry of W by more than 64 2^kcut.  On the 2048-state
+// daily workload that leaves out about a quarter of the K-cell products of
+// the 16 squarings (half of the last rounds') and moves the prices by under
+// 1e-8 relative in exact arithmetic; the error grows steeply only from
+// 2^-28 (1.5e-4).
 // Flush rule: an element whose copies are both +-0 (|W| < 2^-51 for FP16
 // x3; H alone for x1) is stored as FP32 0 as well, so a zero cell is zero
 // in every array of its buffer.  A squaring item whose output cells are
@@ -91,6 +99,15 @@ struct WLive {
     int s, tm, tn, pad;
     uint64_t cells, pad2;
 };
+
+// the cell flag of a nonzero cell whose largest |W| is amax: 96 + E with
+// amax < 2^E (frexp), kept in 1 .. 254
+__device__ inline uint8_t cell_flag(float amax) {
+    int e = 0;
+    (void)frexpf(amax, &e);
+    return (uint8_t)min(max(e + 96, 1), 254);
+}
+constexpr uint8_t kFlagAny = 255;  // may be nonzero, no bound known
 
 template <class T> __device__ inline T *at(char *base, uint64_t off) {
     return reinterpret_cast<T *>(base + off);
@@ -150,6 +167,7 @@ __global__ __launch_bounds__(64) void k_wsplit(char *__restrict__ base,
     const bool was = *flag != 0;  // wave-uniform
     double rs = 0.0;
     unsigned nz = 0;
+    float amax = 0.0f;  // the lane's largest |W| after the flush
     float *Wm = at<float>(base, sc.W);  // W0 itself (flushed in place)
     // all 64 loads issued before any store (the stores may alias W)
     float wrow[64];
@@ -173,11 +191,14 @@ __global__ __launch_bounds__(64) void k_wsplit(char *__restrict__ base,
         th[lane][c] = h;
         if (LO) tl[lane][c] = l;
         nz |= nzv;
+        amax = fmaxf(amax, fabsf(w));
     }
     at<double>(base, sc.part)[(size_t)it.tn * d + i] = rs;  // slot = 64-column block
     // the tile is one cell
     const bool any = __ballot(nz != 0) != 0;
-    if (lane == 0) *flag = any ? 1 : 0;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o, 64));
+    if (lane == 0) *flag = any ? cell_flag(amax) : 0;
     if (!any && !was && !dense) return;  // wave-uniform: no flagged cell to overwrite
     // column-major copies: lane = row, from its own LDS row
 #pragma unroll 8
@@ -230,6 +251,7 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
     __shared__ __attribute__((aligned(16))) uint16_t Bs[2][NOP][BT * PITCH];
     __shared__ double rsum[WGN][BT];
     __shared__ int wnz[NW];
+    __shared__ float wmx[NW];  // each wave's largest |W'| (its block lies in one cell)
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int r = lane & 15, g = lane >> 4;
@@ -404,6 +426,7 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
         const float inv = N::IS * N::IS;
         double rows[TM][4];
         unsigned nzl = 0;  // any nonzero copy among this lane's elements
+        float amax = 0.0f;  // the lane's largest |W'| after the flush
 #pragma unroll
         for (int a = 0; a < TM; a++)
 #pragma unroll
@@ -437,6 +460,7 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
                         lc[v] = 0;
                     }
                     nzl |= nzv;
+                    amax = fmaxf(amax, fabsf(o[v]));
                     rows[a][v] += (double)o[v];
                     Hrn[(size_t)(row + v) * d + col] = hc[v];
                     if (LO) Lrn[(size_t)(row + v) * d + col] = lc[v];
@@ -473,17 +497,27 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
         }
         {
             const bool wz = __ballot(nzl != 0) != 0;
-            if (lane == 0) wnz[wave] = wz ? 1 : 0;
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o, 64));
+            if (lane == 0) {
+                wnz[wave] = wz ? 1 : 0;
+                wmx[wave] = amax;
+            }
         }
         __syncthreads();
         // flags of the tile's cells: a wave's WTM x WTN block lies in one cell
         if (t < TPC * TPC) {
             const int ca = t / TPC, cb = t % TPC;
             int f = 0;
+            float m = 0.0f;
 #pragma unroll
             for (int w = 0; w < NW; w++)
-                if ((w & 1) * WTM / 64 == ca && (w >> 1) * WTN / 64 == cb) f |= wnz[w];
-            at<uint8_t>(base, sc.fln)[(size_t)(it.tm * TPC + ca) * nc + it.tn * TPC + cb] = (uint8_t)f;
+                if ((w & 1) * WTM / 64 == ca && (w >> 1) * WTN / 64 == cb) {
+                    f |= wnz[w];
+                    m = fmaxf(m, wmx[w]);
+                }
+            at<uint8_t>(base, sc.fln)[(size_t)(it.tm * TPC + ca) * nc + it.tn * TPC + cb] =
+                f ? cell_flag(m) : (uint8_t)0;
         }
         // one partial slot per 64-column block, the sum of its CPS wave
         // columns in order (CPS = 64 / WTN)
@@ -521,7 +555,7 @@ template <int TPC>
 __global__ __launch_bounds__(1024) void k_wlive(char *__restrict__ base, const WScen *__restrict__ scen,
                                                 const WItem *__restrict__ items, int nitems,
                                                 WLive *__restrict__ live, int cap, int *__restrict__ counts,
-                                                int wskip) {
+                                                int wskip, int kthr) {
     // block (x, chunk): the 16 items j = x + 8 (16 chunk + w) of XCD x, one
     // per wave w; one atomic per block reserves their slots in list x
     __shared__ int slive[16];
@@ -537,15 +571,17 @@ __global__ __launch_bounds__(1024) void k_wlive(char *__restrict__ base, const W
         const WScen sc = scen[it.s];
         const int nc = sc.d / 64;
         const uint8_t *fl = at<const uint8_t>(base, sc.fl);
-        int fa = 0, fb = 0;
+        int fa = 0, fb = 0;  // the largest flags of the A cells (tile rows, k) and B cells (k, tile columns)
         if (lane < nc) {
 #pragma unroll
             for (int q = 0; q < TPC; q++) {
-                fa |= fl[(size_t)(it.tm * TPC + q) * nc + lane];
-                fb |= fl[(size_t)lane * nc + it.tn * TPC + q];
+                fa = max(fa, (int)fl[(size_t)(it.tm * TPC + q) * nc + lane]);
+                fb = max(fb, (int)fl[(size_t)lane * nc + it.tn * TPC + q]);
             }
         }
-        cells = __ballot(fa != 0 && fb != 0);
+        // K cell `lane`: both operands may be nonzero and their bounds'
+        // product 2^(fa + fb - 192) exceeds 2^kcut (kthr = kcut + 193, or 0)
+        cells = __ballot(fa != 0 && fb != 0 && fa + fb >= kthr);
         bool need = (lane >= it.tn * TPC && lane < it.tn * TPC + TPC && fa != 0) || !wskip;
         if (lane < TPC * TPC)
             need = need || at<const uint8_t>(base, sc.fln)[(size_t)(it.tm * TPC + lane / TPC) * nc +
@@ -624,7 +660,7 @@ __global__ void k_wfix(char *__restrict__ base, const WScen *__restrict__ scen, 
         at<uint16_t>(base, sc.Lrn)[e] = l;
     }
     // the diagonal cells may have become nonzero
-    if ((i & 63) == 0) at<uint8_t>(base, sc.fln)[(size_t)(i >> 6) * (d / 64) + (i >> 6)] = 1;
+    if ((i & 63) == 0) at<uint8_t>(base, sc.fln)[(size_t)(i >> 6) * (d / 64) + (i >> 6)] = kFlagAny;
 }
 
 // ---------------------------------------------------------------------
@@ -1342,6 +1378,11 @@ constexpr int kMaxScen = 4096;
 // the same bits -- for tests and timing)
 int g_wsq_skip = 1;
 
+// K-cell cutoff of the sparse rounds: a K cell whose operand bounds
+// multiply to at most 2^kcut is left out; kcut <= -192 keeps every cell
+// that may be nonzero (the dense loop's bits)
+std::atomic<int> g_wsq_kcut{-36};
+
 template <class F> unsigned launch_mode(int mode, F f) {
     switch (mode) {
     case 0: f(std::integral_constant<bool, false>(), std::integral_constant<bool, false>()); break;
@@ -1560,7 +1601,7 @@ static WsqPlan *make_plan(int n, const unsigned *states, const int *ds, const in
             const int nc = ds[i] / 64;
             hipLaunchKernelGGL(k_fill8, dim3(cdiv(nc * nc, 256)), dim3(256), 0, D.stream,
                                reinterpret_cast<uint8_t *>(ptr(states[i] + (unsigned)wsq_layout(ds[i], mode).fl[b])),
-                               nc * nc, (uint8_t)1);
+                               nc * nc, kFlagAny);
         }
     if (check_launch()) {
         (void)hipFree(P->dev);
@@ -1644,12 +1685,13 @@ OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
                 if (sparse) {
                     lv = P->live[c];
                     cn = P->counts + 8 * c;
+                    const int kc = g_wsq_kcut.load(), kthr = kc <= -192 ? 0 : kc + 193;
                     if (c == 1)
                         hipLaunchKernelGGL((k_wlive<2>), dim3(8 * cdiv(cdiv(nit, 8), 16)), dim3(1024), 0, D.stream, D.base,
-                                           P->sc, its, nit, P->live[c], P->cap[c], P->counts + 8 * c, 1);
+                                           P->sc, its, nit, P->live[c], P->cap[c], P->counts + 8 * c, 1, kthr);
                     else
                         hipLaunchKernelGGL((k_wlive<1>), dim3(8 * cdiv(cdiv(nit, 8), 16)), dim3(1024), 0, D.stream, D.base,
-                                           P->sc, its, nit, P->live[c], P->cap[c], P->counts + 8 * c, 1);
+                                           P->sc, its, nit, P->live[c], P->cap[c], P->counts + 8 * c, 1, kthr);
                 }
                 if (c == 1)
                     hipLaunchKernelGGL((k_wsq<BF, LO, 128, 8>), dim3(g), dim3(512), 0, D.stream, D.base,
@@ -1678,6 +1720,12 @@ OPHIP_EXPORT int ophip_wsq_set_skip(int on) {
     return old;
 }
 
+// the sparse rounds' K-cell cutoff (log2; -192 or below: off), for rounds
+// launched or captured from now on; returns the previous value
+OPHIP_EXPORT int ophip_wsq_set_kcut(int log2_bound) {
+    return g_wsq_kcut.exchange(log2_bound < -192 ? -192 : log2_bound);
+}
+
 // handle of the FP32 W buffer `buf` (0/1) inside a state
 OPHIP_EXPORT unsigned ophip_wsq_w(unsigned state, int d, int mode, int buf) {
     WsqLayout L = wsq_layout(d, mode);
@@ -1689,8 +1737,9 @@ OPHIP_EXPORT unsigned ophip_wsq_w(unsigned state, int d, int mode, int buf) {
 // 6 cell flags of the copies (d/64 x d/64 bytes, row-major; 0 = every
 // FP32 W and row-major copy value of the 64 x 64 cell is +-0, and so is
 // every column-major copy value unless a backward layout has written its
-// fragment copies into that buffer's column-major slots since); 0 for
-// arrays the mode does not have
+// fragment copies into that buffer's column-major slots since; otherwise
+// the magnitude bound 96 + E, 255 unknown); 0 for arrays the mode does not
+// have
 OPHIP_EXPORT unsigned ophip_wsq_array(unsigned state, int d, int mode, int buf, int which) {
     if (d <= 0 || d % 64 || mode < 0 || mode > 2 || buf < 0 || buf > 1 || which < 0 || which > 7)
         return 0;

@@ -92,6 +92,7 @@ SIGNATURES = {
                                I32]),
     "ophip_wsq_exec": (I32, [ULONG, I32]),
     "ophip_wsq_set_skip": (I32, [I32]),
+    "ophip_wsq_set_kcut": (I32, [I32]),
     "ophip_wsq_plan_free": (V, [ULONG]),
     "ophip_w8_bytes": (ULONG, [I32]),
     "ophip_w8_w": (U32, [U32, I32, I32]),

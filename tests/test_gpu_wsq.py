@@ -696,6 +696,7 @@ def test_zero_cell_skip_same_bits(hip, mode, tile):
 
     p = plan(H, lats, tile)
     old = H.lib.ophip_wsq_set_skip(1)
+    oldk = H.lib.ophip_wsq_set_kcut(-192)  # every K cell that may be nonzero
     try:
         for nsq in (1, 2, 7):
             got = {}
@@ -718,6 +719,13 @@ def test_zero_cell_skip_same_bits(hip, mode, tile):
                             assert not np.any((f == 0) & ~z)
                             off = ~np.eye(L.d // 64, dtype=bool)
                             assert np.array_equal((f == 0)[off], z[off])
+                            # a flag 96 + E bounds the cell: max|W| in [2^(E-1), 2^E)
+                            nc = L.d // 64
+                            wmax = np.abs(L.arr(b, 0, np.float32)).reshape(nc, 64, nc, 64).max(axis=(1, 3))
+                            m = (f != 0) & (f != 255)
+                            e = f[m].astype(np.int64) - 96
+                            assert np.all(wmax[m] < 2.0**e) and np.all(wmax[m] >= 2.0**(e - 1))
+                            assert np.all(f[np.eye(nc, dtype=bool)] == 255)
                         zf = (flags(L, nsq & 1) == 0).mean()
                         print(f"[skip] d={L.d} nsq={nsq}: zero cells {zf:.2f}", flush=True)
                         if nsq <= 2 and L.cfg.nr == 1:
@@ -727,6 +735,34 @@ def test_zero_cell_skip_same_bits(hip, mode, tile):
                     assert np.array_equal(x, y)
     finally:
         H.lib.ophip_wsq_set_skip(old)
+        H.lib.ophip_wsq_set_kcut(oldk)
+        H.lib.ophip_wsq_plan_free(p)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_kcut_leaves_out_tiny_products(hip, mode):
+    """the K-cell cutoff: at 2^-36 it moves no entry of W by more than the
+    left-out products allow (at most 64 2^-36 per left-out cell and round,
+    doubled by every later round: under 2^-20 here) against every K cell;
+    a cutoff that bites (2^-8) changes W"""
+    banner(f"wsq K-cell cutoff mode={mode}")
+    H = hip
+    L = Lattice(H, 512, 1, mode)  # d = 512, one band
+    p = plan(H, [L], 128)
+    oldk = H.lib.ophip_wsq_set_kcut(-36)
+    try:
+        out = {}
+        for kc in (-192, -36, -8):
+            H.lib.ophip_wsq_set_kcut(kc)
+            L.first_step()
+            H.check(H.lib.ophip_wsq_exec(p, 9))
+            H.sync()
+            out[kc] = L.arr(1, 0, np.float32).astype(np.float64)
+        full = out[-192]
+        assert np.abs(out[-36] - full).max() <= 2.0**-20
+        assert np.abs(out[-8] - full).max() > 0.0  # a cutoff that bites changes W
+    finally:
+        H.lib.ophip_wsq_set_kcut(oldk)
         H.lib.ophip_wsq_plan_free(p)
 
 
