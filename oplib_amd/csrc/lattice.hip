@@ -11,7 +11,7 @@
 //                     in FP64 and rounded once to FP32 (or kept as FP64);
 //                     the generator is never written to memory
 //   ophip_sck_rows    cumulative tables stored row-contiguous for the
-//                     sampler (one 64-row tile per workgroup, LDS transpose)
+//                     sampler (one pass over 8-row blocks up to d = 2048)
 #include "ophip_internal.h"
 
 using namespace ophip;
@@ -372,15 +372,15 @@ OPHIP_EXPORT void opcuda_sck(int d, int nm, unsigned tpk, unsigned ctpk) {
 // non-decreasing, and the values are within a few ulps of the sequential
 // order (bit for bit where every partial sum is exact).
 //
+// For d > 2048 (k_sck_one below takes the smaller tables in one pass):
 // k_sck_part: one wavefront per 64 rows x one chunk (lane = row: the 64
 // values of its chunk with 64 coalesced loads, 64 consecutive rows per
 // column), writes t_q.  k_sck_fin: the same wavefronts again, each chains
 // its rows' totals (the same adds in the same order in every wavefront,
 // so all of them see the same T), re-forms l_q (the same adds as
 // k_sck_part) and writes C through an LDS tile, so the row-contiguous
-// stores are coalesced.  Grid (row blocks, chunks, tables): 1024
-// wavefronts at d = 2048.  Rows and columns past d are clamped reads that
-// are never used.
+// stores are coalesced.  Grid (row blocks, chunks, tables).  Rows and
+// columns past d are clamped reads that are never used.
 constexpr int kSckChunk = 64;
 
 __device__ inline void sck_chunk_load(const float *__restrict__ src, int d, int y0, float (&v)[kSckChunk]) {
@@ -440,12 +440,87 @@ __global__ __launch_bounds__(64) void k_sck_fin(int d, const float *__restrict__
         if (lane < ny) crow[mb + (size_t)(row0 + r) * d + y0 + lane] = tile[r][lane];
 }
 
+// k_sck_one: the same table in one pass for d <= 2048, the same adds in
+// the same order.  A workgroup takes R rows (R = 8) and holds them whole in
+// registers: lane l of wave w owns row l % R of the block and chunk
+// (64 / R) w + l / R (64 values), so wave w holds 64 (64 / R) consecutive
+// columns of the R rows.  Each thread forms its chunk's running sums, the
+// chunk totals meet in LDS, every thread chains its row's totals in order
+// (carry before its chunk, then on to T), and each wave writes its block
+// through its own LDS tile with coalesced row stores.  Every entry is read
+// once and written once (8 bytes per entry).  Row blocks are dealt to XCDs
+// in contiguous runs (block b runs on XCD b % 8), so the blocks that share
+// the 128-byte lines of 32 consecutive rows read them through one L2.
+#ifndef OPHIP_SCK_R
+#define OPHIP_SCK_R 8
+#endif
+constexpr int kSckRows = OPHIP_SCK_R, kSckCPW = 64 / kSckRows, kSckCP = kSckChunk + 1,
+              kSckPitch = kSckCPW * kSckCP + 1, kSckWaves = 32 / kSckCPW;
+
+__global__ __launch_bounds__(64 * kSckWaves) void k_sck_one(int d, const float *__restrict__ tpk,
+                                                             float *__restrict__ crow, int plus_identity,
+                                                             int per_xcd) {
+    __shared__ float tq[kSckRows][33];
+    __shared__ float st[kSckWaves][kSckRows * kSckPitch];
+    const int nrb = (d + kSckRows - 1) / kSckRows;
+    const int rb = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (rb >= nrb) return;  // whole workgroup, before any barrier
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int nq = (d + kSckChunk - 1) / kSckChunk;  // <= 32
+    const int rl = lane % kSckRows, qq = lane / kSckRows, q = kSckCPW * w + qq;
+    const int row0 = rb * kSckRows, row = row0 + rl, y0 = q * kSckChunk;
+    const size_t mb = (size_t)blockIdx.y * d * d;
+    const bool act = q < nq;
+    float v[kSckChunk];
+    if (act) sck_chunk_load(tpk + mb + min(row, d - 1), d, y0, v);
+    const int ny = act ? min(kSckChunk, d - y0) : 0;
+    float l = 0.0f;
+#pragma unroll
+    for (int c = 0; c < kSckChunk; c++) {
+        if (c < ny) {
+            float x = v[c];
+            if (plus_identity && row == y0 + c) x += 1.0f;
+            l += (x > 0.0f) ? x : 0.0f;
+        }
+        v[c] = l;
+    }
+    if (act) tq[rl][q] = l;
+    __syncthreads();
+    float carry = 0.0f, T = 1.0f;
+    if (act) {
+        for (int p = 0; p < q; p++) carry = carry + tq[rl][p];
+        T = carry;
+        for (int p = q; p < nq; p++) T = T + tq[rl][p];
+    }
+    float *tile = st[w];
+    {
+        float *dst = tile + rl * kSckPitch + qq * kSckCP;
+#pragma unroll
+        for (int c = 0; c < kSckChunk; c++) {
+            const float s = (carry + v[c]) / T;
+            dst[c] = s < 1.0f ? s : 1.0f;
+        }
+    }
+    __syncthreads();
+    const int c0 = 64 * kSckCPW * w, ncol = min(64 * kSckCPW, d - c0);  // this wave's columns
+    if (ncol > 0)
+        for (int r = 0; r < kSckRows && row0 + r < d; r++)
+            for (int c = lane; c < ncol; c += 64)
+                crow[mb + (size_t)(row0 + r) * d + c0 + c] = tile[r * kSckPitch + (c >> 6) * kSckCP + (c & 63)];
+}
+
 static int sck_rows(int d, int nm, unsigned tpk, unsigned crow, int plus_identity) {
     Device &D = dev();
     if (d <= 0 || nm <= 0 || nm > 65535 || !ptr(tpk) || !ptr(crow)) return (int)set_status(kErrBadArg);
     if (!handle_range_ok(tpk, 4 * (size_t)nm * d * d) || !handle_range_ok(crow, 4 * (size_t)nm * d * d))
         return (int)set_status(kErrBadArg);
     const int nq = cdiv(d, kSckChunk);
+    if (nq <= 32) {  // one pass: 64 / R chunks per wavefront
+        const int per_xcd = cdiv(cdiv(d, kSckRows), 8);
+        hipLaunchKernelGGL(k_sck_one, dim3(8 * per_xcd, nm), dim3(64 * cdiv(nq, kSckCPW)), 0, D.stream, d,
+                           P<const float>(tpk), P<float>(crow), plus_identity, per_xcd);
+        return (int)check_launch();
+    }
     float *tq = (float *)workspace(sizeof(float) * (size_t)nm * nq * d);
     if (!tq) return (int)kErrOutOfArena;
     const dim3 g(cdiv(d, 64), nq, nm);

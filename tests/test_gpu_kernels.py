@@ -465,7 +465,7 @@ def stochastic(rng, nm, d, power=6):
     return P.astype(np.float32)
 
 
-@pytest.mark.parametrize("d", [200, 2048])
+@pytest.mark.parametrize("d", [200, 2048, 2100])  # one pass up to d = 2048, two passes beyond
 def test_sck_bit_exact_both_layouts(H, d):
     banner(f"sck d={d}")
     rng = np.random.default_rng(11)
