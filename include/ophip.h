@@ -189,12 +189,13 @@ int ophip_saxpby(int n, float a, unsigned A, float b, unsigned B, unsigned C);
  * reference's per-round active lists, CModel Common.cs:508-545). */
 unsigned long ophip_wsq_bytes(int d, int mode);
 unsigned      ophip_wsq_w(unsigned state, int d, int mode, int buf);
-/* any array of a state: which = 0 W, 1 Hr, 2 Hc, 3 Lr, 4 Lc, 5 partial
- * row sums, 6 cell flags (d/64 x d/64 bytes; 0 = every FP32 W and
- * row-major copy value of that 64 x 64 cell is +-0, and so is every
- * column-major copy value unless a backward layout has since written its
- * fragment copies into that buffer; otherwise 96 + E with the cell's
- * largest |W| below 2^E, or 255 where no bound is known), 7 backward-step tile lists (d/16 + d/64 +
+/* any array of a state: which = 0 W (FP32, column-major), 1-4 the
+ * buffer's four 16-bit copy slots (2 and 4 of the buffer the squarings did
+ * not end in hold the backward layout's fragment-major hi and lo parts of
+ * W; the squarings split W as they stage it and store no copies), 5 partial
+ * row sums, 6 cell flags (d/64 x d/64 bytes; 0 = every W value of that
+ * 64 x 64 cell is +0; otherwise 96 + E with the cell's largest |W| below
+ * 2^E, or 255 where no bound is known), 7 backward-step tile lists (d/16 + d/64 +
  * d/32 lists of d/32 + 1 ints, per 16-row tile, 64-row block and 32-row
  * block: count, then the visited k tiles, kt + 65536 where the lo part is
  * read too, + 2^(17+q) in a block of nr row tiles where its tile q visits

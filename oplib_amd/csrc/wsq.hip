@@ -14,18 +14,23 @@
 // diagonal is corrected so every row of W sums to exactly the FP64 sum of
 // its off-diagonal part with opposite sign (rows of U sum to 1).
 //
-// Operands.  W is kept in FP32 (column-major) and as 16-bit copies scaled by
-// 2^S (S = 15 for FP16 so entries <= 1 stay normal down to 2^-24/2^15;
-// S = 0 for BF16): H = rnd(W 2^S) and, for the three-product variant,
+// Operands.  W is kept in FP32 (column-major) only; a squaring splits it
+// into 16-bit parts scaled by 2^S (S = 15 for FP16 so entries <= 1 stay
+// normal down to 2^-24/2^15; S = 0 for BF16) as it stages its operand
+// tiles into LDS: H = rnd(W 2^S) and, for the three-product variant,
 // L = rnd((W 2^S - H) 2^G) (G = 11 FP16, 8 BF16).  The product is
 //     W*W ~ 2^-2S (H*H + 2^-G (H*L + L*H)).
-// The A operand is read from a row-major copy and the B operand from a
-// column-major copy so that both feed the MFMA with 8 consecutive k per lane.
+// Storing the parts would move three times the bytes of W per round (both
+// layouts, both parts) for reads of the same size; splitting on the way in
+// costs a few VALU operations per element instead.  A tiles are staged
+// through registers as 4-row x 8-column blocks, so each row's 8
+// consecutive k reach LDS as one store: both operands then feed the MFMA
+// with 8 consecutive k per lane.
 //
 // Kernels (all tiles 128x128 or 64x64, every d a multiple of the tile):
-//   k_wsplit   FP32 W -> 16-bit copies (both layouts) + partial row sums
+//   k_wsplit   the first W: flush rule in place, partial row sums, flags
 //   k_wsq      W' = 2W + W*W on v_mfma_f32_16x16x32_{f16,bf16}; epilogue
-//              writes W' (FP32), its 16-bit copies and partial row sums
+//              writes W' (FP32), its partial row sums and cell flags
 //   k_wfix     diagonal correction from the partial row sums (fixed order)
 // A scenario table lets one launch serve many lattices of different d.
 #include "ophip_internal.h"
@@ -59,15 +64,13 @@ struct WScen {
     int d;
     int pad;
     uint64_t W, Wn;                // FP32 W (input), W' (output), column-major
-    uint64_t Hr, Hc, Lr, Lc;       // 16-bit copies of W (row-/column-major)
-    uint64_t Hrn, Hcn, Lrn, Lcn;   // copies of W' (outputs)
     uint64_t part;                 // partial row sums of W': [d/64 column blocks][d]
-    uint64_t fl, fln;              // cell flags of the copies of W, W' (below)
+    uint64_t fl, fln;              // cell flags of W, W' (below)
 };
 
-// Cell flags.  The 16-bit copies are cut into 64 x 64 cells; flag byte
-// [cr][cc] (d/64 x d/64, row-major) is 0 only if every H (and L) value of
-// cell (rows 64 cr.., columns 64 cc..) is +-0, else 96 + E with the cell's
+// Cell flags.  W is cut into 64 x 64 cells; flag byte [cr][cc] (d/64 x
+// d/64, row-major) is 0 only if every value of cell (rows 64 cr..,
+// columns 64 cc..) is +0 (so are its 16-bit parts), else 96 + E with the cell's
 // largest |W| below 2^E (255: no bound known).  FP16 H and L flush entries
 // |W| < 2^-51 to zero, and a banded generator keeps most cells there for
 // many squarings, so the product's K loop visits only the cells k with
@@ -81,9 +84,9 @@ struct WScen {
 // the 16 squarings (half of the last rounds') and moves the prices by under
 // 1e-8 relative in exact arithmetic; the error grows steeply only from
 // 2^-28 (1.5e-4).
-// Flush rule: an element whose copies are both +-0 (|W| < 2^-51 for FP16
-// x3; H alone for x1) is stored as FP32 0 as well, so a zero cell is zero
-// in every array of its buffer.  A squaring item whose output cells are
+// Flush rule: an element whose 16-bit parts are both +-0 (|W| < 2^-51 for
+// FP16 x3; H alone for x1) is stored as FP32 +0, so the parts the next
+// squaring makes of a zero cell are zero.  A squaring item whose output cells are
 // then certainly zero (no K cells, zero input cells) and whose destination
 // cells already hold zeros (flags of the buffer's previous contents) writes
 // nothing at all, and the diagonal fix skips the partial sums of zero cells.
@@ -142,29 +145,22 @@ template <> struct Num<true> {
 };
 
 // ---------------------------------------------------------------------
-// k_wsplit: 64x64 tile per workgroup (64 threads), LDS transpose.  With
-// the sparse round loop (dense = 0) a tile whose copies are all zero and
-// whose destination cell is not flagged stores no copies: that loop reads
-// no unflagged cell.  The dense loop reads every cell's copies, and a
-// backward layout may have left fragment copies in the column-major slots
-// of an unflagged cell, so with dense = 1 every tile stores its copies.
+// k_wsplit: the first W of a solve, one 64x64 tile (one cell) per
+// workgroup (64 threads, lane = row): the flush rule applied in place,
+// the partial row sums and the cell's flag.  The squarings split W into
+// its 16-bit parts as they stage their operands.
 // ---------------------------------------------------------------------
 template <bool BF, bool LO>
 __global__ __launch_bounds__(64) void k_wsplit(char *__restrict__ base,
                                                const WScen *__restrict__ scen,
-                                               const WItem *__restrict__ items, int dense) {
+                                               const WItem *__restrict__ items) {
     using N = Num<BF>;
-    __shared__ uint16_t th[64][66], tl[64][66];
     const WItem it = items[blockIdx.x];
     const WScen sc = scen[it.s];
-    const float *W = at<const float>(base, sc.W);
-    uint16_t *Hcn = at<uint16_t>(base, sc.Hcn), *Hrn = at<uint16_t>(base, sc.Hrn);
-    uint16_t *Lcn = at<uint16_t>(base, sc.Lcn), *Lrn = at<uint16_t>(base, sc.Lrn);
     const int d = sc.d, lane = threadIdx.x;
     const int i0 = it.tm * 64, j0 = it.tn * 64;
     const int i = i0 + lane;
     uint8_t *flag = at<uint8_t>(base, sc.fln) + (size_t)it.tm * (d / 64) + it.tn;
-    const bool was = *flag != 0;  // wave-uniform
     double rs = 0.0;
     unsigned nz = 0;
     float amax = 0.0f;  // the lane's largest |W| after the flush
@@ -172,48 +168,27 @@ __global__ __launch_bounds__(64) void k_wsplit(char *__restrict__ base,
     // all 64 loads issued before any store (the stores may alias W)
     float wrow[64];
 #pragma unroll
-    for (int c = 0; c < 64; c++) wrow[c] = W[(size_t)i + (size_t)d * (j0 + c)];
+    for (int c = 0; c < 64; c++) wrow[c] = Wm[(size_t)i + (size_t)d * (j0 + c)];
 #pragma unroll
     for (int c = 0; c < 64; c++) {
-        const size_t e = (size_t)i + (size_t)d * (j0 + c);
         float w = wrow[c];
-        float ws = w * N::S;
-        uint16_t h = N::enc(ws);
-        uint16_t l = 0;
-        if (LO) l = N::enc((ws - N::dec(h)) * N::G);
+        const float ws = w * N::S;
+        const uint16_t h = N::enc(ws);
+        const uint16_t l = LO ? N::enc((ws - N::dec(h)) * N::G) : (uint16_t)0;
         const unsigned nzv = (h | l) & 0x7fffu;
-        if (!nzv) {  // flush rule; zeros stored as +0 everywhere
-            h = l = 0;
-            if (__builtin_bit_cast(uint32_t, w) != 0u) Wm[e] = 0.0f;
+        if (!nzv) {  // flush rule: both copies +-0 -> FP32 +0
+            if (__builtin_bit_cast(uint32_t, w) != 0u) Wm[(size_t)i + (size_t)d * (j0 + c)] = 0.0f;
             w = 0.0f;
         }
         rs += (double)w;
-        th[lane][c] = h;
-        if (LO) tl[lane][c] = l;
         nz |= nzv;
         amax = fmaxf(amax, fabsf(w));
     }
     at<double>(base, sc.part)[(size_t)it.tn * d + i] = rs;  // slot = 64-column block
-    // the tile is one cell
     const bool any = __ballot(nz != 0) != 0;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o, 64));
     if (lane == 0) *flag = any ? cell_flag(amax) : 0;
-    if (!any && !was && !dense) return;  // wave-uniform: no flagged cell to overwrite
-    // column-major copies: lane = row, from its own LDS row
-#pragma unroll 8
-    for (int c = 0; c < 64; c++) {
-        const size_t e = (size_t)i + (size_t)d * (j0 + c);
-        Hcn[e] = th[lane][c];
-        if (LO) Lcn[e] = tl[lane][c];
-    }
-    __syncthreads();
-    // row-major copies: row r of the tile, lanes over columns
-    for (int r = 0; r < 64; r++) {
-        const size_t e = (size_t)(i0 + r) * d + j0 + lane;
-        Hrn[e] = th[r][lane];
-        if (LO) Lrn[e] = tl[r][lane];
-    }
 }
 
 // ---------------------------------------------------------------------
@@ -243,10 +218,8 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
     // NW waves as 2 (rows) x NW/2 (columns); wave tile WTM x WTN of TM x TN MFMA tiles
     constexpr int WGN = NW / 2, WTM = BT / 2, WTN = BT / WGN, TM = WTM / 16, TN = WTN / 16;
     constexpr int NOP = LO ? 2 : 1;
-    constexpr int LD16 = BT * BK / 8 / (64 * NW);  // 16-byte loads per thread per operand array
     constexpr int PF = BK == 64 ? 2 : 4;     // K-steps of loads in flight
     constexpr int SPC = 64 / BK, TPC = BT / 64;  // K-steps per cell, cells per tile side
-    static_assert(LD16 >= 1, "tile");
     __shared__ __attribute__((aligned(16))) uint16_t As[2][NOP][BT * PITCH];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[2][NOP][BT * PITCH];
     __shared__ double rsum[WGN][BT];
@@ -274,8 +247,6 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
         const WScen sc = scen[it.s];
         const int d = sc.d;
         const int i0 = it.tm * BT, j0 = it.tn * BT;
-        const uint16_t *Hr = at<const uint16_t>(base, sc.Hr), *Hc = at<const uint16_t>(base, sc.Hc);
-        const uint16_t *Lr = at<const uint16_t>(base, sc.Lr), *Lc = at<const uint16_t>(base, sc.Lc);
 
         // the epilogue's W values, loaded before the K loop's first loads:
         // their latency hides behind the loop, and the epilogue issues
@@ -301,32 +272,68 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
             }
 
         auto swz = [](int row) { return CPR == 8 ? (row & 7) : ((row >> 1) & 3); };
-        typedef u4v Stage[2][NOP][LD16];  // [A/B][hi/lo][load]
+        // Operands come from the FP32 W (column-major) and are split into
+        // their 16-bit hi/lo parts on the way to LDS, with the operations
+        // that make the stored copies (the same bits).  Waves 0 .. NW/2-1
+        // stage A, the others B, NF float4 loads per thread and stage.  A:
+        // thread (rg, cg) loads a 4-row x KA-column block (rows 4 rg.., k =
+        // KA cg ..; eight 128-byte column pieces per load instruction) and
+        // writes each row's KA consecutive k as one LDS store.  B: thread
+        // (kq, jb) loads 4 consecutive k (4 kq ..) of columns jb + 16 u.
+        constexpr int HALF = 32 * NW, NF = BT * BK / HALF / 4, KA = NF, CG = BK / KA, KQ = BK / 4;
+        static_assert(BT / 4 * CG == HALF && HALF / KQ == 16 && 16 * NF == BT, "operand staging");
+        const bool isA = wave < NW / 2;  // wave-uniform
+        const int ht = t % HALF;
+        const int rg = ht / CG, cg = ht % CG, kq = ht % KQ, jb = ht / KQ;
+        const float *Wsrc = at<const float>(base, sc.W);
+        const float *ga = Wsrc + (size_t)(i0 + 4 * rg) + (size_t)d * (KA * cg);
+        const float *gb = Wsrc + (size_t)(4 * kq) + (size_t)d * (j0 + jb);
+        typedef f4 Stage[NF];
         auto gload = [&](Stage &x, int k0) {
+            if (isA) {
 #pragma unroll
-            for (int q = 0; q < LD16; q++) {
-                const int idx = t + 64 * NW * q;
-                const int row = idx / CPR, part = idx % CPR;  // CPR x 16 B per row
-                const size_t ea = (size_t)(i0 + row) * d + k0 + 8 * part;  // row-major A
-                const size_t eb = (size_t)(j0 + row) * d + k0 + 8 * part;  // column-major B
-                x[0][0][q] = *(const u4v *)(Hr + ea);
-                x[1][0][q] = *(const u4v *)(Hc + eb);
-                if (LO) {
-                    x[0][NOP - 1][q] = *(const u4v *)(Lr + ea);
-                    x[1][NOP - 1][q] = *(const u4v *)(Lc + eb);
-                }
+                for (int u = 0; u < NF; u++) x[u] = *(const f4 *)(ga + (size_t)d * (k0 + u));
+            } else {
+#pragma unroll
+                for (int u = 0; u < NF; u++) x[u] = *(const f4 *)(gb + k0 + (size_t)d * (16 * u));
             }
         };
+        auto split2 = [](float a, float b, uint32_t &hi, uint32_t &lo) {
+            const float as = a * N::S, bs = b * N::S;
+            const uint16_t ha = N::enc(as), hb = N::enc(bs);
+            hi = (uint32_t)ha | ((uint32_t)hb << 16);
+            lo = 0;
+            if (LO)
+                lo = (uint32_t)N::enc((as - N::dec(ha)) * N::G) |
+                     ((uint32_t)N::enc((bs - N::dec(hb)) * N::G) << 16);
+        };
         auto sstore = [&](const Stage &x, int buf) {
+            if (isA) {
 #pragma unroll
-            for (int q = 0; q < LD16; q++) {
-                const int idx = t + 64 * NW * q;
-                const int row = idx / CPR, part = idx % CPR;
+                for (int v = 0; v < 4; v++) {
+                    const int row = 4 * rg + v, kk = KA * cg;
+                    const int sw = row * PITCH + 8 * ((kk >> 3) ^ swz(row)) + (kk & 7);
+                    uint32_t h[KA / 2], l[KA / 2];
 #pragma unroll
-                for (int o = 0; o < NOP; o++) {
-                    const int sw = row * PITCH + 8 * (part ^ swz(row));
-                    *(u4v *)&As[buf][o][sw] = x[0][o][q];
-                    *(u4v *)&Bs[buf][o][sw] = x[1][o][q];
+                    for (int u = 0; u < KA / 2; u++) split2(x[2 * u][v], x[2 * u + 1][v], h[u], l[u]);
+                    if constexpr (KA == 8) {
+                        *(u4v *)&As[buf][0][sw] = u4v{h[0], h[1], h[2], h[3]};
+                        if (LO) *(u4v *)&As[buf][NOP - 1][sw] = u4v{l[0], l[1], l[2], l[3]};
+                    } else {
+                        *(uint2 *)&As[buf][0][sw] = uint2{h[0], h[1]};
+                        if (LO) *(uint2 *)&As[buf][NOP - 1][sw] = uint2{l[0], l[1]};
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < NF; u++) {
+                    const int col = jb + 16 * u, kk = 4 * kq;
+                    const int sw = col * PITCH + 8 * ((kk >> 3) ^ swz(col)) + (kk & 7);
+                    uint32_t h0, h1, l0, l1;
+                    split2(x[u][0], x[u][1], h0, l0);
+                    split2(x[u][2], x[u][3], h1, l1);
+                    *(uint2 *)&Bs[buf][0][sw] = uint2{h0, h1};
+                    if (LO) *(uint2 *)&Bs[buf][NOP - 1][sw] = uint2{l0, l1};
                 }
             }
         };
@@ -421,8 +428,6 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
 
         // epilogue: lane holds rows 4g+v of column r of every 16x16 tile
         float *Wn = at<float>(base, sc.Wn);
-        uint16_t *Hrn = at<uint16_t>(base, sc.Hrn), *Hcn = at<uint16_t>(base, sc.Hcn);
-        uint16_t *Lrn = at<uint16_t>(base, sc.Lrn), *Lcn = at<uint16_t>(base, sc.Lcn);
         const float inv = N::IS * N::IS;
         double rows[TM][4];
         unsigned nzl = 0;  // any nonzero copy among this lane's elements
@@ -442,39 +447,23 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
                 f4 p = acc[a][b];
                 if (LO) p += cor[a][b] * N::IG;
                 f4 o = 2.0f * w + p * inv;
-                uint16_t hc[4], lc[4];
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
-                    // split the stored FP32 value: no fused multiply-add-
-                    // convert, whose single rounding differs at FP16 ties
+                    // the split of the stored FP32 value decides the flush:
+                    // no fused multiply-add-convert, whose single rounding
+                    // differs at FP16 ties
                     float ov = o[v];
                     asm volatile("" : "+v"(ov));
-                    float os = ov * N::S;
-                    hc[v] = N::enc(os);
-                    lc[v] = 0;
-                    if (LO) lc[v] = N::enc((os - N::dec(hc[v])) * N::G);
-                    const unsigned nzv = (hc[v] | lc[v]) & 0x7fffu;
-                    if (!nzv) {  // flush rule; zeros stored as +0 everywhere
-                        o[v] = 0.0f;
-                        hc[v] = 0;
-                        lc[v] = 0;
-                    }
+                    const float os = ov * N::S;
+                    const uint16_t hc = N::enc(os);
+                    const uint16_t lc = LO ? N::enc((os - N::dec(hc)) * N::G) : (uint16_t)0;
+                    const unsigned nzv = (hc | lc) & 0x7fffu;
+                    if (!nzv) o[v] = 0.0f;  // flush rule: both copies +-0 -> FP32 +0
                     nzl |= nzv;
                     amax = fmaxf(amax, fabsf(o[v]));
                     rows[a][v] += (double)o[v];
-                    Hrn[(size_t)(row + v) * d + col] = hc[v];
-                    if (LO) Lrn[(size_t)(row + v) * d + col] = lc[v];
                 }
                 *(f4 *)(Wn + e) = o;
-                uint2 ph, pl;
-                ph.x = (uint32_t)hc[0] | ((uint32_t)hc[1] << 16);
-                ph.y = (uint32_t)hc[2] | ((uint32_t)hc[3] << 16);
-                *(uint2 *)(Hcn + e) = ph;
-                if (LO) {
-                    pl.x = (uint32_t)lc[0] | ((uint32_t)lc[1] << 16);
-                    pl.y = (uint32_t)lc[2] | ((uint32_t)lc[3] << 16);
-                    *(uint2 *)(Lcn + e) = pl;
-                }
             }
         }
         // row sums over this wave's WTN columns: reduce across the 16 lanes r
@@ -611,10 +600,8 @@ __global__ __launch_bounds__(256) void k_fill8(uint8_t *__restrict__ p, int n, u
 // diagonal correction: W'_ii -= sum_j W'_ij (fixed summation order).
 // One wavefront per 64 rows; the lanes of a row group read the d/64
 // partial slots of their rows in a fixed order.
-template <bool BF, bool LO>
 __global__ void k_wfix(char *__restrict__ base, const WScen *__restrict__ scen, int nscen,
                        const int *__restrict__ nsq, int round, int *__restrict__ counts) {
-    using N = Num<BF>;
     // the live-list counters of the next round start at zero
     if (counts && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 16) counts[threadIdx.x] = 0;
     const int s = blockIdx.y;
@@ -648,17 +635,7 @@ __global__ void k_wfix(char *__restrict__ base, const WScen *__restrict__ scen, 
         for (int u = 0; u < 16; u++)
             if (q + u < nslots) rs += f[u] ? v[u] : 0.0;
     }
-    const float w = (float)((double)w0 - rs);
-    Wn[e] = w;
-    const float ws = w * N::S;
-    const uint16_t h = N::enc(ws);
-    at<uint16_t>(base, sc.Hcn)[e] = h;
-    at<uint16_t>(base, sc.Hrn)[e] = h;  // the diagonal sits at the same offset in both layouts
-    if (LO) {
-        const uint16_t l = N::enc((ws - N::dec(h)) * N::G);
-        at<uint16_t>(base, sc.Lcn)[e] = l;
-        at<uint16_t>(base, sc.Lrn)[e] = l;
-    }
+    Wn[e] = (float)((double)w0 - rs);
     // the diagonal cells may have become nonzero
     if ((i & 63) == 0) at<uint8_t>(base, sc.fln)[(size_t)(i >> 6) * (d / 64) + (i >> 6)] = kFlagAny;
 }
@@ -691,12 +668,26 @@ __device__ inline float x_from_copies(const uint16_t *xt, size_t n, int row, int
     return N::dec(xt[o]) + N::dec(xt[o + n]) * N::IG;
 }
 
-// row-major H (and L) of a d x d matrix -> fragment-major; one thread per
-// 16-byte fragment
-template <bool LO>
-__global__ __launch_bounds__(256) void k_wfrag(int d, const uint16_t *__restrict__ Hr,
-                                               const uint16_t *__restrict__ Lr,
-                                               uint16_t *__restrict__ Hf, uint16_t *__restrict__ Lf) {
+// the 16-bit hi/lo parts of eight FP32 values, as the squarings split W
+template <bool BF>
+__device__ inline void split8(const float (&w)[8], uint4 &hi, uint4 &lo) {
+    using N = Num<BF>;
+    uint16_t h[8], l[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const float ws = w[e] * N::S;
+        h[e] = N::enc(ws);
+        l[e] = N::enc((ws - N::dec(h[e])) * N::G);
+    }
+    hi = *(const uint4 *)h;
+    lo = *(const uint4 *)l;
+}
+
+// FP32 W (d x d column-major) -> its hi/lo parts, fragment-major; one
+// thread per 16-byte fragment (8 consecutive columns of one row)
+template <bool BF>
+__global__ __launch_bounds__(256) void k_wfrag(int d, const float *__restrict__ W, uint16_t *__restrict__ Hf,
+                                               uint16_t *__restrict__ Lf) {
     const int nkt = d / 32;
     const size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // fragment index
     if (f >= (size_t)d * d / 8) return;
@@ -704,15 +695,20 @@ __global__ __launch_bounds__(256) void k_wfrag(int d, const uint16_t *__restrict
     const size_t tile = f >> 6;
     const int it = (int)(tile / nkt), kt = (int)(tile % nkt);
     const int r = l & 15, g = l >> 4;
-    const size_t src = (size_t)(16 * it + r) * d + 32 * kt + 8 * g;
-    *(uint4 *)(Hf + f * 8) = *(const uint4 *)(Hr + src);
-    if (LO) *(uint4 *)(Lf + f * 8) = *(const uint4 *)(Lr + src);
+    const float *src = W + (size_t)(16 * it + r) + (size_t)d * (32 * kt + 8 * g);
+    float w[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) w[e] = src[(size_t)d * e];
+    uint4 hi, lo;
+    split8<BF>(w, hi, lo);
+    *(uint4 *)(Hf + f * 8) = hi;
+    *(uint4 *)(Lf + f * 8) = lo;
 }
 
 // the same in another state order: fragment rows/columns are states i of
-// the output order, read from the copies' state perm[i] (a permutation)
-__global__ __launch_bounds__(256) void k_wfragp(int d, const uint16_t *__restrict__ Hr,
-                                                const uint16_t *__restrict__ Lr,
+// the output order, read from W's state perm[i] (a permutation)
+template <bool BF>
+__global__ __launch_bounds__(256) void k_wfragp(int d, const float *__restrict__ W,
                                                 const int *__restrict__ perm,
                                                 uint16_t *__restrict__ Hf, uint16_t *__restrict__ Lf) {
     const int nkt = d / 32;
@@ -722,22 +718,20 @@ __global__ __launch_bounds__(256) void k_wfragp(int d, const uint16_t *__restric
     const size_t tile = f >> 6;
     const int it = (int)(tile / nkt), kt = (int)(tile % nkt);
     const int r = l & 15, g = l >> 4;
-    const size_t row = (size_t)perm[16 * it + r] * d;
-    uint16_t h[8], lo[8];
+    const float *row = W + perm[16 * it + r];
+    float w[8];
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
-        const int k = perm[32 * kt + 8 * g + e];
-        h[e] = Hr[row + k];
-        lo[e] = Lr[row + k];
-    }
-    *(uint4 *)(Hf + f * 8) = *(const uint4 *)h;
-    *(uint4 *)(Lf + f * 8) = *(const uint4 *)lo;
+    for (int e = 0; e < 8; e++) w[e] = row[(size_t)d * perm[32 * kt + 8 * g + e]];
+    uint4 hi, lo;
+    split8<BF>(w, hi, lo);
+    *(uint4 *)(Hf + f * 8) = hi;
+    *(uint4 *)(Lf + f * 8) = lo;
 }
 
 // one lattice of a batched fragment layout and its tile lists (arena
-// offsets): row-major copies in, fragment copies, flags and lists out
+// offsets): FP32 W in, fragment copies, flags and lists out
 struct LayScen {
-    uint64_t Hr, Lr, Hf, Lf, flags, tl;
+    uint64_t W, Hf, Lf, flags, tl;
 };
 constexpr int kLayBatch = 32;
 struct LayBatch {
@@ -745,28 +739,36 @@ struct LayBatch {
 };
 
 // The same for the regime permutation of a lattice squared node-major
-// (copies' state nr x + r) stepped in the reference order (fragment state
-// x + nx r), through LDS: a workgroup reads one 64 x 64 block of the
-// row-major copies (64/nr nodes x nr regimes per side) with 16-byte loads
-// and writes every 8-column run of its fragment rows as one 16-byte store.
-// nr in {1, 2, 4, 8}, nx % 8 == 0, d = nx nr, d % 64 == 0.  Lattice
-// blockIdx.y of the batch.
+// (W's state nr x + r) stepped in the reference order (fragment state
+// x + nx r), through LDS: a workgroup reads one 64 x 64 block of W
+// (64/nr nodes x nr regimes per side; 16-byte loads down its columns),
+// keeps its hi/lo parts column by column in LDS and writes every 8-column
+// run of its fragment rows as one 16-byte store.  nr in {1, 2, 4, 8},
+// nx % 8 == 0, d = nx nr, d % 64 == 0.  Lattice blockIdx.y of the batch.
+template <bool BF>
 __global__ __launch_bounds__(256) void k_wfragk(int d, int nx, int nr, char *__restrict__ base,
                                                 const LayBatch bt) {
+    using N = Num<BF>;
     const LayScen &S = bt.s[blockIdx.y];
-    const uint16_t *__restrict__ Hr = at<const uint16_t>(base, S.Hr);
-    const uint16_t *__restrict__ Lr = at<const uint16_t>(base, S.Lr);
+    const float *__restrict__ W = at<const float>(base, S.W);
     uint16_t *__restrict__ Hf = at<uint16_t>(base, S.Hf);
     uint16_t *__restrict__ Lf = at<uint16_t>(base, S.Lf);
-    __shared__ __attribute__((aligned(16))) uint16_t sh[2][64][72];
+    __shared__ __attribute__((aligned(16))) uint16_t sh[2][64][68];  // [hi/lo][column][row]
     const int nb = d / 64, bi = blockIdx.x / nb, bj = blockIdx.x % nb, t = threadIdx.x;
     const int nkt = d / 32, NB = 64 / nr;
 #pragma unroll
-    for (int u = 0; u < 2; u++) {
-        const int q = t + 256 * u, row = q >> 3, ch = q & 7;  // 512 16-byte chunks per array
-        const size_t src = (size_t)(64 * bi + row) * d + 64 * bj + 8 * ch;
-        *(uint4 *)&sh[0][row][8 * ch] = *(const uint4 *)(Hr + src);
-        *(uint4 *)&sh[1][row][8 * ch] = *(const uint4 *)(Lr + src);
+    for (int u = 0; u < 4; u++) {
+        const int q = t + 256 * u, col = q >> 4, rq = q & 15;  // 4 rows of one column per thread
+        const f4 w = *(const f4 *)(W + (size_t)(64 * bi + 4 * rq) + (size_t)d * (64 * bj + col));
+        uint16_t h[4], l[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const float ws = w[e] * N::S;
+            h[e] = N::enc(ws);
+            l[e] = N::enc((ws - N::dec(h[e])) * N::G);
+        }
+        *(uint2 *)&sh[0][col][4 * rq] = *(const uint2 *)h;
+        *(uint2 *)&sh[1][col][4 * rq] = *(const uint2 *)l;
     }
     __syncthreads();
 #pragma unroll
@@ -780,8 +782,8 @@ __global__ __launch_bounds__(256) void k_wfragk(int d, int nx, int nr, char *__r
         uint16_t h[8], l[8];
 #pragma unroll
         for (int b = 0; b < 8; b++) {
-            h[b] = sh[0][rho][nr * (8 * c + b) + rp];
-            l[b] = sh[1][rho][nr * (8 * c + b) + rp];
+            h[b] = sh[0][nr * (8 * c + b) + rp][rho];
+            l[b] = sh[1][nr * (8 * c + b) + rp][rho];
         }
         *(uint4 *)(Hf + o) = *(const uint4 *)h;
         *(uint4 *)(Lf + o) = *(const uint4 *)l;
@@ -1356,16 +1358,6 @@ WScen make_scen(unsigned state, int d, int mode, int src) {
     const int dst = src ^ 1;
     s.W = p + (src ? L.W1 : L.W0);
     s.Wn = p + (dst ? L.W1 : L.W0);
-    s.Hr = p + L.H[src][0];
-    s.Hc = p + L.H[src][1];
-    s.Hrn = p + L.H[dst][0];
-    s.Hcn = p + L.H[dst][1];
-    // the one-product mode has no low parts: point them at the high parts
-    // (never read or written when LO is false)
-    s.Lr = mode ? p + L.H[src][2] : s.Hr;
-    s.Lc = mode ? p + L.H[src][3] : s.Hc;
-    s.Lrn = mode ? p + L.H[dst][2] : s.Hrn;
-    s.Lcn = mode ? p + L.H[dst][3] : s.Hcn;
     s.part = p + L.part;
     s.fl = p + L.fl[src];
     s.fln = p + L.fl[dst];
@@ -1661,9 +1653,9 @@ OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
         // every diagonal fix clears the counters for the next round
         const bool sparse = g_wsq_skip && P->sparse_ok;
         hipLaunchKernelGGL((k_wsplit<BF, LO>), dim3(P->n_split), dim3(64), 0, D.stream, D.base,
-                           P->sc_split, P->it_split, sparse ? 0 : 1);
+                           P->sc_split, P->it_split);
         int *counts = sparse ? P->counts : nullptr;
-        hipLaunchKernelGGL((k_wfix<BF, LO>), fixgrid, dim3(64), 0, D.stream, D.base, P->sc_split,
+        hipLaunchKernelGGL(k_wfix, fixgrid, dim3(64), 0, D.stream, D.base, P->sc_split,
                            P->n, (const int *)nullptr, 0, counts);
         for (int rr = 0; rr < rounds; rr++) {
             const int src = rr & 1;
@@ -1704,7 +1696,7 @@ OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
             // the fix kernel indexes lattices 0..n-1 of the table half for
             // this direction
             const WScen *scn = P->sc + (src ? P->n : 0);
-            hipLaunchKernelGGL((k_wfix<BF, LO>), fixgrid, dim3(64), 0, D.stream, D.base, scn, P->n,
+            hipLaunchKernelGGL(k_wfix, fixgrid, dim3(64), 0, D.stream, D.base, scn, P->n,
                                per ? (const int *)P->nsq_dev : (const int *)nullptr, rr, counts);
         }
     });
@@ -1909,12 +1901,11 @@ OPHIP_EXPORT int ophip_wsq_bwd_prep(int d, int ncp, int mode, unsigned X, unsign
 }
 
 namespace {
-// one lattice's entry of a batched layout: the copies of buffer buf, the
-// fragment copies in the other buffer's column-major slots
+// one lattice's entry of a batched layout: W of buffer buf, its fragment
+// copies in the other buffer's column-major copy slots
 LayScen lay_scen(const WsqLayout &L, unsigned state, int buf) {
     LayScen S;
-    S.Hr = state + L.H[buf][0];
-    S.Lr = state + L.H[buf][2];
+    S.W = state + (buf ? L.W1 : L.W0);
     S.Hf = state + L.H[1 - buf][1];
     S.Lf = state + L.H[1 - buf][3];
     S.flags = state + L.tf;
@@ -1942,27 +1933,28 @@ void bwd_tile_lists(Device &D, const WsqLayout &L, unsigned state, int d, int mo
 }
 }  // namespace
 
-// W's hi/lo copies of buffer `buf`, fragment-major, into the column-major
-// slots of the other buffer (free once the squarings are done); once per
-// solve, before the steps
+// W of buffer `buf` split into its hi/lo parts (as the squarings split
+// it), fragment-major, into the column-major copy slots of the other
+// buffer; once per solve, before the steps
 OPHIP_EXPORT int ophip_wsq_bwd_layout(unsigned state, int d, int mode, int buf) {
     if (!bwd_args_ok(d, 16, mode) || buf < 0 || buf > 1) return (int)set_status(kErrBadArg);
     const WsqLayout L = wsq_layout(d, mode);
     if (!handle_range_ok(state, L.total)) return (int)set_status(kErrBadHandle);
     Device &D = dev();
-    const uint16_t *Hr = P<const uint16_t>(state + (unsigned)L.H[buf][0]);
-    const uint16_t *Lr = P<const uint16_t>(state + (unsigned)L.H[buf][2]);
+    const float *W = P<const float>(state + (unsigned)(buf ? L.W1 : L.W0));
     uint16_t *Hf = P<uint16_t>(state + (unsigned)L.H[1 - buf][1]);
     uint16_t *Lf = P<uint16_t>(state + (unsigned)L.H[1 - buf][3]);
-    const size_t nf = (size_t)d * d / 8;
-    hipLaunchKernelGGL((k_wfrag<true>), dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, D.stream, d, Hr,
-                       Lr, Hf, Lf);
+    const dim3 grid((unsigned)(((size_t)d * d / 8 + 255) / 256));
+    if (mode == 2)
+        hipLaunchKernelGGL((k_wfrag<true>), grid, dim3(256), 0, D.stream, d, W, Hf, Lf);
+    else
+        hipLaunchKernelGGL((k_wfrag<false>), grid, dim3(256), 0, D.stream, d, W, Hf, Lf);
     bwd_tile_lists(D, L, state, d, mode, buf);
     return (int)check_launch();
 }
 
-// ophip_wsq_bwd_layout with the states reordered: fragment state i is the
-// copies' state perm[i] (perm: handle of d ints, a permutation of 0..d-1),
+// ophip_wsq_bwd_layout with the states reordered: fragment state i is W's
+// state perm[i] (perm: handle of d ints, a permutation of 0..d-1),
 // so the steps run in the order of perm's domain.  The step arithmetic is
 // the same in every order, its FP32 accumulation is not: a lattice squared
 // in a band order steps in its natural order.
@@ -1972,13 +1964,14 @@ OPHIP_EXPORT int ophip_wsq_bwd_layout_perm(unsigned state, int d, int mode, int 
     if (!handle_range_ok(state, L.total) || !handle_range_ok(perm, 4 * (size_t)d))
         return (int)set_status(kErrBadHandle);
     Device &D = dev();
-    const uint16_t *Hr = P<const uint16_t>(state + (unsigned)L.H[buf][0]);
-    const uint16_t *Lr = P<const uint16_t>(state + (unsigned)L.H[buf][2]);
+    const float *W = P<const float>(state + (unsigned)(buf ? L.W1 : L.W0));
     uint16_t *Hf = P<uint16_t>(state + (unsigned)L.H[1 - buf][1]);
     uint16_t *Lf = P<uint16_t>(state + (unsigned)L.H[1 - buf][3]);
-    const size_t nf = (size_t)d * d / 8;
-    hipLaunchKernelGGL(k_wfragp, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, D.stream, d, Hr, Lr,
-                       P<const int>(perm), Hf, Lf);
+    const dim3 grid((unsigned)(((size_t)d * d / 8 + 255) / 256));
+    if (mode == 2)
+        hipLaunchKernelGGL((k_wfragp<true>), grid, dim3(256), 0, D.stream, d, W, P<const int>(perm), Hf, Lf);
+    else
+        hipLaunchKernelGGL((k_wfragp<false>), grid, dim3(256), 0, D.stream, d, W, P<const int>(perm), Hf, Lf);
     bwd_tile_lists(D, L, state, d, mode, buf);
     return (int)check_launch();
 }
@@ -2129,8 +2122,11 @@ OPHIP_EXPORT int ophip_wsq_bwd_layout_regime_batch(int n, const unsigned *states
         const int m = std::min(kLayBatch, n - i0);
         LayBatch bt{};
         for (int i = 0; i < m; i++) bt.s[i] = lay_scen(L, states[i0 + i], buf);
-        hipLaunchKernelGGL(k_wfragk, dim3((unsigned)((d / 64) * (d / 64)), m), dim3(256), 0, D.stream, d, nx, nr,
-                           D.base, bt);
+        const dim3 grid((unsigned)((d / 64) * (d / 64)), m);
+        if (mode == 2)
+            hipLaunchKernelGGL((k_wfragk<true>), grid, dim3(256), 0, D.stream, d, nx, nr, D.base, bt);
+        else
+            hipLaunchKernelGGL((k_wfragk<false>), grid, dim3(256), 0, D.stream, d, nx, nr, D.base, bt);
         bwd_tile_lists(D, d, mode, bt, m);
         const unsigned e = check_launch();
         if (e) return (int)e;

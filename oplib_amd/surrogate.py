@@ -101,10 +101,13 @@ class PriceSolver:
     """The operator solver over batches of nb draws: one captured graph,
     re-parameterized per batch (opm_batch_update)."""
 
-    def __init__(self, nb=64, precision=M.OPM_PREC_F16X3):
+    def __init__(self, nb=64, precision=M.OPM_PREC_F16X3, tile=0):
+        """tile 0: the squaring plan the batch picks on timed solves (its
+        K-cell cutoff leaves out products per tile, so the bits depend on
+        the plan); 64 or 128: that tile for every lattice"""
         self.nb = nb
         mid = 0.5 * (PARAM_LO + PARAM_HI)
-        self.batch = M.Batch([config(mid, precision) for _ in range(nb)])
+        self.batch = M.Batch([config(mid, precision) for _ in range(nb)], tile=tile)
         self.precision = precision
         self.batch.capture()
 

@@ -367,13 +367,16 @@ def test_surrogate_gain_and_fallback(S, hip):
 
 def test_batch_update_matches_fresh_batch(S, hip):
     """re-parameterising a captured batch (opm_batch_update) gives the bits a
-    batch created for those parameters gives; mismatched grids are refused"""
+    batch created for those parameters gives, with the same squaring plan
+    (one tile for both: a plan picked on timed solves may differ from batch
+    to batch, and the K-cell cutoff is taken per tile); mismatched grids
+    are refused"""
     from oplib_amd import opmodel as M
     banner("batch update = fresh batch")
     prm = S.draw_params(8, 555)
-    sol = S.PriceSolver(nb=8)
+    sol = S.PriceSolver(nb=8, tile=128)
     got = sol.solve(prm)
-    fresh = M.Batch([S.config(p) for p in prm])
+    fresh = M.Batch([S.config(p) for p in prm], tile=128)
     fresh.solve()
     ref = fresh.prices().reshape(8, -1)
     fresh.close()

@@ -138,15 +138,11 @@ def test_split_and_one_squaring(hip, mode, tile):
         W0 = L.arr(0, 0, np.float32)
         assert np.abs(W0 - L.W0).max() <= 1e-6 * np.abs(L.W0).max() + 1e-12  # diag fix only
         assert np.abs(W0.astype(np.float64).sum(axis=1)).max() < 1e-6 * np.abs(W0).max()
-        hr = L.arr(0, 1, np.uint16, "C")  # row-major copy: element (i,j) at i*d + j
-        hc = L.arr(0, 2, np.uint16, "F")
+        # flush rule: an element whose 16-bit parts are both +-0 is stored as +0
         want_h, want_l = enc(W0, mode)
-        assert np.array_equal(hc, bits16(want_h if mode == 2 else want_h, mode))
-        assert np.array_equal(hr, hc)
-        if mode > 0:
-            lr, lc = L.arr(0, 3, np.uint16, "C"), L.arr(0, 4, np.uint16, "F")
-            assert np.array_equal(lc, bits16(want_l, mode))
-            assert np.array_equal(lr, lc)
+        flushed = (want_h == 0) & ((want_l == 0) | (mode == 0))
+        assert np.all(W0.view(np.uint32)[flushed] == 0)
+        assert np.any(flushed)  # the banded lattice has such elements
         # one squaring
         L.first_step()
         H.check(H.lib.ophip_wsq_exec(p, 1))
@@ -313,9 +309,10 @@ def _backward_step_case(H, mode, dense):
     H.sync()
     H.lib.ophip_wsq_plan_free(p)
     d, ncp, buf = L.d, 48, 1
-    W = L.arr(buf, 0, np.float32).astype(np.float64)
-    Hr = dec16(L.arr(buf, 1, np.uint16, "C"), mode).astype(np.float64)
-    Lr = dec16(L.arr(buf, 3, np.uint16, "C"), mode).astype(np.float64)
+    W32 = L.arr(buf, 0, np.float32)
+    W = W32.astype(np.float64)
+    h32, lo32 = enc(W32, mode)  # the parts the layout splits W into
+    Hr, Lr = h32.astype(np.float64), lo32.astype(np.float64)
     S, G = (1.0, G_BF) if mode == 2 else (S16, G16)
     X = np.random.RandomState(3).uniform(0.0, 300.0, size=(d, ncp)).astype(np.float32)
     Xd = H.DeviceArray.from_numpy(X, order="C")
@@ -326,7 +323,7 @@ def _backward_step_case(H, mode, dense):
     H.check(H.lib.ophip_wsq_bwd_prep(d, ncp, mode, Xd.h, XT.h))
     ev, el = get_cutoffs(H)
     vis, lo = (np.kron(m, np.ones((16, 32))) for m in
-               tile_flags(L.arr(buf, 1, np.uint16, "C"), d, mode, (ev, el)))
+               tile_flags(bits16(h32, mode), d, mode, (ev, el)))
     df = 0.9999
     H.check(H.lib.ophip_wsq_bwd_step(L.state.h, d, mode, buf, ncp, np.float32(df), Xd.h, XT.h,
                                      Yd.h, YT.h))
@@ -575,8 +572,8 @@ def test_wform_backward_layout_perm(hip):
     H.sync()
     H.lib.ophip_wsq_plan_free(p)
     d = L.d
-    Hr = L.arr(buf, 1, np.uint16, "C")
-    Lr = L.arr(buf, 3, np.uint16, "C")
+    h32, lo32 = enc(L.arr(buf, 0, np.float32), mode)  # the parts the layout splits W into
+    Hr, Lr = bits16(h32, mode), bits16(lo32, mode)
     perm = np.random.RandomState(7).permutation(d).astype(np.int32)
     pd = H.DeviceArray.from_numpy(perm)
     H.check(H.lib.ophip_wsq_bwd_layout_perm(L.state.h, d, mode, buf, pd.h))
@@ -671,8 +668,8 @@ def test_fp8_split_and_one_squaring(hip):
 @pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("tile", [64, 128])
 def test_zero_cell_skip_same_bits(hip, mode, tile):
-    """zero-cell skipping: the flags are exact (flag 0 <=> the cell's copies
-    are all +-0, checked cell by cell on every buffer), the skip is real on a
+    """zero-cell skipping: the flags are exact (flag 0 <=> the cell's FP32
+    values are all +0, checked cell by cell on every buffer), the skip is real on a
     banded lattice (most cells are zero after the first squarings), and the
     squarings give the same bits as the dense K loop"""
     banner(f"wsq zero-cell skip mode={mode} tile={tile}")
@@ -681,11 +678,8 @@ def test_zero_cell_skip_same_bits(hip, mode, tile):
 
     def cells_zero(L, buf):
         d, nc = L.d, L.d // 64
-        z = np.ones((nc, nc), dtype=bool)
-        for which in ((1, 2, 3, 4) if mode else (1, 2)):
-            a = L.arr(buf, which, np.uint16, "C" if which in (1, 3) else "F") & 0x7FFF
-            z &= ~a.reshape(nc, 64, nc, 64).any(axis=(1, 3))
-        return z
+        a = L.arr(buf, 0, np.uint32)  # +0 only: the flush stores +0
+        return ~(a != 0).reshape(nc, 64, nc, 64).any(axis=(1, 3))
 
     def flags(L, buf):
         nc = L.d // 64
@@ -706,9 +700,7 @@ def test_zero_cell_skip_same_bits(hip, mode, tile):
                     L.first_step()
                 H.check(H.lib.ophip_wsq_exec(p, nsq))
                 H.sync()
-                got[skip] = [[L.arr(b, w, np.uint32 if w == 0 else np.uint16) for w in
-                              ((0, 1, 2, 3, 4) if mode else (0, 1, 2))] for L in lats
-                             for b in (0, 1)]
+                got[skip] = [[L.arr(b, 0, np.uint32)] for L in lats for b in (0, 1)]
                 if skip:
                     for L in lats:
                         for b in (0, 1):
@@ -772,8 +764,7 @@ def test_later_solve_same_bits(hip, mode):
     solve left values (the destination's flags), gives the bits of a fresh
     state's solve: after a wide-band solve (10 squarings) and a backward
     layout (fragment copies written into the other buffer's column-major
-    slots), a narrow one (4 squarings); both buffers' FP32 W and row-major
-    copies, and the column-major copies of every flagged cell"""
+    copy slots), a narrow one (4 squarings): both buffers' W, every bit"""
     banner(f"wsq later solve after a wide one and a layout, mode={mode}")
     H = hip
     A, B = Lattice(H, 128, 2, mode), Lattice(H, 128, 2, mode)
@@ -793,15 +784,12 @@ def test_later_solve_same_bits(hip, mode):
             fa = np.empty(nc * nc, dtype=np.uint8)
             H.check(H.lib.opcuda_memcpy_d2h(H.lib.ophip_wsq_array(A.state.h, d, mode, buf, 6),
                                             fa.ctypes.data, fa.nbytes))
-            cell = np.kron(fa.reshape(nc, nc) != 0, np.ones((64, 64), dtype=bool))
-            for which in (0, 1, 2, 3, 4):
-                dt = np.uint32 if which == 0 else np.uint16
-                order = "F" if which in (2, 4) else "C"
-                a, b = A.arr(buf, which, dt, order), B.arr(buf, which, dt, order)
-                if which in (2, 4):
-                    assert np.array_equal(a[cell], b[cell]), (buf, which)
-                else:
-                    assert np.array_equal(a, b), (buf, which)
+            assert np.array_equal(A.arr(buf, 0, np.uint32), B.arr(buf, 0, np.uint32)), buf
+            # the flags of both states agree too
+            fb = np.empty(nc * nc, dtype=np.uint8)
+            H.check(H.lib.opcuda_memcpy_d2h(H.lib.ophip_wsq_array(B.state.h, d, mode, buf, 6),
+                                            fb.ctypes.data, fb.nbytes))
+            assert np.array_equal(fa, fb), buf
     finally:
         H.lib.ophip_wsq_plan_free(pa)
         H.lib.ophip_wsq_plan_free(pb)
@@ -809,11 +797,10 @@ def test_later_solve_same_bits(hip, mode):
 
 @pytest.mark.parametrize("mode", [1, 2])
 def test_dense_solve_after_layout_same_bits(hip, mode):
-    """the dense K loop (ophip_wsq_set_skip(0)) reads every cell's copies:
+    """the dense K loop (ophip_wsq_set_skip(0)) reads every cell of W:
     after an odd-count solve and a backward layout (fragment copies written
-    into buffer 0's column-major slots, where many cells are flagged zero),
-    a dense solve gives a fresh state's bits in every array of both
-    buffers"""
+    into buffer 0's column-major copy slots), a dense solve gives a fresh
+    state's bits in W of both buffers"""
     banner(f"wsq dense solve after a layout, mode={mode}")
     H = hip
     A, B = Lattice(H, 512, 1, mode), Lattice(H, 512, 1, mode)  # one band: zero cells
@@ -833,9 +820,7 @@ def test_dense_solve_after_layout_same_bits(hip, mode):
             H.check(H.lib.ophip_wsq_exec(pb, nsq))
             H.sync()
             for buf in (0, 1):
-                for which in (0, 1, 2, 3, 4):
-                    dt = np.uint32 if which == 0 else np.uint16
-                    assert np.array_equal(A.arr(buf, which, dt), B.arr(buf, which, dt)), (nsq, buf, which)
+                assert np.array_equal(A.arr(buf, 0, np.uint32), B.arr(buf, 0, np.uint32)), (nsq, buf)
             H.check(H.lib.ophip_wsq_bwd_layout(A.state.h, d, mode, nsq & 1))
             H.check(H.lib.ophip_wsq_bwd_layout(B.state.h, d, mode, nsq & 1))
     finally:
