@@ -1164,7 +1164,14 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd_blk(char *__restrict__ base, c
     const size_t n = (size_t)d * NC, rstride = (size_t)nkt * 512;  // elements per 16-row strip
     const int *lst = at<const int>(base, S.tl) + (RB == 4 ? tl_list4(d) : tl_list2(d)) +
                      (size_t)rb * (nkt + 1);
-    const int cnt = __builtin_amdgcn_readfirstlane(lst[0]);
+    // the list's head in one load: lane l holds lst[l] (the count, then
+    // the first 63 entries), read back with v_readlane; later entries
+    // (d > 2016) are loaded when needed
+    const int head = lane <= nkt ? lst[lane] : 0;
+    const int cnt = __builtin_amdgcn_readfirstlane(head);
+    auto entry = [&](int idx) {  // idx wave-uniform, < cnt
+        return idx < 63 ? __builtin_amdgcn_readlane(head, 1 + idx) : __builtin_amdgcn_readfirstlane(lst[1 + idx]);
+    };
     const uint16_t *ha = at<const uint16_t>(base, S.Hf) + frag_off(RB * rb, 0, lane, nkt);
     const uint16_t *la = at<const uint16_t>(base, S.Lf) + frag_off(RB * rb, 0, lane, nkt);
     const uint16_t *xh = at<const uint16_t>(base, S.XT) + frag_off(0, 0, lane, nkt);
@@ -1212,7 +1219,7 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd_blk(char *__restrict__ base, c
         s8 w[E][2 * RB], x[E][2 * NT];
 #pragma unroll
         for (int e = 0; e < E; e++) {
-            const int ent = __builtin_amdgcn_readfirstlane(lst[1 + min(e0 + e * NW, cnt - 1)]);
+            const int ent = entry(min(e0 + e * NW, cnt - 1));
             const size_t k = (size_t)512 * (ent & 0xffff);
 #pragma unroll
             for (int i = 0; i < RB; i++) {  // uniform: does row tile i visit this k tile, read its lo part
@@ -1980,6 +1987,10 @@ namespace {
 // backward-step kernel: 0 dense k_wbwd1, 1 automatic, 2 k_wbwd_sp (16-row
 // tiles), 3 k_wbwd_blk with 64-row blocks, 4 with 32-row blocks
 int g_bwd_kernel = 1;
+// list entries each wave of the 32-row-block step has in flight
+#ifndef OPHIP_BWD_E
+#define OPHIP_BWD_E 2
+#endif
 }  // namespace
 
 OPHIP_EXPORT int ophip_wsq_bwd_step(unsigned state, int d, int mode, int buf, int ncp, float df,
@@ -2085,10 +2096,10 @@ OPHIP_EXPORT int ophip_wsq_bwd_step_batch(int nb, const unsigned *states, int d,
                 hipLaunchKernelGGL((k_wbwd_blk<true, 4, 4, 2>), grid, dim3(256), 0, D.stream, D.base, bt, m, d,
                                    per_xcd);
             else if (mode == 1)
-                hipLaunchKernelGGL((k_wbwd_blk<false, 2, 8, 2>), grid, dim3(512), 0, D.stream, D.base, bt, m, d,
+                hipLaunchKernelGGL((k_wbwd_blk<false, 2, 8, OPHIP_BWD_E>), grid, dim3(512), 0, D.stream, D.base, bt, m, d,
                                    per_xcd);
             else
-                hipLaunchKernelGGL((k_wbwd_blk<true, 2, 8, 2>), grid, dim3(512), 0, D.stream, D.base, bt, m, d,
+                hipLaunchKernelGGL((k_wbwd_blk<true, 2, 8, OPHIP_BWD_E>), grid, dim3(512), 0, D.stream, D.base, bt, m, d,
                                    per_xcd);
         }
         const unsigned e = check_launch();
