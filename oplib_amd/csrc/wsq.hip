@@ -1149,7 +1149,10 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd_blk(char *__restrict__ base, c
     constexpr int NT = 3, NC = 48, NTILE = RB * NT, ROWS = 16 * RB;
     static_assert(RB == 2 || RB == 4, "row blocks of 32 or 64 rows");
     __shared__ f4 red[NW][NTILE][64];
-    __shared__ uint16_t th[NC][ROWS], tlo[NC][ROWS];
+    // Y's copies column by column; 8 elements of padding per column (an
+    // 80-byte pitch for 32 rows) spread the columns of a wave's 2-byte
+    // writes and 16-byte reads over the banks
+    __shared__ __attribute__((aligned(16))) uint16_t th[NC][ROWS + 8], tlo[NC][ROWS + 8];
     constexpr int NKB = ROWS / 32;  // k tiles of this block's rows
     __shared__ __attribute__((aligned(16))) uint16_t xs[2][NKB][NT][64][8];  // own rows' value copies (X = 0)
     const int nrb = d / ROWS;
