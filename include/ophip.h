@@ -189,18 +189,18 @@ int ophip_saxpby(int n, float a, unsigned A, float b, unsigned B, unsigned C);
  * reference's per-round active lists, CModel Common.cs:508-545). */
 unsigned long ophip_wsq_bytes(int d, int mode);
 unsigned      ophip_wsq_w(unsigned state, int d, int mode, int buf);
-/* any array of a state: which = 0 W (FP32, column-major), 1-4 the
- * buffer's four 16-bit copy slots (2 and 4 of the buffer the squarings did
- * not end in hold the backward layout's fragment-major hi and lo parts of
- * W; the squarings split W as they stage it and store no copies), 5 partial
- * row sums, 6 cell flags (d/64 x d/64 bytes; 0 = every W value of that
- * 64 x 64 cell is +0; otherwise 96 + E with the cell's largest |W| below
- * 2^E, or 255 where no bound is known), 7 backward-step tile lists (d/16 + d/64 +
+/* any array of a state: which = 0 W of buffer buf (FP32, column-major),
+ * 2 and 4 the backward layout's fragment-major hi and lo parts of W (buf
+ * ignored; modes 1 and 2; the squarings split W as they stage it and store
+ * no copies), 5 partial row sums, 6 cell flags of buffer buf (d/64 x d/64
+ * bytes; 0 = every W value of that 64 x 64 cell is +0; otherwise 96 + E with
+ * the cell's largest |W| below 2^E, or 255 where no bound is known), 7
+ * backward-step tile lists (d/16 + d/64 +
  * d/32 lists of d/32 + 1 ints, per 16-row tile, 64-row block and 32-row
  * block: count, then the visited k tiles, kt + 65536 where the lo part is
  * read too, + 2^(17+q) in a block of nr row tiles where its tile q visits
  * kt and + 2^(17+nr+q) where that tile reads the lo part; buf ignored)
- * (0 when the mode has no such array) */
+ * (0 for which = 1 or 3 and when the mode has no such array) */
 unsigned      ophip_wsq_array(unsigned state, int d, int mode, int buf, int which);
 /* Zero-cell skipping: the squaring visits only the K cells whose A and B
  * cells are flagged nonzero (the skipped products are exact zeros: same

@@ -533,10 +533,7 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
 // as all ones when a plan is made (k_fill8) and the split and every round
 // keep them a superset of their buffers' nonzero cells, so a lattice's
 // first round (and split) of a later solve writes only where the previous
-// solve left values.  (The backward layout writes its fragment copies into the
-// column-major copy slots of the buffer the squarings did not end in:
-// those slots are rewritten by the round that next flags their cells, and
-// nothing reads an unflagged cell.)
+// solve left values.
 // Live items are appended to the list of their XCD (static position j % 8:
 // the xcd_order placement); the list order within an XCD does not change
 // any result.  Requires d <= 4096 (64 cells).
@@ -1327,7 +1324,7 @@ __global__ __launch_bounds__(64 * NW) void k_wbwd_blk(char *__restrict__ base, c
 // mode: 0 = FP16 one product, 1 = FP16 three products (hi/lo),
 //       2 = BF16 three products
 struct WsqLayout {
-    size_t W0, W1, H[2][4], part, fl[2], tl, tf, total;  // byte offsets; H[buf][Hr,Hc,Lr,Lc]
+    size_t W0, W1, Fh, Fl, part, fl[2], tl, tf, total;  // byte offsets; Fh/Fl: -1 in mode 0
 };
 
 static WsqLayout wsq_layout(int d, int mode) {
@@ -1340,9 +1337,10 @@ static WsqLayout wsq_layout(int d, int mode) {
     };
     L.W0 = take(4 * dd);
     L.W1 = take(4 * dd);
-    int narr = mode == 0 ? 2 : 4;
-    for (int b = 0; b < 2; b++)
-        for (int a = 0; a < 4; a++) L.H[b][a] = a < narr ? take(2 * dd) : (size_t)-1;
+    // the backward layout's fragment-major hi/lo parts of the final W (the
+    // three-product modes only: the steps read both)
+    L.Fh = mode ? take(2 * dd) : (size_t)-1;
+    L.Fl = mode ? take(2 * dd) : (size_t)-1;
     L.part = take(8 * (size_t)(d / 64) * d);
     for (int b = 0; b < 2; b++) L.fl[b] = take((size_t)(d / 64) * (d / 64));  // cell flags
     // backward-step tile lists: per 16-row tile, 64-row block, 32-row block
@@ -1734,14 +1732,13 @@ OPHIP_EXPORT unsigned ophip_wsq_w(unsigned state, int d, int mode, int buf) {
     return state + (unsigned)(buf ? L.W1 : L.W0);
 }
 
-// handle of one array of a state: which = 0 W (FP32), 1 Hr, 2 Hc, 3 Lr,
-// 4 Lc (16-bit copies, row-/column-major), 5 partial row sums (FP64),
-// 6 cell flags of the copies (d/64 x d/64 bytes, row-major; 0 = every
-// FP32 W and row-major copy value of the 64 x 64 cell is +-0, and so is
-// every column-major copy value unless a backward layout has written its
-// fragment copies into that buffer's column-major slots since; otherwise
-// the magnitude bound 96 + E, 255 unknown); 0 for arrays the mode does not
-// have
+// handle of one array of a state: which = 0 W of buffer buf (FP32,
+// column-major), 2 and 4 the backward layout's fragment-major hi and lo
+// parts of W (buf ignored; three-product modes), 5 partial row sums (FP64),
+// 6 cell flags of buffer buf (d/64 x d/64 bytes, row-major; 0 = every W
+// value of the 64 x 64 cell is +0, otherwise the magnitude bound 96 + E,
+// 255 unknown), 7 backward-step tile lists; 0 for 1, 3 and arrays the mode
+// does not have
 OPHIP_EXPORT unsigned ophip_wsq_array(unsigned state, int d, int mode, int buf, int which) {
     if (d <= 0 || d % 64 || mode < 0 || mode > 2 || buf < 0 || buf > 1 || which < 0 || which > 7)
         return 0;
@@ -1751,7 +1748,9 @@ OPHIP_EXPORT unsigned ophip_wsq_array(unsigned state, int d, int mode, int buf, 
     else if (which == 5) off = L.part;
     else if (which == 6) off = L.fl[buf];
     else if (which == 7) off = L.tl;
-    else off = L.H[buf][which - 1];
+    else if (which == 2) off = L.Fh;
+    else if (which == 4) off = L.Fl;
+    else off = (size_t)-1;  // 1, 3: no such array
     if (off == (size_t)-1) return 0;
     return state + (unsigned)off;
 }
@@ -1911,13 +1910,13 @@ OPHIP_EXPORT int ophip_wsq_bwd_prep(int d, int ncp, int mode, unsigned X, unsign
 }
 
 namespace {
-// one lattice's entry of a batched layout: W of buffer buf, its fragment
-// copies in the other buffer's column-major copy slots
+// one lattice's entry of a batched layout: W of buffer buf, the state's
+// fragment arrays
 LayScen lay_scen(const WsqLayout &L, unsigned state, int buf) {
     LayScen S;
     S.W = state + (buf ? L.W1 : L.W0);
-    S.Hf = state + L.H[1 - buf][1];
-    S.Lf = state + L.H[1 - buf][3];
+    S.Hf = state + L.Fh;
+    S.Lf = state + L.Fl;
     S.flags = state + L.tf;
     S.tl = state + L.tl;
     return S;
@@ -1944,16 +1943,16 @@ void bwd_tile_lists(Device &D, const WsqLayout &L, unsigned state, int d, int mo
 }  // namespace
 
 // W of buffer `buf` split into its hi/lo parts (as the squarings split
-// it), fragment-major, into the column-major copy slots of the other
-// buffer; once per solve, before the steps
+// it), fragment-major, into the state's fragment arrays; once per solve,
+// before the steps
 OPHIP_EXPORT int ophip_wsq_bwd_layout(unsigned state, int d, int mode, int buf) {
     if (!bwd_args_ok(d, 16, mode) || buf < 0 || buf > 1) return (int)set_status(kErrBadArg);
     const WsqLayout L = wsq_layout(d, mode);
     if (!handle_range_ok(state, L.total)) return (int)set_status(kErrBadHandle);
     Device &D = dev();
     const float *W = P<const float>(state + (unsigned)(buf ? L.W1 : L.W0));
-    uint16_t *Hf = P<uint16_t>(state + (unsigned)L.H[1 - buf][1]);
-    uint16_t *Lf = P<uint16_t>(state + (unsigned)L.H[1 - buf][3]);
+    uint16_t *Hf = P<uint16_t>(state + (unsigned)L.Fh);
+    uint16_t *Lf = P<uint16_t>(state + (unsigned)L.Fl);
     const dim3 grid((unsigned)(((size_t)d * d / 8 + 255) / 256));
     if (mode == 2)
         hipLaunchKernelGGL((k_wfrag<true>), grid, dim3(256), 0, D.stream, d, W, Hf, Lf);
@@ -1975,8 +1974,8 @@ OPHIP_EXPORT int ophip_wsq_bwd_layout_perm(unsigned state, int d, int mode, int 
         return (int)set_status(kErrBadHandle);
     Device &D = dev();
     const float *W = P<const float>(state + (unsigned)(buf ? L.W1 : L.W0));
-    uint16_t *Hf = P<uint16_t>(state + (unsigned)L.H[1 - buf][1]);
-    uint16_t *Lf = P<uint16_t>(state + (unsigned)L.H[1 - buf][3]);
+    uint16_t *Hf = P<uint16_t>(state + (unsigned)L.Fh);
+    uint16_t *Lf = P<uint16_t>(state + (unsigned)L.Fl);
     const dim3 grid((unsigned)(((size_t)d * d / 8 + 255) / 256));
     if (mode == 2)
         hipLaunchKernelGGL((k_wfragp<true>), grid, dim3(256), 0, D.stream, d, W, P<const int>(perm), Hf, Lf);
@@ -2074,8 +2073,8 @@ OPHIP_EXPORT int ophip_wsq_bwd_step_batch(int nb, const unsigned *states, int d,
         for (int i = 0; i < m; i++) {
             const unsigned st = states[i0 + i];
             BwdScen &S = bt.s[i];
-            S.Hf = st + L.H[1 - buf][1];
-            S.Lf = st + L.H[1 - buf][3];
+            S.Hf = st + L.Fh;
+            S.Lf = st + L.Fl;
             S.tl = st + L.tl;
             S.X = X[i0 + i];
             S.XT = XT[i0 + i];
@@ -2160,8 +2159,8 @@ OPHIP_EXPORT int ophip_wsq_bwd_step(unsigned state, int d, int mode, int buf, in
         return (int)set_status(kErrBadHandle);
     if (g_bwd_kernel) return ophip_wsq_bwd_step_batch(1, &state, d, mode, buf, ncp, &df, &X, &XT, &Y, &YT);
     Device &D = dev();
-    const uint16_t *Hf = P<const uint16_t>(state + (unsigned)L.H[1 - buf][1]);
-    const uint16_t *Lf = P<const uint16_t>(state + (unsigned)L.H[1 - buf][3]);
+    const uint16_t *Hf = P<const uint16_t>(state + (unsigned)L.Fh);
+    const uint16_t *Lf = P<const uint16_t>(state + (unsigned)L.Fl);
     const uint16_t *xt = P<const uint16_t>(XT);
     const float *x = P<const float>(X);
     float *y = P<float>(Y);
