@@ -91,6 +91,9 @@ struct WScen {
 // cells already hold zeros (flags of the buffer's previous contents) writes
 // nothing at all, and the diagonal fix skips the partial sums of zero cells.
 
+// most 16-item chunks in one XCD's live list of a round
+constexpr int kLiveChunks = 1024;
+
 // work item = (scenario, tile)
 struct WItem {
     int s, tm, tn, pad;
@@ -205,8 +208,11 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
                                              const int *__restrict__ counts, int cap) {
     // Work: the static list items[0..nitems) (every K-step of every item),
     // or, when live is given, the live items of the round (k_wlive): list x
-    // = live + x cap (counts[x] items) is taken by the workgroups on XCD x
-    // (b % 8 == x; the grid is a multiple of 8), with each item's K cells.
+    // = live + x cap, in chunks of 16 slots of which counts[x cap / 16 + c]
+    // are used, is taken in its static order by the workgroups on XCD x
+    // (b % 8 == x; the grid is a multiple of 8), with each item's K cells:
+    // the workgroups of an XCD then work on neighbouring tiles of one
+    // lattice at a time, whose operand panels its L2 shares.
     using N = Num<BF>;
     // K-step BK = 64 for 128-tiles (two MFMA k-steps per LDS stage, two
     // register stages in flight), 32 for 64-tiles (four stages).  LDS rows
@@ -225,20 +231,49 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
     __shared__ double rsum[WGN][BT];
     __shared__ int wnz[NW];
     __shared__ float wmx[NW];  // each wave's largest |W'| (its block lies in one cell)
+    __shared__ int pref[kLiveChunks + 1];  // live items before chunk c of this XCD's list
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int r = lane & 15, g = lane >> 4;
     const int wm0 = (wave & 1) * WTM, wn0 = (wave >> 1) * WTN;
 
     const WLive *list = live ? live + (size_t)(blockIdx.x & 7) * cap : nullptr;
-    const int nlist = live ? counts[blockIdx.x & 7] : nitems;
+    int nlist = nitems, nch = 0;
+    if (live) {
+        // the chunk counts of this XCD chained in order (wave 0, 64 at a time)
+        nch = ((nitems + 7) / 8 + 15) / 16;  // the chunks k_wlive covered
+        const int *cnt = counts + (blockIdx.x & 7) * (cap / 16);
+        if (wave == 0) {
+            int run = 0;
+            if (lane == 0) pref[0] = 0;
+            for (int c0 = 0; c0 < nch; c0 += 64) {
+                int v = c0 + lane < nch ? cnt[c0 + lane] : 0;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int u = __shfl_up(v, o, 64);
+                    if (lane >= o) v += u;
+                }
+                if (c0 + lane < nch) pref[c0 + lane + 1] = run + v;
+                run += __shfl(v, 63, 64);
+            }
+        }
+        __syncthreads();
+        nlist = pref[nch];
+    }
     const int first = live ? (int)(blockIdx.x >> 3) : (int)blockIdx.x;
     const int stride = live ? (int)(gridDim.x >> 3) : (int)gridDim.x;
     for (int item = first; item < nlist; item += stride) {
         WItem it;
         uint64_t cells = 0;
         if (live) {
-            const WLive lv = list[item];
+            // the chunk holding list entry `item`: the last c with pref[c] <= item
+            int lo = 0, hi = nch - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (pref[mid] <= item) lo = mid;
+                else hi = mid - 1;
+            }
+            const WLive lv = list[16 * lo + (item - pref[lo])];
             it = WItem{lv.s, lv.tm, lv.tn, 0};
             cells = lv.cells;
         } else {
@@ -543,12 +578,13 @@ __global__ __launch_bounds__(1024) void k_wlive(char *__restrict__ base, const W
                                                 WLive *__restrict__ live, int cap, int *__restrict__ counts,
                                                 int wskip, int kthr) {
     // block (x, chunk): the 16 items j = x + 8 (16 chunk + w) of XCD x, one
-    // per wave w; one atomic per block reserves their slots in list x
+    // per wave w; the live ones go, in w order, to slots 16 chunk .. of
+    // list x, and their number to counts[x cap / 16 + chunk], so the list
+    // keeps the static order (the squaring kernel chains the counts)
     __shared__ int slive[16];
-    __shared__ int sbase;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int x = blockIdx.x & 7;
-    const int j = x + 8 * (16 * (int)(blockIdx.x >> 3) + w);
+    const int x = blockIdx.x & 7, chunk = (int)(blockIdx.x >> 3);
+    const int j = x + 8 * (16 * chunk + w);
     bool ok = false;
     uint64_t cells = 0;
     WItem it{};
@@ -579,13 +615,12 @@ __global__ __launch_bounds__(1024) void k_wlive(char *__restrict__ base, const W
     if (threadIdx.x == 0) {
         int n = 0;
         for (int q = 0; q < 16; q++) n += slive[q];
-        sbase = n ? atomicAdd(&counts[x], n) : 0;
+        counts[x * (cap / 16) + chunk] = n;
     }
-    __syncthreads();
     if (ok && lane == 0) {
         int rank = 0;
         for (int q = 0; q < w; q++) rank += slive[q];
-        live[(size_t)x * cap + sbase + rank] = WLive{it.s, it.tm, it.tn, 0, cells, 0};
+        live[(size_t)x * cap + 16 * chunk + rank] = WLive{it.s, it.tm, it.tn, 0, cells, 0};
     }
 }
 
@@ -598,9 +633,7 @@ __global__ __launch_bounds__(256) void k_fill8(uint8_t *__restrict__ p, int n, u
 // One wavefront per 64 rows; the lanes of a row group read the d/64
 // partial slots of their rows in a fixed order.
 __global__ void k_wfix(char *__restrict__ base, const WScen *__restrict__ scen, int nscen,
-                       const int *__restrict__ nsq, int round, int *__restrict__ counts) {
-    // the live-list counters of the next round start at zero
-    if (counts && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 16) counts[threadIdx.x] = 0;
+                       const int *__restrict__ nsq, int round) {
     const int s = blockIdx.y;
     if (s >= nscen) return;
     if (nsq && round >= nsq[s]) return;  // this lattice has finished its squarings
@@ -1416,7 +1449,8 @@ struct WsqPlan {
     WItem *it_split = nullptr, *it_all[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     std::vector<WItem *> it_round[2];
     // sparse rounds (every lattice d <= 4096): per class, 8 XCD lists of
-    // cap live items and their counters (counts[8 c + x])
+    // cap live items (16-item chunks in static order) and the chunks'
+    // counts (class c: counts + 8 cap[0] / 16 c, XCD x: + x cap[c] / 16)
     bool sparse_ok = false;
     int cap[2] = {0, 0};
     WLive *live[2] = {nullptr, nullptr};
@@ -1544,11 +1578,14 @@ static WsqPlan *make_plan(int n, const unsigned *states, const int *ds, const in
     size_t nitems = 0;
     for (auto &l : lists) nitems += l.size();
     P->sparse_ok = P->dmax <= 4096;
-    for (int c = 0; c < 2; c++) P->cap[c] = (int)((lists[1 + c * (2 + P->rounds)].size() + 7) / 8);
+    // each XCD's list: 16-slot chunks for the 16-item blocks of k_wlive
+    for (int c = 0; c < 2; c++)
+        P->cap[c] = 16 * (int)(((lists[1 + c * (2 + P->rounds)].size() + 7) / 8 + 15) / 16);
+    if (P->cap[0] / 16 > kLiveChunks || P->cap[1] / 16 > kLiveChunks) P->sparse_ok = false;
     size_t bytes = sizeof(WScen) * 3 * n + sizeof(int) * (n + 4) + sizeof(WItem) * nitems;
     bytes = (bytes + 255) & ~(size_t)255;
     const size_t live_off = bytes;
-    bytes += sizeof(WLive) * 8 * (size_t)(P->cap[0] + P->cap[1]) + 16 * sizeof(int);
+    bytes += sizeof(WLive) * 8 * (size_t)(P->cap[0] + P->cap[1]) + sizeof(int) * (P->cap[0] + P->cap[1]) / 2;
     if (hipMalloc(&P->dev, bytes) != hipSuccess) {
         delete P;
         set_status(kErrOutOfArena);
@@ -1586,7 +1623,7 @@ static WsqPlan *make_plan(int n, const unsigned *states, const int *ds, const in
     for (size_t l = 0; l < lists.size() && e == hipSuccess; l++)
         if (!lists[l].empty())
             e = hipMemcpy(where[l], lists[l].data(), sizeof(WItem) * lists[l].size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(P->counts, 0, 16 * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(P->counts, 0, sizeof(int) * (P->cap[0] + P->cap[1]) / 2);
     if (e != hipSuccess) {
         (void)hipFree(P->dev);
         delete P;
@@ -1662,9 +1699,8 @@ OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
         const bool sparse = g_wsq_skip && P->sparse_ok;
         hipLaunchKernelGGL((k_wsplit<BF, LO>), dim3(P->n_split), dim3(64), 0, D.stream, D.base,
                            P->sc_split, P->it_split);
-        int *counts = sparse ? P->counts : nullptr;
         hipLaunchKernelGGL(k_wfix, fixgrid, dim3(64), 0, D.stream, D.base, P->sc_split,
-                           P->n, (const int *)nullptr, 0, counts);
+                           P->n, (const int *)nullptr, 0);
         for (int rr = 0; rr < rounds; rr++) {
             const int src = rr & 1;
             bool any = false;
@@ -1684,14 +1720,14 @@ OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
                 const int *cn = nullptr;
                 if (sparse) {
                     lv = P->live[c];
-                    cn = P->counts + 8 * c;
+                    cn = P->counts + (c ? P->cap[0] / 2 : 0);
                     const int kc = g_wsq_kcut.load(), kthr = kc <= -192 ? 0 : kc + 193;
                     if (c == 1)
                         hipLaunchKernelGGL((k_wlive<2>), dim3(8 * cdiv(cdiv(nit, 8), 16)), dim3(1024), 0, D.stream, D.base,
-                                           P->sc, its, nit, P->live[c], P->cap[c], P->counts + 8 * c, 1, kthr);
+                                           P->sc, its, nit, P->live[c], P->cap[c], P->counts + (c ? P->cap[0] / 2 : 0), 1, kthr);
                     else
                         hipLaunchKernelGGL((k_wlive<1>), dim3(8 * cdiv(cdiv(nit, 8), 16)), dim3(1024), 0, D.stream, D.base,
-                                           P->sc, its, nit, P->live[c], P->cap[c], P->counts + 8 * c, 1, kthr);
+                                           P->sc, its, nit, P->live[c], P->cap[c], P->counts + (c ? P->cap[0] / 2 : 0), 1, kthr);
                 }
                 if (c == 1)
                     hipLaunchKernelGGL((k_wsq<BF, LO, 128, 8>), dim3(g), dim3(512), 0, D.stream, D.base,
@@ -1705,7 +1741,7 @@ OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
             // this direction
             const WScen *scn = P->sc + (src ? P->n : 0);
             hipLaunchKernelGGL(k_wfix, fixgrid, dim3(64), 0, D.stream, D.base, scn, P->n,
-                               per ? (const int *)P->nsq_dev : (const int *)nullptr, rr, counts);
+                               per ? (const int *)P->nsq_dev : (const int *)nullptr, rr);
         }
     });
 }
