@@ -179,7 +179,7 @@ __global__ __launch_bounds__(64) void k_wsplit(char *__restrict__ base,
         const uint16_t h = N::enc(ws);
         const uint16_t l = LO ? N::enc((ws - N::dec(h)) * N::G) : (uint16_t)0;
         const unsigned nzv = (h | l) & 0x7fffu;
-        if (!nzv) {  // flush rule: both copies +-0 -> FP32 +0
+        if (!nzv) {  // flush rule: both 16-bit parts +-0 -> FP32 +0
             if (__builtin_bit_cast(uint32_t, w) != 0u) Wm[(size_t)i + (size_t)d * (j0 + c)] = 0.0f;
             w = 0.0f;
         }
@@ -308,8 +308,9 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
 
         auto swz = [](int row) { return CPR == 8 ? (row & 7) : ((row >> 1) & 3); };
         // Operands come from the FP32 W (column-major) and are split into
-        // their 16-bit hi/lo parts on the way to LDS, with the operations
-        // that make the stored copies (the same bits).  Waves 0 .. NW/2-1
+        // their 16-bit hi/lo parts on the way to LDS, with the same
+        // operations as the epilogue's flush test and the backward layout
+        // (the same bits).  Waves 0 .. NW/2-1
         // stage A, the others B, NF float4 loads per thread and stage.  A:
         // thread (rg, cg) loads a 4-row x KA-column block (rows 4 rg.., k =
         // KA cg ..; eight 128-byte column pieces per load instruction) and
@@ -465,7 +466,7 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
         float *Wn = at<float>(base, sc.Wn);
         const float inv = N::IS * N::IS;
         double rows[TM][4];
-        unsigned nzl = 0;  // any nonzero copy among this lane's elements
+        unsigned nzl = 0;  // any nonzero 16-bit part among this lane's elements
         float amax = 0.0f;  // the lane's largest |W'| after the flush
 #pragma unroll
         for (int a = 0; a < TM; a++)
@@ -493,7 +494,7 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
                     const uint16_t hc = N::enc(os);
                     const uint16_t lc = LO ? N::enc((os - N::dec(hc)) * N::G) : (uint16_t)0;
                     const unsigned nzv = (hc | lc) & 0x7fffu;
-                    if (!nzv) o[v] = 0.0f;  // flush rule: both copies +-0 -> FP32 +0
+                    if (!nzv) o[v] = 0.0f;  // flush rule: both 16-bit parts +-0 -> FP32 +0
                     nzl |= nzv;
                     amax = fmaxf(amax, fabsf(o[v]));
                     rows[a][v] += (double)o[v];
@@ -1550,7 +1551,7 @@ static WsqPlan *make_plan(int n, const unsigned *states, const int *ds, const in
     for (int i = 0; i < n; i++) {
         sc[i] = make_scen(states[i], ds[i], mode, 0);
         sc[n + i] = make_scen(states[i], ds[i], mode, 1);
-        // the split reads W0 and writes buffer 0's copies and diagonal
+        // the first split and diagonal fix work on W0 in place (buffer 0)
         sc_split[i] = make_scen(states[i], ds[i], mode, 1);
         sc_split[i].W = sc[i].W;
         P->dmax = std::max(P->dmax, ds[i]);
@@ -1907,7 +1908,7 @@ OPHIP_EXPORT int ophip_wsq_autotune(int n, const unsigned *states, const int *ds
 }
 
 // ---------------------------------------------------------------------
-// W-form backward induction (modes 1 and 2: the states hold L copies)
+// W-form backward induction (modes 1 and 2: the layout makes lo parts)
 // ---------------------------------------------------------------------
 namespace {
 template <class F> unsigned launch_bwd(int mode, int ncp, F f) {
