@@ -218,6 +218,23 @@ unsigned long ophip_wsq_plan(int n, const unsigned *states, const int *ds, const
                              int mode, int tile, int grid);
 int           ophip_wsq_exec(unsigned long plan, int nsq);
 void          ophip_wsq_plan_free(unsigned long plan);
+/* ophip_sek_rs_batch's FP64 assembly of W0 = dt*L (out_kind 2; node_major:
+ * 6) into buffer 0 of each lattice's state, fused with the first stage of
+ * ophip_wsq_exec: the flush rule, the partial row sums and buffer 0's cell
+ * flags.  ophip_wsq_exec_ex(plan, nsq, 1) then leaves that stage out; the
+ * results are the same bits as ophip_sek_rs_batch + ophip_wsq_exec.  Cells
+ * that are structurally zero and already flagged zero are not written, so
+ * buffer 0's flags must describe its contents (as plan creation and every
+ * exec leave them).  d = nx*nr % 64 == 0; mode as the plan's.  The
+ * reference assembles and exponentiates in separate passes (sek.cu:40-103,
+ * CModel Device.cs:216-228). */
+int           ophip_wsq_assemble_rs(int n, const int *nx, const int *nr, const unsigned *xval,
+                                    const unsigned *drift, const unsigned *vol, const double *q_off,
+                                    const double *dt, int node_major, int mode,
+                                    const unsigned *states);
+/* ophip_wsq_exec with flags: 1 = buffer 0 was written by
+ * ophip_wsq_assemble_rs (no split) */
+int           ophip_wsq_exec_ex(unsigned long plan, int nsq, int flags);
 /* The W form with FP8 e4m3 operands: per round, row / column power-of-two
  * scales, hi/lo e4m3 copies (fragment-major), W' = 2W + W*W on
  * v_mfma_f32_16x16x32_fp8_fp8 (three products, FP32 accumulate) and the

@@ -13,6 +13,7 @@
 //   ophip_sck_rows    cumulative tables stored row-contiguous for the
 //                     sampler (one pass over 8-row blocks up to d = 2048)
 #include "ophip_internal.h"
+#include "rs_lattice.h"
 
 using namespace ophip;
 
@@ -119,64 +120,9 @@ OPHIP_EXPORT void opcuda_sglvg(unsigned gen, int ni, int d, unsigned invm, unsig
 }
 
 // ---------------------------------------------------------------------
-// FP64 regime-switching local-vol lattice (LV is nr = 1).
-// States y = x + nx*r (S2DGrid.cs:89).  Row y: rates to x-1 and x+1 in the
-// same regime from the closed-form inverse of the 2x2 stencil of
-// S1DGrid.cs:78-147 applied to (drift, vol^2) as in CLVModel.cs:99-114;
-// rate q_off to every other regime at the same x; diagonal = -sum.  Rows at
-// x = 0 and x = nx-1 have no price moves (absorbing in x).
+// FP64 regime-switching local-vol lattice (RSLattice, rs_elem:
+// rs_lattice.h)
 // ---------------------------------------------------------------------
-
-struct RSLattice {
-    int nx, nr;
-    const double *xval;   // nx nodes
-    const double *drift;  // per state (nx*nr), in price units
-    const double *vol;    // per state (nx*nr), absolute vol in price units
-    double q_off;
-};
-
-__device__ inline void rs_rates(const RSLattice &L, int y, double &am, double &ap) {
-    int x = y % L.nx;
-    am = 0.0;
-    ap = 0.0;
-    if (x <= 0 || x >= L.nx - 1) return;
-    double m00 = L.xval[x - 1] - L.xval[x];
-    double m01 = L.xval[x + 1] - L.xval[x];
-    double m10 = m00 * m00, m11 = m01 * m01;
-    double det = m00 * m11 - m01 * m10;
-    double i00 = m11 / det, i10 = -m10 / det, i01 = -m01 / det, i11 = m00 / det;
-    double mu = L.drift[y], v = L.vol[y];
-    double s2 = v * v;
-    am = i00 * mu + i01 * s2;
-    ap = i10 * mu + i11 * s2;
-}
-
-// MODE 0: generator (FP64 out); MODE 1: I + dt*L; MODE 2: dt*L (U0 - I).
-// NM: node-major states y = nr*x + r (the same operator in another basis:
-// the 8 regimes of a node are neighbours, so the matrix is one band and
-// the W form's zero-cell skipping applies); drift/vol stay indexed by the
-// canonical state x + nx*r.
-template <int MODE, bool NM>
-__device__ inline double rs_elem(const RSLattice &L, double dt, int i, int j) {
-    const int nx = L.nx, nr = L.nr;
-    const int xi = NM ? i / nr : i % nx, ri = NM ? i % nr : i / nx;
-    const int xj = NM ? j / nr : j % nx, rj = NM ? j % nr : j / nx;
-    double g = 0.0;
-    if (ri == rj) {
-        if (xj == xi - 1 || xj == xi + 1 || xj == xi) {
-            double am, ap;
-            rs_rates(L, xi + nx * ri, am, ap);
-            if (xj == xi - 1) g = am;
-            else if (xj == xi + 1) g = ap;
-            else g = -(am + ap + (double)(L.nr - 1) * L.q_off);
-        }
-    } else if (xi == xj) {
-        g = L.q_off;
-    }
-    if (MODE == 0) return g;
-    if (MODE == 1) return (i == j ? 1.0 : 0.0) + dt * g;
-    return dt * g;
-}
 
 template <int MODE, typename OUT, bool NM>
 __global__ void k_rs_assemble(RSLattice L, double dt, OUT *__restrict__ out) {
@@ -220,16 +166,6 @@ __global__ __launch_bounds__(256) void k_rs_assemble_batch(const RSBatch b) {
             reinterpret_cast<double2 *>(o)[1] = make_double2(v[2], v[3]);
         }
     }
-}
-
-static bool rs_args(int nx, int nr, unsigned xval, unsigned drift, unsigned vol, RSLattice &L) {
-    if (nx < 3 || nr < 1 || (long)nx * nr > 65535) return false;
-    L.nx = nx;
-    L.nr = nr;
-    L.xval = P<const double>(xval);
-    L.drift = P<const double>(drift);
-    L.vol = P<const double>(vol);
-    return L.xval && L.drift && L.vol;
 }
 
 // FP64 generator (d = nx*nr, d*d doubles at `gen`)

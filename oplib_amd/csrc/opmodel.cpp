@@ -385,13 +385,23 @@ static int solve_body(opm_solver *s) {
         s->u_final = c->nsq & 1;
         s->kernel_h = s->U[s->u_final].h;
     } else {
-        // W0 = dt*L in FP32 straight from the FP64 assembly, then the
-        // W-form squarings with renormalisation
+        // W0 = dt*L in FP32 straight from the FP64 assembly (fused with
+        // the first split in the 16-bit modes), then the W-form squarings
+        // with renormalisation
         const bool f8 = s->wmode == 3;
-        const unsigned w0 = f8 ? ophip_w8_w(s->wstate.h, d, 0) : ophip_wsq_w(s->wstate.h, d, s->wmode, 0);
-        st = ophip_sek_rs(nx, nr, s->x.h, s->drift.h, s->vol.h, c->q_off, s->dt0, s->nm ? 6 : 2, w0);
-        if (st) return st;
-        st = f8 ? ophip_w8_exec(s->wstate.h, d, c->nsq) : ophip_wsq_exec(s->wplan, c->nsq);
+        if (f8) {
+            st = ophip_sek_rs(nx, nr, s->x.h, s->drift.h, s->vol.h, c->q_off, s->dt0, s->nm ? 6 : 2,
+                              ophip_w8_w(s->wstate.h, d, 0));
+            if (st) return st;
+            st = ophip_w8_exec(s->wstate.h, d, c->nsq);
+        } else {
+            const unsigned h = s->wstate.h;
+            const double q = c->q_off;
+            st = ophip_wsq_assemble_rs(1, &nx, &nr, &s->x.h, &s->drift.h, &s->vol.h, &q, &s->dt0,
+                                       s->nm ? 1 : 0, s->wmode, &h);
+            if (st) return st;
+            st = ophip_wsq_exec_ex(s->wplan, c->nsq, 1);
+        }
         if (st) return st;
         s->u_final = c->nsq & 1;
         s->kernel_h = f8 ? ophip_w8_w(s->wstate.h, d, s->u_final)
@@ -662,9 +672,10 @@ static int group_body(opm_group *g) {
     opm_solver *a = g->s[0];
     const int d = a->d, wmode = a->wmode, nsq = a->cfg.nsq, nsteps = a->cfg.nsteps;
     int st;
-    {  // every scenario's FP64 assembly and first step, one launch per 32
+    {  // every scenario's FP64 assembly and first step, fused with the
+       // squarings' first split, one launch per 32
         std::vector<int> nx(n), nr(n);
-        std::vector<unsigned> xv(n), dr(n), vo(n), dst(n);
+        std::vector<unsigned> xv(n), dr(n), vo(n), states(n);
         std::vector<double> q(n), dt(n);
         for (int i = 0; i < n; i++) {
             const opm_solver *s = g->s[i];
@@ -675,13 +686,13 @@ static int group_body(opm_group *g) {
             vo[i] = s->vol.h;
             q[i] = s->cfg.q_off;
             dt[i] = s->dt0;
-            dst[i] = ophip_wsq_w(s->wstate.h, d, wmode, 0);
+            states[i] = s->wstate.h;
         }
-        st = ophip_sek_rs_batch(n, nx.data(), nr.data(), xv.data(), dr.data(), vo.data(), q.data(), dt.data(),
-                                a->nm ? 6 : 2, dst.data());
+        st = ophip_wsq_assemble_rs(n, nx.data(), nr.data(), xv.data(), dr.data(), vo.data(), q.data(),
+                                   dt.data(), a->nm ? 1 : 0, wmode, states.data());
         if (st) return st;
     }
-    st = ophip_wsq_exec(g->wplan, nsq);
+    st = ophip_wsq_exec_ex(g->wplan, nsq, 1);
     if (st) return st;
     const int buf = nsq & 1;
     std::vector<unsigned> states(n), X[2], XT[2], P0(n), PT0(n);
@@ -968,6 +979,7 @@ opm_batch *opm_batch_create(const opm_config *cfgs, int n, int tile) {
 static int batch_body(opm_batch *b) {
     int st;
     {  // every lattice's FP64 assembly and first step, one launch per 32
+       // (W form: fused with the squarings' first split)
         const int n = b->n;
         std::vector<int> nx(n), nr(n, 1);
         std::vector<unsigned> xv(n), dr(n), vo(n), dst(n);
@@ -977,14 +989,18 @@ static int batch_body(opm_batch *b) {
             xv[i] = b->x[i].h;
             dr[i] = b->drift[i].h;
             vo[i] = b->vol[i].h;
-            dst[i] = b->wmode >= 0 ? ophip_wsq_w(b->state[i].h, b->d[i], b->wmode, 0) : b->state[i].h;
+            dst[i] = b->state[i].h;
         }
-        st = ophip_sek_rs_batch(n, nx.data(), nr.data(), xv.data(), dr.data(), vo.data(), q.data(),
-                                b->dt0.data(), b->wmode >= 0 ? 2 : 0, dst.data());
+        if (b->wmode >= 0)
+            st = ophip_wsq_assemble_rs(n, nx.data(), nr.data(), xv.data(), dr.data(), vo.data(), q.data(),
+                                       b->dt0.data(), 0, b->wmode, dst.data());
+        else
+            st = ophip_sek_rs_batch(n, nx.data(), nr.data(), xv.data(), dr.data(), vo.data(), q.data(),
+                                    b->dt0.data(), 0, dst.data());
         if (st) return st;
     }
     if (b->wmode >= 0) {
-        st = ophip_wsq_exec(b->wplan, -1);
+        st = ophip_wsq_exec_ex(b->wplan, -1, 1);
         if (st) return st;
     } else {
         for (int r = 0; r < b->rounds; r++)

@@ -34,6 +34,7 @@
 //   k_wfix     diagonal correction from the partial row sums (fixed order)
 // A scenario table lets one launch serve many lattices of different d.
 #include "ophip_internal.h"
+#include "rs_lattice.h"
 
 #include <algorithm>
 #include <array>
@@ -1681,12 +1682,131 @@ OPHIP_EXPORT void ophip_wsq_plan_free(unsigned long plan) {
     delete P;
 }
 
+// ---------------------------------------------------------------------
+// k_rs_wsplit: the FP64 assembly of W0 = dt*L (rs_elem) fused with
+// k_wsplit's work (flush rule, partial row sums, cell flags of buffer 0):
+// one 64-row stripe of one lattice per workgroup, wave w takes the cells
+// w, w+4, ... of the stripe, lane = row.  A cell whose entries are all
+// structurally zero (rs_block_may_be_nonzero) writes nothing when buffer
+// 0's flag already says it holds zeros (the diagonal fix selects its
+// partial sums away), else zeros and flag 0.  Every other cell is written
+// with the values, partial sums and flag that k_rs_assemble_batch (dt*L)
+// followed by k_wsplit leave.
+// ---------------------------------------------------------------------
+namespace {
+
+struct RSWEntry {
+    RSLattice L;
+    double dt;
+    float *W;      // buffer 0
+    double *part;  // partial row sums, [d/64 column blocks][d]
+    uint8_t *fl;   // buffer 0's cell flags
+};
+constexpr int kRSWBatch = 32;
+struct RSWBatch {
+    RSWEntry e[kRSWBatch];
+};
+
+template <bool NM, bool BF, bool LO>
+__global__ __launch_bounds__(256) void k_rs_wsplit(const RSWBatch b) {
+    using N = Num<BF>;
+    const RSWEntry &E = b.e[blockIdx.y];
+    const int d = E.L.nx * E.L.nr, nc = d / 64, tm = blockIdx.x;
+    if (tm >= nc) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = tm * 64 + lane;
+    for (int tn = wave; tn < nc; tn += 4) {
+        const int j0 = tn * 64;
+        uint8_t *flag = E.fl + (size_t)tm * nc + tn;
+        float *col = E.W + (size_t)d * j0 + i;
+        if (!rs_block_may_be_nonzero<NM>(E.L.nx, E.L.nr, tm * 64, j0, 64)) {
+            if (*flag != 0) {
+#pragma unroll 16
+                for (int c = 0; c < 64; c++) col[(size_t)d * c] = 0.0f;
+                if (lane == 0) *flag = 0;
+            }
+            continue;
+        }
+        double rs = 0.0;
+        unsigned nz = 0;
+        float amax = 0.0f;
+        for (int c = 0; c < 64; c++) {
+            float w = (float)rs_elem<2, NM>(E.L, E.dt, i, j0 + c);
+            const float ws = w * N::S;
+            const uint16_t h = N::enc(ws);
+            const uint16_t l = LO ? N::enc((ws - N::dec(h)) * N::G) : (uint16_t)0;
+            const unsigned nzv = (h | l) & 0x7fffu;
+            if (!nzv) w = 0.0f;  // flush rule: both 16-bit parts +-0 -> FP32 +0
+            col[(size_t)d * c] = w;
+            rs += (double)w;
+            nz |= nzv;
+            amax = fmaxf(amax, fabsf(w));
+        }
+        E.part[(size_t)tn * d + i] = rs;
+        const bool any = __ballot(nz != 0) != 0;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o, 64));
+        if (lane == 0) *flag = any ? cell_flag(amax) : 0;
+    }
+}
+
+}  // namespace
+
+// ophip_sek_rs_batch (out_kind 2, node_major: 6) into buffer 0 of each
+// lattice's state, fused with the first stage of ophip_wsq_exec (the flush
+// rule, partial row sums and buffer 0's cell flags); ophip_wsq_exec_ex with
+// presplit = 1 then leaves that stage out.  Needs buffer 0's flags to
+// describe its contents, as every plan and exec leaves them.
+OPHIP_EXPORT int ophip_wsq_assemble_rs(int n, const int *nx, const int *nr, const unsigned *xval,
+                                       const unsigned *drift, const unsigned *vol,
+                                       const double *q_off, const double *dt, int node_major,
+                                       int mode, const unsigned *states) {
+    if (n <= 0 || !nx || !nr || !xval || !drift || !vol || !q_off || !dt || !states || mode < 0 ||
+        mode > 2)
+        return (int)set_status(kErrBadArg);
+    for (int k = 0; k < n; k++) {
+        RSLattice L;
+        const int d = nx[k] * nr[k];
+        if (!rs_args(nx[k], nr[k], xval[k], drift[k], vol[k], L) || d % 64 ||
+            !handle_range_ok(states[k], wsq_layout(d, mode).total))
+            return (int)set_status(kErrBadArg);
+    }
+    Device &D = dev();
+    for (int k0 = 0; k0 < n; k0 += kRSWBatch) {
+        const int m = n - k0 < kRSWBatch ? n - k0 : kRSWBatch;
+        RSWBatch b{};
+        int most = 0;
+        for (int k = 0; k < m; k++) {
+            RSWEntry &E = b.e[k];
+            rs_args(nx[k0 + k], nr[k0 + k], xval[k0 + k], drift[k0 + k], vol[k0 + k], E.L);
+            E.L.q_off = q_off[k0 + k];
+            E.dt = dt[k0 + k];
+            const int d = nx[k0 + k] * nr[k0 + k];
+            const WsqLayout Y = wsq_layout(d, mode);
+            E.W = P<float>(states[k0 + k] + (unsigned)Y.W0);
+            E.part = P<double>(states[k0 + k] + (unsigned)Y.part);
+            E.fl = P<uint8_t>(states[k0 + k] + (unsigned)Y.fl[0]);
+            most = d / 64 > most ? d / 64 : most;
+        }
+        const unsigned e = launch_mode(mode, [&](auto bf, auto lo) {
+            constexpr bool BF = decltype(bf)::value, LO = decltype(lo)::value;
+            if (node_major)
+                hipLaunchKernelGGL((k_rs_wsplit<true, BF, LO>), dim3(most, m), dim3(256), 0, D.stream, b);
+            else
+                hipLaunchKernelGGL((k_rs_wsplit<false, BF, LO>), dim3(most, m), dim3(256), 0, D.stream, b);
+        });
+        if (e) return (int)e;
+    }
+    return 0;
+}
+
 // Split + diagonal fix of W0 (buffer 0, written by ophip_sek_rs out_kind
 // 2), then the squarings, each followed by the diagonal fix.  nsq >= 0:
 // every lattice squares nsq times; nsq < 0: the per-lattice counts given
 // to the plan.  Lattice i ends in buffer (its count) & 1.  Kernel launches
-// only.
-OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
+// only.  presplit: buffer 0 was written by ophip_wsq_assemble_rs (the
+// split is left out).
+static int wsq_exec(unsigned long plan, int nsq, bool presplit) {
     Device &D = dev();
     WsqPlan *P = (WsqPlan *)(uintptr_t)plan;
     if (!P) return (int)set_status(kErrBadArg);
@@ -1698,8 +1818,9 @@ OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
         // sparse rounds: live-item compaction before each squaring launch;
         // every diagonal fix clears the counters for the next round
         const bool sparse = g_wsq_skip && P->sparse_ok;
-        hipLaunchKernelGGL((k_wsplit<BF, LO>), dim3(P->n_split), dim3(64), 0, D.stream, D.base,
-                           P->sc_split, P->it_split);
+        if (!presplit)
+            hipLaunchKernelGGL((k_wsplit<BF, LO>), dim3(P->n_split), dim3(64), 0, D.stream, D.base,
+                               P->sc_split, P->it_split);
         hipLaunchKernelGGL(k_wfix, fixgrid, dim3(64), 0, D.stream, D.base, P->sc_split,
                            P->n, (const int *)nullptr, 0);
         for (int rr = 0; rr < rounds; rr++) {
@@ -1745,6 +1866,14 @@ OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) {
                                per ? (const int *)P->nsq_dev : (const int *)nullptr, rr);
         }
     });
+}
+
+OPHIP_EXPORT int ophip_wsq_exec(unsigned long plan, int nsq) { return wsq_exec(plan, nsq, false); }
+
+// flags: 1 = presplit (buffer 0 written by ophip_wsq_assemble_rs)
+OPHIP_EXPORT int ophip_wsq_exec_ex(unsigned long plan, int nsq, int flags) {
+    if (flags & ~1) return (int)set_status(kErrBadArg);
+    return wsq_exec(plan, nsq, (flags & 1) != 0);
 }
 
 // zero-cell skipping of the squaring kernel: 1 on (default), 0 off (every

@@ -91,6 +91,10 @@ SIGNATURES = {
     "ophip_wsq_plan": (ULONG, [I32, C.POINTER(U32), C.POINTER(I32), C.POINTER(I32), I32, I32,
                                I32]),
     "ophip_wsq_exec": (I32, [ULONG, I32]),
+    "ophip_wsq_assemble_rs": (I32, [I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(U32),
+                                    C.POINTER(U32), C.POINTER(U32), C.POINTER(F64),
+                                    C.POINTER(F64), I32, I32, C.POINTER(U32)]),
+    "ophip_wsq_exec_ex": (I32, [ULONG, I32, I32]),
     "ophip_wsq_set_skip": (I32, [I32]),
     "ophip_wsq_set_kcut": (I32, [I32]),
     "ophip_wsq_plan_free": (V, [ULONG]),

@@ -922,3 +922,69 @@ def test_wform_backward_layout_regime_batch(hip):
         H.sync()
         for a, b in zip(g, grab(L)):
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("nm", [0, 1])
+def test_fused_assembly_matches_two_kernel_path(hip, mode, nm):
+    """ophip_wsq_assemble_rs + ophip_wsq_exec_ex(presplit) against
+    ophip_sek_rs_batch + ophip_wsq_exec: the same bits in both W buffers and
+    both flag arrays after every call of a sequence (the fused kernel skips
+    zero cells already flagged zero from the second call on), and the same
+    partial sums in every cell flagged nonzero after a call without
+    squarings"""
+    banner(f"wsq fused assembly vs two-kernel path mode={mode} nm={nm}")
+    import ctypes as C
+    H = hip
+    shapes = [(64, 2), (96, 4), (128, 2), (256, 8)]  # d = 128, 384, 256, 2048
+    la = [Lattice(H, nx, nr, mode) for nx, nr in shapes]
+    lb = [Lattice(H, nx, nr, mode) for nx, nr in shapes]
+    pa, pb = plan(H, la, 64), plan(H, lb, 64)
+    n = len(shapes)
+
+    def arrs(L):
+        return [(L.arr(b, 0, np.uint32).copy(), grab_flags(L, b)) for b in (0, 1)]
+
+    def grab_flags(L, b):
+        nc = L.d // 64
+        f = np.empty(nc * nc, dtype=np.uint8)
+        H.check(H.lib.opcuda_memcpy_d2h(H.lib.ophip_wsq_array(L.state.h, L.d, mode, b, 6),
+                                        f.ctypes.data, f.nbytes))
+        return f.reshape(nc, nc)
+
+    def grab_part(L):
+        nc = L.d // 64
+        p = np.empty(nc * L.d, dtype=np.float64)
+        H.check(H.lib.opcuda_memcpy_d2h(H.lib.ophip_wsq_array(L.state.h, L.d, mode, 0, 5),
+                                        p.ctypes.data, p.nbytes))
+        return p.reshape(nc, L.d)  # [column block][row]
+
+    nx = (C.c_int * n)(*[L.cfg.nx for L in la])
+    nr = (C.c_int * n)(*[L.cfg.nr for L in la])
+    xv = lambda ls: (C.c_uint * n)(*[L.xd.h for L in ls])
+    dr = lambda ls: (C.c_uint * n)(*[L.dd.h for L in ls])
+    vo = lambda ls: (C.c_uint * n)(*[L.vd.h for L in ls])
+    q = (C.c_double * n)(*[L.cfg.q_off for L in la])
+    try:
+        for k, (nsq, scale) in enumerate([(3, 1.0), (2, 0.5), (0, 0.25), (4, 1.0), (0, 2.0)]):
+            dt = (C.c_double * n)(*[L.dt * scale for L in la])
+            w0 = (C.c_uint * n)(*[H.lib.ophip_wsq_array(L.state.h, L.d, mode, 0, 0) for L in la])
+            H.check(H.lib.ophip_sek_rs_batch(n, nx, nr, xv(la), dr(la), vo(la), q, dt, 6 if nm else 2, w0))
+            H.check(H.lib.ophip_wsq_exec(pa, nsq))
+            st = (C.c_uint * n)(*[L.state.h for L in lb])
+            H.check(H.lib.ophip_wsq_assemble_rs(n, nx, nr, xv(lb), dr(lb), vo(lb), q, dt, nm, mode, st))
+            H.check(H.lib.ophip_wsq_exec_ex(pb, nsq, 1))
+            H.sync()
+            for A, B in zip(la, lb):
+                for (wa, fa), (wb, fb) in zip(arrs(A), arrs(B)):
+                    assert np.array_equal(wa, wb), (k, A.d)
+                    assert np.array_equal(fa, fb), (k, A.d)
+                if nsq == 0:
+                    f0 = grab_flags(A, 0)
+                    sel = np.repeat(f0.T != 0, 64, axis=1)  # [column block][row]
+                    assert np.array_equal(grab_part(A)[sel], grab_part(B)[sel]), (k, A.d)
+        assert H.lib.ophip_wsq_exec_ex(pb, 1, 2) == 0x10003  # unknown flag bits
+        H.lib.ophip_set_status(0)
+    finally:
+        H.lib.ophip_wsq_plan_free(pa)
+        H.lib.ophip_wsq_plan_free(pb)
