@@ -926,14 +926,15 @@ def test_wform_backward_layout_regime_batch(hip):
 
 @pytest.mark.parametrize("mode", [0, 1, 2])
 @pytest.mark.parametrize("nm", [0, 1])
-def test_fused_assembly_matches_two_kernel_path(hip, mode, nm):
+@pytest.mark.parametrize("skip", [1, 0])
+def test_fused_assembly_matches_two_kernel_path(hip, mode, nm, skip):
     """ophip_wsq_assemble_rs + ophip_wsq_exec_ex(presplit) against
     ophip_sek_rs_batch + ophip_wsq_exec: the same bits in both W buffers and
     both flag arrays after every call of a sequence (the fused kernel skips
     zero cells already flagged zero from the second call on), and the same
     partial sums in every cell flagged nonzero after a call without
-    squarings"""
-    banner(f"wsq fused assembly vs two-kernel path mode={mode} nm={nm}")
+    squarings; with the sparse and with the dense K loop"""
+    banner(f"wsq fused assembly vs two-kernel path mode={mode} nm={nm} skip={skip}")
     import ctypes as C
     H = hip
     shapes = [(64, 2), (96, 4), (128, 2), (256, 8)]  # d = 128, 384, 256, 2048
@@ -965,6 +966,7 @@ def test_fused_assembly_matches_two_kernel_path(hip, mode, nm):
     dr = lambda ls: (C.c_uint * n)(*[L.dd.h for L in ls])
     vo = lambda ls: (C.c_uint * n)(*[L.vd.h for L in ls])
     q = (C.c_double * n)(*[L.cfg.q_off for L in la])
+    old_skip = H.lib.ophip_wsq_set_skip(skip)
     try:
         for k, (nsq, scale) in enumerate([(3, 1.0), (2, 0.5), (0, 0.25), (4, 1.0), (0, 2.0)]):
             dt = (C.c_double * n)(*[L.dt * scale for L in la])
@@ -986,5 +988,6 @@ def test_fused_assembly_matches_two_kernel_path(hip, mode, nm):
         assert H.lib.ophip_wsq_exec_ex(pb, 1, 2) == 0x10003  # unknown flag bits
         H.lib.ophip_set_status(0)
     finally:
+        H.lib.ophip_wsq_set_skip(old_skip)
         H.lib.ophip_wsq_plan_free(pa)
         H.lib.ophip_wsq_plan_free(pb)
