@@ -247,9 +247,12 @@ unsigned      ophip_w8_array(unsigned state, int d, int which);
 int           ophip_w8_exec(unsigned state, int d, int nsq);
 
 /* Backward induction in the W form on the matrix cores (modes 1, 2):
- * Y = df (X + W X).  ophip_wsq_bwd_layout copies the hi/lo copies of W
- * (buffer `buf`, the one the squarings ended in) into an MFMA
- * fragment-major layout kept in the other buffer; X/Y are d x ncp
+ * Y = df (X + W X).  ophip_wsq_bwd_layout splits W (buffer `buf`, the one
+ * the squarings ended in) into its hi/lo parts in the state's MFMA
+ * fragment-major arrays (ophip_wsq_array 2 and 4) and builds the tile
+ * lists; the layouts in W's own state order and in the regime order do not
+ * read the 64 x 64 cells that buffer's flags mark zero, so the flags must
+ * describe W (as plan creation and every exec leave them); X/Y are d x ncp
  * row-major FP32 (ncp 16, 32 or 48) with 16-bit hi/lo copies XT/YT
  * (2 * ncp * d values, fragment-major); ophip_wsq_bwd_prep writes XT from
  * X.  A step given X = 0 takes X from XT (hi + lo 2^-G: 22 significant bits

@@ -764,6 +764,7 @@ __global__ __launch_bounds__(256) void k_wfragp(int d, const float *__restrict__
 // offsets): FP32 W in, fragment copies, flags and lists out
 struct LayScen {
     uint64_t W, Hf, Lf, flags, tl;
+    uint64_t cf;  // W's cell flags (0 = every value of a 64 x 64 cell is +0)
 };
 constexpr int kLayBatch = 32;
 struct LayBatch {
@@ -788,10 +789,13 @@ __global__ __launch_bounds__(256) void k_wfragk(int d, int nx, int nr, char *__r
     __shared__ __attribute__((aligned(16))) uint16_t sh[2][64][68];  // [hi/lo][column][row]
     const int nb = d / 64, bi = blockIdx.x / nb, bj = blockIdx.x % nb, t = threadIdx.x;
     const int nkt = d / 32, NB = 64 / nr;
+    // a cell flagged zero holds +0 only: its values are not read
+    const bool zero = at<const uint8_t>(base, S.cf)[(size_t)bi * nb + bj] == 0;
 #pragma unroll
     for (int u = 0; u < 4; u++) {
         const int q = t + 256 * u, col = q >> 4, rq = q & 15;  // 4 rows of one column per thread
-        const f4 w = *(const f4 *)(W + (size_t)(64 * bi + 4 * rq) + (size_t)d * (64 * bj + col));
+        const f4 w = zero ? f4{0.f, 0.f, 0.f, 0.f}
+                          : *(const f4 *)(W + (size_t)(64 * bi + 4 * rq) + (size_t)d * (64 * bj + col));
         uint16_t h[4], l[4];
 #pragma unroll
         for (int e = 0; e < 4; e++) {
@@ -993,20 +997,65 @@ inline uint32_t hi_bits(int mode, int e) {
 std::atomic<int> g_cut_visit{-24}, g_cut_lo{-12};
 }  // namespace
 
+// The 64-state cells [c0, c1] that hold the W states of fragment states
+// a .. a + n - 1: pmode 1, the same states; pmode 2, the regime order
+// (fragment state x + nx r is W's state nr x + r, increasing in x within
+// a regime).  A range over more than two regimes gets every cell.
+__device__ inline void perm_cells(int pmode, int d, int nx, int nr, int a, int n, int &c0, int &c1) {
+    const int b = a + n - 1;
+    if (pmode == 1) {
+        c0 = a >> 6;
+        c1 = b >> 6;
+        return;
+    }
+    const int ra = a / nx, rb = b / nx;
+    const int pa = nr * (a - ra * nx) + ra, pb = nr * (b - rb * nx) + rb;
+    if (ra == rb) {
+        c0 = pa >> 6;
+        c1 = pb >> 6;
+    } else if (rb == ra + 1) {  // the end of regime ra, the start of rb
+        c0 = min(pa, rb) >> 6;
+        c1 = max(nr * (nx - 1) + ra, pb) >> 6;
+    } else {
+        c0 = 0;
+        c1 = (d - 1) >> 6;
+    }
+}
+
 // k_wbwd_flags: byte flags[it][kt] (d/16 x d/32): bit 0 = the fragment is
 // visited, bit 1 = its lo part is read; a wave per (it, four k tiles), the
-// four 16-byte loads in flight; lattice blockIdx.y of the batch.
+// four 16-byte loads in flight; lattice blockIdx.y of the batch.  pmode 1
+// (fragments in W's state order) or 2 (regime order, nx, nr): a fragment
+// whose W cells are all flagged zero holds zeros and is not read; pmode 0
+// (any other order): every fragment is read.
 __global__ __launch_bounds__(256) void k_wbwd_flags(int d, char *__restrict__ base, const LayBatch bt,
-                                                    uint32_t tvis, uint32_t tlo) {
+                                                    uint32_t tvis, uint32_t tlo, int pmode, int nx, int nr) {
     const uint16_t *__restrict__ Hf = at<const uint16_t>(base, bt.s[blockIdx.y].Hf);
     uint8_t *__restrict__ flags = at<uint8_t>(base, bt.s[blockIdx.y].flags);
     const int nkt = d / 32, lane = threadIdx.x & 63;
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);  // (it, k0 / 4)
     if (w >= d / 16 * (nkt / 4)) return;  // wave-uniform
     const int it = w / (nkt / 4), k0 = 4 * (w % (nkt / 4));
+    bool read[4] = {true, true, true, true};
+    if (pmode) {  // uniform
+        const uint8_t *cf = at<const uint8_t>(base, bt.s[blockIdx.y].cf);
+        const int nb = d / 64;
+        int r0, r1;
+        perm_cells(pmode, d, nx, nr, 16 * it, 16, r0, r1);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            int c0, c1;
+            perm_cells(pmode, d, nx, nr, 32 * (k0 + u), 32, c0, c1);
+            bool any = false;
+            for (int cr = r0; cr <= r1 && !any; cr++)
+                for (int cc = c0; cc <= c1 && !any; cc++) any = cf[(size_t)cr * nb + cc] != 0;
+            read[u] = any;
+        }
+    }
     uint4 h[4];
 #pragma unroll
-    for (int u = 0; u < 4; u++) h[u] = *(const uint4 *)(Hf + frag_off(it, k0 + u, lane, nkt));
+    for (int u = 0; u < 4; u++)
+        h[u] = read[u] ? *(const uint4 *)(Hf + frag_off(it, k0 + u, lane, nkt)) : uint4{0u, 0u, 0u, 0u};
     unsigned word = 0;
 #pragma unroll
     for (int u = 0; u < 4; u++) {
@@ -2085,26 +2134,27 @@ LayScen lay_scen(const WsqLayout &L, unsigned state, int buf) {
     S.Lf = state + L.Fl;
     S.flags = state + L.tf;
     S.tl = state + L.tl;
+    S.cf = state + L.fl[buf];
     return S;
 }
 
 // the tile lists of m lattices' fragment copies (after their layout), from
-// the hi copies
-void bwd_tile_lists(Device &D, int d, int mode, const LayBatch &bt, int m) {
+// the hi copies; pmode, nx, nr: the fragments' state order (k_wbwd_flags)
+void bwd_tile_lists(Device &D, int d, int mode, const LayBatch &bt, int m, int pmode, int nx, int nr) {
     const int waves = d / 16 * (d / 128);
     const int ev = g_cut_visit.load(), el = g_cut_lo.load();
     const uint32_t tvis = hi_bits(mode, ev), tlo = hi_bits(mode, mode == 2 ? el - 3 : el);
     hipLaunchKernelGGL(k_wbwd_flags, dim3(cdiv(waves, 4), m), dim3(256), 0, D.stream, d, D.base, bt, tvis,
-                       tlo);
+                       tlo, pmode, nx, nr);
     hipLaunchKernelGGL(k_wbwd_lists, dim3(cdiv(d / 16 + d / 64 + d / 32, 4), m), dim3(256), 0, D.stream, d,
                        D.base, bt);
 }
 
 // the same for one lattice
-void bwd_tile_lists(Device &D, const WsqLayout &L, unsigned state, int d, int mode, int buf) {
+void bwd_tile_lists(Device &D, const WsqLayout &L, unsigned state, int d, int mode, int buf, int pmode) {
     LayBatch bt{};
     bt.s[0] = lay_scen(L, state, buf);
-    bwd_tile_lists(D, d, mode, bt, 1);
+    bwd_tile_lists(D, d, mode, bt, 1, pmode, 0, 0);
 }
 }  // namespace
 
@@ -2124,7 +2174,7 @@ OPHIP_EXPORT int ophip_wsq_bwd_layout(unsigned state, int d, int mode, int buf) 
         hipLaunchKernelGGL((k_wfrag<true>), grid, dim3(256), 0, D.stream, d, W, Hf, Lf);
     else
         hipLaunchKernelGGL((k_wfrag<false>), grid, dim3(256), 0, D.stream, d, W, Hf, Lf);
-    bwd_tile_lists(D, L, state, d, mode, buf);
+    bwd_tile_lists(D, L, state, d, mode, buf, 1);
     return (int)check_launch();
 }
 
@@ -2147,7 +2197,7 @@ OPHIP_EXPORT int ophip_wsq_bwd_layout_perm(unsigned state, int d, int mode, int 
         hipLaunchKernelGGL((k_wfragp<true>), grid, dim3(256), 0, D.stream, d, W, P<const int>(perm), Hf, Lf);
     else
         hipLaunchKernelGGL((k_wfragp<false>), grid, dim3(256), 0, D.stream, d, W, P<const int>(perm), Hf, Lf);
-    bwd_tile_lists(D, L, state, d, mode, buf);
+    bwd_tile_lists(D, L, state, d, mode, buf, 0);
     return (int)check_launch();
 }
 
@@ -2306,7 +2356,7 @@ OPHIP_EXPORT int ophip_wsq_bwd_layout_regime_batch(int n, const unsigned *states
             hipLaunchKernelGGL((k_wfragk<true>), grid, dim3(256), 0, D.stream, d, nx, nr, D.base, bt);
         else
             hipLaunchKernelGGL((k_wfragk<false>), grid, dim3(256), 0, D.stream, d, nx, nr, D.base, bt);
-        bwd_tile_lists(D, d, mode, bt, m);
+        bwd_tile_lists(D, d, mode, bt, m, 2, nx, nr);
         const unsigned e = check_launch();
         if (e) return (int)e;
     }
