@@ -1038,19 +1038,19 @@ __global__ __launch_bounds__(256) void k_wbwd_flags(int d, char *__restrict__ ba
     const int it = w / (nkt / 4), k0 = 4 * (w % (nkt / 4));
     bool read[4] = {true, true, true, true};
     if (pmode) {  // uniform
+        // lanes 16 u .. 16 u + 15 read the flags of fragment u's W cells
+        // (a box of at most 16 cells, else the fragment is read) in one load
         const uint8_t *cf = at<const uint8_t>(base, bt.s[blockIdx.y].cf);
-        const int nb = d / 64;
-        int r0, r1;
+        const int nb = d / 64, u = lane >> 4, c = lane & 15;
+        int r0, r1, c0, c1;
         perm_cells(pmode, d, nx, nr, 16 * it, 16, r0, r1);
+        perm_cells(pmode, d, nx, nr, 32 * (k0 + u), 32, c0, c1);
+        const int nrow = r1 - r0 + 1, ncol = c1 - c0 + 1;
+        bool nz = nrow * ncol > 16 && c == 0;
+        if (nrow * ncol <= 16 && c < nrow * ncol) nz = cf[(size_t)(r0 + c / ncol) * nb + c0 + c % ncol] != 0;
+        const uint64_t m = __ballot(nz);
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            int c0, c1;
-            perm_cells(pmode, d, nx, nr, 32 * (k0 + u), 32, c0, c1);
-            bool any = false;
-            for (int cr = r0; cr <= r1 && !any; cr++)
-                for (int cc = c0; cc <= c1 && !any; cc++) any = cf[(size_t)cr * nb + cc] != 0;
-            read[u] = any;
-        }
+        for (int q = 0; q < 4; q++) read[q] = ((m >> (16 * q)) & 0xffffu) != 0;
     }
     uint4 h[4];
 #pragma unroll
