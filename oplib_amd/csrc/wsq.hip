@@ -1735,7 +1735,8 @@ OPHIP_EXPORT void ophip_wsq_plan_free(unsigned long plan) {
 // k_rs_wsplit: the FP64 assembly of W0 = dt*L (rs_elem) fused with
 // k_wsplit's work (flush rule, partial row sums, cell flags of buffer 0):
 // one 64-row stripe of one lattice per workgroup, wave w takes the cells
-// w, w+4, ... of the stripe, lane = row.  A cell whose entries are all
+// w, w+4, ... of the stripe, lane = row (the row's rates evaluated once,
+// the wave's cell flags read in one load).  A cell whose entries are all
 // structurally zero (rs_block_may_be_nonzero) writes nothing when buffer
 // 0's flag already says it holds zeros (the diagonal fix selects its
 // partial sums away), else zeros and flag 0.  Every other cell is written
@@ -1760,42 +1761,64 @@ template <bool NM, bool BF, bool LO>
 __global__ __launch_bounds__(256) void k_rs_wsplit(const RSWBatch b) {
     using N = Num<BF>;
     const RSWEntry &E = b.e[blockIdx.y];
-    const int d = E.L.nx * E.L.nr, nc = d / 64, tm = blockIdx.x;
+    const int nx = E.L.nx, nr = E.L.nr, d = nx * nr, nc = d / 64, tm = blockIdx.x;
     if (tm >= nc) return;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = tm * 64 + lane;
-    for (int tn = wave; tn < nc; tn += 4) {
-        const int j0 = tn * 64;
-        uint8_t *flag = E.fl + (size_t)tm * nc + tn;
-        float *col = E.W + (size_t)d * j0 + i;
-        if (!rs_block_may_be_nonzero<NM>(E.L.nx, E.L.nr, tm * 64, j0, 64)) {
-            if (*flag != 0) {
+    // the row's node, regime and rates: rs_elem's terms, evaluated once
+    const int xi = NM ? i / nr : i % nx, ri = NM ? i % nr : i / nx;
+    double am, ap;
+    rs_rates(E.L, xi + nx * ri, am, ap);
+    const double gd = -(am + ap + (double)(nr - 1) * E.L.q_off);
+    uint8_t *frow = E.fl + (size_t)tm * nc;
+    for (int q0 = 0; wave + 4 * q0 < nc; q0 += 64) {
+        // buffer 0's flags of the wave's next 64 cells, one per lane
+        const int tq = wave + 4 * (q0 + lane);
+        const int fv = tq < nc ? (int)frow[tq] : 0;
+        for (int q = q0; q < q0 + 64; q++) {
+            const int tn = wave + 4 * q;
+            if (tn >= nc) break;
+            const int j0 = tn * 64;
+            float *col = E.W + (size_t)d * j0 + i;
+            if (!rs_block_may_be_nonzero<NM>(nx, nr, tm * 64, j0, 64)) {
+                if (__builtin_amdgcn_readlane(fv, q - q0) != 0) {
 #pragma unroll 16
-                for (int c = 0; c < 64; c++) col[(size_t)d * c] = 0.0f;
-                if (lane == 0) *flag = 0;
+                    for (int c = 0; c < 64; c++) col[(size_t)d * c] = 0.0f;
+                    if (lane == 0) frow[tn] = 0;
+                }
+                continue;
             }
-            continue;
-        }
-        double rs = 0.0;
-        unsigned nz = 0;
-        float amax = 0.0f;
-        for (int c = 0; c < 64; c++) {
-            float w = (float)rs_elem<2, NM>(E.L, E.dt, i, j0 + c);
-            const float ws = w * N::S;
-            const uint16_t h = N::enc(ws);
-            const uint16_t l = LO ? N::enc((ws - N::dec(h)) * N::G) : (uint16_t)0;
-            const unsigned nzv = (h | l) & 0x7fffu;
-            if (!nzv) w = 0.0f;  // flush rule: both 16-bit parts +-0 -> FP32 +0
-            col[(size_t)d * c] = w;
-            rs += (double)w;
-            nz |= nzv;
-            amax = fmaxf(amax, fabsf(w));
-        }
-        E.part[(size_t)tn * d + i] = rs;
-        const bool any = __ballot(nz != 0) != 0;
+            double rs = 0.0;
+            unsigned nz = 0;
+            float amax = 0.0f;
+            for (int c = 0; c < 64; c++) {
+                const int j = j0 + c;
+                const int xj = NM ? j / nr : j % nx, rj = NM ? j % nr : j / nx;
+                double g = 0.0;
+                if (ri == rj) {
+                    if (xj == xi - 1) g = am;
+                    else if (xj == xi + 1) g = ap;
+                    else if (xj == xi) g = gd;
+                } else if (xi == xj) {
+                    g = E.L.q_off;
+                }
+                float w = (float)(E.dt * g);  // rs_elem<2, NM>(L, dt, i, j)
+                const float ws = w * N::S;
+                const uint16_t h = N::enc(ws);
+                const uint16_t l = LO ? N::enc((ws - N::dec(h)) * N::G) : (uint16_t)0;
+                const unsigned nzv = (h | l) & 0x7fffu;
+                if (!nzv) w = 0.0f;  // flush rule: both 16-bit parts +-0 -> FP32 +0
+                col[(size_t)d * c] = w;
+                rs += (double)w;
+                nz |= nzv;
+                amax = fmaxf(amax, fabsf(w));
+            }
+            E.part[(size_t)tn * d + i] = rs;
+            const bool any = __ballot(nz != 0) != 0;
 #pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o, 64));
-        if (lane == 0) *flag = any ? cell_flag(amax) : 0;
+            for (int o = 32; o >= 1; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o, 64));
+            if (lane == 0) frow[tn] = any ? cell_flag(amax) : 0;
+        }
     }
 }
 
