@@ -991,3 +991,35 @@ def test_fused_assembly_matches_two_kernel_path(hip, mode, nm, skip):
         H.lib.ophip_wsq_set_skip(old_skip)
         H.lib.ophip_wsq_plan_free(pa)
         H.lib.ophip_wsq_plan_free(pb)
+
+
+def test_fused_assembly_rejects_bad_arguments(hip):
+    """ophip_wsq_assemble_rs refuses what it cannot assemble (status
+    0x10003, nothing launched): a size that is not a multiple of 64, a
+    missing array, a state handle too small for the layout, a bad mode"""
+    banner("wsq fused assembly argument checks")
+    import ctypes as C
+    H = hip
+    L = Lattice(H, 64, 2, 1)  # d = 128
+    c = L.cfg
+
+    def call(nx=64, nr=2, state=None, mode=1, xv=True):
+        n = 1
+        a = lambda t, v: (t * n)(v)
+        st = a(C.c_uint, L.state.h if state is None else state)
+        return H.lib.ophip_wsq_assemble_rs(n, a(C.c_int, nx), a(C.c_int, nr),
+                                           a(C.c_uint, L.xd.h) if xv else None, a(C.c_uint, L.dd.h),
+                                           a(C.c_uint, L.vd.h), a(C.c_double, c.q_off), a(C.c_double, L.dt),
+                                           1, mode, st)
+
+    try:
+        assert call() == 0
+        H.sync()
+        assert call(nx=65, nr=2) == 0x10003  # d = 130
+        assert call(xv=False) == 0x10003
+        small = H.DeviceArray(1024, np.uint8)
+        assert call(state=small.h) == 0x10003
+        assert call(mode=3) == 0x10003
+        assert H.lib.ophip_wsq_assemble_rs(0, None, None, None, None, None, None, None, 1, 1, None) == 0x10003
+    finally:
+        H.lib.ophip_set_status(0)
