@@ -263,9 +263,7 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
     }
     const int first = live ? (int)(blockIdx.x >> 3) : (int)blockIdx.x;
     const int stride = live ? (int)(gridDim.x >> 3) : (int)gridDim.x;
-    for (int item = first; item < nlist; item += stride) {
-        WItem it;
-        uint64_t cells = 0;
+    auto fetch = [&](int item, WItem &it, uint64_t &cells) {
         if (live) {
             // the chunk holding list entry `item`: the last c with pref[c] <= item
             int lo = 0, hi = nch - 1;
@@ -279,8 +277,92 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
             cells = lv.cells;
         } else {
             it = items[item];
+            cells = 0;
         }
-        const WScen sc = scen[it.s];
+    };
+    // Operands come from the FP32 W (column-major) and are split into
+    // their 16-bit hi/lo parts on the way to LDS, with the same operations
+    // as the epilogue's flush test and the backward layout (the same bits).
+    // Waves 0 .. NW/2-1 stage A, the others B, NF float4 loads per thread
+    // and stage.  A: thread (rg, cg) loads a 4-row x KA-column block (rows
+    // 4 rg.., k = KA cg ..; eight 128-byte column pieces per load
+    // instruction) and writes each row's KA consecutive k as one LDS store.
+    // B: thread (kq, jb) loads 4 consecutive k (4 kq ..) of columns jb + 16 u.
+    constexpr int HALF = 32 * NW, NF = BT * BK / HALF / 4, KA = NF, CG = BK / KA, KQ = BK / 4;
+    static_assert(BT / 4 * CG == HALF && HALF / KQ == 16 && 16 * NF == BT, "operand staging");
+    const bool isA = wave < NW / 2;  // wave-uniform
+    const int ht = t % HALF;
+    const int rg = ht / CG, cg = ht % CG, kq = ht % KQ, jb = ht / KQ;
+    typedef f4 Stage[NF];
+    Stage s0, s1, s2, s3;
+    // The load side: the item whose operand loads are being issued (an
+    // item's first PF K-steps go out during the previous item's epilogue).
+    // Its K-steps: every K-step (static list) or every K-step of its K
+    // cells (wave-uniform: scalar registers); the cursor gives the K offset
+    // of the next step to load and, past the last step, repeats the last
+    // one (loads stay unconditional).
+    const bool dense = live == nullptr;
+    const float *ga = nullptr, *gb = nullptr;
+    int ld = 0, lns = 0, lq = 0, lh = 0, klast = 0;
+    uint64_t lm = 0;
+    auto gload = [&](Stage &x, int k0) {
+        if (isA) {
+#pragma unroll
+            for (int u = 0; u < NF; u++) x[u] = *(const f4 *)(ga + (size_t)ld * (k0 + u));
+        } else {
+#pragma unroll
+            for (int u = 0; u < NF; u++) x[u] = *(const f4 *)(gb + k0 + (size_t)ld * (16 * u));
+        }
+    };
+    auto next_k = [&]() {
+        int k0;
+        if (dense) {
+            k0 = min(lq, lns - 1) * BK;
+            lq++;
+        } else if (lm) {
+            k0 = 64 * __builtin_ctzll(lm) + BK * lh;
+            if (SPC == 1 || lh) {
+                lm &= lm - 1;
+                lh = 0;
+            } else {
+                lh = 1;
+            }
+            klast = k0;
+        } else {
+            k0 = klast;
+        }
+        return k0;
+    };
+    // set the load side to an item and issue its first PF K-steps; returns
+    // its K-step count
+    auto begin = [&](const WItem &bi, uint64_t bcells, const WScen &bsc) {
+        ld = bsc.d;
+        const float *Wsrc = at<const float>(base, bsc.W);
+        ga = Wsrc + (size_t)(bi.tm * BT + 4 * rg) + (size_t)ld * (KA * cg);
+        gb = Wsrc + (size_t)(4 * kq) + (size_t)ld * (bi.tn * BT + jb);
+        lm = bcells;
+        lq = lh = klast = 0;
+        lns = dense ? ld / BK : __popcll(bcells) * SPC;
+        if (lns > 0) {
+            gload(s0, next_k());
+            gload(s1, next_k());
+            if (PF == 4) {
+                gload(s2, next_k());
+                gload(s3, next_k());
+            }
+        }
+        return lns;
+    };
+    WItem it{};
+    WScen sc{};
+    int ns = 0;
+    if (first < nlist) {
+        uint64_t c0 = 0;
+        fetch(first, it, c0);
+        sc = scen[it.s];
+        ns = begin(it, c0, sc);
+    }
+    for (int item = first; item < nlist; item += stride) {
         const int d = sc.d;
         const int i0 = it.tm * BT, j0 = it.tn * BT;
 
@@ -308,33 +390,6 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
             }
 
         auto swz = [](int row) { return CPR == 8 ? (row & 7) : ((row >> 1) & 3); };
-        // Operands come from the FP32 W (column-major) and are split into
-        // their 16-bit hi/lo parts on the way to LDS, with the same
-        // operations as the epilogue's flush test and the backward layout
-        // (the same bits).  Waves 0 .. NW/2-1
-        // stage A, the others B, NF float4 loads per thread and stage.  A:
-        // thread (rg, cg) loads a 4-row x KA-column block (rows 4 rg.., k =
-        // KA cg ..; eight 128-byte column pieces per load instruction) and
-        // writes each row's KA consecutive k as one LDS store.  B: thread
-        // (kq, jb) loads 4 consecutive k (4 kq ..) of columns jb + 16 u.
-        constexpr int HALF = 32 * NW, NF = BT * BK / HALF / 4, KA = NF, CG = BK / KA, KQ = BK / 4;
-        static_assert(BT / 4 * CG == HALF && HALF / KQ == 16 && 16 * NF == BT, "operand staging");
-        const bool isA = wave < NW / 2;  // wave-uniform
-        const int ht = t % HALF;
-        const int rg = ht / CG, cg = ht % CG, kq = ht % KQ, jb = ht / KQ;
-        const float *Wsrc = at<const float>(base, sc.W);
-        const float *ga = Wsrc + (size_t)(i0 + 4 * rg) + (size_t)d * (KA * cg);
-        const float *gb = Wsrc + (size_t)(4 * kq) + (size_t)d * (j0 + jb);
-        typedef f4 Stage[NF];
-        auto gload = [&](Stage &x, int k0) {
-            if (isA) {
-#pragma unroll
-                for (int u = 0; u < NF; u++) x[u] = *(const f4 *)(ga + (size_t)d * (k0 + u));
-            } else {
-#pragma unroll
-                for (int u = 0; u < NF; u++) x[u] = *(const f4 *)(gb + k0 + (size_t)d * (16 * u));
-            }
-        };
         auto split2 = [](float a, float b, uint32_t &hi, uint32_t &lo) {
             const float as = a * N::S, bs = b * N::S;
             const uint16_t ha = N::enc(as), hb = N::enc(bs);
@@ -404,42 +459,8 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
           }
         };
 
-        // K-steps to visit: every K-step (static list), or every K-step of
-        // the item's K cells (wave-uniform: scalar registers)
         const int nc = d / 64;
-        const bool dense = live == nullptr;
-        const int ns = dense ? d / BK : __popcll(cells) * SPC;
-        // load cursor: the K offset of the next step to load; past the last
-        // step it repeats the last one (loads stay unconditional)
-        uint64_t lm = cells;
-        int lq = 0, lh = 0, klast = 0;
-        auto next_k = [&]() {
-            int k0;
-            if (dense) {
-                k0 = min(lq, ns - 1) * BK;
-                lq++;
-            } else if (lm) {
-                k0 = 64 * __builtin_ctzll(lm) + BK * lh;
-                if (SPC == 1 || lh) {
-                    lm &= lm - 1;
-                    lh = 0;
-                } else {
-                    lh = 1;
-                }
-                klast = k0;
-            } else {
-                k0 = klast;
-            }
-            return k0;
-        };
-        if (ns > 0) {
-            Stage s0, s1, s2, s3;
-            gload(s0, next_k());
-            gload(s1, next_k());
-            if (PF == 4) {
-                gload(s2, next_k());
-                gload(s3, next_k());
-            }
+        if (ns > 0) {  // the first PF K-steps' loads are in flight (begin)
             // step kt: its registers go to LDS buffer kt&1 (last read by
             // compute(kt-2), finished by every thread before the barrier of
             // step kt-1), the registers are refilled with step kt+PF, then
@@ -462,6 +483,18 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
 #undef OPHIP_WSQ_STAGE
         }
         __syncthreads();  // all LDS reads done before the next item's stores
+
+        // the next item's first K-steps go out now, behind this epilogue
+        const int nxt = item + stride;
+        WItem itn = it;
+        WScen scn = sc;
+        int nsn = 0;
+        if (nxt < nlist) {  // workgroup-uniform
+            uint64_t cn = 0;
+            fetch(nxt, itn, cn);
+            scn = scen[itn.s];
+            nsn = begin(itn, cn, scn);
+        }
 
         // epilogue: lane holds rows 4g+v of column r of every 16x16 tile
         float *Wn = at<float>(base, sc.Wn);
@@ -559,6 +592,9 @@ __global__ __launch_bounds__(64 * NW) void k_wsq(char *__restrict__ base,
                 part[(size_t)((BT / 64) * it.tn + h) * d + row] = x;
             }
         }
+        it = itn;
+        sc = scn;
+        ns = nsn;
     }
 }
 
